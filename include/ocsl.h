@@ -1,0 +1,167 @@
+/*
+ * ocsl.h -- C ABI of the OpenCSL MI355X sparse-Krylov hot path (libocsl_hip.so).
+ *
+ * Drop-in device replacement for the linear-solver half of the reference's
+ * CMatrix_solver (cmatrix_solver.h:27-95): the CSR system it owns
+ * (matrix / b_vector / x_vector), its two solvers CG_solve and
+ * ILU_BICGSTAB_solve, the dispatching solve(), and its BLAS helpers.  Plain C
+ * types only; every entry point returns an ocsl_status and never exits or
+ * throws.  One handle = one CSR structure + one HIP stream; calls on a handle
+ * are serialised by the caller, distinct handles are independent.
+ *
+ * CSR arrays use the reference's layout (CMatrix_solver::CCSR,
+ * cmatrix_solver.h:33-51): uint64 row_ptr[n+1], uint64 column_index[nnz]
+ * ascending per row, double value[nnz].  Column (uint64_t)-1, the sentinel
+ * CFVM::define_matrix_structure puts in boundary rows (cfvm.cpp:90-93), is
+ * accepted and ignored, as is any column >= n.
+ */
+#ifndef OCSL_H
+#define OCSL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ocsl_solver* ocsl_handle;
+
+typedef enum {
+    OCSL_OK = 0,
+    /* > 0: the solve ran; warning flags (may be OR-ed) */
+    OCSL_WARN_MAXITER = 1,     /* tolerance not reached within max_iter */
+    OCSL_WARN_BREAKDOWN = 2,   /* a Krylov denominator was zero */
+    OCSL_WARN_NONFINITE = 4,   /* residual became NaN/Inf */
+    /* < 0: hard errors, nothing was solved */
+    OCSL_ERR_ARG = -1,
+    OCSL_ERR_STRUCTURE = -2,   /* missing diagonal, unsorted row, bad row_ptr */
+    OCSL_ERR_NOMEM = -3,
+    OCSL_ERR_HIP = -4,
+    OCSL_ERR_STATE = -5,       /* values / rhs not set */
+    OCSL_ERR_NODEVICE = -6
+} ocsl_status;
+
+typedef enum { OCSL_CG = 0, OCSL_ILU_BICGSTAB = 1 } ocsl_method;
+
+/* REFERENCE: the reference's fixed-iteration semantics (CG: <= 4 iterations,
+ * stop on absolute ||r||^2 < 1e-6; BiCGSTAB: <= 3 iterations, stop on
+ * ||r||^2/||b||^2 < 1e-20; 3-sweep ILU, exact triangular solves).
+ * TOLERANCE: textbook CG / right-preconditioned ILU(0)-BiCGSTAB to
+ * ||b - A x|| / ||b|| <= tol. */
+typedef enum { OCSL_MODE_REFERENCE = 0, OCSL_MODE_TOLERANCE = 1 } ocsl_mode;
+
+/* FP64: all arithmetic in double.  MIXED: FP32 matrix, vectors and inner
+ * iterations; FP64 solution accumulator, true residual b - A x, refinement
+ * update and dot-product accumulation (TOLERANCE mode only). */
+typedef enum { OCSL_FP64 = 0, OCSL_MIXED = 1 } ocsl_precision;
+
+typedef enum { OCSL_HOST = 0, OCSL_DEVICE = 1 } ocsl_where;
+
+typedef enum { OCSL_TRI_EXACT = 0, OCSL_TRI_JACOBI = 1 } ocsl_tri_mode;
+
+typedef struct {
+    double tol;          /* relative residual target (TOLERANCE); default 1e-8 */
+    int max_iter;        /* default 20000 */
+    int ilu_sweeps;      /* fine-grained ILU(0) sweeps; default 3 (0 = exact ILU(0)) */
+    int tri_mode;        /* ocsl_tri_mode for TOLERANCE BiCGSTAB; default exact */
+    int tri_sweeps;      /* Jacobi sweeps per triangular solve; default 3 */
+    double tri_weight;   /* Jacobi damping; default 0.9 */
+    double replace_ratio;/* MIXED: FP64 residual replacement when ||r|| drops by this factor; default 0.1 */
+    int check_every;     /* iterations per device chunk between host checks; default 32 */
+    int reorder;         /* 1: reverse Cuthill-McKee row order for CG (default); 0: input order */
+    int use_graph;       /* 1: replay iteration chunks as hipGraphs (default) */
+} ocsl_options;
+
+typedef struct {
+    int status;          /* ocsl_status of the solve */
+    int iterations;      /* Krylov iterations performed */
+    int replacements;    /* MIXED: FP64 residual replacements */
+    int history_len;     /* entries available from ocsl_get_history */
+    double residual;     /* REFERENCE: the value CG_solve / ILU_BICGSTAB_solve return;
+                            TOLERANCE: final ||b - A x||/||b|| in FP64 */
+    double true_relres;  /* ||b - A x||/||b|| in FP64 (both modes) */
+    double solve_ms;     /* device time of the solve (HIP events), excluding uploads */
+    double bytes;        /* algorithmic HBM bytes of the Krylov iterations */
+} ocsl_report;
+
+/* Fills *opt with the defaults listed above. */
+void ocsl_default_options(ocsl_options* opt);
+
+/* Number of HIP devices (0 when none); never fails. */
+int ocsl_device_count(void);
+
+/* Creates a handle for an n x n CSR structure (the reference's CCSR layout).
+ * Replaces CFVM::define_matrix_structure's CMatrix_solver construction
+ * (cfvm.cpp:74-98: CMatrix_solver(n, n, nnz), M = matrix,
+ * extract_diag_indexes()).  Sentinel columns are stripped; every row must hold
+ * its diagonal.  Builds the device structure, reorderings and ILU tables. */
+ocsl_status ocsl_create(ocsl_handle* h, int64_t n, const uint64_t* row_ptr,
+                        const uint64_t* column_index);
+ocsl_status ocsl_destroy(ocsl_handle h);
+
+/* Row count and stored-entry count (including sentinels) of the handle. */
+ocsl_status ocsl_get_size(ocsl_handle h, int64_t* n, int64_t* nnz);
+
+/* Values of matrix / b_vector / x_vector (cmatrix_solver.h:87-89), in the
+ * reference's order, from host or device memory.  A device pointer is read
+ * during ocsl_solve, not copied here: keep it alive until the solve returns. */
+ocsl_status ocsl_set_values(ocsl_handle h, const double* value, ocsl_where where);
+ocsl_status ocsl_set_rhs(ocsl_handle h, const double* b, ocsl_where where);
+ocsl_status ocsl_set_x(ocsl_handle h, const double* x0, ocsl_where where);
+/* Copies x_vector out after a solve. */
+ocsl_status ocsl_get_x(ocsl_handle h, double* x, ocsl_where where);
+
+/* One linear solve.  method/mode/precision replace the global
+ * ematrix_solver_type that CMatrix_solver::solve() reads
+ * (cmatrix_solver.cpp:764-780).  opt may be NULL (defaults); rep may be NULL. */
+ocsl_status ocsl_solve(ocsl_handle h, int method, int mode, int precision,
+                       const ocsl_options* opt, ocsl_report* rep);
+
+/* CMatrix_solver::CG_solve (cmatrix_solver.cpp:635-680) and
+ * CMatrix_solver::ILU_BICGSTAB_solve (cmatrix_solver.cpp:682-762) in
+ * REFERENCE mode; *res receives their return value. */
+ocsl_status ocsl_CG_solve(ocsl_handle h, double* res);
+ocsl_status ocsl_ILU_BICGSTAB_solve(ocsl_handle h, double* res);
+
+/* Residual history of the last solve: REFERENCE mode, the value each
+ * iteration computes (||r'||^2 for CG, ||r||^2/||b||^2 + DBL_MIN for
+ * BiCGSTAB); TOLERANCE mode, ||r_k||/||b|| for k = 0..iterations. */
+ocsl_status ocsl_get_history(ocsl_handle h, double* out, int cap, int* len);
+
+/* CMatrix_solver::ABSsum / sum / norm (cmatrix_solver.cpp:233-263) of
+ * n-vectors, on the handle's device and stream. */
+ocsl_status ocsl_ABSsum(ocsl_handle h, const double* v, ocsl_where where, double* out);
+ocsl_status ocsl_sum(ocsl_handle h, const double* v, ocsl_where where, double* out);
+ocsl_status ocsl_norm(ocsl_handle h, const double* a, const double* b, ocsl_where where,
+                      double* out);
+
+/* CMatrix_solver::gemv (cmatrix_solver.cpp:338-453): out = a*(A x) + b*z
+ * with the handle's current values; z may be NULL. */
+ocsl_status ocsl_gemv(ocsl_handle h, double a, const double* x, double b, const double* z,
+                      double* out, ocsl_where where);
+
+/* CMatrix_solver::ILU_decompose (cmatrix_solver.cpp:472-552): the ILU factor
+ * values M in the reference's slot order after `sweeps` sweeps. */
+ocsl_status ocsl_ILU_decompose(ocsl_handle h, int sweeps, double* m_out, ocsl_where where);
+
+/* Measurement hook (no counterpart in the reference): launches the tolerance-
+ * mode CG SpMV kernel (q = A p fused with p.q) `reps` times on the handle's
+ * stream between two HIP events and returns its average duration and the
+ * algorithmic bytes of one launch: nnz*(sv+4) + 4*(slices+1) + 2*n*sv with
+ * sv = 8 (FP64) or 4 (MIXED).  Values must be set. */
+ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reorder, int reps,
+                              double* avg_ms, double* bytes_per_launch);
+
+/* HIP stream of the handle (a hipStream_t), for callers that pass device
+ * pointers produced on another stream. */
+void* ocsl_get_stream(ocsl_handle h);
+
+/* Last error message of the calling thread ("" if none). */
+const char* ocsl_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCSL_H */

@@ -1,0 +1,144 @@
+// The solver handle: one CSR structure on one device with one stream.
+#pragma once
+
+#include <map>
+#include <memory>
+
+#include "ocsl_common.h"
+
+namespace ocsl {
+
+// Host-side analysis of the caller's CSR structure (setup only).
+struct HostCSR {
+    int n = 0;
+    int64_t nnz_in = 0;       // entries in the caller's arrays, sentinels included
+    std::vector<int> rp, col;  // real entries only, int32
+    std::vector<int> src;      // real entry -> slot in the caller's arrays
+    std::vector<int> diag;     // slot (real CSR) of (i,i)
+};
+
+// A SELL-64 copy of the matrix in some row order (identity or RCM).
+struct Sell {
+    int n = 0, n_slices = 0;
+    int64_t total = 0;         // padded entries
+    DBuf<int> slice_ptr, col, src, perm, iperm;  // perm: new -> old (empty for identity)
+    DBuf<double> val64;
+    DBuf<float> val32;
+    bool have64 = false, have32 = false;
+    bool identity = true;
+    template <class V>
+    SellView<V> view() const;
+};
+template <>
+inline SellView<double> Sell::view<double>() const {
+    return {n, n_slices, slice_ptr.p, col.p, val64.p};
+}
+template <>
+inline SellView<float> Sell::view<float>() const {
+    return {n, n_slices, slice_ptr.p, col.p, val32.p};
+}
+
+// Level schedule of a triangular solve: rows grouped by dependency depth.
+struct Levels {
+    int n_levels = 0;
+    int max_width = 0;
+    std::vector<int> ptr_host;  // n_levels + 1
+    DBuf<int> rows, ptr;
+};
+
+// Fine-grained ILU(0) sweep tables (cmatrix_solver.cpp:519-551): for each
+// real entry p the (k, q) slot pairs whose products enter its sum, in the
+// reference's k order, and the slot of the dividing diagonal (-1 for upper).
+struct IluTables {
+    DBuf<int> pair_ptr, pair_k, pair_q, den;
+    int64_t n_pairs = 0;
+};
+
+struct Work {
+    DBuf<double> d[16];
+    DBuf<float> f[12];
+};
+
+}  // namespace ocsl
+
+struct ocsl_solver {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ocsl::HostCSR host;
+
+    // device CSR of the real entries (input order)
+    ocsl::DBuf<int> rp, col, src, diag;
+    ocsl::DBuf<double> csr_val;  // gathered values (input order, real entries)
+    ocsl::DBuf<float> csr_val32;
+    bool csr_val_valid = false, csr_val32_valid = false;
+
+    // caller data
+    ocsl::DBuf<double> val_in_own, b_own, x_own;
+    const double* val_in = nullptr;  // device pointer to the caller's value array
+    const double* b_dev = nullptr;   // device pointer to b (input order)
+    bool have_vals = false, have_b = false;
+    uint64_t val_epoch = 0;          // bumped on every ocsl_set_values
+
+    ocsl::Sell sell_id, sell_rcm;
+    uint64_t sell_id_epoch64 = ~0ull, sell_id_epoch32 = ~0ull;
+    uint64_t sell_rcm_epoch64 = ~0ull, sell_rcm_epoch32 = ~0ull;
+
+    ocsl::Levels lev_fwd, lev_bwd;
+    bool levels_built = false;
+    ocsl::IluTables ilu;
+    bool ilu_built = false;
+    ocsl::DBuf<double> ilu_m, ilu_old;
+    ocsl::DBuf<float> ilu_m32;
+
+    ocsl::Work work;
+    ocsl::DBuf<ocsl::Scalars> S;
+    ocsl::Scalars* S_host = nullptr;  // pinned mirror
+    ocsl::DBuf<double> red_partial;
+    ocsl::DBuf<unsigned> red_counter;
+    ocsl::DBuf<double> dscal;          // small scalar outputs of k_dots
+    double* dscal_host = nullptr;      // pinned mirror
+    ocsl::DBuf<double> hist;
+    std::vector<double> hist_host;
+    int hist_len = 0;
+
+    // hipGraph cache of iteration chunks, keyed by (kind, chunk)
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+
+    ocsl::RedScratch red() const { return {red_partial.p, red_counter.p}; }
+};
+
+namespace ocsl {
+
+// setup (ocsl_setup.hip)
+void analyse_structure(int64_t n, const uint64_t* rp, const uint64_t* col, HostCSR& out);
+std::vector<int> rcm_order(const HostCSR& a);
+void build_sell(const HostCSR& a, const std::vector<int>* order, Sell& s, hipStream_t st);
+void build_levels(const HostCSR& a, bool lower, Levels& L, hipStream_t st);
+void build_ilu_tables(const HostCSR& a, IluTables& t, hipStream_t st);
+
+// helpers shared by drivers (ocsl_api.hip)
+void ensure_csr_values(ocsl_solver* h, bool need32);
+void ensure_sell_values(ocsl_solver* h, Sell& s, bool need64, bool need32);
+Sell& sell_for(ocsl_solver* h, bool reorder);
+void sync(ocsl_solver* h);
+void drop_graphs(ocsl_solver* h);
+double device_dot(ocsl_solver* h, const double* a, const double* b, int n);
+void device_dots(ocsl_solver* h, int n, int nr, const double* const* a, const double* const* b,
+                 const int* kind, double* out);
+
+inline int grid_for(int64_t n, int per_block = kBlock, int cap = 4096) {
+    int64_t g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+// solvers (ocsl_cg.hip, ocsl_bicgstab.hip)
+void cg_reference(ocsl_solver* h, ocsl_report* rep);
+void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report* rep);
+void bicgstab_reference(ocsl_solver* h, ocsl_report* rep);
+void bicgstab_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report* rep);
+void ilu_decompose_device(ocsl_solver* h, int sweeps, bool want32);
+
+}  // namespace ocsl

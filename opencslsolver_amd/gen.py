@@ -1,0 +1,173 @@
+"""Seeded synthetic systems (host only, no GPU).
+
+Thin numpy wrappers over ``libocsl_gen.so`` (include/ocsl_gen.h), which fixes
+the RNG contract: std::mt19937_64 engine, U = (e >> 11) * 2^-53, Box-Muller
+normals, Fisher-Yates permutation from the top.  Matrices come back in the
+reference's CSR layout (uint64 row_ptr / column_index, float64 values), the
+layout of ``CMatrix_solver::CCSR`` (cmatrix_solver.h:33-51).
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from ._native import gen_lib
+
+SENTINEL = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+@dataclasses.dataclass
+class CSR:
+    """Square CSR matrix in the reference layout."""
+
+    row_ptr: np.ndarray  # uint64[n+1]
+    col: np.ndarray  # uint64[nnz], sentinel = 2**64-1
+    val: np.ndarray  # float64[nnz]
+
+    @property
+    def n(self) -> int:
+        return int(self.row_ptr.shape[0] - 1)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.shape[0])
+
+    def to_dense(self) -> np.ndarray:
+        n = self.n
+        a = np.zeros((n, n))
+        for i in range(n):
+            for p in range(int(self.row_ptr[i]), int(self.row_ptr[i + 1])):
+                c = int(self.col[p])
+                if c < n:
+                    a[i, c] += self.val[p]
+        return a
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        n = self.n
+        rows = np.repeat(np.arange(n), np.diff(self.row_ptr.astype(np.int64)))
+        mask = self.col < np.uint64(n)
+        out = np.zeros(n)
+        np.add.at(out, rows[mask], self.val[mask] * x[self.col[mask].astype(np.int64)])
+        return out
+
+    def with_values(self, val: np.ndarray) -> "CSR":
+        return CSR(self.row_ptr, self.col, np.ascontiguousarray(val, dtype=np.float64))
+
+
+def hex_sizes(n: int, sentinels: bool = False) -> tuple[int, int]:
+    rows, nnz = ctypes.c_int64(), ctypes.c_int64()
+    rc = gen_lib().ocsl_gen_hex_sizes(n, int(sentinels), ctypes.byref(rows), ctypes.byref(nnz))
+    if rc != 0:
+        raise ValueError(f"ocsl_gen_hex_sizes({n}) failed: {rc}")
+    return rows.value, nnz.value
+
+
+def poisson3d(n: int, seed_mesh: int = 1, seed_rhs: int | None = 2, sigma: float = 0.5,
+              leak: float = 1e-6, permute: bool = True, sentinels: bool = False):
+    """SPD pressure-correction system on an n^3 hex mesh; returns (A, b, perm).
+
+    ``seed_rhs=None`` draws b from the continuation of the mesh stream.
+    """
+    rows, nnz = hex_sizes(n, sentinels)
+    rp = np.empty(rows + 1, np.uint64)
+    col = np.empty(nnz, np.uint64)
+    val = np.empty(nnz, np.float64)
+    b = np.empty(rows, np.float64)
+    perm = np.empty(rows, np.int64)
+    sr = 0xFFFFFFFFFFFFFFFF if seed_rhs is None else seed_rhs
+    rc = gen_lib().ocsl_gen_poisson3d(n, ctypes.c_uint64(seed_mesh), ctypes.c_uint64(sr),
+                                      ctypes.c_double(sigma), ctypes.c_double(leak), int(permute),
+                                      int(sentinels), _p(rp, _u64p), _p(col, _u64p),
+                                      _p(val, _f64p), _p(b, _f64p), _p(perm, _i64p))
+    if rc != 0:
+        raise ValueError(f"ocsl_gen_poisson3d failed: {rc}")
+    return CSR(rp, col, val), b, perm
+
+
+def convdiff3d(n: int, seed_perm: int = 3, rho: float = 1.2, dt: float = 1e-3,
+               peclet: float = 10.0, permute: bool = True, sentinels: bool = False):
+    """Taylor-Green upwind convection-diffusion system; returns (A, perm)."""
+    rows, nnz = hex_sizes(n, sentinels)
+    rp = np.empty(rows + 1, np.uint64)
+    col = np.empty(nnz, np.uint64)
+    val = np.empty(nnz, np.float64)
+    perm = np.empty(rows, np.int64)
+    rc = gen_lib().ocsl_gen_convdiff3d(n, ctypes.c_uint64(seed_perm), ctypes.c_double(rho),
+                                       ctypes.c_double(dt), ctypes.c_double(peclet), int(permute),
+                                       int(sentinels), _p(rp, _u64p), _p(col, _u64p),
+                                       _p(val, _f64p), _p(perm, _i64p))
+    if rc != 0:
+        raise ValueError(f"ocsl_gen_convdiff3d failed: {rc}")
+    return CSR(rp, col, val), perm
+
+
+def uniform(seed: int, n: int, lo: float = -1.0, hi: float = 1.0) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    rc = gen_lib().ocsl_gen_uniform(ctypes.c_uint64(seed), ctypes.c_int64(n), ctypes.c_double(lo),
+                                    ctypes.c_double(hi), _p(out, _f64p))
+    if rc != 0:
+        raise ValueError("ocsl_gen_uniform failed")
+    return out
+
+
+def normal(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    rc = gen_lib().ocsl_gen_normal(ctypes.c_uint64(seed), ctypes.c_int64(n), _p(out, _f64p))
+    if rc != 0:
+        raise ValueError("ocsl_gen_normal failed")
+    return out
+
+
+def batch_plan(seed: int = 7, m: int = 64, lo: int = 22, hi: int = 80):
+    """Sizes n_k and per-system (mesh, rhs) seeds of m independent systems."""
+    n = np.empty(m, np.int32)
+    sm = np.empty(m, np.uint64)
+    sr = np.empty(m, np.uint64)
+    rc = gen_lib().ocsl_gen_batch_plan(ctypes.c_uint64(seed), m, lo, hi,
+                                       n.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       _p(sm, _u64p), _p(sr, _u64p))
+    if rc != 0:
+        raise ValueError("ocsl_gen_batch_plan failed")
+    return n, sm, sr
+
+
+def sample_slots(seed: int, a: CSR, count: int) -> np.ndarray:
+    out = np.empty(count, np.int64)
+    rc = gen_lib().ocsl_gen_sample_slots(ctypes.c_uint64(seed), ctypes.c_int64(a.n),
+                                         _p(a.row_ptr, _u64p), _p(a.col, _u64p), count,
+                                         _p(out, _i64p))
+    if rc != 0:
+        raise ValueError(f"ocsl_gen_sample_slots failed: {rc}")
+    return out
+
+
+def system_hash(a: CSR, b: np.ndarray | None = None) -> int:
+    return int(gen_lib().ocsl_gen_hash(ctypes.c_int64(a.n), _p(a.row_ptr, _u64p),
+                                       _p(a.col, _u64p), _p(a.val, _f64p), _p(b, _f64p)))
+
+
+def tridiag_unit_test_system():
+    """The 10x10 system of the reference's CG unit test
+    (cmatrix_solver_supply.h:29-51): diagonal 5, off-diagonals 1, b = 2, x0 = 1."""
+    n = 10
+    rp = [0]
+    cols, vals = [], []
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                cols.append(j)
+                vals.append(5.0 if j == i else 1.0)
+        rp.append(len(cols))
+    a = CSR(np.array(rp, np.uint64), np.array(cols, np.uint64), np.array(vals, np.float64))
+    return a, np.full(n, 2.0), np.full(n, 1.0)
+

@@ -1,0 +1,205 @@
+"""Python handle over the C ABI (include/ocsl.h).
+
+``Solver`` owns one ``ocsl_handle``: a CSR structure in the reference's
+layout (uint64 row_ptr / column_index, sentinel columns allowed) on the
+current HIP device.  Data may be numpy arrays (host) or torch CUDA tensors
+(device, read in place on the handle's stream after the caller's stream is
+synchronised).  There is no CPU fallback: without a HIP device or without
+``libocsl_hip.so`` every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._native import hip_lib
+
+_f64p = ctypes.POINTER(ctypes.c_double)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+METHODS = {"cg": _abi.CG, "ilu_bicgstab": _abi.ILU_BICGSTAB, "bicgstab": _abi.ILU_BICGSTAB}
+MODES = {"reference": _abi.MODE_REFERENCE, "tolerance": _abi.MODE_TOLERANCE}
+PRECISIONS = {"fp64": _abi.FP64, "mixed": _abi.MIXED}
+
+
+class OcslError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"ocsl status {status}: {what}")
+        self.status = status
+
+
+def _check(st: int, allow_warn: bool = False) -> int:
+    if st < 0 or (st > 0 and not allow_warn):
+        msg = hip_lib().ocsl_last_error()
+        raise OcslError(st, msg.decode() if msg else "")
+    return st
+
+
+def _is_torch_cuda(a) -> bool:
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(a, torch.Tensor) and a.is_cuda
+
+
+def _ptr(a, dtype=np.float64):
+    """(pointer, where, keepalive) of a numpy array or a torch CUDA tensor."""
+    if _is_torch_cuda(a):
+        import torch
+
+        if a.dtype != torch.float64 or not a.is_contiguous():
+            raise TypeError("device arrays must be contiguous float64 tensors")
+        torch.cuda.current_stream(a.device).synchronize()
+        return ctypes.c_void_p(a.data_ptr()), _abi.DEVICE, a
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return arr.ctypes.data_as(ctypes.c_void_p), _abi.HOST, arr
+
+
+def default_options(**kw) -> _abi.Options:
+    o = _abi.Options()
+    hip_lib().ocsl_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown option {k}")
+        setattr(o, k, v)
+    return o
+
+
+def device_count() -> int:
+    return int(hip_lib().ocsl_device_count())
+
+
+class Solver:
+    """One CSR structure on the GPU with its solvers (CMatrix_solver's solve path)."""
+
+    def __init__(self, row_ptr, column_index):
+        self._lib = hip_lib()
+        self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+        self.col = np.ascontiguousarray(column_index, dtype=np.uint64)
+        self.n = int(self.row_ptr.shape[0] - 1)
+        self.nnz = int(self.col.shape[0])
+        h = _abi.c_handle()
+        _check(self._lib.ocsl_create(ctypes.byref(h), self.n, self.row_ptr.ctypes.data_as(_u64p),
+                                     self.col.ctypes.data_as(_u64p)))
+        self._h = h
+        self._keep = {}
+
+    # -- lifetime --------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.ocsl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def stream(self) -> int:
+        return int(self._lib.ocsl_get_stream(self._h) or 0)
+
+    # -- data --------------------------------------------------------------
+    def set_values(self, val) -> None:
+        p, where, keep = _ptr(val)
+        self._keep["val"] = keep
+        _check(self._lib.ocsl_set_values(self._h, p, where))
+
+    def set_rhs(self, b) -> None:
+        p, where, keep = _ptr(b)
+        self._keep["b"] = keep
+        _check(self._lib.ocsl_set_rhs(self._h, p, where))
+
+    def set_x(self, x0) -> None:
+        p, where, keep = _ptr(x0)
+        _check(self._lib.ocsl_set_x(self._h, p, where))
+
+    def get_x(self, out=None) -> np.ndarray:
+        if out is not None and _is_torch_cuda(out):
+            _check(self._lib.ocsl_get_x(self._h, ctypes.c_void_p(out.data_ptr()), _abi.DEVICE))
+            return out
+        x = np.empty(self.n)
+        _check(self._lib.ocsl_get_x(self._h, x.ctypes.data_as(ctypes.c_void_p), _abi.HOST))
+        return x
+
+    # -- solves ------------------------------------------------------------
+    def solve(self, method="cg", mode="tolerance", precision="fp64", allow_warn=True, **opts) -> dict:
+        o = default_options(**opts)
+        rep = _abi.Report()
+        st = self._lib.ocsl_solve(self._h, METHODS[method], MODES[mode], PRECISIONS[precision],
+                                  ctypes.byref(o), ctypes.byref(rep))
+        _check(st, allow_warn=allow_warn)
+        d = rep.as_dict()
+        d["status"] = st
+        return d
+
+    def CG_solve(self) -> float:
+        """CMatrix_solver::CG_solve (cmatrix_solver.cpp:635-680)."""
+        res = ctypes.c_double()
+        _check(self._lib.ocsl_CG_solve(self._h, ctypes.byref(res)), allow_warn=True)
+        return res.value
+
+    def ILU_BICGSTAB_solve(self) -> float:
+        """CMatrix_solver::ILU_BICGSTAB_solve (cmatrix_solver.cpp:682-762)."""
+        res = ctypes.c_double()
+        _check(self._lib.ocsl_ILU_BICGSTAB_solve(self._h, ctypes.byref(res)), allow_warn=True)
+        return res.value
+
+    def history(self) -> np.ndarray:
+        ln = ctypes.c_int()
+        _check(self._lib.ocsl_get_history(self._h, None, 0, ctypes.byref(ln)))
+        out = np.empty(ln.value)
+        _check(self._lib.ocsl_get_history(self._h, out.ctypes.data_as(_f64p), ln.value,
+                                          ctypes.byref(ln)))
+        return out
+
+    # -- reference helpers -------------------------------------------------
+    def abs_sum(self, v) -> float:
+        p, where, _k = _ptr(v)
+        out = ctypes.c_double()
+        _check(self._lib.ocsl_ABSsum(self._h, p, where, ctypes.byref(out)))
+        return out.value
+
+    def sum(self, v) -> float:
+        p, where, _k = _ptr(v)
+        out = ctypes.c_double()
+        _check(self._lib.ocsl_sum(self._h, p, where, ctypes.byref(out)))
+        return out.value
+
+    def norm(self, a, b) -> float:
+        pa, where, _ka = _ptr(a)
+        pb, _w, _kb = _ptr(b)
+        out = ctypes.c_double()
+        _check(self._lib.ocsl_norm(self._h, pa, pb, where, ctypes.byref(out)))
+        return out.value
+
+    def gemv(self, a: float, x, b: float = 0.0, z=None) -> np.ndarray:
+        px, _w, _kx = _ptr(x)
+        pz, _w2, _kz = _ptr(z) if z is not None else (None, _abi.HOST, None)
+        out = np.empty(self.n)
+        _check(self._lib.ocsl_gemv(self._h, a, px, b, pz, out.ctypes.data_as(ctypes.c_void_p),
+                                   _abi.HOST))
+        return out
+
+    def ilu_decompose(self, sweeps: int = 3) -> np.ndarray:
+        out = np.empty(self.nnz)
+        _check(self._lib.ocsl_ILU_decompose(self._h, sweeps, out.ctypes.data_as(ctypes.c_void_p),
+                                            _abi.HOST))
+        return out
+
+    def time_cg_spmv(self, precision="fp64", reorder=1, reps=50):
+        """(average ms, algorithmic bytes) of one CG SpMV launch on the handle's stream."""
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.ocsl_time_cg_spmv(self._h, PRECISIONS[precision], reorder, reps,
+                                           ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value

@@ -1,0 +1,174 @@
+"""CG on the GPU through the C ABI, against the reference's golden vectors and
+the oracle."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from opencslsolver_amd import gen
+from opencslsolver_amd.gen import CSR
+from opencslsolver_amd.solver import OcslError, Solver
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+SOLVER_FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "solver_*.npz")))
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _solver(a, b, x0=None):
+    s = Solver(a.row_ptr, a.col)
+    s.set_values(a.val)
+    s.set_rhs(b)
+    s.set_x(np.zeros(a.n) if x0 is None else x0)
+    return s
+
+
+@pytest.mark.parametrize("path", SOLVER_FIXTURES, ids=lambda p: os.path.basename(p))
+def test_cg_reference_mode_matches_reference(path):
+    d = np.load(path)
+    a = CSR(d["row_ptr"], d["col"], d["val"])
+    with _solver(a, d["b"], d["x0"]) as s:
+        res = s.CG_solve()
+        x = s.get_x()
+        hist = s.history()
+    assert res == pytest.approx(float(d["cg_res"]), rel=1e-10)
+    assert _rel(x, d["cg_x"]) <= 1e-10
+    _, _, h_or = oracle.cg_reference(a, d["b"], d["x0"])
+    assert len(hist) == len(h_or)
+    assert np.allclose(hist, h_or, rtol=1e-10, atol=0)
+
+
+def test_cg_reference_mode_on_sod_systems():
+    d = np.load(os.path.join(GOLDEN, "sod_split_2steps.npz"))
+    rp, col = d["row_ptr"], d["col"]
+    checked = 0
+    for k in range(d["dump_val"].shape[0]):
+        if d["dump_method"][k] != 0:
+            continue
+        a = CSR(rp, col, d["dump_val"][k])
+        with _solver(a, d["dump_b"][k], d["dump_x0"][k]) as s:
+            res = s.CG_solve()
+            x = s.get_x()
+        assert res == pytest.approx(d["res_hist"][k], rel=1e-10, abs=1e-28)
+        assert _rel(x, d["dump_x"][k]) <= 1e-10
+        checked += 1
+    assert checked >= 2
+
+
+def test_blas_helpers_match_reference():
+    d = np.load(os.path.join(GOLDEN, "solver_poisson6s.npz"))
+    a = CSR(d["row_ptr"], d["col"], d["val"])
+    with _solver(a, d["b"], d["x0"]) as s:
+        assert _rel(s.gemv(-1.0, d["x0"], 1.0, d["b"]), d["gemv_r"]) <= 1e-14
+        assert s.norm(d["b"], d["b"]) == pytest.approx(float(d["norm_bb"]), rel=1e-13)
+        assert s.abs_sum(d["b"]) == pytest.approx(float(d["abssum_b"]), rel=1e-13)
+        assert s.sum(d["b"]) == pytest.approx(float(np.sum(d["b"])), rel=1e-12, abs=1e-12)
+
+
+@pytest.mark.parametrize("reorder", [1, 0])
+def test_cg_tolerance_fp64_matches_oracle(reorder):
+    a, b, _ = gen.poisson3d(16, seed_mesh=1, seed_rhs=2)
+    oracle.set_threads(1)
+    it_o, x_o, h_o, tr_o = oracle.cg_tol(a, b, tol=1e-8, maxit=5000)
+    with _solver(a, b) as s:
+        rep = s.solve("cg", "tolerance", "fp64", tol=1e-8, max_iter=5000, reorder=reorder)
+        x = s.get_x()
+        h = s.history()
+    assert rep["status"] == 0, rep
+    assert abs(rep["iterations"] - it_o) <= 2
+    assert rep["true_relres"] <= 1.2e-8
+    # residual histories agree closely until rounding differences accumulate
+    k = min(50, len(h), len(h_o))
+    assert np.allclose(h[:k], h_o[:k], rtol=1e-7)
+    r = b - a.matvec(x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.2e-8
+    assert _rel(x, x_o) <= 1e-5
+
+
+def test_cg_mixed_reaches_fp64_tolerance():
+    a, b, _ = gen.poisson3d(24, seed_mesh=1, seed_rhs=2)
+    with _solver(a, b) as s:
+        rep64 = s.solve("cg", "tolerance", "fp64", tol=1e-8)
+        x64 = s.get_x()
+        s.set_x(np.zeros(a.n))
+        rep = s.solve("cg", "tolerance", "mixed", tol=1e-8)
+        xm = s.get_x()
+    assert rep["status"] == 0, rep
+    assert rep["replacements"] >= 1
+    r = b - a.matvec(xm)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-8
+    assert rep["iterations"] <= 1.5 * rep64["iterations"]
+    # error against the FP64 solution, bounded by the conditioning of the
+    # near-null boundary-leak mode
+    assert np.linalg.norm(xm - x64) / np.linalg.norm(x64) <= 1e-5
+
+
+def test_cg_deterministic_and_graph_invariant():
+    a, b, _ = gen.poisson3d(20, seed_mesh=3, seed_rhs=4)
+    xs = []
+    with _solver(a, b) as s:
+        for graph in (1, 1, 0):
+            s.set_x(np.zeros(a.n))
+            s.solve("cg", "tolerance", "fp64", tol=1e-9, use_graph=graph, check_every=16)
+            xs.append(s.get_x())
+    assert np.array_equal(xs[0], xs[1])
+    assert np.array_equal(xs[0], xs[2])
+
+
+def test_cg_sentinel_input_equals_clean_input():
+    a, b, _ = gen.poisson3d(10, sentinels=True)
+    a2, b2, _ = gen.poisson3d(10, sentinels=False)
+    with _solver(a, b) as s1, _solver(a2, b2) as s2:
+        s1.solve("cg", "tolerance", "fp64", tol=1e-9)
+        s2.solve("cg", "tolerance", "fp64", tol=1e-9)
+        assert np.array_equal(s1.get_x(), s2.get_x())
+
+
+def test_api_errors_are_reported():
+    a, b, _ = gen.poisson3d(4)
+    s = Solver(a.row_ptr, a.col)
+    with pytest.raises(OcslError) as e:
+        s.solve("cg", "tolerance", "fp64")
+    assert e.value.status == -5  # values / rhs not set
+    s.set_values(a.val)
+    s.set_rhs(b)
+    with pytest.raises(OcslError):
+        s.solve("cg", "reference", "mixed")
+    bad = a.col.copy()
+    bad[0], bad[1] = bad[1], bad[0]
+    with pytest.raises(OcslError) as e:
+        Solver(a.row_ptr, bad)
+    assert e.value.status == -2
+    s.close()
+
+
+def test_cg_zero_rhs_returns_zero():
+    a, _, _ = gen.poisson3d(6)
+    with _solver(a, np.zeros(a.n)) as s:
+        rep = s.solve("cg", "tolerance", "fp64")
+        assert rep["iterations"] == 0
+        assert np.all(s.get_x() == 0)
+
+
+def test_cg_full_size_headline_system():
+    """128^3 permuted pressure system: the FP64 residual of both precisions is
+    checked by an independent host SpMV."""
+    a, b, _ = gen.poisson3d(128, seed_mesh=1, seed_rhs=2)
+    with _solver(a, b) as s:
+        rep64 = s.solve("cg", "tolerance", "fp64", tol=1e-8)
+        x64 = s.get_x()
+        s.set_x(np.zeros(a.n))
+        repm = s.solve("cg", "tolerance", "mixed", tol=1e-8)
+        xm = s.get_x()
+    for x, rep in ((x64, rep64), (xm, repm)):
+        assert rep["status"] == 0, rep
+        rel = np.linalg.norm(b - a.matvec(x)) / np.linalg.norm(b)
+        assert rel <= 1.05e-8, (rel, rep)
+    # the oracle takes 1993 iterations on this system
+    assert abs(rep64["iterations"] - 1993) <= 20

@@ -1,0 +1,227 @@
+"""The oracle pinned before it is trusted (CPU only).
+
+How each oracle function is pinned:
+
+| oracle function          | pinned by |
+|--------------------------|-----------|
+| or_gemv / or_norm / or_abssum | reference golden vectors (solver_*.npz), dense products |
+| or_diag_index            | reference ILU / triangular golden vectors (wrong slot breaks them) |
+| or_ilu_decompose         | reference golden ILU factors; fixed point = exact ILU(0) = LU for a tridiagonal matrix |
+| or_ilu_exact             | L U = A on the pattern (dense), LU of a tridiagonal matrix |
+| or_forsub / or_backsub   | reference golden vectors; L y = b, U x = y (dense) |
+| or_*_jacobi              | w = 1 with enough sweeps equals the exact solve |
+| or_cg_reference          | reference golden results (unit-test system, seeded 3D systems, Sod systems); brute-force textbook CG in numpy |
+| or_ilu_bicgstab_reference| reference golden results (seeded systems and Sod momentum/enthalpy systems) |
+| or_cg_tol / or_bicgstab_tol | numpy dense solve; residual bound; n-step exactness |
+| or_transpose / or_sampled_outer | dense transpose / outer product |
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from opencslsolver_amd import gen
+from opencslsolver_amd.gen import CSR
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+SOLVER_FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "solver_*.npz")))
+
+
+@pytest.fixture(autouse=True)
+def _one_thread():
+    oracle.set_threads(1)
+
+
+def _load(path):
+    d = np.load(path)
+    a = CSR(d["row_ptr"], d["col"], d["val"])
+    return a, d
+
+
+def _close(a, b, rtol=1e-10, atol=0.0):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    scale = max(np.max(np.abs(b)), 1e-300) if b.size else 1.0
+    return np.max(np.abs(a - b)) <= rtol * scale + atol if a.size else True
+
+
+# ---------------------------------------------------------------- golden vectors
+@pytest.mark.parametrize("path", SOLVER_FIXTURES, ids=lambda p: os.path.basename(p))
+def test_oracle_matches_reference_golden(path):
+    a, d = _load(path)
+    b, x0 = d["b"], d["x0"]
+    res, x, hist = oracle.cg_reference(a, b, x0)
+    assert res == pytest.approx(float(d["cg_res"]), rel=1e-10)
+    assert _close(x, d["cg_x"])
+    assert hist[-1] == res
+    res, x, hist = oracle.ilu_bicgstab_reference(a, b, x0)
+    ref = float(d["bicg_res"])
+    assert res == pytest.approx(ref, rel=1e-8, abs=1e-28)
+    assert _close(x, d["bicg_x"], rtol=1e-9)
+    m = oracle.ilu_decompose(a)
+    assert _close(m, d["ilu_m"])
+    y = oracle.forsub(a, d["ilu_m"], b)
+    assert _close(y, d["forsub_y"])
+    z = oracle.backsub(a, d["ilu_m"], d["forsub_y"])
+    assert _close(z, d["backsub_x"])
+    assert _close(oracle.gemv(a, x0, -1.0, 1.0, b), d["gemv_r"], rtol=1e-14)
+    assert oracle.norm(b, b) == pytest.approx(float(d["norm_bb"]), rel=1e-14)
+    assert oracle.abssum(b) == pytest.approx(float(d["abssum_b"]), rel=1e-14)
+
+
+@pytest.mark.parametrize("tag", ["cg", "bicgstab", "split"])
+def test_oracle_reproduces_reference_on_sod_systems(tag):
+    """Every dumped shock-tube solve (momentum, pressure, enthalpy) re-run by the oracle."""
+    d = np.load(os.path.join(GOLDEN, f"sod_{tag}_2steps.npz"))
+    rp, col = d["row_ptr"], d["col"]
+    hist = d["res_hist"]
+    for k in range(d["dump_val"].shape[0]):
+        a = CSR(rp, col, d["dump_val"][k])
+        b, x0, xr = d["dump_b"][k], d["dump_x0"][k], d["dump_x"][k]
+        if d["dump_method"][k] == 0:
+            res, x, _ = oracle.cg_reference(a, b, x0)
+        else:
+            res, x, _ = oracle.ilu_bicgstab_reference(a, b, x0)
+        if not np.all(np.isfinite(xr)):
+            continue
+        # ||r||^2/||b||^2 below 1e-28 is rounding noise (relative residual ~1e-14)
+        assert res == pytest.approx(hist[k], rel=1e-8, abs=1e-28), k
+        assert _close(x, xr, rtol=1e-9), k
+
+
+def test_sentinels_are_inert():
+    a, b, _ = gen.poisson3d(5, sentinels=True)
+    a2, b2, _ = gen.poisson3d(5, sentinels=False)
+    assert np.array_equal(b, b2)
+    x0 = gen.uniform(3, a.n)
+    r1 = oracle.cg_reference(a, b, x0)
+    r2 = oracle.cg_reference(a2, b2, x0)
+    assert r1[0] == r2[0] and np.array_equal(r1[1], r2[1])
+
+
+# ---------------------------------------------------------------- mathematics
+def _textbook_cg4(A, b, x):
+    """CG_solve's mathematics (two SpMVs, 1e-40 guards, absolute 1e-6 stop)."""
+    x = x.copy()
+    r = b - A @ x
+    p = r.copy()
+    res = 0.0
+    for _ in range(4):
+        q = A @ p
+        a = (r @ r) / (p @ q + 1e-40)
+        x = x + a * p
+        rn = -a * (A @ p) + r
+        res = rn @ rn
+        if res < 1e-6:
+            break
+        beta = res / (r @ r + 1e-40)
+        p = rn + beta * p
+        r = rn
+    return res, x
+
+
+def test_unit_test_system_known_answer():
+    """cmatrix_solver_supply.h:29-82 prints the CG residual without an expected
+    value; brute force gives it."""
+    a, b, x0 = gen.tridiag_unit_test_system()
+    res, x = _textbook_cg4(a.to_dense(), b, x0)
+    r2, x2, hist = oracle.cg_reference(a, b, x0)
+    assert r2 == pytest.approx(res, rel=1e-12)
+    assert _close(x2, x, rtol=1e-13)
+    assert len(hist) == 4
+
+
+def test_cg_reference_exact_in_n_steps():
+    """CG on an SPD system of size <= 4 is exact after at most 4 iterations."""
+    dense = np.array([[4.0, -1, 0, -1], [-1, 4, -1, 0], [0, -1, 4, -1], [-1, 0, -1, 4]])
+    rp = np.array([0, 3, 6, 9, 12], np.uint64)
+    col = np.array([0, 1, 3, 0, 1, 2, 1, 2, 3, 0, 2, 3], np.uint64)
+    val = np.array([dense[i, j] for i in range(4) for j in range(4) if dense[i, j] != 0])
+    A = CSR(rp, col, val)
+    bb = np.array([1.0, 2.0, 3.0, 4.0]) * 1e3
+    res, x, _ = oracle.cg_reference(A, bb, np.zeros(4))
+    assert _close(x, np.linalg.solve(dense, bb), rtol=1e-12)
+
+
+def test_ilu_sweeps_reach_lu_of_tridiagonal():
+    n = 12
+    rp, col, val = [0], [], []
+    rng = np.random.default_rng(5)
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                col.append(j)
+                val.append(6.0 + rng.random() if j == i else -1.0 - rng.random())
+        rp.append(len(col))
+    A = CSR(np.array(rp, np.uint64), np.array(col, np.uint64), np.array(val))
+    dense = A.to_dense()
+    m_exact = oracle.ilu_exact(A)
+    m_sweep = oracle.ilu_decompose(A, sweeps=2 * n)
+    assert _close(m_sweep, m_exact, rtol=1e-14)
+    L = np.eye(n)
+    U = np.zeros((n, n))
+    for i in range(n):
+        for p in range(rp[i], rp[i + 1]):
+            j = col[p]
+            (L if j < i else U)[i, j] = m_exact[p]
+    assert np.allclose(L @ U, dense, rtol=0, atol=1e-13)
+    # three sweeps (the reference's count) are exact for the first rows only
+    m3 = oracle.ilu_decompose(A, sweeps=3)
+    assert np.allclose(m3[:4], m_exact[:4], rtol=1e-14)
+    assert not np.allclose(m3, m_exact, rtol=1e-14)
+
+
+def test_triangular_solves_dense():
+    a, b, _ = gen.poisson3d(4)
+    m = oracle.ilu_decompose(a)
+    n = a.n
+    L = np.eye(n)
+    U = np.zeros((n, n))
+    for i in range(n):
+        for p in range(int(a.row_ptr[i]), int(a.row_ptr[i + 1])):
+            j = int(a.col[p])
+            if j < n:
+                (L if j < i else U)[i, j] = m[p]
+    y = oracle.forsub(a, m, b)
+    assert np.allclose(L @ y, b, rtol=0, atol=1e-12)
+    x = oracle.backsub(a, m, y)
+    assert np.allclose(U @ x, y, rtol=0, atol=1e-12)
+    # w = 1 Jacobi sweeps reach the exact solves after depth-many sweeps
+    yj = oracle.forsub_jacobi(a, m, b, sweeps=n, w=1.0)
+    assert _close(yj, y, rtol=1e-12)
+    xj = oracle.backsub_jacobi(a, m, y, sweeps=n, w=1.0)
+    assert _close(xj, x, rtol=1e-12)
+
+
+def test_cg_tolerance_vs_dense_solve():
+    a, b, _ = gen.poisson3d(6, leak=1e-2)
+    it, x, hist, tr = oracle.cg_tol(a, b, tol=1e-12, maxit=5000)
+    assert tr <= 1e-11
+    xs = np.linalg.solve(a.to_dense(), b)
+    assert _close(x, xs, rtol=1e-8)
+    assert hist[0] == pytest.approx(1.0)
+    assert hist[-1] <= 1e-12 and len(hist) == it + 1
+
+
+@pytest.mark.parametrize("ilu_sweeps,tri_sweeps", [(3, 0), (0, 0), (3, 3)])
+def test_bicgstab_tolerance_vs_dense_solve(ilu_sweeps, tri_sweeps):
+    a, _ = gen.convdiff3d(6)
+    b = gen.uniform(4, a.n)
+    it, x, hist, tr = oracle.bicgstab_tol(a, b, tol=1e-12, ilu_sweeps=ilu_sweeps,
+                                          tri_sweeps=tri_sweeps)
+    assert tr <= 1e-11
+    assert _close(x, np.linalg.solve(a.to_dense(), b), rtol=1e-9)
+
+
+def test_transpose_and_sampled_outer():
+    a, _ = gen.convdiff3d(4, sentinels=True)
+    at = oracle.transpose(a)
+    assert np.array_equal(at.to_dense(), a.to_dense().T)
+    lam, x = gen.normal(1, a.n), gen.normal(2, a.n)
+    d = oracle.sampled_outer(a, lam, x)
+    dense = -np.outer(lam, x)
+    for i in range(a.n):
+        for p in range(int(a.row_ptr[i]), int(a.row_ptr[i + 1])):
+            c = int(a.col[p])
+            assert d[p] == (dense[i, c] if c < a.n else 0.0)
