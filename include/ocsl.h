@@ -63,7 +63,7 @@ typedef enum { OCSL_TRI_EXACT = 0, OCSL_TRI_JACOBI = 1 } ocsl_tri_mode;
 typedef struct {
     double tol;          /* relative residual target (TOLERANCE); default 1e-8 */
     int max_iter;        /* default 20000 */
-    int ilu_sweeps;      /* fine-grained ILU(0) sweeps; default 3 (0 = exact ILU(0)) */
+    int ilu_sweeps;      /* fine-grained ILU(0) sweeps (>= 1); default 3 */
     int tri_mode;        /* ocsl_tri_mode for TOLERANCE BiCGSTAB; default exact */
     int tri_sweeps;      /* Jacobi sweeps per triangular solve; default 3 */
     double tri_weight;   /* Jacobi damping; default 0.9 */
@@ -71,6 +71,8 @@ typedef struct {
     int check_every;     /* iterations per device chunk between host checks; default 32 */
     int reorder;         /* 1: reverse Cuthill-McKee row order for CG (default); 0: input order */
     int use_graph;       /* 1: replay iteration chunks as hipGraphs (default) */
+    int block_size;      /* threads per workgroup of the CG kernels, 64..1024; 0 = default (1024) */
+    int grid_blocks;     /* workgroups of the CG kernels; 0 = default (512) */
 } ocsl_options;
 
 typedef struct {
@@ -117,6 +119,18 @@ ocsl_status ocsl_get_x(ocsl_handle h, double* x, ocsl_where where);
  * (cmatrix_solver.cpp:764-780).  opt may be NULL (defaults); rep may be NULL. */
 ocsl_status ocsl_solve(ocsl_handle h, int method, int mode, int precision,
                        const ocsl_options* opt, ocsl_report* rep);
+
+/* k right-hand sides sharing the handle's matrix (and ILU factors), solved
+ * together by TOLERANCE-mode ILU(0)-BiCGSTAB: every matrix / factor pass
+ * serves all k.  Replaces the three momentum solves per iteration of
+ * CFlow_solver::iterate (cflow_solver.cpp:259-301), whose matrices coincide
+ * unless no-slip viscous walls couple the components
+ * (cflow_solver_cell_face_loops.cpp:94-113).  B and X hold k column blocks of
+ * n values (B[c*n + i]); X holds the initial guesses on entry and the
+ * solutions on return.  1 <= k <= 4; reps (optional) receives k reports. */
+ocsl_status ocsl_solve_multi(ocsl_handle h, int method, int precision, int k, const double* B,
+                             double* X, ocsl_where where, const ocsl_options* opt,
+                             ocsl_report* reps);
 
 /* CMatrix_solver::CG_solve (cmatrix_solver.cpp:635-680) and
  * CMatrix_solver::ILU_BICGSTAB_solve (cmatrix_solver.cpp:682-762) in

@@ -44,6 +44,8 @@ class Options(ctypes.Structure):
         ("check_every", ctypes.c_int),
         ("reorder", ctypes.c_int),
         ("use_graph", ctypes.c_int),
+        ("block_size", ctypes.c_int),
+        ("grid_blocks", ctypes.c_int),
     ]
 
 
@@ -76,6 +78,9 @@ SIGNATURES = {
     "ocsl_get_x": (ctypes.c_int, [c_handle, ctypes.c_void_p, ctypes.c_int]),
     "ocsl_solve": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(Options), ctypes.POINTER(Report)]),
+    "ocsl_solve_multi": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.POINTER(Options), ctypes.POINTER(Report)]),
     "ocsl_CG_solve": (ctypes.c_int, [c_handle, _f64p]),
     "ocsl_ILU_BICGSTAB_solve": (ctypes.c_int, [c_handle, _f64p]),
     "ocsl_get_history": (ctypes.c_int, [c_handle, _f64p, ctypes.c_int, _i32p]),

@@ -82,18 +82,70 @@ void ensure_sell_values(ocsl_solver* h, Sell& s, bool need64, bool need32) {
     if (want32) e32 = h->val_epoch;
 }
 
+SellP& sellp_for(ocsl_solver* h, bool reorder) {
+    SellP& s = reorder ? h->sellp_rcm : h->sellp_id;
+    if (s.n == 0) {
+        drop_graphs(h);
+        if (reorder) {
+            std::vector<int> order = rcm_order(h->host);
+            build_sellp(h->host, &order, s, h->stream);
+        } else {
+            build_sellp(h->host, nullptr, s, h->stream);
+        }
+    }
+    return s;
+}
+
+// FP64 values (off-diagonals + diagonal) and, for the FP32 copy, the
+// difference form: off-diagonals rounded, diagonal replaced by the FP64 row
+// sum rounded once.
+void ensure_sellp_values(ocsl_solver* h, SellP& s, bool need64, bool need32) {
+    OCSL_REQUIRE(h->have_vals, OCSL_ERR_STATE, "values not set");
+    uint64_t& e64 = (&s == &h->sellp_rcm) ? h->sellp_rcm_epoch64 : h->sellp_id_epoch64;
+    uint64_t& e32 = (&s == &h->sellp_rcm) ? h->sellp_rcm_epoch32 : h->sellp_id_epoch32;
+    const bool want64 = (need64 || need32) && e64 != h->val_epoch;
+    const bool want32 = need32 && e32 != h->val_epoch;
+    if (!want64 && !want32) return;
+    if (!s.val64.p || !s.diag64.p) {
+        drop_graphs(h);
+        s.val64.alloc(s.total);
+        s.diag64.alloc(s.n);
+    }
+    if (need32 && (!s.val32.p || !s.diag32.p)) {
+        drop_graphs(h);
+        s.val32.alloc(s.total);
+        s.diag32.alloc(s.n);
+    }
+    const int g = grid_for(s.total);
+    if (want64) {
+        k_gather_vals<double><<<g, kBlock, 0, h->stream>>>(s.total, s.src.p, h->val_in, s.val64.p);
+        k_gather_vals<double><<<grid_for(s.n), kBlock, 0, h->stream>>>(s.n, s.diag_src.p, h->val_in, s.diag64.p);
+        e64 = h->val_epoch;
+    }
+    if (want32) {
+        k_gather_vals<float><<<g, kBlock, 0, h->stream>>>(s.total, s.src.p, h->val_in, s.val32.p);
+        k_sellp_rowsum<<<grid_for((int64_t)s.n_slices * kWave), kBlock, 0, h->stream>>>(s.view<double>(),
+                                                                                       s.diag32.p);
+        e32 = h->val_epoch;
+    }
+    OCSL_HIP(hipGetLastError());
+}
+
 void device_dots(ocsl_solver* h, int n, int nr, const double* const* a, const double* const* b,
                  const int* kind, double* out) {
     const int g = red_blocks(kBlock * 4, n);
     if (nr == 1) {
         DotArgs<1> d{{a[0]}, {b[0]}, {kind[0]}};
-        k_dots<1><<<g, kBlock, 0, h->stream>>>(n, d, h->red(), h->dscal.p);
+        k_dots<1><<<g, kBlock, 0, h->stream>>>(n, d, h->red());
+        k_finalize<1, FinStore<1>><<<1, kBlock, 0, h->stream>>>(h->red(), g, FinStore<1>{h->dscal.p});
     } else if (nr == 2) {
         DotArgs<2> d{{a[0], a[1]}, {b[0], b[1]}, {kind[0], kind[1]}};
-        k_dots<2><<<g, kBlock, 0, h->stream>>>(n, d, h->red(), h->dscal.p);
+        k_dots<2><<<g, kBlock, 0, h->stream>>>(n, d, h->red());
+        k_finalize<2, FinStore<2>><<<1, kBlock, 0, h->stream>>>(h->red(), g, FinStore<2>{h->dscal.p});
     } else {
         DotArgs<3> d{{a[0], a[1], a[2]}, {b[0], b[1], b[2]}, {kind[0], kind[1], kind[2]}};
-        k_dots<3><<<g, kBlock, 0, h->stream>>>(n, d, h->red(), h->dscal.p);
+        k_dots<3><<<g, kBlock, 0, h->stream>>>(n, d, h->red());
+        k_finalize<3, FinStore<3>><<<1, kBlock, 0, h->stream>>>(h->red(), g, FinStore<3>{h->dscal.p});
     }
     OCSL_HIP(hipGetLastError());
     OCSL_HIP(hipMemcpyAsync(h->dscal_host, h->dscal.p, sizeof(double) * nr, hipMemcpyDeviceToHost,
@@ -158,6 +210,8 @@ void ocsl_default_options(ocsl_options* o) {
     o->check_every = 32;
     o->reorder = 1;
     o->use_graph = 1;
+    o->block_size = 0;
+    o->grid_blocks = 0;
 }
 
 int ocsl_device_count(void) {
@@ -191,8 +245,6 @@ ocsl_status ocsl_create(ocsl_handle* out, int64_t n, const uint64_t* row_ptr,
         h->S.alloc(1);
         OCSL_HIP(hipHostMalloc((void**)&h->S_host, sizeof(Scalars), hipHostMallocDefault));
         h->red_partial.alloc(kMaxRed * kMaxRedBlocks);
-        h->red_counter.alloc(1);
-        OCSL_HIP(hipMemsetAsync(h->red_counter.p, 0, sizeof(unsigned), h->stream));
         h->dscal.alloc(8);
         OCSL_HIP(hipHostMalloc((void**)&h->dscal_host, sizeof(double) * 8, hipHostMallocDefault));
         sync(h);
@@ -299,6 +351,43 @@ ocsl_status ocsl_solve(ocsl_handle h, int method, int mode, int precision, const
     });
     if (st != OCSL_OK) return st;
     return (ocsl_status)r->status;
+}
+
+ocsl_status ocsl_solve_multi(ocsl_handle h, int method, int precision, int k, const double* B,
+                             double* X, ocsl_where where, const ocsl_options* opt,
+                             ocsl_report* reps) {
+    if (!h || !B || !X || k < 1 || k > 4) return OCSL_ERR_ARG;
+    std::vector<ocsl_report> local(k);
+    ocsl_report* r = reps ? reps : local.data();
+    std::memset(r, 0, sizeof(ocsl_report) * k);
+    ocsl_options o;
+    ocsl_default_options(&o);
+    if (opt) o = *opt;
+    ocsl_status st = guarded([&] {
+        OCSL_REQUIRE(method == OCSL_ILU_BICGSTAB, OCSL_ERR_ARG, "multi-RHS solves use ILU_BICGSTAB");
+        OCSL_REQUIRE(precision == OCSL_FP64 || precision == OCSL_MIXED, OCSL_ERR_ARG, "bad precision");
+        OCSL_REQUIRE(o.max_iter > 0 && o.check_every > 0 && o.tol >= 0.0, OCSL_ERR_ARG, "bad options");
+        OCSL_REQUIRE(h->have_vals, OCSL_ERR_STATE, "values not set");
+        const size_t cnt = (size_t)h->host.n * k;
+        DBuf<double> bd, xd;
+        const double* Bd = B;
+        double* Xd = X;
+        if (where == OCSL_HOST) {
+            bd.upload(B, cnt, h->stream);
+            xd.upload(X, cnt, h->stream);
+            Bd = bd.p;
+            Xd = xd.p;
+        }
+        bicgstab_multi(h, precision == OCSL_MIXED, o, k, Bd, Xd, r);
+        if (where == OCSL_HOST) {
+            OCSL_HIP(hipMemcpyAsync(X, Xd, sizeof(double) * cnt, hipMemcpyDeviceToHost, h->stream));
+            sync(h);
+        }
+    });
+    if (st != OCSL_OK) return st;
+    int worst = 0;
+    for (int c = 0; c < k; ++c) worst |= r[c].status;
+    return (ocsl_status)worst;
 }
 
 ocsl_status ocsl_CG_solve(ocsl_handle h, double* res) {

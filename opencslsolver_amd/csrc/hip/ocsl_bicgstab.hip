@@ -1,17 +1,819 @@
-// ILU(0)-BiCGSTAB drivers (filled in below).
+// ILU(0)-BiCGSTAB.
+//   ilu_decompose_device : the reference's fine-grained ILU(0) sweeps
+//                          (cmatrix_solver.cpp:519-551), one thread per entry
+//                          over precomputed (k, q) slot pairs.
+//   triangular solves    : exact, level-scheduled (cmatrix_solver.cpp:555-571,
+//                          595-610), or damped Jacobi sweeps (the reference's
+//                          commented-out variants, :573-592, :612-631).
+//   bicgstab_reference   : CMatrix_solver::ILU_BICGSTAB_solve
+//                          (cmatrix_solver.cpp:682-762) with its quirks.
+//   bicgstab_tolerance   : right-preconditioned BiCGSTAB (van der Vorst 1992)
+//                          to a relative residual, device-resident scalars,
+//                          1 or 3 right-hand sides sharing A and M (one matrix
+//                          pass serves all of them), FP64 or MIXED (FP32
+//                          storage, FP64 arithmetic, FP64 solution and
+//                          residual replacement).
+#include <cfloat>
+#include <cmath>
+
 #include "ocsl_handle.h"
 #include "ocsl_kernels.h"
 
 namespace ocsl {
 
-void ilu_decompose_device(ocsl_solver*, int, bool) {
-    throw Error{OCSL_ERR_ARG, "ILU not implemented yet"};
+// ------------------------------------------------------------------ ILU sweeps
+__global__ void k_ilu_sweep(int nnz, const double* __restrict__ a, const double* __restrict__ old,
+                            double* __restrict__ nw, const int* __restrict__ pptr,
+                            const int* __restrict__ pk, const int* __restrict__ pq,
+                            const int* __restrict__ den) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += gridDim.x * blockDim.x) {
+        double dum = 0.0;
+        for (int t = pptr[p]; t < pptr[p + 1]; ++t) dum += old[pk[t]] * old[pq[t]];
+        const int d = den[p];
+        nw[p] = d >= 0 ? (a[p] - dum) / old[d] : a[p] - dum;
+    }
 }
-void bicgstab_reference(ocsl_solver*, ocsl_report*) {
-    throw Error{OCSL_ERR_ARG, "BiCGSTAB not implemented yet"};
+
+__global__ void k_to_float(int n, const double* __restrict__ in, float* __restrict__ out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
 }
-void bicgstab_tolerance(ocsl_solver*, bool, const ocsl_options&, ocsl_report*) {
-    throw Error{OCSL_ERR_ARG, "BiCGSTAB not implemented yet"};
+
+void ilu_decompose_device(ocsl_solver* h, int sweeps, bool want32) {
+    OCSL_REQUIRE(sweeps >= 0, OCSL_ERR_ARG, "ILU sweeps must be >= 0");
+    if (!h->ilu_built) {
+        build_ilu_tables(h->host, h->ilu, h->stream);
+        h->ilu_built = true;
+    }
+    const int nnz = (int)h->host.col.size();
+    h->ilu_m.alloc(nnz);
+    h->ilu_old.alloc(nnz);
+    // M^0 = A (copy_same_structure, cmatrix_solver.cpp:702)
+    double* cur = h->ilu_m.p;
+    double* prev = h->ilu_old.p;
+    if (sweeps % 2 == 1) std::swap(cur, prev);  // the last sweep lands in ilu_m
+    OCSL_HIP(hipMemcpyAsync(cur, h->csr_val.p, sizeof(double) * nnz, hipMemcpyDeviceToDevice, h->stream));
+    const int g = grid_for(nnz);
+    for (int s = 0; s < sweeps; ++s) {
+        std::swap(cur, prev);
+        k_ilu_sweep<<<g, kBlock, 0, h->stream>>>(nnz, h->csr_val.p, prev, cur, h->ilu.pair_ptr.p,
+                                                 h->ilu.pair_k.p, h->ilu.pair_q.p, h->ilu.den.p);
+    }
+    OCSL_HIP(hipGetLastError());
+    if (want32) {
+        h->ilu_m32.alloc(nnz);
+        k_to_float<<<g, kBlock, 0, h->stream>>>(nnz, h->ilu_m.p, h->ilu_m32.p);
+        OCSL_HIP(hipGetLastError());
+    }
+}
+
+// ------------------------------------------------------------------ triangular solves
+struct TriCSR {
+    const int* rp;
+    const int* col;
+    const int* diag;
+};
+
+// lower: out_i = in_i - sum_{col < i} M x_col (unit diagonal)
+// upper: out_i = (in_i - sum_{col > i} M x_col) / M_ii
+// The sum is formed first and subtracted once, as the reference does.
+template <class VM, class X, int NR>
+__device__ __forceinline__ void tri_row(int i, TriCSR t, const VM* __restrict__ m, const X* in, X* out,
+                                        bool lower) {
+    double sum[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) sum[c] = 0.0;
+    const int d = t.diag[i];
+    const int p0 = lower ? t.rp[i] : d + 1;
+    const int p1 = lower ? d : t.rp[i + 1];
+    for (int p = p0; p < p1; ++p) {
+        const double mv = (double)m[p];
+        const int j = t.col[p];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) sum[c] += mv * (double)out[(size_t)j * NR + c];
+    }
+    const double dv = lower ? 1.0 : (double)m[d];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+        const double v = (double)in[(size_t)i * NR + c] - sum[c];
+        out[(size_t)i * NR + c] = (X)(lower ? v : v / dv);
+    }
+}
+
+template <class VM, class X, int NR>
+__global__ void k_tri_level(const int* __restrict__ rows, int begin, int count, TriCSR t,
+                            const VM* __restrict__ m, const X* in, X* out, int lower,
+                            const Scalars* S) {
+    if (S && S->halt) return;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x)
+        tri_row<VM, X, NR>(rows[begin + k], t, m, in, out, lower != 0);
+}
+
+// All levels in one workgroup; a barrier separates consecutive levels.
+constexpr int kTriBlock = 1024;
+template <class VM, class X, int NR>
+__global__ void __launch_bounds__(kTriBlock) k_tri_block(const int* __restrict__ rows,
+                                                         const int* __restrict__ lptr, int n_levels,
+                                                         TriCSR t, const VM* __restrict__ m, const X* in,
+                                                         X* out, int lower, const Scalars* S) {
+    if (S && S->halt) return;
+    for (int l = 0; l < n_levels; ++l) {
+        const int b = lptr[l], e = lptr[l + 1];
+        for (int k = b + threadIdx.x; k < e; k += kTriBlock) tri_row<VM, X, NR>(rows[k], t, m, in, out, lower != 0);
+        __syncthreads();
+    }
+}
+
+// One damped Jacobi sweep: out = w * (in - off(M) old) [/ M_ii] + (1 - w) old
+template <class VM, class X, int NR>
+__global__ void k_tri_jacobi(int n, TriCSR t, const VM* __restrict__ m, const X* __restrict__ in,
+                             const X* __restrict__ old, X* __restrict__ out, int lower, double w,
+                             int first, const Scalars* S) {
+    if (S && S->halt) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double sum[NR];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) sum[c] = 0.0;
+        const int d = t.diag[i];
+        if (!first) {
+            const int p0 = lower ? t.rp[i] : d + 1;
+            const int p1 = lower ? d : t.rp[i + 1];
+            for (int p = p0; p < p1; ++p) {
+                const double mv = (double)m[p];
+                const int j = t.col[p];
+#pragma unroll
+                for (int c = 0; c < NR; ++c) sum[c] += mv * (double)old[(size_t)j * NR + c];
+            }
+        }
+        const double inv = lower ? 1.0 : 1.0 / (double)m[d];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+            const double o = first ? 0.0 : (double)old[(size_t)i * NR + c];
+            out[(size_t)i * NR + c] = (X)(w * inv * ((double)in[(size_t)i * NR + c] - sum[c]) + (1.0 - w) * o);
+        }
+    }
+}
+
+namespace {
+
+TriCSR tri_csr(ocsl_solver* h) { return {h->rp.p, h->col.p, h->diag.p}; }
+
+void ensure_levels(ocsl_solver* h) {
+    if (h->levels_built) return;
+    build_levels(h->host, true, h->lev_fwd, h->stream);
+    build_levels(h->host, false, h->lev_bwd, h->stream);
+    h->levels_built = true;
+}
+
+bool use_block_tri(const Levels& L, int n) {
+    return n <= 65536 || (int64_t)L.n_levels * 256 > n;
+}
+
+// Exact solve (lower or upper) of NR interleaved vectors.
+template <class VM, class X, int NR>
+void tri_exact(ocsl_solver* h, const VM* m, const X* in, X* out, bool lower, const Scalars* S) {
+    const Levels& L = lower ? h->lev_fwd : h->lev_bwd;
+    const int n = h->host.n;
+    if (use_block_tri(L, n)) {
+        k_tri_block<VM, X, NR><<<1, kTriBlock, 0, h->stream>>>(L.rows.p, L.ptr.p, L.n_levels, tri_csr(h), m, in,
+                                                               out, lower, S);
+    } else {
+        for (int l = 0; l < L.n_levels; ++l) {
+            const int b = L.ptr_host[l], cnt = L.ptr_host[l + 1] - b;
+            k_tri_level<VM, X, NR><<<grid_for(cnt), kBlock, 0, h->stream>>>(L.rows.p, b, cnt, tri_csr(h), m,
+                                                                           in, out, lower, S);
+        }
+    }
+}
+
+// Jacobi sweeps into out, tmp as the ping-pong partner.
+template <class VM, class X, int NR>
+void tri_jacobi(ocsl_solver* h, const VM* m, const X* in, X* out, X* tmp, bool lower, int sweeps, double w,
+                const Scalars* S) {
+    const int n = h->host.n;
+    X* bufs[2] = {out, tmp};
+    int cur = (sweeps % 2 == 1) ? 0 : 1;  // the last sweep writes `out`
+    for (int s = 0; s < sweeps; ++s) {
+        X* dst = bufs[cur];
+        const X* src = bufs[cur ^ 1];
+        k_tri_jacobi<VM, X, NR><<<grid_for(n), kBlock, 0, h->stream>>>(n, tri_csr(h), m, in, src, dst, lower, w,
+                                                                       s == 0, S);
+        cur ^= 1;
+    }
+}
+
+}  // namespace
+
+// M^-1 in = U^-1 L^-1 in, through tmp (and tmp2 for Jacobi).
+template <class VM, class X, int NR>
+static void apply_precond(ocsl_solver* h, const VM* m, const X* in, X* tmp, X* tmp2, X* out,
+                          const ocsl_options& o, const Scalars* S) {
+    if (o.tri_mode == OCSL_TRI_JACOBI) {
+        tri_jacobi<VM, X, NR>(h, m, in, tmp, tmp2, true, o.tri_sweeps, o.tri_weight, S);
+        tri_jacobi<VM, X, NR>(h, m, tmp, out, tmp2, false, o.tri_sweeps, o.tri_weight, S);
+    } else {
+        tri_exact<VM, X, NR>(h, m, in, tmp, true, S);
+        tri_exact<VM, X, NR>(h, m, tmp, out, false, S);
+    }
+}
+
+// ------------------------------------------------------------------ reference mode
+void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
+    const int n = h->host.n;
+    ensure_csr_values(h, false);
+    ensure_levels(h);
+    Sell& s = sell_for(h, false);
+    ensure_sell_values(h, s, true, false);
+    Work& w = h->work;
+    for (int i = 0; i < 11; ++i) work_buf(h, w.d[i], n);
+    double *r = w.d[0].p, *p = w.d[1].p, *v = w.d[2].p, *r0 = w.d[3].p, *sv = w.d[4].p, *dum = w.d[5].p;
+    double *hh = w.d[6].p, *t = w.d[7].p, *y = w.d[8].p, *z = w.d[9].p, *dum2 = w.d[10].p;
+    double* x = h->x_own.p;
+    const double* b = h->b_dev;
+    ocsl_options o;
+    ocsl_default_options(&o);
+    const int gv = grid_for(n);
+    auto spmv = [&](double a, const double* xin, double bz, const double* zin, double* out) {
+        k_sell_axpby<double><<<grid_for((int64_t)s.n_slices * kWave), kBlock, 0, h->stream>>>(
+            s.view<double>(), a, xin, bz, zin, out);
+    };
+    auto axpby = [&](double a, const double* xa, double bb, const double* ya, double* out) {
+        k_axpby<<<gv, kBlock, 0, h->stream>>>(n, a, xa, bb, ya, out);
+    };
+    auto dot2 = [&](const double* a1, const double* b1, const double* a2, const double* b2, double* out) {
+        const double* aa[2] = {a1, a2};
+        const double* bb[2] = {b1, b2};
+        const int kind[2] = {0, 0};
+        device_dots(h, n, 2, aa, bb, kind, out);
+    };
+    std::vector<double> hist;
+    OCSL_HIP(hipEventRecord(h->ev0, h->stream));
+    ilu_decompose_device(h, 3, false);  // M.copy_same_structure + ILU_decompose
+    const double* m = h->ilu_m.p;       // allocated by the call above on first use
+    OCSL_HIP(hipMemsetAsync(p, 0, sizeof(double) * n, h->stream));
+    OCSL_HIP(hipMemsetAsync(v, 0, sizeof(double) * n, h->stream));
+    spmv(-1.0, x, 1.0, b, r);
+    OCSL_HIP(hipMemcpyAsync(r0, r, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+    double alpha = 1.0, omega = 1.0, res = 0.0;
+    int it = 0;
+    for (; it < 3; ++it) {
+        const double rhon = device_dot(h, r0, r, n);
+        axpby(1.0, p, -1.0 * omega, v, dum);
+        axpby(1.0, r, omega, dum, p);
+        tri_exact<double, double, 1>(h, m, p, dum, true, nullptr);
+        tri_exact<double, double, 1>(h, m, dum, y, false, nullptr);
+        spmv(1.0, y, 0.0, nullptr, v);
+        const double dum1 = device_dot(h, r0, v, n);
+        alpha = rhon / (dum1 + DBL_MIN);
+        axpby(1.0, x, alpha, y, hh);
+        double d[2];
+        dot2(r, r, b, b, d);
+        res = d[0] / d[1] + DBL_MIN;
+        hist.push_back(res);
+        if (res < 1.0e-20) { ++it; break; }
+        axpby(-1.0 * alpha, v, 1.0, r, sv);
+        tri_exact<double, double, 1>(h, m, sv, dum, true, nullptr);
+        tri_exact<double, double, 1>(h, m, dum, z, false, nullptr);
+        spmv(1.0, z, 0.0, nullptr, t);
+        tri_exact<double, double, 1>(h, m, t, dum, true, nullptr);
+        tri_exact<double, double, 1>(h, m, sv, dum2, true, nullptr);
+        dot2(dum, dum2, dum, dum, d);
+        omega = d[0] / (d[1] + 1.0e-40);
+        axpby(1.0, hh, omega, z, x);
+        axpby(-1.0 * omega, t, 1.0, sv, r);
+    }
+    OCSL_HIP(hipGetLastError());
+    OCSL_HIP(hipEventRecord(h->ev1, h->stream));
+    sync(h);
+    float ms = 0.f;
+    OCSL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->hist_host = hist;
+    h->hist_len = (int)hist.size();
+    rep->status = std::isfinite(res) ? OCSL_OK : OCSL_WARN_NONFINITE;
+    rep->iterations = it;
+    rep->residual = res;
+    {
+        const int g = red_blocks(kBlock * 4, n);
+        k_true_residual<double><<<g, kBlock, 0, h->stream>>>(s.view<double>(), b, x, h->red());
+        k_finalize<2, FinStore<2>><<<1, kBlock, 0, h->stream>>>(h->red(), g, FinStore<2>{h->dscal.p});
+        OCSL_HIP(hipMemcpyAsync(h->dscal_host, h->dscal.p, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                h->stream));
+        sync(h);
+        const double rr = h->dscal_host[0], bb = h->dscal_host[1];
+        rep->true_relres = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    }
+    rep->history_len = h->hist_len;
+    rep->solve_ms = ms;
+    const double nnz = (double)h->host.col.size();
+    const double b_spmv = nnz * 12.0 + 16.0 * n;
+    const double b_tri = nnz * 12.0 + 8.0 * n + 16.0 * n;
+    rep->bytes = it * (2.0 * b_spmv + 3.0 * b_tri + 20.0 * 8.0 * n);
+}
+
+// ------------------------------------------------------------------ tolerance mode
+template <class V, class X, int NR>
+struct BiArgs {
+    SellView<V> A;
+    const X* rh;  // shadow residual r^
+    X *r, *p, *v, *s, *t, *yh, *zh;
+    double* x;
+    Scalars* S;
+    double* hist;
+    RedScratch red;
+    int n;
+};
+
+// p = r + beta (p - omega v); on the first iteration beta = 0, p = v = 0.
+template <class V, class X, int NR>
+__global__ void __launch_bounds__(kBlock) k_bi_p(BiArgs<V, X, NR> c) {
+    Scalars* S = c.S;
+    if (S->done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) S->halt = 1;
+        return;
+    }
+    double beta[NR], omega[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        beta[k] = S->donek[k] ? 0.0 : S->betak[k];
+        omega[k] = S->omegak[k];
+    }
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < c.n; i += gridDim.x * kBlock) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const size_t q = (size_t)i * NR + k;
+            c.p[q] = (X)((double)c.r[q] + beta[k] * ((double)c.p[q] - omega[k] * (double)c.v[q]));
+        }
+    }
+}
+
+// v = A yh ; alpha = rho / (r^ . v)
+template <class V, class X, int NR>
+__global__ void __launch_bounds__(kBlock) k_bi_spmv_v(BiArgs<V, X, NR> c) {
+    Scalars* S = c.S;
+    if (S->halt) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int g = gridDim.x;
+    const int lb = xcd_block(blockIdx.x, g);
+    const int spb = (c.A.n_slices + g - 1) / g;
+    const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc[k] = 0.0;
+    for (int sl = s0 + threadIdx.x / kWave; sl < s1; sl += kWavesPerBlock) {
+        const int row = sl * kWave + lane;
+        double out[NR];
+        sell_rows_nr<V, X, NR>(c.A, sl, lane, row, c.yh, out);
+        if (row < c.A.n) {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const X vv = (X)out[k];
+                c.v[(size_t)row * NR + k] = vv;
+                acc[k] += (double)c.rh[(size_t)row * NR + k] * (double)vv;
+            }
+        }
+    }
+    block_partials<NR>(acc, c.red);
+}
+
+// alpha = rho / (r^ . v) per right-hand side
+template <int NR>
+struct FinBiSpmvV {
+    Scalars* S;
+    __device__ bool active() const { return !S->halt; }
+    __device__ void operator()(const double* t) const {
+        for (int k = 0; k < NR; ++k) {
+            if (S->donek[k]) { S->alphak[k] = 0.0; continue; }
+            if (t[k] == 0.0 || !isfinite(t[k])) {
+                S->breakdown |= (t[k] == 0.0);
+                S->nonfinite |= !isfinite(t[k]);
+                S->donek[k] = 1;
+                S->alphak[k] = 0.0;
+            } else {
+                S->alphak[k] = S->rhok[k] / t[k];
+            }
+        }
+    }
+};
+
+// s = r - alpha v ; early exit test on ||s||
+template <class V, class X, int NR>
+__global__ void __launch_bounds__(kBlock) k_bi_s(BiArgs<V, X, NR> c) {
+    Scalars* S = c.S;
+    if (S->halt) return;
+    double alpha[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) alpha[k] = S->alphak[k];
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc[k] = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < c.n; i += gridDim.x * kBlock) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const size_t q = (size_t)i * NR + k;
+            const X sv = (X)((double)c.r[q] - alpha[k] * (double)c.v[q]);
+            c.s[q] = sv;
+            acc[k] += (double)sv * (double)sv;
+        }
+    }
+    block_partials<NR>(acc, c.red);
+}
+
+// ||s||: right-hand sides already converged at s exit early (x += alpha yh)
+template <int NR>
+struct FinBiS {
+    Scalars* S;
+    __device__ bool active() const { return !S->halt; }
+    __device__ void operator()(const double* t) const {
+        for (int k = 0; k < NR; ++k) {
+            S->ssk[k] = t[k];
+            const double scale = S->bbk[k] > 0.0 ? S->bbk[k] : 1.0;
+            S->earlyk[k] = (!S->donek[k] && t[k] <= S->tol2 * scale) ? 1 : 0;
+        }
+    }
+};
+
+// t = A zh ; omega = (t . s) / (t . t)
+template <class V, class X, int NR>
+__global__ void __launch_bounds__(kBlock) k_bi_spmv_t(BiArgs<V, X, NR> c) {
+    Scalars* S = c.S;
+    if (S->halt) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int g = gridDim.x;
+    const int lb = xcd_block(blockIdx.x, g);
+    const int spb = (c.A.n_slices + g - 1) / g;
+    const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
+    double acc[2 * NR];
+#pragma unroll
+    for (int k = 0; k < 2 * NR; ++k) acc[k] = 0.0;
+    for (int sl = s0 + threadIdx.x / kWave; sl < s1; sl += kWavesPerBlock) {
+        const int row = sl * kWave + lane;
+        double out[NR];
+        sell_rows_nr<V, X, NR>(c.A, sl, lane, row, c.zh, out);
+        if (row < c.A.n) {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const X tv = (X)out[k];
+                c.t[(size_t)row * NR + k] = tv;
+                acc[k] += (double)tv * (double)c.s[(size_t)row * NR + k];
+                acc[NR + k] += (double)tv * (double)tv;
+            }
+        }
+    }
+    block_partials<2 * NR>(acc, c.red);
+}
+
+// omega = (t . s) / (t . t) per right-hand side
+template <int NR>
+struct FinBiSpmvT {
+    Scalars* S;
+    __device__ bool active() const { return !S->halt; }
+    __device__ void operator()(const double* t) const {
+        for (int k = 0; k < NR; ++k) {
+            if (S->donek[k] || S->earlyk[k]) { S->omegak[k] = 0.0; continue; }
+            if (t[NR + k] == 0.0 || !isfinite(t[NR + k])) {
+                S->breakdown |= (t[NR + k] == 0.0);
+                S->nonfinite |= !isfinite(t[NR + k]);
+                S->omegak[k] = 0.0;
+                S->donek[k] = 1;
+            } else {
+                S->omegak[k] = t[k] / t[NR + k];
+            }
+        }
+    }
+};
+
+// x += alpha yh + omega zh ; r = s - omega t ; rr, rho = r^ . r, beta
+template <class V, class X, int NR>
+__global__ void __launch_bounds__(kBlock) k_bi_xr(BiArgs<V, X, NR> c) {
+    Scalars* S = c.S;
+    if (S->halt) return;
+    double alpha[NR], omega[NR];
+    bool skip[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        alpha[k] = S->alphak[k];
+        omega[k] = S->omegak[k];
+        skip[k] = S->donek[k] && !S->earlyk[k];
+    }
+    double acc[2 * NR];
+#pragma unroll
+    for (int k = 0; k < 2 * NR; ++k) acc[k] = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < c.n; i += gridDim.x * kBlock) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const size_t q = (size_t)i * NR + k;
+            double rv = (double)c.r[q];
+            if (!skip[k]) {
+                c.x[q] += alpha[k] * (double)c.yh[q] + omega[k] * (double)c.zh[q];
+                rv = (double)c.s[q] - omega[k] * (double)c.t[q];
+                c.r[q] = (X)rv;
+                rv = (double)(X)rv;
+            }
+            acc[k] += rv * rv;
+            acc[NR + k] += (double)c.rh[q] * rv;
+        }
+    }
+    block_partials<2 * NR>(acc, c.red);
+}
+
+// rho_new = r^ . r, beta, iteration count, history, convergence
+template <int NR>
+struct FinBiXr {
+    Scalars* S;
+    double* hist;
+    __device__ bool active() const { return !S->halt; }
+    __device__ void operator()(const double* t) const {
+        const int it = ++S->iter;
+        bool all = true;
+        for (int k = 0; k < NR; ++k) {
+            const double scale = S->bbk[k] > 0.0 ? S->bbk[k] : 1.0;
+            if (!S->donek[k]) {
+                const double rho_new = t[NR + k];
+                S->betak[k] = (rho_new / S->rhok[k]) * (S->alphak[k] / S->omegak[k]);
+                S->rhok[k] = rho_new;
+                S->rrk[k] = t[k];
+                if (S->earlyk[k] || t[k] <= S->tol2 * scale) S->donek[k] = 1;
+                if (!isfinite(t[k])) { S->nonfinite = 1; S->donek[k] = 1; }
+                if (rho_new == 0.0 && !S->donek[k]) { S->breakdown = 1; S->donek[k] = 1; }
+                if (sqrt(t[k]) < S->replace_ratio * S->rrefk[k]) S->need_update = 1;
+            }
+            S->earlyk[k] = 0;
+            if (hist && (it + 1) * NR <= S->hist_cap) hist[(size_t)it * NR + k] = sqrt(S->rrk[k] / scale);
+            all = all && S->donek[k];
+        }
+        if (all || it >= S->max_iter) S->done = 1;
+    }
+};
+
+// r = b - A x in FP64 (standard form), stored in X; on the first call also
+// r^ = r, p = v = 0 and the scalar state; later calls (residual replacement)
+// recompute rho = r^ . r.
+template <class X, int NR>
+__global__ void __launch_bounds__(kBlock) k_bi_residual(SellView<double> A, const double* __restrict__ b,
+                                                       const double* __restrict__ x, X* r, X* rh, X* p,
+                                                       X* v, RedScratch red, int first) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int g = gridDim.x;
+    const int lb = xcd_block(blockIdx.x, g);
+    const int spb = (A.n_slices + g - 1) / g;
+    const int s0 = lb * spb, s1 = min(A.n_slices, s0 + spb);
+    double acc[3 * NR];
+#pragma unroll
+    for (int k = 0; k < 3 * NR; ++k) acc[k] = 0.0;
+    for (int sl = s0 + threadIdx.x / kWave; sl < s1; sl += kWavesPerBlock) {
+        const int row = sl * kWave + lane;
+        double ax[NR];
+        sell_rows_nr<double, double, NR>(A, sl, lane, row, x, ax);
+        if (row < A.n) {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const size_t q = (size_t)row * NR + k;
+                const double bv = b[q];
+                const X rv = (X)(bv - ax[k]);
+                r[q] = rv;
+                if (first) {
+                    rh[q] = rv;
+                    p[q] = (X)0;
+                    v[q] = (X)0;
+                }
+                acc[k] += (double)rv * (double)rv;
+                acc[NR + k] += bv * bv;
+                acc[2 * NR + k] += (double)rh[q] * (double)rv;
+            }
+        }
+    }
+    block_partials<3 * NR>(acc, red);
+}
+
+// (Re)starts from the residual b - A x: bb and the initial scalars on the
+// first call; convergence judged on this true residual, also for right-hand
+// sides the recurrence had already declared converged.
+template <int NR>
+struct FinBiResidual {
+    Scalars* S;
+    double* hist;
+    int first;
+    __device__ bool active() const { return true; }
+    __device__ void operator()(const double* t) const {
+        bool all = true;
+        for (int k = 0; k < NR; ++k) {
+            if (first) {
+                S->bbk[k] = t[NR + k];
+                S->alphak[k] = 1.0;
+                S->omegak[k] = 1.0;
+                S->betak[k] = 0.0;
+                S->donek[k] = 0;
+                S->earlyk[k] = 0;
+            }
+            const double scale = S->bbk[k] > 0.0 ? S->bbk[k] : 1.0;
+            S->rrk[k] = t[k];
+            S->rrefk[k] = sqrt(t[k]);
+            if (first && hist) hist[k] = sqrt(t[k] / scale);
+            const int was_done = S->donek[k];
+            S->donek[k] = (t[k] <= S->tol2 * scale) ? 1 : 0;
+            if (!isfinite(t[k])) { S->nonfinite = 1; S->donek[k] = 1; }
+            if (!S->donek[k]) {
+                S->rhok[k] = t[2 * NR + k];
+                if (was_done) { S->betak[k] = 0.0; S->alphak[k] = 1.0; S->omegak[k] = 1.0; }
+            }
+            all = all && S->donek[k];
+        }
+        S->done = all ? 1 : 0;
+        S->halt = S->done;
+        S->need_update = 0;
+    }
+};
+
+namespace {
+
+template <class V, class X, int NR>
+void bi_iteration(ocsl_solver* h, const BiArgs<V, X, NR>& c, const void* m, X* tmp, X* tmp2,
+                  const ocsl_options& o, int g_spmv, int g_vec) {
+    k_bi_p<V, X, NR><<<g_vec, kBlock, 0, h->stream>>>(c);
+    apply_precond<V, X, NR>(h, (const V*)m, c.p, tmp, tmp2, c.yh, o, c.S);
+    k_bi_spmv_v<V, X, NR><<<g_spmv, kBlock, 0, h->stream>>>(c);
+    k_finalize<NR, FinBiSpmvV<NR>><<<1, kBlock, 0, h->stream>>>(c.red, g_spmv, FinBiSpmvV<NR>{c.S});
+    k_bi_s<V, X, NR><<<g_vec, kBlock, 0, h->stream>>>(c);
+    k_finalize<NR, FinBiS<NR>><<<1, kBlock, 0, h->stream>>>(c.red, g_vec, FinBiS<NR>{c.S});
+    apply_precond<V, X, NR>(h, (const V*)m, c.s, tmp, tmp2, c.zh, o, c.S);
+    k_bi_spmv_t<V, X, NR><<<g_spmv, kBlock, 0, h->stream>>>(c);
+    k_finalize<2 * NR, FinBiSpmvT<NR>><<<1, kBlock, 0, h->stream>>>(c.red, g_spmv, FinBiSpmvT<NR>{c.S});
+    k_bi_xr<V, X, NR><<<g_vec, kBlock, 0, h->stream>>>(c);
+    k_finalize<2 * NR, FinBiXr<NR>><<<1, kBlock, 0, h->stream>>>(c.red, g_vec, FinBiXr<NR>{c.S, c.hist});
+}
+
+int round8b(int g) { return g >= kXcds ? (g / kXcds) * kXcds : g; }
+
+template <class V, class X, int NR>
+void bicgstab_run(ocsl_solver* h, bool mixed, const ocsl_options& o, const double* b_il, double* x_il,
+                  ocsl_report* reps) {
+    const int n = h->host.n;
+    Sell& s = sell_for(h, false);
+    ensure_sell_values(h, s, true, mixed);
+    ensure_csr_values(h, false);
+    if (o.tri_mode == OCSL_TRI_EXACT) ensure_levels(h);
+    Work& w = h->work;
+    // X vectors: r, rh, p, v, s, t, yh, zh, tmp, tmp2 -> 10 x n*NR
+    X* vec[10];
+    for (int i = 0; i < 10; ++i) {
+        if (sizeof(X) == 8) vec[i] = (X*)work_buf(h, w.d[i], (size_t)n * NR);
+        else vec[i] = (X*)work_buf(h, w.f[i], (size_t)n * NR);
+    }
+    work_buf(h, h->hist, (size_t)(o.max_iter + 2) * NR);
+
+    Scalars init{};
+    init.tol2 = o.tol * o.tol;
+    init.max_iter = o.max_iter;
+    init.replace_ratio = mixed ? o.replace_ratio : 0.0;
+    init.hist_cap = (o.max_iter + 2) * NR;
+    OCSL_HIP(hipMemcpyAsync(h->S.p, &init, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
+
+    OCSL_HIP(hipEventRecord(h->ev0, h->stream));
+    ilu_decompose_device(h, o.ilu_sweeps, mixed);
+    const void* m = mixed ? (const void*)h->ilu_m32.p : (const void*)h->ilu_m.p;
+    BiArgs<V, X, NR> c{s.view<V>(), vec[1], vec[0], vec[2], vec[3], vec[4], vec[5], vec[6], vec[7],
+                       x_il, h->S.p, h->hist.p, h->red(), n};
+    const int g_spmv = round8b(std::min(kMaxRedBlocks, (s.n_slices + kWavesPerBlock - 1) / kWavesPerBlock));
+    const int g_vec = round8b(red_blocks(kBlock * 4, n));
+    auto residual = [&](int first) {
+        k_bi_residual<X, NR><<<g_spmv, kBlock, 0, h->stream>>>(s.view<double>(), b_il, x_il, c.r, (X*)c.rh, c.p,
+                                                               c.v, h->red(), first);
+        k_finalize<3 * NR, FinBiResidual<NR>><<<1, kBlock, 0, h->stream>>>(
+            h->red(), g_spmv, FinBiResidual<NR>{h->S.p, h->hist.p, first});
+        OCSL_HIP(hipGetLastError());
+    };
+    auto read_S = [&] {
+        OCSL_HIP(hipMemcpyAsync(h->S_host, h->S.p, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+        sync(h);
+    };
+    residual(1);
+    read_S();
+    int replacements = 0;
+    const int kind = 16 + (mixed ? 1 : 0) + 2 * NR + 64 * o.tri_mode + 256 * o.tri_sweeps;
+    int done_iters = 0;
+    while (!h->S_host->done) {
+        const int left = o.max_iter - h->S_host->iter;
+        if (left <= 0) break;
+        // geometric chunk growth: few wasted (early-exiting) launches on fast
+        // convergence, few host checks on slow convergence
+        int chunk = std::max(1, std::min({o.check_every, left, std::max(1, done_iters)}));
+        const bool graph = o.use_graph && chunk == o.check_every;
+        if (graph) {
+            auto key = std::make_pair(kind, chunk);
+            auto it = h->graphs.find(key);
+            if (it == h->graphs.end()) {
+                hipGraph_t gr;
+                OCSL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                for (int k = 0; k < chunk; ++k) bi_iteration<V, X, NR>(h, c, m, vec[8], vec[9], o, g_spmv, g_vec);
+                OCSL_HIP(hipStreamEndCapture(h->stream, &gr));
+                hipGraphExec_t ex;
+                OCSL_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+                OCSL_HIP(hipGraphDestroy(gr));
+                it = h->graphs.emplace(key, ex).first;
+            }
+            OCSL_HIP(hipGraphLaunch(it->second, h->stream));
+        } else {
+            for (int k = 0; k < chunk; ++k) bi_iteration<V, X, NR>(h, c, m, vec[8], vec[9], o, g_spmv, g_vec);
+            OCSL_HIP(hipGetLastError());
+        }
+        read_S();
+        done_iters = h->S_host->iter;
+        const Scalars& S = *h->S_host;
+        if (S.nonfinite) break;
+        if (mixed && (S.need_update || S.done)) {
+            residual(0);
+            ++replacements;
+            read_S();
+        }
+    }
+    OCSL_HIP(hipEventRecord(h->ev1, h->stream));
+    read_S();
+    float ms = 0.f;
+    OCSL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    const Scalars S = *h->S_host;
+    const int iters = S.iter;
+    h->hist_len = std::min(iters + 1, o.max_iter + 2) * NR;
+    h->hist_host.assign(h->hist_len, 0.0);
+    OCSL_HIP(hipMemcpy(h->hist_host.data(), h->hist.p, sizeof(double) * h->hist_len, hipMemcpyDeviceToHost));
+    // true residual of each right-hand side
+    residual(0);
+    read_S();
+    const double nnz = (double)h->host.col.size();
+    const double sv = mixed ? 4.0 : 8.0;
+    const double b_spmv = nnz * (sv + 4.0) + 4.0 * (s.n_slices + 1) + (1.0 + NR * 3.0) * n * sv;
+    const double b_tri = nnz * (sv + 4.0) + 4.0 * (n + 1) + 4.0 * n + 2.0 * NR * n * sv;
+    const double b_vec = NR * n * (sv * 4.0 + sv * 3.0 + sv * 6.0 + 16.0 + 2.0 * sv);
+    for (int k = 0; k < NR; ++k) {
+        ocsl_report* r = reps + k;
+        const double scale = h->S_host->bbk[k] > 0.0 ? h->S_host->bbk[k] : 1.0;
+        const double tr = std::sqrt(h->S_host->rrk[k] / scale);
+        r->iterations = iters;
+        r->replacements = replacements;
+        r->history_len = h->hist_len;
+        r->true_relres = tr;
+        r->residual = tr;
+        r->solve_ms = ms;
+        r->bytes = iters * (2.0 * b_spmv + 4.0 * b_tri + b_vec) / NR;
+        int st = OCSL_OK;
+        if (S.breakdown) st |= OCSL_WARN_BREAKDOWN;
+        if (S.nonfinite || !std::isfinite(tr)) st |= OCSL_WARN_NONFINITE;
+        if (!(tr <= o.tol * 1.0000001) && !(S.donek[k] && !mixed && st == OCSL_OK)) st |= OCSL_WARN_MAXITER;
+        r->status = st;
+    }
+}
+
+}  // namespace
+
+// Interleaves k vectors of n (column blocks) into n x k rows.
+template <class T>
+__global__ void k_interleave(int n, int k, const T* __restrict__ in, T* __restrict__ out, int to_rows) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int c = 0; c < k; ++c) {
+            if (to_rows) out[(size_t)i * k + c] = in[(size_t)c * n + i];
+            else out[(size_t)c * n + i] = in[(size_t)i * k + c];
+        }
+}
+
+void bicgstab_multi(ocsl_solver* h, bool mixed, const ocsl_options& o, int k, const double* B_dev,
+                    double* X_dev, ocsl_report* reps) {
+    OCSL_REQUIRE(o.ilu_sweeps >= 1, OCSL_ERR_ARG, "ilu_sweeps must be >= 1");
+    const int n = h->host.n;
+    Work& w = h->work;
+    double* b_il = work_buf(h, w.d[14], (size_t)n * k);
+    double* x_il = work_buf(h, w.d[15], (size_t)n * k);
+    const int gv = grid_for(n);
+    if (k == 1) {
+        OCSL_HIP(hipMemcpyAsync(b_il, B_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+        OCSL_HIP(hipMemcpyAsync(x_il, X_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+    } else {
+        k_interleave<double><<<gv, kBlock, 0, h->stream>>>(n, k, B_dev, b_il, 1);
+        k_interleave<double><<<gv, kBlock, 0, h->stream>>>(n, k, X_dev, x_il, 1);
+    }
+    OCSL_HIP(hipGetLastError());
+    if (k == 1) {
+        if (mixed) bicgstab_run<float, float, 1>(h, true, o, b_il, x_il, reps);
+        else bicgstab_run<double, double, 1>(h, false, o, b_il, x_il, reps);
+    } else if (k == 2) {
+        if (mixed) bicgstab_run<float, float, 2>(h, true, o, b_il, x_il, reps);
+        else bicgstab_run<double, double, 2>(h, false, o, b_il, x_il, reps);
+    } else if (k == 3) {
+        if (mixed) bicgstab_run<float, float, 3>(h, true, o, b_il, x_il, reps);
+        else bicgstab_run<double, double, 3>(h, false, o, b_il, x_il, reps);
+    } else {
+        if (mixed) bicgstab_run<float, float, 4>(h, true, o, b_il, x_il, reps);
+        else bicgstab_run<double, double, 4>(h, false, o, b_il, x_il, reps);
+    }
+    if (k == 1)
+        OCSL_HIP(hipMemcpyAsync(X_dev, x_il, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+    else
+        k_interleave<double><<<gv, kBlock, 0, h->stream>>>(n, k, x_il, X_dev, 0);
+    OCSL_HIP(hipGetLastError());
+    sync(h);
+}
+
+void bicgstab_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report* rep) {
+    bicgstab_multi(h, mixed, o, 1, h->b_dev, h->x_own.p, rep);
+    h->hist_len = std::min<int>(h->hist_len, (int)h->hist_host.size());
 }
 
 }  // namespace ocsl

@@ -31,9 +31,27 @@ static void launch_spmv(ocsl_solver* h, const Sell& s, double a, const double* x
     OCSL_HIP(hipGetLastError());
 }
 
+// ||b - A x|| / ||b|| over SELL-P: the residual kernel writing into scratch.
+static double true_relres_p(ocsl_solver* h, const SellP& s, const double* b, const double* x) {
+    const int n = s.n;
+    double* r = work_buf(h, h->work.d[11], n);
+    const int g = std::min(kMaxRedBlocks, (s.n_slices + 15) / 16 * 1);
+    k_cg_residual<false><<<std::max(1, g), 1024, 0, h->stream>>>(s.view<double>(), b, x, r, nullptr, nullptr,
+                                                                 nullptr, h->red(), 0);
+    k_finalize<2, FinStore<2>><<<1, kBlock, 0, h->stream>>>(h->red(), std::max(1, g), FinStore<2>{h->dscal.p},
+                                                            kSlotRR);
+    OCSL_HIP(hipGetLastError());
+    OCSL_HIP(hipMemcpyAsync(h->dscal_host, h->dscal.p, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                            h->stream));
+    sync(h);
+    const double rr = h->dscal_host[0], bb = h->dscal_host[1];
+    return bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+}
+
 static double true_relres(ocsl_solver* h, const Sell& s, const double* b, const double* x) {
-    k_true_residual<double><<<red_blocks(kBlock * 4, s.n), kBlock, 0, h->stream>>>(
-        s.view<double>(), b, x, h->red(), h->dscal.p);
+    const int g = red_blocks(kBlock * 4, s.n);
+    k_true_residual<double><<<g, kBlock, 0, h->stream>>>(s.view<double>(), b, x, h->red());
+    k_finalize<2, FinStore<2>><<<1, kBlock, 0, h->stream>>>(h->red(), g, FinStore<2>{h->dscal.p});
     OCSL_HIP(hipGetLastError());
     OCSL_HIP(hipMemcpyAsync(h->dscal_host, h->dscal.p, 2 * sizeof(double), hipMemcpyDeviceToHost,
                             h->stream));
@@ -47,7 +65,7 @@ void cg_reference(ocsl_solver* h, ocsl_report* rep) {
     Sell& s = sell_for(h, false);
     ensure_sell_values(h, s, true, false);
     Work& w = h->work;
-    for (int i = 0; i < 5; ++i) w.d[i].alloc(n);
+    for (int i = 0; i < 5; ++i) work_buf(h, w.d[i], n);
     double *r = w.d[0].p, *rn = w.d[1].p, *p = w.d[2].p, *pn = w.d[3].p, *q = w.d[4].p;
     double* x = h->x_own.p;
     const double* b = h->b_dev;
@@ -95,41 +113,54 @@ void cg_reference(ocsl_solver* h, ocsl_report* rep) {
 
 namespace {
 
-struct CgBuffers {
-    double *b, *x64, *r64, *p64, *q64;
-    float *r32, *p32, *q32;
+struct CgLaunch {
+    int grid, block;
 };
-
-template <class V, class X>
-void launch_cg_iter(ocsl_solver* h, const CgArgs<V, X>& c, int n, int g_spmv, int g_vec) {
-    k_cg_spmv<V, X><<<g_spmv, kBlock, 0, h->stream>>>(c);
-    k_cg_update_r<V, X><<<g_vec, kBlock, 0, h->stream>>>(c, n);
-    k_cg_update_xp<V, X><<<g_vec, kBlock, 0, h->stream>>>(c, n);
-}
 
 int round8(int g) { return g >= kXcds ? (g / kXcds) * kXcds : g; }
 
+// Default launch: 512 blocks of 1024 threads (2 blocks, 8 waves per SIMD, on
+// each of the 256 CUs).  Every block of a consuming kernel re-adds one
+// partial per producing block, so fewer, larger blocks keep that cheap.  Small
+// systems get one wave per slice.  All kernels of a solve share the launch:
+// each re-adds partial sums the others stored.
+CgLaunch cg_launch(int n_slices, const ocsl_options& o) {
+    int block = o.block_size > 0 ? o.block_size : 1024;
+    block = std::max(kWave, std::min(kCgMaxBlock, block / kWave * kWave));
+    const int waves = block / kWave;
+    int cap = o.grid_blocks > 0 ? o.grid_blocks : 512;
+    cap = std::min(cap, kMaxRedBlocks);
+    return {round8(std::max(1, std::min(cap, (n_slices + waves - 1) / waves))), block};
+}
+
 template <class V, class X>
-void run_chunk(ocsl_solver* h, const CgArgs<V, X>& c, int n, int chunk, bool use_graph, int kind) {
-    const int g_spmv = round8(std::min(kMaxRedBlocks, (c.A.n_slices + kWavesPerBlock - 1) / kWavesPerBlock));
-    const int g_vec = round8(red_blocks(kBlock * 4, n));
+void launch_iter(ocsl_solver* h, const CgArgs<V, X>& c, int n, CgLaunch L) {
+    k_cg_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
+    k_cg_update_r<V, X><<<L.grid, L.block, 0, h->stream>>>(c, n);
+    k_cg_update_xp<V, X><<<L.grid, L.block, 0, h->stream>>>(c, n);
+}
+
+// `chunk` iterations, replayed as one hipGraph when use_graph.  Kernel arguments are fixed per handle: every buffer
+// involved is allocated once at the handle's size (work_buf drops the cached
+// graphs if one is reallocated).
+template <class V, class X>
+void run_chunk(ocsl_solver* h, const CgArgs<V, X>& c, int n, int chunk, bool use_graph, int kind,
+               CgLaunch L) {
     if (!use_graph) {
-        for (int k = 0; k < chunk; ++k) launch_cg_iter(h, c, n, g_spmv, g_vec);
+        for (int k = 0; k < chunk; ++k) launch_iter(h, c, n, L);
         OCSL_HIP(hipGetLastError());
         return;
     }
-    // Graphs are keyed by (kind, chunk); kernel arguments are fixed per handle
-    // because every buffer involved is allocated once at the handle's size.
-    auto key = std::make_pair(kind, chunk);
+    auto key = std::make_pair(kind + 4 * L.grid + (1 << 20) * L.block, chunk);
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
-        hipGraph_t g;
+        hipGraph_t gr;
         OCSL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < chunk; ++k) launch_cg_iter(h, c, n, g_spmv, g_vec);
-        OCSL_HIP(hipStreamEndCapture(h->stream, &g));
+        for (int k = 0; k < chunk; ++k) launch_iter(h, c, n, L);
+        OCSL_HIP(hipStreamEndCapture(h->stream, &gr));
         hipGraphExec_t ex;
-        OCSL_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-        OCSL_HIP(hipGraphDestroy(g));
+        OCSL_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+        OCSL_HIP(hipGraphDestroy(gr));
         it = h->graphs.emplace(key, ex).first;
     }
     OCSL_HIP(hipGraphLaunch(it->second, h->stream));
@@ -143,21 +174,25 @@ void read_scalars(ocsl_solver* h) {
 }  // namespace
 
 void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report* rep) {
-    Sell& s = sell_for(h, o.reorder != 0);
-    ensure_sell_values(h, s, true, mixed);
+    SellP& s = sellp_for(h, o.reorder != 0);
+    ensure_sellp_values(h, s, true, mixed);
     const int n = s.n;
     const int* perm = s.identity ? nullptr : s.perm.p;
     Work& w = h->work;
-    // FP64: d[5..9]; MIXED: d[5..6] + f[0..2]
-    for (int i = 5; i < 10; ++i) w.d[i].alloc(n);
-    if (mixed)
-        for (int i = 0; i < 3; ++i) w.f[i].alloc(n);
-    CgBuffers B{w.d[5].p, w.d[6].p, w.d[7].p, w.d[8].p, w.d[9].p,
-                mixed ? w.f[0].p : nullptr, mixed ? w.f[1].p : nullptr,
-                mixed ? w.f[2].p : nullptr};
-    const double* hist_before = h->hist.p;
-    h->hist.alloc((size_t)o.max_iter + 2);
-    if (h->hist.p != hist_before) drop_graphs(h);
+    // b, x in FP64 (d5, d6); r, p, q in the Krylov precision
+    double* b = work_buf(h, w.d[5], n);
+    double* x = work_buf(h, w.d[6], n);
+    CgArgs<double, double> c64{};
+    CgArgs<float, float> c32{};
+    if (mixed) {
+        c32 = {s.view<float>(), work_buf(h, w.f[0], n), work_buf(h, w.f[1], n), work_buf(h, w.f[2], n), x,
+               h->S.p, nullptr, h->red()};
+    } else {
+        c64 = {s.view<double>(), work_buf(h, w.d[7], n), work_buf(h, w.d[8], n), work_buf(h, w.d[9], n), x,
+               h->S.p, nullptr, h->red()};
+    }
+    work_buf(h, h->hist, (size_t)o.max_iter + 2);
+    c32.hist = c64.hist = h->hist.p;
 
     Scalars init{};
     init.tol2 = o.tol * o.tol;
@@ -167,54 +202,45 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report
     OCSL_HIP(hipMemcpyAsync(h->S.p, &init, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
 
     const int gv = grid_for(n);
-    k_permute_in<double><<<gv, kBlock, 0, h->stream>>>(n, perm, h->b_dev, B.b);
-    k_permute_in<double><<<gv, kBlock, 0, h->stream>>>(n, perm, h->x_own.p, B.x64);
+    k_permute_in<double><<<gv, kBlock, 0, h->stream>>>(n, perm, h->b_dev, b);
+    k_permute_in<double><<<gv, kBlock, 0, h->stream>>>(n, perm, h->x_own.p, x);
     OCSL_HIP(hipGetLastError());
 
     OCSL_HIP(hipEventRecord(h->ev0, h->stream));
-    const int g_res = round8(std::min(kMaxRedBlocks, (s.n_slices + kWavesPerBlock - 1) / kWavesPerBlock));
+    const CgLaunch L = cg_launch(s.n_slices, o);
     auto residual = [&](int first) {
         if (mixed)
-            k_cg_residual<true><<<g_res, kBlock, 0, h->stream>>>(s.view<double>(), B.b, B.x64, nullptr,
-                                                                B.r32, B.p32, nullptr, h->S.p,
-                                                                h->hist.p, h->red(), first);
+            k_cg_residual<true><<<L.grid, L.block, 0, h->stream>>>(s.view<double>(), b, x, nullptr, c32.r,
+                                                                  nullptr, c32.p, h->red(), first);
         else
-            k_cg_residual<false><<<g_res, kBlock, 0, h->stream>>>(s.view<double>(), B.b, B.x64, B.r64,
-                                                                 nullptr, nullptr, B.p64, h->S.p,
-                                                                 h->hist.p, h->red(), first);
+            k_cg_residual<false><<<L.grid, L.block, 0, h->stream>>>(s.view<double>(), b, x, c64.r, nullptr,
+                                                                   c64.p, nullptr, h->red(), first);
+        k_finalize<2, FinCgResidual><<<1, kBlock, 0, h->stream>>>(
+            h->red(), L.grid, FinCgResidual{h->S.p, h->hist.p, first}, kSlotRR);
         OCSL_HIP(hipGetLastError());
     };
     residual(1);
     read_scalars(h);
     int replacements = 0;
     const int kind = (mixed ? 1 : 0) + (s.identity ? 0 : 2);
-    if (!h->S_host->done) {
-        while (true) {
-            const int left = o.max_iter - h->S_host->iter;
-            const int chunk = std::min(o.check_every, std::max(left, 1));
-            if (mixed) {
-                CgArgs<float, float> c{s.view<float>(), B.r32, B.p32, B.q32, B.x64, h->S.p, h->hist.p, h->red()};
-                run_chunk(h, c, n, chunk, o.use_graph && chunk == o.check_every, kind);
-            } else {
-                CgArgs<double, double> c{s.view<double>(), B.r64, B.p64, B.q64, B.x64, h->S.p, h->hist.p, h->red()};
-                run_chunk(h, c, n, chunk, o.use_graph && chunk == o.check_every, kind);
-            }
+    while (!h->S_host->done) {
+        const int left = o.max_iter - h->S_host->iter;
+        const bool graph = o.use_graph && left >= o.check_every;
+        const int chunk = graph ? o.check_every : std::max(1, std::min(o.check_every, left));
+        if (mixed) run_chunk(h, c32, n, chunk, graph, kind, L);
+        else run_chunk(h, c64, n, chunk, graph, kind, L);
+        read_scalars(h);
+        const Scalars& S = *h->S_host;
+        if (S.nonfinite || S.breakdown) break;
+        if (mixed && (S.need_update || S.done)) {
+            residual(0);
+            ++replacements;
             read_scalars(h);
-            const Scalars& S = *h->S_host;
-            if (S.nonfinite || S.breakdown) break;
-            if (mixed && (S.need_update || S.done)) {
-                residual(0);
-                ++replacements;
-                read_scalars(h);
-                if (h->S_host->done || h->S_host->nonfinite) break;
-            } else if (!mixed && S.done) {
-                break;
-            }
-            if (h->S_host->iter >= o.max_iter) break;
         }
+        if (h->S_host->iter >= o.max_iter) break;
     }
     OCSL_HIP(hipEventRecord(h->ev1, h->stream));
-    k_permute_out<double><<<gv, kBlock, 0, h->stream>>>(n, perm, B.x64, h->x_own.p);
+    k_permute_out<double><<<gv, kBlock, 0, h->stream>>>(n, perm, x, h->x_own.p);
     OCSL_HIP(hipGetLastError());
     read_scalars(h);
     float ms = 0.f;
@@ -224,7 +250,7 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report
     h->hist_len = std::min(iters + 1, o.max_iter + 2);
     h->hist_host.assign(h->hist_len, 0.0);
     OCSL_HIP(hipMemcpy(h->hist_host.data(), h->hist.p, sizeof(double) * h->hist_len, hipMemcpyDeviceToHost));
-    const double tr = true_relres(h, s, B.b, B.x64);
+    const double tr = true_relres_p(h, s, b, x);
     rep->iterations = iters;
     rep->replacements = replacements;
     rep->history_len = h->hist_len;
@@ -235,31 +261,29 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report
     if (S.breakdown) st |= OCSL_WARN_BREAKDOWN;
     if (S.nonfinite || !std::isfinite(tr)) st |= OCSL_WARN_NONFINITE;
     const double scale = S.bb > 0.0 ? S.bb : 1.0;
-    const bool converged = mixed ? (tr <= o.tol) : (S.rr <= S.tol2 * scale);
+    const bool converged = mixed ? (tr <= o.tol) : (S.rr_last <= S.tol2 * scale);
     if (!converged && st == OCSL_OK) st |= OCSL_WARN_MAXITER;
     rep->status = st;
-    // algorithmic bytes of the iterations: SpMV (values + columns + slice
-    // pointers + p gather + q write) + r update (r, q read, r write) + x/p
-    // update (x read/write in FP64, p, r read, p write)
-    const double nnz = (double)h->host.col.size();
+    // algorithmic bytes: K1 (values + columns + slice pointers + p gathered +
+    // q written) + K2 (r, q read, r written) + K3 (x read/write in FP64, p, r
+    // read, p written); the FP64 residual replacements
+    const double off = (double)h->host.col.size() - n;
     const double sv = mixed ? 4.0 : 8.0;
-    const double it_bytes = nnz * (sv + 4.0) + 4.0 * (s.n_slices + 1) + 2.0 * n * sv +
-                            3.0 * n * sv + (16.0 + 3.0 * sv) * n;
-    // FP64 residual replacement: FP64 SpMV of x, b read, r32 write
-    const double repl_bytes = nnz * 12.0 + 4.0 * (s.n_slices + 1) + n * (8.0 + 8.0 + 4.0);
+    const double it_bytes = off * (sv + 4.0) + n * sv + 4.0 * (s.n_slices + 1) + 2.0 * n * sv + 3.0 * n * sv +
+                            (16.0 + 3.0 * sv) * n;
+    const double repl_bytes = off * 12.0 + 8.0 * n + 4.0 * (s.n_slices + 1) + n * (8.0 + 8.0 + sv);
     rep->bytes = iters * it_bytes + replacements * repl_bytes;
 }
 
 template <class V, class X>
-static double time_spmv(ocsl_solver* h, Sell& s, X* p, X* q, int reps) {
+static double time_spmv(ocsl_solver* h, SellP& s, X* p, X* q, int reps, const ocsl_options& o) {
     CgArgs<V, X> c{s.view<V>(), nullptr, p, q, nullptr, h->S.p, nullptr, h->red()};
-    const int g = round8(std::min(kMaxRedBlocks, (s.n_slices + kWavesPerBlock - 1) / kWavesPerBlock));
+    const CgLaunch L = cg_launch(s.n_slices, o);
     Scalars init{};
-    init.rr = 1.0;
     OCSL_HIP(hipMemcpyAsync(h->S.p, &init, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
-    k_cg_spmv<V, X><<<g, kBlock, 0, h->stream>>>(c);  // warm
+    k_cg_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);  // warm
     OCSL_HIP(hipEventRecord(h->ev0, h->stream));
-    for (int i = 0; i < reps; ++i) k_cg_spmv<V, X><<<g, kBlock, 0, h->stream>>>(c);
+    for (int i = 0; i < reps; ++i) k_cg_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
     OCSL_HIP(hipEventRecord(h->ev1, h->stream));
     OCSL_HIP(hipGetLastError());
     sync(h);
@@ -275,9 +299,11 @@ extern "C" ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reord
     using namespace ocsl;
     if (!h || !avg_ms || !bytes_per_launch || reps <= 0) return OCSL_ERR_ARG;
     try {
+        ocsl_options o;
+        ocsl_default_options(&o);
         const bool mixed = precision == OCSL_MIXED;
-        Sell& s = sell_for(h, reorder != 0);
-        ensure_sell_values(h, s, true, mixed);
+        SellP& s = sellp_for(h, reorder != 0);
+        ensure_sellp_values(h, s, true, mixed);
         const int n = s.n;
         DBuf<double> p64, q64;
         DBuf<float> p32, q32;
@@ -285,17 +311,20 @@ extern "C" ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reord
         if (mixed) {
             p32.alloc(n);
             q32.alloc(n);
-            OCSL_HIP(hipMemsetAsync(p32.p, 0, sizeof(float) * n, h->stream));
-            ms = time_spmv<float, float>(h, s, p32.p, q32.p, reps);
+            k_fill<float><<<grid_for(n), kBlock, 0, h->stream>>>(n, 1.0f, p32.p);
+            ms = time_spmv<float, float>(h, s, p32.p, q32.p, reps, o);
         } else {
             p64.alloc(n);
             q64.alloc(n);
-            OCSL_HIP(hipMemsetAsync(p64.p, 0, sizeof(double) * n, h->stream));
-            ms = time_spmv<double, double>(h, s, p64.p, q64.p, reps);
+            k_fill<double><<<grid_for(n), kBlock, 0, h->stream>>>(n, 1.0, p64.p);
+            ms = time_spmv<double, double>(h, s, p64.p, q64.p, reps, o);
         }
         const double sv = mixed ? 4.0 : 8.0;
         *avg_ms = ms;
-        *bytes_per_launch = (double)h->host.col.size() * (sv + 4.0) + 4.0 * (s.n_slices + 1) + 2.0 * n * sv;
+        // off-diagonal values and columns, the diagonal (no column index),
+        // slice pointers, p gathered once, q written
+        const double off = (double)h->host.col.size() - n;
+        *bytes_per_launch = off * (sv + 4.0) + n * sv + 4.0 * (s.n_slices + 1) + 2.0 * n * sv;
         return OCSL_OK;
     } catch (const Error& e) {
         set_last_error(e.what);

@@ -18,7 +18,7 @@ constexpr int kBlock = 256;       // threads per workgroup (4 waves)
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kXcds = 8;          // MI355X: 8 XCDs, workgroups dealt round-robin
 constexpr int kMaxRedBlocks = 2048;
-constexpr int kMaxRed = 4;        // reductions finalised by one kernel
+constexpr int kMaxRed = 16;       // reductions finalised by one kernel
 
 // ---------------------------------------------------------------- errors
 struct Error {
@@ -74,8 +74,8 @@ struct DBuf {
 // Device-side scalar state of one solve.  Written by the last block of each
 // reducing kernel; read by later kernels (kernel boundaries order them).
 struct Scalars {
-    double rr, rr0, bb, pq, alpha, beta, omega, rho, rho_old, ts, tt, ss, r_ref, tol2;
-    double rrk[4], alphak[4], omegak[4], rhok[4], rho_oldk[4], betak[4], bbk[4], ssk[4];
+    double rr, rr0, rr_last, bb, pq, alpha, beta, omega, rho, rho_old, ts, tt, ss, r_ref, tol2;
+    double rrk[4], alphak[4], omegak[4], rhok[4], rho_oldk[4], betak[4], bbk[4], ssk[4], rrefk[4];
     double aux[8];
     int iter, max_iter, done, halt, early, need_update, breakdown, nonfinite;
     int donek[4], earlyk[4];
@@ -84,11 +84,10 @@ struct Scalars {
     int pad[3];
 };
 
-// Scratch of the deterministic grid reductions: partial sums per block per
-// reduction, and an arrival counter that the last block resets.
+// Scratch of the deterministic grid reductions: one partial sum per block
+// per reduction.
 struct RedScratch {
-    double* partial;    // [kMaxRed][kMaxRedBlocks]
-    unsigned* counter;  // 1
+    double* partial;  // [kMaxRed][kMaxRedBlocks]
 };
 
 // ---------------------------------------------------------------- device helpers
@@ -105,55 +104,58 @@ __device__ __forceinline__ int xcd_block(int b, int g) {
     return (b % kXcds) * (g / kXcds) + b / kXcds;
 }
 
-// Grid-wide deterministic sum of NR per-thread values.  Every block stores
-// its partials; the last block to arrive adds them in block order and calls
-// fin(totals) on thread 0.  Returns true in the finalising block.
-template <int NR, class Fin>
-__device__ __forceinline__ void grid_reduce(const double (&v)[NR], RedScratch red, Fin fin) {
-    __shared__ double sh[NR][kWavesPerBlock];
-    __shared__ int is_last;
+// Deterministic grid-wide reductions in two launches.  The producing kernel
+// stores one partial sum per block and reduction (block_partials, plain
+// stores: no fences, no atomics); a one-block k_finalize launched after it on
+// the same stream adds the partials in block order and hands the totals to a
+// finaliser functor on thread 0.  The kernel boundary between the two makes
+// every partial visible, across XCDs too; an in-kernel "last block" scheme
+// would need an agent-scope fence per block, which on CDNA writes back the
+// XCD's L2 each time.
+template <int NR>
+__device__ __forceinline__ void block_partials(const double (&v)[NR], RedScratch red, int slot0 = 0) {
+    static_assert(NR <= kMaxRed, "reduction scratch holds kMaxRed partial sums per block");
+    constexpr int kMaxWaves = 16;
+    __shared__ double sh[NR][kMaxWaves];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int nw = blockDim.x / kWave;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         const double s = wave_sum(v[r]);
         if (lane == 0) sh[r][wid] = s;
     }
     __syncthreads();
-    const int G = gridDim.x;
     if (tid == 0) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             double s = 0.0;
-            for (int w = 0; w < kWavesPerBlock; ++w) s += sh[r][w];
-            __hip_atomic_store(&red.partial[r * kMaxRedBlocks + blockIdx.x], s, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            for (int w = 0; w < nw; ++w) s += sh[r][w];
+            red.partial[(slot0 + r) * kMaxRedBlocks + blockIdx.x] = s;
         }
-        __threadfence();
-        const unsigned t = atomicAdd(red.counter, 1u);
-        is_last = (t == (unsigned)(G - 1));
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    double tot[NR];
+}
+
+template <int NR, class Fin>
+__global__ void __launch_bounds__(kBlock) k_finalize(RedScratch red, int G, Fin fin, int slot0 = 0) {
+    if (!fin.active()) return;
+    __shared__ double sh[NR][kWavesPerBlock];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         double s = 0.0;
-        for (int i = tid; i < G; i += kBlock)
-            s += __hip_atomic_load(&red.partial[r * kMaxRedBlocks + i], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+        for (int i = tid; i < G; i += kBlock) s += red.partial[(slot0 + r) * kMaxRedBlocks + i];
         s = wave_sum(s);
         if (lane == 0) sh[r][wid] = s;
     }
     __syncthreads();
     if (tid == 0) {
+        double tot[NR];
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             double s = 0.0;
             for (int w = 0; w < kWavesPerBlock; ++w) s += sh[r][w];
             tot[r] = s;
         }
-        *red.counter = 0u;
         fin(tot);
     }
 }
@@ -178,7 +180,14 @@ struct SellView {
     const int* slice_ptr;  // n_slices + 1, element offsets (multiples of 64)
     const int* col;
     const V* val;
+    int max_w;             // widest slice (entries per row)
 };
+
+// Entries of a SELL row are fetched kBatch at a time: all column indices and
+// values of a batch are loaded before any x gather, so each wavefront keeps
+// up to 2*kBatch streaming loads and then kBatch gathers in flight.  The batch
+// predicates depend only on the slice width and are uniform across the wave.
+constexpr int kBatch = 8;
 
 // Row sum of slice-row `row` (lane of slice s) against x, in stored order.
 template <class V, class X, class Acc>
@@ -188,10 +197,18 @@ __device__ __forceinline__ Acc sell_row(const SellView<V>& A, int s, int lane, c
     Acc acc = 0;
     const int* cp = A.col + base + lane;
     const V* vp = A.val + base + lane;
-    for (int k = 0; k < w; ++k) {
-        const int c = __builtin_nontemporal_load(cp + k * kWave);
-        const V v = __builtin_nontemporal_load(vp + k * kWave);
-        acc += (Acc)v * (Acc)x[c];
+    for (int k0 = 0; k0 < w; k0 += kBatch) {
+        int c[kBatch];
+        V v[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j)
+            if (k0 + j < w) {
+                c[j] = __builtin_nontemporal_load(cp + (k0 + j) * kWave);
+                v[j] = __builtin_nontemporal_load(vp + (k0 + j) * kWave);
+            }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j)
+            if (k0 + j < w) acc += (Acc)v[j] * (Acc)x[c[j]];
     }
     return acc;
 }
@@ -213,12 +230,117 @@ __device__ __forceinline__ double sell_row_diff(const SellView<V>& A, int s, int
     double acc = 0.0;
     const int* cp = A.col + base + lane;
     const V* vp = A.val + base + lane;
-    for (int k = 0; k < w; ++k) {
-        const int c = __builtin_nontemporal_load(cp + k * kWave);
-        const V v = __builtin_nontemporal_load(vp + k * kWave);
-        acc += (double)v * (c == row ? xi : (double)x[c] - xi);
+    for (int k0 = 0; k0 < w; k0 += kBatch) {
+        int c[kBatch];
+        V v[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j)
+            if (k0 + j < w) {
+                c[j] = __builtin_nontemporal_load(cp + (k0 + j) * kWave);
+                v[j] = __builtin_nontemporal_load(vp + (k0 + j) * kWave);
+            }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j)
+            if (k0 + j < w) acc += (double)v[j] * (c[j] == row ? xi : (double)x[c[j]] - xi);
     }
     return acc;
+}
+
+// SELL-P: the layout of the tolerance-mode CG kernels.  The diagonal is a
+// dense vector of its own (no column index); the off-diagonal entries of each
+// 64-row slice are stored in pairs, pair t of lane l at base + 128 t + 2 l, so
+// a lane fetches two columns (int2) and two values (float2 / double2) per
+// load: 512 or 1024 contiguous bytes per wavefront instruction.  Slices are
+// padded to an even width with value 0 and column = the row itself.
+template <class V>
+struct SellPView {
+    int n;
+    int n_slices;
+    const int* slice_ptr;  // n_slices + 1, element offsets (multiples of 128)
+    const int* col;
+    const V* val;
+    const V* diag;         // a_ii, or the row sum in the FP32 difference form
+};
+
+template <class V>
+struct Pair2;
+template <>
+struct Pair2<float> {
+    using T = float2;
+};
+template <>
+struct Pair2<double> {
+    using T = double2;
+};
+
+// Row of A x over SELL-P.  kDiff: difference form, diag = row sum s_i:
+// s_i x_i + sum_off a_ij (x_j - x_i); otherwise a_ii x_i + sum_off a_ij x_j.
+// Arithmetic in FP64 whatever the storage.
+template <class V, class X, bool kDiff>
+__device__ __forceinline__ double sellp_row(const SellPView<V>& A, int s, int lane, int row,
+                                            const X* __restrict__ x, double xi) {
+    using V2 = typename Pair2<V>::T;
+    const int base = A.slice_ptr[s];
+    const int np = (A.slice_ptr[s + 1] - base) >> 7;  // pairs per lane
+    const int2* cp = reinterpret_cast<const int2*>(A.col + base) + lane;
+    const V2* vp = reinterpret_cast<const V2*>(A.val + base) + lane;
+    double acc = row < A.n ? (double)A.diag[row] * xi : 0.0;
+    constexpr int kPairs = 4;
+    for (int t0 = 0; t0 < np; t0 += kPairs) {
+        int2 c[kPairs];
+        V2 v[kPairs];
+#pragma unroll
+        for (int j = 0; j < kPairs; ++j)
+            if (t0 + j < np) {
+                c[j] = cp[(t0 + j) * kWave];
+                v[j] = vp[(t0 + j) * kWave];
+            }
+#pragma unroll
+        for (int j = 0; j < kPairs; ++j)
+            if (t0 + j < np) {
+                if constexpr (kDiff) {
+                    acc += (double)v[j].x * ((double)x[c[j].x] - xi);
+                    acc += (double)v[j].y * ((double)x[c[j].y] - xi);
+                } else {
+                    acc += (double)v[j].x * (double)x[c[j].x];
+                    acc += (double)v[j].y * (double)x[c[j].y];
+                }
+            }
+    }
+    return acc;
+}
+
+// NR interleaved vectors (x[row * NR + k]) against one SELL row: one pass
+// over the matrix entries serves all right-hand sides.  FP32 storage uses
+// the difference form of sell_row_diff; FP64 the plain row sum.
+template <class V, class X, int NR>
+__device__ __forceinline__ void sell_rows_nr(const SellView<V>& A, int s, int lane, int row,
+                                             const X* __restrict__ x, double (&out)[NR]) {
+    const int base = A.slice_ptr[s];
+    const int w = (A.slice_ptr[s + 1] - base) >> 6;
+    const int* cp = A.col + base + lane;
+    const V* vp = A.val + base + lane;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) out[k] = 0.0;
+    if constexpr (sizeof(V) == 4) {
+        double xi[NR];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) xi[k] = row < A.n ? (double)x[(size_t)row * NR + k] : 0.0;
+        for (int j = 0; j < w; ++j) {
+            const int c = __builtin_nontemporal_load(cp + j * kWave);
+            const double v = (double)__builtin_nontemporal_load(vp + j * kWave);
+#pragma unroll
+            for (int k = 0; k < NR; ++k)
+                out[k] += v * (c == row ? xi[k] : (double)x[(size_t)c * NR + k] - xi[k]);
+        }
+    } else {
+        for (int j = 0; j < w; ++j) {
+            const int c = __builtin_nontemporal_load(cp + j * kWave);
+            const double v = (double)__builtin_nontemporal_load(vp + j * kWave);
+#pragma unroll
+            for (int k = 0; k < NR; ++k) out[k] += v * (double)x[(size_t)c * NR + k];
+        }
+    }
 }
 
 }  // namespace ocsl

@@ -19,7 +19,7 @@ struct HostCSR {
 
 // A SELL-64 copy of the matrix in some row order (identity or RCM).
 struct Sell {
-    int n = 0, n_slices = 0;
+    int n = 0, n_slices = 0, max_w = 0;
     int64_t total = 0;         // padded entries
     DBuf<int> slice_ptr, col, src, perm, iperm;  // perm: new -> old (empty for identity)
     DBuf<double> val64;
@@ -31,11 +31,31 @@ struct Sell {
 };
 template <>
 inline SellView<double> Sell::view<double>() const {
-    return {n, n_slices, slice_ptr.p, col.p, val64.p};
+    return {n, n_slices, slice_ptr.p, col.p, val64.p, max_w};
 }
 template <>
 inline SellView<float> Sell::view<float>() const {
-    return {n, n_slices, slice_ptr.p, col.p, val32.p};
+    return {n, n_slices, slice_ptr.p, col.p, val32.p, max_w};
+}
+
+// SELL-P copy of the matrix (tolerance-mode CG), RCM or input row order.
+struct SellP {
+    int n = 0, n_slices = 0;
+    int64_t total = 0;                      // padded off-diagonal entries
+    DBuf<int> slice_ptr, col, src, diag_src, perm;  // perm: new -> old (empty for identity)
+    DBuf<double> val64, diag64;
+    DBuf<float> val32, diag32;              // diag32: row sums (difference form)
+    bool identity = true;
+    template <class V>
+    SellPView<V> view() const;
+};
+template <>
+inline SellPView<double> SellP::view<double>() const {
+    return {n, n_slices, slice_ptr.p, col.p, val64.p, diag64.p};
+}
+template <>
+inline SellPView<float> SellP::view<float>() const {
+    return {n, n_slices, slice_ptr.p, col.p, val32.p, diag32.p};
 }
 
 // Level schedule of a triangular solve: rows grouped by dependency depth.
@@ -81,6 +101,9 @@ struct ocsl_solver {
     uint64_t val_epoch = 0;          // bumped on every ocsl_set_values
 
     ocsl::Sell sell_id, sell_rcm;
+    ocsl::SellP sellp_id, sellp_rcm;
+    uint64_t sellp_id_epoch64 = ~0ull, sellp_id_epoch32 = ~0ull;
+    uint64_t sellp_rcm_epoch64 = ~0ull, sellp_rcm_epoch32 = ~0ull;
     uint64_t sell_id_epoch64 = ~0ull, sell_id_epoch32 = ~0ull;
     uint64_t sell_rcm_epoch64 = ~0ull, sell_rcm_epoch32 = ~0ull;
 
@@ -95,7 +118,6 @@ struct ocsl_solver {
     ocsl::DBuf<ocsl::Scalars> S;
     ocsl::Scalars* S_host = nullptr;  // pinned mirror
     ocsl::DBuf<double> red_partial;
-    ocsl::DBuf<unsigned> red_counter;
     ocsl::DBuf<double> dscal;          // small scalar outputs of k_dots
     double* dscal_host = nullptr;      // pinned mirror
     ocsl::DBuf<double> hist;
@@ -105,7 +127,7 @@ struct ocsl_solver {
     // hipGraph cache of iteration chunks, keyed by (kind, chunk)
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;
 
-    ocsl::RedScratch red() const { return {red_partial.p, red_counter.p}; }
+    ocsl::RedScratch red() const { return {red_partial.p}; }
 };
 
 namespace ocsl {
@@ -114,6 +136,7 @@ namespace ocsl {
 void analyse_structure(int64_t n, const uint64_t* rp, const uint64_t* col, HostCSR& out);
 std::vector<int> rcm_order(const HostCSR& a);
 void build_sell(const HostCSR& a, const std::vector<int>* order, Sell& s, hipStream_t st);
+void build_sellp(const HostCSR& a, const std::vector<int>* order, SellP& s, hipStream_t st);
 void build_levels(const HostCSR& a, bool lower, Levels& L, hipStream_t st);
 void build_ilu_tables(const HostCSR& a, IluTables& t, hipStream_t st);
 
@@ -121,11 +144,24 @@ void build_ilu_tables(const HostCSR& a, IluTables& t, hipStream_t st);
 void ensure_csr_values(ocsl_solver* h, bool need32);
 void ensure_sell_values(ocsl_solver* h, Sell& s, bool need64, bool need32);
 Sell& sell_for(ocsl_solver* h, bool reorder);
+SellP& sellp_for(ocsl_solver* h, bool reorder);
+void ensure_sellp_values(ocsl_solver* h, SellP& s, bool need64, bool need32);
 void sync(ocsl_solver* h);
 void drop_graphs(ocsl_solver* h);
 double device_dot(ocsl_solver* h, const double* a, const double* b, int n);
 void device_dots(ocsl_solver* h, int n, int nr, const double* const* a, const double* const* b,
                  const int* kind, double* out);
+
+// Work buffer of at least `count` elements; a reallocation invalidates the
+// cached hipGraphs, whose kernel arguments hold the old addresses.
+template <class T>
+inline T* work_buf(ocsl_solver* h, DBuf<T>& b, size_t count) {
+    if (!b.p || count > b.n) {
+        drop_graphs(h);
+        b.alloc(count);
+    }
+    return b.p;
+}
 
 inline int grid_for(int64_t n, int per_block = kBlock, int cap = 4096) {
     int64_t g = (n + per_block - 1) / per_block;
@@ -140,5 +176,7 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report
 void bicgstab_reference(ocsl_solver* h, ocsl_report* rep);
 void bicgstab_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report* rep);
 void ilu_decompose_device(ocsl_solver* h, int sweeps, bool want32);
+void bicgstab_multi(ocsl_solver* h, bool mixed, const ocsl_options& o, int k, const double* B_dev,
+                    double* X_dev, ocsl_report* reps);
 
 }  // namespace ocsl

@@ -45,6 +45,11 @@ __global__ void k_permute_out(int n, const int* __restrict__ perm, const T* __re
         out[perm ? perm[i] : i] = in[i];
 }
 
+template <class T>
+__global__ void k_fill(int n, T v, T* __restrict__ out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = v;
+}
+
 // ------------------------------------------------------------------ BLAS-1
 // out = a*x + b*y (cmatrix_solver.cpp:284-335)
 [[maybe_unused]] static __global__ void k_axpby(int n, double a, const double* __restrict__ x, double b,
@@ -63,8 +68,7 @@ struct DotArgs {
 };
 
 template <int NR>
-__global__ void __launch_bounds__(kBlock) k_dots(int n, DotArgs<NR> args, RedScratch red,
-                                                 double* out) {
+__global__ void __launch_bounds__(kBlock) k_dots(int n, DotArgs<NR> args, RedScratch red) {
     double acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = 0.0;
@@ -75,11 +79,18 @@ __global__ void __launch_bounds__(kBlock) k_dots(int n, DotArgs<NR> args, RedScr
             acc[r] += args.kind[r] == 0 ? av * args.b[r][i] : (args.kind[r] == 1 ? av : fabs(av));
         }
     }
-    grid_reduce<NR>(acc, red, [&](const double* t) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) out[r] = t[r];
-    });
+    block_partials<NR>(acc, red);
 }
+
+// Copies the totals out (k_dots, k_true_residual).
+template <int NR>
+struct FinStore {
+    double* out;
+    __device__ bool active() const { return true; }
+    __device__ void operator()(const double* t) const {
+        for (int r = 0; r < NR; ++r) out[r] = t[r];
+    }
+};
 
 // ------------------------------------------------------------------ SpMV
 // out = a*(A x) + b*z over a SELL matrix whose row order is the vector order
@@ -106,9 +117,27 @@ __global__ void __launch_bounds__(kBlock) k_sell_axpby(SellView<V> A, double a,
 // ------------------------------------------------------------------ CG (tolerance mode)
 // Vectors live in the SELL row order.  X is the Krylov vector type (double or
 // float); dot products always accumulate in double.
+//
+// One iteration is three kernels, with no host round trip and no separate
+// finalisation launch: each kernel stores per-block partial sums, and the
+// kernels after it re-add those partials themselves, every block in the same
+// fixed order, so all blocks derive bitwise identical scalars.
+//   K1 k_cg_spmv      q = A p, partials of p.q   (block 0: S->rr = sum r.r)
+//   K2 k_cg_update_r  alpha = rr / pq; r -= alpha q, partials of r.r
+//   K3 k_cg_update_xp alpha, beta = rr_new / rr; x += alpha p; p = r + beta p
+//                     (block 0: iteration count, history, convergence flags)
+// Flags: `done` (written by K3 block 0, or K2 block 0 on breakdown) stops K1
+// and K2; K1 turns it into `halt`, which stops K3 -- K3 cannot test `done`
+// because its own block 0 writes it.  Only block 0 writes shared scalars, and
+// never one that other blocks of the same kernel read.
+constexpr int kCgMaxBlock = 1024;  // CG kernels run with 64..1024 threads per block
+constexpr int kSlotPQ = 0;  // p.q of K1
+constexpr int kSlotRR = 1;  // r.r of K2 and of the residual kernel
+constexpr int kSlotBB = 2;  // b.b of the residual kernel
+
 template <class V, class X>
 struct CgArgs {
-    SellView<V> A;
+    SellPView<V> A;
     X* r;
     X* p;
     X* q;
@@ -118,25 +147,83 @@ struct CgArgs {
     RedScratch red;
 };
 
-// r = b - A x (FP64), stored as r64 or r32; p = r on the first call; rr, bb.
-// Finaliser (re)starts the recurrence: rr, bb, r_ref, history entry 0 when
-// `first`, convergence flag.
+// Sum of the G partials of one slot, identical in every block (fixed order,
+// butterfly wave sums give every lane the same bits); all threads get it.
+__device__ __forceinline__ double block_sum_partials(RedScratch red, int slot, int G) {
+    __shared__ double sh[16];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int nw = blockDim.x / kWave;
+    double s = 0.0;
+    for (int i = tid; i < G; i += blockDim.x) s += red.partial[slot * kMaxRedBlocks + i];
+    s = wave_sum(s);
+    if (lane == 0) sh[wid] = s;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += sh[w];
+    __syncthreads();
+    return t;
+}
+
+// 16-byte vector of X: each lane of the vector kernels moves kVec<X>
+// consecutive elements per load or store.
+template <class X>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    static constexpr int N = 4;
+    using T = float4;
+};
+template <>
+struct Vec16<double> {
+    static constexpr int N = 2;
+    using T = double2;
+};
+
+template <class X>
+__device__ __forceinline__ void vload(const X* __restrict__ p, int v, X (&e)[Vec16<X>::N]) {
+    const typename Vec16<X>::T t = reinterpret_cast<const typename Vec16<X>::T*>(p)[v];
+    __builtin_memcpy(e, &t, sizeof(t));
+}
+template <class X>
+__device__ __forceinline__ void vstore(X* __restrict__ p, int v, const X (&e)[Vec16<X>::N]) {
+    typename Vec16<X>::T t;
+    __builtin_memcpy(&t, e, sizeof(t));
+    reinterpret_cast<typename Vec16<X>::T*>(p)[v] = t;
+}
+// N doubles starting at element v*N (N = 2 or 4), as double2 loads / stores
+template <int N>
+__device__ __forceinline__ void dload(const double* __restrict__ p, int v, double (&e)[N]) {
+#pragma unroll
+    for (int h = 0; h < N / 2; ++h) {
+        const double2 t = reinterpret_cast<const double2*>(p)[v * (N / 2) + h];
+        e[2 * h] = t.x;
+        e[2 * h + 1] = t.y;
+    }
+}
+template <int N>
+__device__ __forceinline__ void dstore(double* __restrict__ p, int v, const double (&e)[N]) {
+#pragma unroll
+    for (int h = 0; h < N / 2; ++h) reinterpret_cast<double2*>(p)[v * (N / 2) + h] = make_double2(e[2 * h], e[2 * h + 1]);
+}
+
+// r = b - A x (FP64), stored as r64 or r32; p = r on the first call; partials
+// of r.r (kSlotRR) and b.b (kSlotBB).
 template <bool kMixed>
-__global__ void __launch_bounds__(kBlock) k_cg_residual(SellView<double> A, const double* __restrict__ b,
-                                                       const double* __restrict__ x,
-                                                       double* __restrict__ r64, float* __restrict__ r32,
-                                                       float* __restrict__ p32,
-                                                       double* __restrict__ p64, Scalars* S, double* hist,
-                                                       RedScratch red, int first) {
+__global__ void __launch_bounds__(kCgMaxBlock) k_cg_residual(SellPView<double> A, const double* __restrict__ b,
+                                                            const double* __restrict__ x,
+                                                            double* __restrict__ r64, float* __restrict__ r32,
+                                                            double* __restrict__ p64, float* __restrict__ p32,
+                                                            RedScratch red, int first) {
     const int lane = threadIdx.x & (kWave - 1);
     const int g = gridDim.x;
     const int lb = xcd_block(blockIdx.x, g);
     const int spb = (A.n_slices + g - 1) / g;
     const int s0 = lb * spb, s1 = min(A.n_slices, s0 + spb);
     double acc[2] = {0.0, 0.0};
-    for (int s = s0 + threadIdx.x / kWave; s < s1; s += kWavesPerBlock) {
+    for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
         const int row = s * kWave + lane;
-        const double ax = sell_row<double, double, double>(A, s, lane, x);
+        const double xi = row < A.n ? x[row] : 0.0;
+        const double ax = sellp_row<double, double, false>(A, s, lane, row, x, xi);
         if (row < A.n) {
             const double bv = b[row];
             const double rv = bv - ax;
@@ -151,101 +238,227 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(SellView<double> A, cons
             }
         }
     }
-    grid_reduce<2>(acc, red, [&](const double* t) {
+    block_partials<2>(acc, red, kSlotRR);
+}
+
+// After the residual kernel: bb and history entry 0 on the first call;
+// convergence judged on this (true) residual; r_ref for the replacement test.
+struct FinCgResidual {
+    Scalars* S;
+    double* hist;
+    int first;
+    __device__ bool active() const { return true; }
+    __device__ void operator()(const double* t) const {
         const double rr = t[0], bb = t[1];
         S->rr = rr;
+        S->rr_last = rr;
         S->r_ref = sqrt(rr);
         if (first) {
             S->bb = bb;
             S->rr0 = rr;
-            const double scale = bb > 0.0 ? bb : 1.0;
-            if (hist && S->hist_cap > 0) hist[0] = sqrt(rr / scale);
-            S->done = (rr <= S->tol2 * scale) ? 1 : 0;
-        } else {
-            const double scale = S->bb > 0.0 ? S->bb : 1.0;
-            S->done = (rr <= S->tol2 * scale) ? 1 : 0;
         }
+        const double scale = S->bb > 0.0 ? S->bb : 1.0;
+        if (first && hist && S->hist_cap > 0) hist[0] = sqrt(rr / scale);
+        S->done = (rr <= S->tol2 * scale) ? 1 : 0;
         S->halt = S->done;
         S->need_update = 0;
         if (!isfinite(rr)) { S->nonfinite = 1; S->done = 1; S->halt = 1; }
-    });
-}
+    }
+};
 
-// q = A p ; alpha = rr / (p.q)
+// K1: q = A p, partials of p.q.  FP32 storage uses the difference form.
 template <class V, class X>
-__global__ void __launch_bounds__(kBlock) k_cg_spmv(CgArgs<V, X> c) {
+__global__ void __launch_bounds__(kCgMaxBlock) k_cg_spmv(CgArgs<V, X> c) {
     Scalars* S = c.S;
     if (S->done) {
         if (blockIdx.x == 0 && threadIdx.x == 0) S->halt = 1;
         return;
     }
-    const int lane = threadIdx.x & (kWave - 1);
     const int g = gridDim.x;
+    if (blockIdx.x == 0) {
+        const double rr = block_sum_partials(c.red, kSlotRR, g);
+        if (threadIdx.x == 0) S->rr = rr;
+    }
+    const int lane = threadIdx.x & (kWave - 1);
     const int lb = xcd_block(blockIdx.x, g);
     const int spb = (c.A.n_slices + g - 1) / g;
     const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
     double acc[1] = {0.0};
-    for (int s = s0 + threadIdx.x / kWave; s < s1; s += kWavesPerBlock) {
+    for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
         const int row = s * kWave + lane;
-        X qv;
-        if constexpr (sizeof(V) == 4)
-            qv = (X)sell_row_diff<V, X>(c.A, s, lane, row, c.p);
-        else
-            qv = sell_row<V, X, X>(c.A, s, lane, c.p);
+        const X pi = row < c.A.n ? c.p[row] : (X)0;
+        const X qv = (X)sellp_row<V, X, sizeof(V) == 4>(c.A, s, lane, row, c.p, (double)pi);
         if (row < c.A.n) {
             c.q[row] = qv;
-            acc[0] += (double)c.p[row] * (double)qv;
+            acc[0] += (double)pi * (double)qv;
         }
     }
-    grid_reduce<1>(acc, c.red, [&](const double* t) {
-        const double pq = t[0];
-        if (pq == 0.0 || !isfinite(pq)) {
+    block_partials<1>(acc, c.red, kSlotPQ);
+}
+
+__device__ __forceinline__ bool cg_pq_ok(double pq) { return pq != 0.0 && isfinite(pq); }
+
+// Each thread's first kPre vectors are loaded before the partial sums are
+// re-added, so the two memory latencies overlap.  Elements past the last full
+// vector are handled by the last block.
+constexpr int kPre = 2;
+
+// K2: alpha = rr / pq ; r -= alpha q ; partials of r.r
+template <class V, class X>
+__global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_r(CgArgs<V, X> c, int n) {
+    constexpr int N = Vec16<X>::N;
+    Scalars* S = c.S;
+    if (S->done) return;
+    const int nv = n / N;
+    const int stride = gridDim.x * blockDim.x;
+    const int v0 = blockIdx.x * blockDim.x + threadIdx.x;
+    X rv[kPre][N], qv[kPre][N];
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        const int v = v0 + k * stride;
+        if (v < nv) {
+            vload(c.r, v, rv[k]);
+            vload(c.q, v, qv[k]);
+        }
+    }
+    const double pq = block_sum_partials(c.red, kSlotPQ, gridDim.x);
+    if (!cg_pq_ok(pq)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
             S->breakdown = (pq == 0.0);
             S->nonfinite = !isfinite(pq);
             S->done = 1;
-            S->alpha = 0.0;
-        } else {
-            S->alpha = S->rr / pq;
         }
-        S->pq = pq;
-    });
+        return;
+    }
+    const double alpha = S->rr / pq;
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        const int v = v0 + k * stride;
+        if (v < nv) {
+            X out[N];
+#pragma unroll
+            for (int e = 0; e < N; ++e) {
+                out[e] = (X)((double)rv[k][e] - alpha * (double)qv[k][e]);
+                acc[0] += (double)out[e] * (double)out[e];
+            }
+            vstore(c.r, v, out);
+        }
+    }
+    for (int v = v0 + kPre * stride; v < nv; v += stride) {
+        X a[N], b[N], out[N];
+        vload(c.r, v, a);
+        vload(c.q, v, b);
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
+            out[e] = (X)((double)a[e] - alpha * (double)b[e]);
+            acc[0] += (double)out[e] * (double)out[e];
+        }
+        vstore(c.r, v, out);
+    }
+    if (blockIdx.x == gridDim.x - 1) {
+        const int i = nv * N + (int)threadIdx.x;
+        if (i < n) {
+            const X r2 = (X)((double)c.r[i] - alpha * (double)c.q[i]);
+            c.r[i] = r2;
+            acc[0] += (double)r2 * (double)r2;
+        }
+    }
+    block_partials<1>(acc, c.red, kSlotRR);
 }
 
-// r -= alpha q ; rr_new = r.r ; beta = rr_new / rr ; convergence tests
+// K3: x += alpha p (in FP64 for both precisions) ; p = r + beta p ; block 0
+// records the iteration.
 template <class V, class X>
-__global__ void __launch_bounds__(kBlock) k_cg_update_r(CgArgs<V, X> c, int n) {
+__global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_xp(CgArgs<V, X> c, int n) {
+    constexpr int N = Vec16<X>::N;
     Scalars* S = c.S;
-    if (S->done) return;
-    const double alpha = S->alpha;
-    double acc[1] = {0.0};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const X rv = (X)((double)c.r[i] - alpha * (double)c.q[i]);
-        c.r[i] = rv;
-        acc[0] += (double)rv * (double)rv;
+    if (S->halt) return;
+    const int g = gridDim.x;
+    const int nv = n / N;
+    const int stride = g * blockDim.x;
+    const int v0 = blockIdx.x * blockDim.x + threadIdx.x;
+    double xv[kPre][N];
+    X pv[kPre][N], rv[kPre][N];
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        const int v = v0 + k * stride;
+        if (v < nv) {
+            dload<N>(c.x, v, xv[k]);
+            vload(c.p, v, pv[k]);
+            vload(c.r, v, rv[k]);
+        }
     }
-    grid_reduce<1>(acc, c.red, [&](const double* t) {
-        const double rr_new = t[0];
-        S->beta = rr_new / S->rr;
-        S->rr = rr_new;
+    const double pq = block_sum_partials(c.red, kSlotPQ, g);
+    if (!cg_pq_ok(pq)) return;
+    const double rr_new = block_sum_partials(c.red, kSlotRR, g);
+    const double rr = S->rr;
+    const double alpha = rr / pq, beta = rr_new / rr;
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        const int v = v0 + k * stride;
+        if (v < nv) {
+            X pn[N];
+#pragma unroll
+            for (int e = 0; e < N; ++e) {
+                const double p0 = (double)pv[k][e];
+                xv[k][e] += alpha * p0;
+                pn[e] = (X)((double)rv[k][e] + beta * p0);
+            }
+            dstore<N>(c.x, v, xv[k]);
+            vstore(c.p, v, pn);
+        }
+    }
+    for (int v = v0 + kPre * stride; v < nv; v += stride) {
+        double xe[N];
+        X pe[N], re[N], pn[N];
+        dload<N>(c.x, v, xe);
+        vload(c.p, v, pe);
+        vload(c.r, v, re);
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
+            const double p0 = (double)pe[e];
+            xe[e] += alpha * p0;
+            pn[e] = (X)((double)re[e] + beta * p0);
+        }
+        dstore<N>(c.x, v, xe);
+        vstore(c.p, v, pn);
+    }
+    if (blockIdx.x == g - 1) {
+        const int i = nv * N + (int)threadIdx.x;
+        if (i < n) {
+            const double p0 = (double)c.p[i];
+            c.x[i] += alpha * p0;
+            c.p[i] = (X)((double)c.r[i] + beta * p0);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->alpha = alpha;
+        S->beta = beta;
+        S->rr_last = rr_new;
         const int it = ++S->iter;
         const double scale = S->bb > 0.0 ? S->bb : 1.0;
         if (c.hist && it < S->hist_cap) c.hist[it] = sqrt(rr_new / scale);
         if (rr_new <= S->tol2 * scale || it >= S->max_iter) S->done = 1;
         if (!isfinite(rr_new)) { S->nonfinite = 1; S->done = 1; }
         if (sqrt(rr_new) < S->replace_ratio * S->r_ref) S->need_update = 1;
-    });
+    }
 }
 
-// x += alpha p (in FP64 for both precisions) ; p = r + beta p
-template <class V, class X>
-__global__ void __launch_bounds__(kBlock) k_cg_update_xp(CgArgs<V, X> c, int n) {
-    Scalars* S = c.S;
-    if (S->halt) return;
-    const double alpha = S->alpha, beta = S->beta;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const double pv = (double)c.p[i];
-        c.x[i] += alpha * pv;
-        c.p[i] = (X)((double)c.r[i] + beta * pv);
+// SELL-P FP32 difference form: diag32 = row sum of the FP64 values.
+[[maybe_unused]] static __global__ void __launch_bounds__(kBlock) k_sellp_rowsum(SellPView<double> A,
+                                                                                 float* __restrict__ diag32) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * kWavesPerBlock;
+    for (int s = wave; s < A.n_slices; s += nwaves) {
+        const int row = s * kWave + lane;
+        if (row >= A.n) continue;
+        const int base = A.slice_ptr[s];
+        const int w = (A.slice_ptr[s + 1] - base) >> 6;
+        double sum = A.diag[row];
+        for (int k = 0; k < w; ++k) sum += A.val[base + (k / 2) * 2 * kWave + 2 * lane + (k & 1)];
+        diag32[row] = (float)sum;
     }
 }
 
@@ -274,8 +487,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_xp(CgArgs<V, X> c, int n) 
 // ||b - A x||^2 and ||b||^2 without writing anything (true residual).
 template <class V>
 __global__ void __launch_bounds__(kBlock) k_true_residual(SellView<V> A, const double* __restrict__ b,
-                                                          const double* __restrict__ x, RedScratch red,
-                                                          double* out) {
+                                                          const double* __restrict__ x, RedScratch red) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
     const int nwaves = gridDim.x * kWavesPerBlock;
@@ -289,10 +501,7 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(SellView<V> A, const d
             acc[1] += b[row] * b[row];
         }
     }
-    grid_reduce<2>(acc, red, [&](const double* t) {
-        out[0] = t[0];
-        out[1] = t[1];
-    });
+    block_partials<2>(acc, red);
 }
 
 }  // namespace ocsl

@@ -162,6 +162,8 @@ void build_sell(const HostCSR& a, const std::vector<int>* order, Sell& s, hipStr
     s.n = n;
     s.n_slices = ns;
     s.total = total;
+    s.max_w = 0;
+    for (int sl = 0; sl < ns; ++sl) s.max_w = std::max(s.max_w, (sp[sl + 1] - sp[sl]) / kWave);
     s.identity = order == nullptr;
     s.slice_ptr.upload(sp.data(), sp.size(), st);
     s.col.upload(col.data(), col.size(), st);
@@ -171,6 +173,66 @@ void build_sell(const HostCSR& a, const std::vector<int>* order, Sell& s, hipStr
         s.iperm.upload(iperm.data(), n, st);
     }
     s.have64 = s.have32 = false;
+    OCSL_HIP(hipStreamSynchronize(st));
+}
+
+// SELL-P (see SellPView): diagonal apart, off-diagonals of each row in the
+// reference's ascending column order, slices padded to an even width.
+void build_sellp(const HostCSR& a, const std::vector<int>* order, SellP& s, hipStream_t st) {
+    const int n = a.n;
+    const int ns = (n + kWave - 1) / kWave;
+    std::vector<int> iperm;
+    if (order) {
+        iperm.assign(n, 0);
+        for (int i = 0; i < n; ++i) iperm[(*order)[i]] = i;
+    }
+    auto old_of = [&](int r) { return order ? (*order)[r] : r; };
+    std::vector<int> sp(ns + 1, 0);
+    for (int sl = 0; sl < ns; ++sl) {
+        int w = 0;
+        for (int l = 0; l < kWave; ++l) {
+            const int r = sl * kWave + l;
+            if (r >= n) break;
+            const int o = old_of(r);
+            w = std::max(w, a.rp[o + 1] - a.rp[o] - 1);
+        }
+        w += w & 1;
+        sp[sl + 1] = sp[sl] + w * kWave;
+    }
+    const int64_t total = sp[ns];
+    std::vector<int> col(total), src(total), dsrc(n);
+    for (int sl = 0; sl < ns; ++sl) {
+        const int w = (sp[sl + 1] - sp[sl]) / kWave;
+        for (int l = 0; l < kWave; ++l) {
+            const int r = sl * kWave + l;
+            int k = 0;
+            if (r < n) {
+                const int o = old_of(r);
+                dsrc[r] = a.src[a.diag[o]];
+                for (int p = a.rp[o]; p < a.rp[o + 1]; ++p) {
+                    if (p == a.diag[o]) continue;
+                    const int64_t q = sp[sl] + (int64_t)(k / 2) * 2 * kWave + 2 * l + (k & 1);
+                    col[q] = order ? iperm[a.col[p]] : a.col[p];
+                    src[q] = a.src[p];
+                    ++k;
+                }
+            }
+            for (; k < w; ++k) {
+                const int64_t q = sp[sl] + (int64_t)(k / 2) * 2 * kWave + 2 * l + (k & 1);
+                col[q] = r < n ? r : 0;
+                src[q] = -1;
+            }
+        }
+    }
+    s.n = n;
+    s.n_slices = ns;
+    s.total = total;
+    s.identity = order == nullptr;
+    s.slice_ptr.upload(sp.data(), sp.size(), st);
+    s.col.upload(col.data(), col.size(), st);
+    s.src.upload(src.data(), src.size(), st);
+    s.diag_src.upload(dsrc.data(), dsrc.size(), st);
+    if (order) s.perm.upload(order->data(), n, st);
     OCSL_HIP(hipStreamSynchronize(st));
 }
 

@@ -143,6 +143,25 @@ class Solver:
         d["status"] = st
         return d
 
+    def solve_multi(self, B, X0=None, method="ilu_bicgstab", precision="fp64", **opts):
+        """k right-hand sides (rows of B, shape (k, n)) sharing the matrix.
+        Returns (X of shape (k, n), list of k reports)."""
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        k = B.shape[0]
+        X = np.zeros_like(B) if X0 is None else np.array(X0, dtype=np.float64, order="C")
+        o = default_options(**opts)
+        reps = (_abi.Report * k)()
+        st = self._lib.ocsl_solve_multi(self._h, METHODS[method], PRECISIONS[precision], k,
+                                        B.ctypes.data_as(ctypes.c_void_p),
+                                        X.ctypes.data_as(ctypes.c_void_p), _abi.HOST,
+                                        ctypes.byref(o), reps)
+        _check(st, allow_warn=True)
+        out = []
+        for r in reps:
+            d = r.as_dict()
+            out.append(d)
+        return X, out
+
     def CG_solve(self) -> float:
         """CMatrix_solver::CG_solve (cmatrix_solver.cpp:635-680)."""
         res = ctypes.c_double()
