@@ -72,7 +72,8 @@ typedef struct {
     int reorder;         /* 1: reverse Cuthill-McKee row order for CG (default); 0: input order */
     int use_graph;       /* 1: replay iteration chunks as hipGraphs (default) */
     int block_size;      /* threads per workgroup of the CG kernels, 64..1024; 0 = default (1024) */
-    int grid_blocks;     /* workgroups of the CG kernels; 0 = default (512) */
+    int grid_blocks;     /* workgroups of the CG SpMV kernels; 0 = default (512) */
+    int vec_blocks;      /* workgroups of the CG vector-update kernels; 0 = as grid_blocks */
 } ocsl_options;
 
 typedef struct {
@@ -160,10 +161,12 @@ ocsl_status ocsl_gemv(ocsl_handle h, double a, const double* x, double b, const 
 ocsl_status ocsl_ILU_decompose(ocsl_handle h, int sweeps, double* m_out, ocsl_where where);
 
 /* Measurement hook (no counterpart in the reference): launches the tolerance-
- * mode CG SpMV kernel (q = A p fused with p.q) `reps` times on the handle's
- * stream between two HIP events and returns its average duration and the
- * algorithmic bytes of one launch: nnz*(sv+4) + 4*(slices+1) + 2*n*sv with
- * sv = 8 (FP64) or 4 (MIXED).  Values must be set. */
+ * mode CG SpMV kernel (q = A p with its fused dot products) `reps` times on
+ * the handle's stream between two HIP events and returns its average duration
+ * and the algorithmic bytes of one launch: off-diagonals * (sv + cb) + n*sv
+ * (diagonal) + 4*(slices+1) + 2*n*sv (p gathered once, q written), plus n*sv
+ * (r read) for MIXED; sv = 8 (FP64) or 4 (MIXED), cb = 2 with packed 16-bit
+ * column offsets, else 4.  Values must be set. */
 ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reorder, int reps,
                               double* avg_ms, double* bytes_per_launch);
 

@@ -46,6 +46,7 @@ class Options(ctypes.Structure):
         ("use_graph", ctypes.c_int),
         ("block_size", ctypes.c_int),
         ("grid_blocks", ctypes.c_int),
+        ("vec_blocks", ctypes.c_int),
     ]
 
 

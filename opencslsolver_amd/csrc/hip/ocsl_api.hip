@@ -212,6 +212,7 @@ void ocsl_default_options(ocsl_options* o) {
     o->use_graph = 1;
     o->block_size = 0;
     o->grid_blocks = 0;
+    o->vec_blocks = 0;
 }
 
 int ocsl_device_count(void) {

@@ -114,7 +114,8 @@ void cg_reference(ocsl_solver* h, ocsl_report* rep) {
 namespace {
 
 struct CgLaunch {
-    int grid, block;
+    int grid, block;  // SpMV and residual kernels
+    int vgrid;        // vector-update kernels
 };
 
 int round8(int g) { return g >= kXcds ? (g / kXcds) * kXcds : g; }
@@ -124,20 +125,32 @@ int round8(int g) { return g >= kXcds ? (g / kXcds) * kXcds : g; }
 // partial per producing block, so fewer, larger blocks keep that cheap.  Small
 // systems get one wave per slice.  All kernels of a solve share the launch:
 // each re-adds partial sums the others stored.
-CgLaunch cg_launch(int n_slices, const ocsl_options& o) {
+CgLaunch cg_launch(int n_slices, int n, const ocsl_options& o) {
     int block = o.block_size > 0 ? o.block_size : 1024;
     block = std::max(kWave, std::min(kCgMaxBlock, block / kWave * kWave));
     const int waves = block / kWave;
     int cap = o.grid_blocks > 0 ? o.grid_blocks : 512;
     cap = std::min(cap, kMaxRedBlocks);
-    return {round8(std::max(1, std::min(cap, (n_slices + waves - 1) / waves))), block};
+    const int grid = round8(std::max(1, std::min(cap, (n_slices + waves - 1) / waves)));
+    int vcap = o.vec_blocks > 0 ? o.vec_blocks : grid;
+    vcap = std::min(vcap, kMaxRedBlocks);
+    // one 16-byte vector per thread at most
+    const int vneed = (int)std::max<int64_t>(1, ((int64_t)n / 2 + block - 1) / block);
+    return {grid, block, round8(std::max(1, std::min(vcap, vneed)))};
 }
 
+// FP64: the textbook three-kernel iteration; MIXED: the two-kernel iteration
+// with merged reductions.
 template <class V, class X>
 void launch_iter(ocsl_solver* h, const CgArgs<V, X>& c, int n, CgLaunch L) {
-    k_cg_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
-    k_cg_update_r<V, X><<<L.grid, L.block, 0, h->stream>>>(c, n);
-    k_cg_update_xp<V, X><<<L.grid, L.block, 0, h->stream>>>(c, n);
+    if constexpr (sizeof(X) == 4) {
+        k_cgm_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
+        k_cgm_update<V, X><<<L.vgrid, L.block, 0, h->stream>>>(c, n);
+    } else {
+        k_cg_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
+        k_cg_update_r<V, X><<<L.vgrid, L.block, 0, h->stream>>>(c, n);
+        k_cg_update_xp<V, X><<<L.vgrid, L.block, 0, h->stream>>>(c, n);
+    }
 }
 
 // `chunk` iterations, replayed as one hipGraph when use_graph.  Kernel arguments are fixed per handle: every buffer
@@ -151,7 +164,7 @@ void run_chunk(ocsl_solver* h, const CgArgs<V, X>& c, int n, int chunk, bool use
         OCSL_HIP(hipGetLastError());
         return;
     }
-    auto key = std::make_pair(kind + 4 * L.grid + (1 << 20) * L.block, chunk);
+    auto key = std::make_pair(kind + 4 * L.grid + (1 << 13) * L.vgrid + (1 << 25) * (L.block / kWave), chunk);
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
         hipGraph_t gr;
@@ -207,7 +220,11 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report
     OCSL_HIP(hipGetLastError());
 
     OCSL_HIP(hipEventRecord(h->ev0, h->stream));
-    const CgLaunch L = cg_launch(s.n_slices, o);
+    // the iteration kernels re-add the partial sums of one another and of the
+    // residual kernel: every launch of the solve uses this geometry
+    const CgLaunch L = cg_launch(s.n_slices, n, o);
+    c32.g_spmv = c64.g_spmv = L.grid;
+    c32.g_vec = c64.g_vec = L.vgrid;
     auto residual = [&](int first) {
         if (mixed)
             k_cg_residual<true><<<L.grid, L.block, 0, h->stream>>>(s.view<double>(), b, x, nullptr, c32.r,
@@ -216,7 +233,7 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report
             k_cg_residual<false><<<L.grid, L.block, 0, h->stream>>>(s.view<double>(), b, x, c64.r, nullptr,
                                                                    c64.p, nullptr, h->red(), first);
         k_finalize<2, FinCgResidual><<<1, kBlock, 0, h->stream>>>(
-            h->red(), L.grid, FinCgResidual{h->S.p, h->hist.p, first}, kSlotRR);
+            h->red(), L.grid, FinCgResidual{h->S.p, h->hist.p, first, L.grid}, kSlotRR);
         OCSL_HIP(hipGetLastError());
     };
     residual(1);
@@ -264,26 +281,37 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report
     const bool converged = mixed ? (tr <= o.tol) : (S.rr_last <= S.tol2 * scale);
     if (!converged && st == OCSL_OK) st |= OCSL_WARN_MAXITER;
     rep->status = st;
-    // algorithmic bytes: K1 (values + columns + slice pointers + p gathered +
-    // q written) + K2 (r, q read, r written) + K3 (x read/write in FP64, p, r
-    // read, p written); the FP64 residual replacements
+    // algorithmic bytes of an iteration: the SpMV (off-diagonal values and
+    // columns -- 2 or 4 bytes each --, the diagonal, slice pointers, p
+    // gathered, q written) and the vector updates.  FP64 (three kernels):
+    // r, q read, r written; x read/written, p, r read, p written.  MIXED (two
+    // kernels): r read in the SpMV; x (FP64), r, q, p read, x, r, p written.
     const double off = (double)h->host.col.size() - n;
     const double sv = mixed ? 4.0 : 8.0;
-    const double it_bytes = off * (sv + 4.0) + n * sv + 4.0 * (s.n_slices + 1) + 2.0 * n * sv + 3.0 * n * sv +
-                            (16.0 + 3.0 * sv) * n;
-    const double repl_bytes = off * 12.0 + 8.0 * n + 4.0 * (s.n_slices + 1) + n * (8.0 + 8.0 + sv);
+    const double cb = s.compact ? 2.0 : 4.0;
+    const double spmv_bytes = off * (sv + cb) + n * sv + 4.0 * (s.n_slices + 1) + 2.0 * n * sv;
+    const double it_bytes = mixed ? spmv_bytes + n * sv + (16.0 + 3.0 * sv) * n + (16.0 + 2.0 * sv) * n
+                                  : spmv_bytes + 3.0 * n * sv + (16.0 + 3.0 * sv) * n;
+    const double repl_bytes = off * (8.0 + cb) + 8.0 * n + 4.0 * (s.n_slices + 1) + n * (8.0 + 8.0 + sv);
     rep->bytes = iters * it_bytes + replacements * repl_bytes;
 }
 
 template <class V, class X>
 static double time_spmv(ocsl_solver* h, SellP& s, X* p, X* q, int reps, const ocsl_options& o) {
-    CgArgs<V, X> c{s.view<V>(), nullptr, p, q, nullptr, h->S.p, nullptr, h->red()};
-    const CgLaunch L = cg_launch(s.n_slices, o);
+    const CgLaunch L = cg_launch(s.n_slices, s.n, o);
+    CgArgs<V, X> c{s.view<V>(), p, p, q, nullptr, h->S.p, nullptr, h->red(), L.grid, L.vgrid};
     Scalars init{};
+    init.max_iter = 1 << 30;
+    init.tol2 = -1.0;
+    init.rr_blocks = 1;
     OCSL_HIP(hipMemcpyAsync(h->S.p, &init, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
-    k_cg_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);  // warm
+    auto launch = [&] {
+        if constexpr (sizeof(X) == 4) k_cgm_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
+        else k_cg_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
+    };
+    launch();  // warm
     OCSL_HIP(hipEventRecord(h->ev0, h->stream));
-    for (int i = 0; i < reps; ++i) k_cg_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
+    for (int i = 0; i < reps; ++i) launch();
     OCSL_HIP(hipEventRecord(h->ev1, h->stream));
     OCSL_HIP(hipGetLastError());
     sync(h);
@@ -321,10 +349,12 @@ extern "C" ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reord
         }
         const double sv = mixed ? 4.0 : 8.0;
         *avg_ms = ms;
-        // off-diagonal values and columns, the diagonal (no column index),
-        // slice pointers, p gathered once, q written
+        // off-diagonal values and columns (2 bytes when packed), the diagonal
+        // (no column index), slice pointers, p gathered once, q written; the
+        // mixed kernel also reads r
         const double off = (double)h->host.col.size() - n;
-        *bytes_per_launch = off * (sv + 4.0) + n * sv + 4.0 * (s.n_slices + 1) + 2.0 * n * sv;
+        const double cb = s.compact ? 2.0 : 4.0;
+        *bytes_per_launch = off * (sv + cb) + n * sv + 4.0 * (s.n_slices + 1) + 2.0 * n * sv + (mixed ? n * sv : 0.0);
         return OCSL_OK;
     } catch (const Error& e) {
         set_last_error(e.what);

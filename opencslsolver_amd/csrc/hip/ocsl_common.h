@@ -78,6 +78,7 @@ struct Scalars {
     double rrk[4], alphak[4], omegak[4], rhok[4], rho_oldk[4], betak[4], bbk[4], ssk[4], rrefk[4];
     double aux[8];
     int iter, max_iter, done, halt, early, need_update, breakdown, nonfinite;
+    int rr_blocks;  // producing grid of the r.r partials the next CG SpMV re-adds
     int donek[4], earlyk[4];
     double replace_ratio;
     int hist_cap;
@@ -249,17 +250,21 @@ __device__ __forceinline__ double sell_row_diff(const SellView<V>& A, int s, int
 // SELL-P: the layout of the tolerance-mode CG kernels.  The diagonal is a
 // dense vector of its own (no column index); the off-diagonal entries of each
 // 64-row slice are stored in pairs, pair t of lane l at base + 128 t + 2 l, so
-// a lane fetches two columns (int2) and two values (float2 / double2) per
-// load: 512 or 1024 contiguous bytes per wavefront instruction.  Slices are
-// padded to an even width with value 0 and column = the row itself.
+// a lane fetches two values (float2 / double2) per load: 512 or 1024
+// contiguous bytes per wavefront instruction.  Columns are either int2 pairs
+// at the same positions, or -- when every |column - row| fits in 16 bits, as
+// it does in a bandwidth-reducing order -- one 32-bit word per pair holding
+// the two signed 16-bit offsets, word t of lane l at base/2 + 64 t + l.
+// Slices are padded to an even width with value 0 and column = the row.
 template <class V>
 struct SellPView {
     int n;
     int n_slices;
-    const int* slice_ptr;  // n_slices + 1, element offsets (multiples of 128)
-    const int* col;
+    const int* slice_ptr;   // n_slices + 1, element offsets (multiples of 128)
+    const int* col;         // int32 columns (null when colw is used)
+    const unsigned* colw;   // packed 16-bit column offsets, or null
     const V* val;
-    const V* diag;         // a_ii, or the row sum in the FP32 difference form
+    const V* diag;          // a_ii, or the row sum in the FP32 difference form
 };
 
 template <class V>
@@ -282,19 +287,30 @@ __device__ __forceinline__ double sellp_row(const SellPView<V>& A, int s, int la
     using V2 = typename Pair2<V>::T;
     const int base = A.slice_ptr[s];
     const int np = (A.slice_ptr[s + 1] - base) >> 7;  // pairs per lane
-    const int2* cp = reinterpret_cast<const int2*>(A.col + base) + lane;
     const V2* vp = reinterpret_cast<const V2*>(A.val + base) + lane;
     double acc = row < A.n ? (double)A.diag[row] * xi : 0.0;
     constexpr int kPairs = 4;
     for (int t0 = 0; t0 < np; t0 += kPairs) {
         int2 c[kPairs];
         V2 v[kPairs];
+        if (A.colw) {
+            const unsigned* wp = A.colw + (base >> 1) + lane;
+#pragma unroll
+            for (int j = 0; j < kPairs; ++j)
+                if (t0 + j < np) {
+                    const unsigned wd = wp[(t0 + j) * kWave];
+                    c[j].x = row + (int)(short)(wd & 0xffffu);
+                    c[j].y = row + (int)(short)(wd >> 16);
+                }
+        } else {
+            const int2* cp = reinterpret_cast<const int2*>(A.col + base) + lane;
+#pragma unroll
+            for (int j = 0; j < kPairs; ++j)
+                if (t0 + j < np) c[j] = cp[(t0 + j) * kWave];
+        }
 #pragma unroll
         for (int j = 0; j < kPairs; ++j)
-            if (t0 + j < np) {
-                c[j] = cp[(t0 + j) * kWave];
-                v[j] = vp[(t0 + j) * kWave];
-            }
+            if (t0 + j < np) v[j] = vp[(t0 + j) * kWave];
 #pragma unroll
         for (int j = 0; j < kPairs; ++j)
             if (t0 + j < np) {

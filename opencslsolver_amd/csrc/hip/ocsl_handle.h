@@ -43,6 +43,8 @@ struct SellP {
     int n = 0, n_slices = 0;
     int64_t total = 0;                      // padded off-diagonal entries
     DBuf<int> slice_ptr, col, src, diag_src, perm;  // perm: new -> old (empty for identity)
+    DBuf<unsigned> colw;                    // packed 16-bit column offsets (compact)
+    bool compact = false;
     DBuf<double> val64, diag64;
     DBuf<float> val32, diag32;              // diag32: row sums (difference form)
     bool identity = true;
@@ -51,11 +53,11 @@ struct SellP {
 };
 template <>
 inline SellPView<double> SellP::view<double>() const {
-    return {n, n_slices, slice_ptr.p, col.p, val64.p, diag64.p};
+    return {n, n_slices, slice_ptr.p, compact ? nullptr : col.p, compact ? colw.p : nullptr, val64.p, diag64.p};
 }
 template <>
 inline SellPView<float> SellP::view<float>() const {
-    return {n, n_slices, slice_ptr.p, col.p, val32.p, diag32.p};
+    return {n, n_slices, slice_ptr.p, compact ? nullptr : col.p, compact ? colw.p : nullptr, val32.p, diag32.p};
 }
 
 // Level schedule of a triangular solve: rows grouped by dependency depth.

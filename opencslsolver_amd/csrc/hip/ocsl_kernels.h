@@ -134,6 +134,8 @@ constexpr int kCgMaxBlock = 1024;  // CG kernels run with 64..1024 threads per b
 constexpr int kSlotPQ = 0;  // p.q of K1
 constexpr int kSlotRR = 1;  // r.r of K2 and of the residual kernel
 constexpr int kSlotBB = 2;  // b.b of the residual kernel
+constexpr int kSlotRQ = 3;  // r.q of the merged K1 (mixed precision)
+constexpr int kSlotQQ = 4;  // q.q of the merged K1
 
 template <class V, class X>
 struct CgArgs {
@@ -145,6 +147,8 @@ struct CgArgs {
     Scalars* S;
     double* hist;
     RedScratch red;
+    int g_spmv;     // blocks of the SpMV kernels (and of the residual kernel)
+    int g_vec;      // blocks of the vector-update kernels
 };
 
 // Sum of the G partials of one slot, identical in every block (fixed order,
@@ -162,6 +166,30 @@ __device__ __forceinline__ double block_sum_partials(RedScratch red, int slot, i
     for (int w = 0; w < nw; ++w) t += sh[w];
     __syncthreads();
     return t;
+}
+
+// Several slots at once: one pass, one pair of barriers.
+template <int K>
+__device__ __forceinline__ void block_sum_partials_k(RedScratch red, const int (&slot)[K], int G,
+                                                     double (&out)[K]) {
+    __shared__ double sh[K][16];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int nw = blockDim.x / kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        for (int i = tid; i < G; i += blockDim.x) s += red.partial[slot[k] * kMaxRedBlocks + i];
+        s = wave_sum(s);
+        if (lane == 0) sh[k][wid] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double t = 0.0;
+        for (int w = 0; w < nw; ++w) t += sh[k][w];
+        out[k] = t;
+    }
+    __syncthreads();
 }
 
 // 16-byte vector of X: each lane of the vector kernels moves kVec<X>
@@ -247,9 +275,11 @@ struct FinCgResidual {
     Scalars* S;
     double* hist;
     int first;
+    int blocks;  // grid of the residual kernel: the partial count of its r.r
     __device__ bool active() const { return true; }
     __device__ void operator()(const double* t) const {
         const double rr = t[0], bb = t[1];
+        S->rr_blocks = blocks;
         S->rr = rr;
         S->rr_last = rr;
         S->r_ref = sqrt(rr);
@@ -276,7 +306,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_spmv(CgArgs<V, X> c) {
     }
     const int g = gridDim.x;
     if (blockIdx.x == 0) {
-        const double rr = block_sum_partials(c.red, kSlotRR, g);
+        const double rr = block_sum_partials(c.red, kSlotRR, S->rr_blocks);
         if (threadIdx.x == 0) S->rr = rr;
     }
     const int lane = threadIdx.x & (kWave - 1);
@@ -321,7 +351,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_r(CgArgs<V, X> c, int
             vload(c.q, v, qv[k]);
         }
     }
-    const double pq = block_sum_partials(c.red, kSlotPQ, gridDim.x);
+    const double pq = block_sum_partials(c.red, kSlotPQ, c.g_spmv);
     if (!cg_pq_ok(pq)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             S->breakdown = (pq == 0.0);
@@ -365,6 +395,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_r(CgArgs<V, X> c, int
         }
     }
     block_partials<1>(acc, c.red, kSlotRR);
+    if (blockIdx.x == 0 && threadIdx.x == 0) S->rr_blocks = gridDim.x;
 }
 
 // K3: x += alpha p (in FP64 for both precisions) ; p = r + beta p ; block 0
@@ -389,7 +420,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_xp(CgArgs<V, X> c, in
             vload(c.r, v, rv[k]);
         }
     }
-    const double pq = block_sum_partials(c.red, kSlotPQ, g);
+    const double pq = block_sum_partials(c.red, kSlotPQ, c.g_spmv);
     if (!cg_pq_ok(pq)) return;
     const double rr_new = block_sum_partials(c.red, kSlotRR, g);
     const double rr = S->rr;
@@ -442,6 +473,161 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_xp(CgArgs<V, X> c, in
         if (rr_new <= S->tol2 * scale || it >= S->max_iter) S->done = 1;
         if (!isfinite(rr_new)) { S->nonfinite = 1; S->done = 1; }
         if (sqrt(rr_new) < S->replace_ratio * S->r_ref) S->need_update = 1;
+    }
+}
+
+// ------------------------------------------------------------------ CG, merged reductions
+// The mixed-precision iteration in two kernels (kernel fusion with merged
+// reductions).  K1 also forms r.q and q.q, which give
+//   ||r_{k+1}||^2 = rr_k - 2 alpha (r.q) + alpha^2 (q.q),
+// so K2 can compute beta = ||r_{k+1}||^2 / rr_k and update x, r and p in one
+// pass.  The true r.r of the stored r_{k+1} (partials of K2) still drives
+// alpha of the next iteration, the convergence test and the FP64 residual
+// replacement trigger; only beta comes from the recurrence.
+//   K1m k_cgm_spmv    block 0: rr = sum r.r, convergence, history;
+//                     q = A p; partials of p.q, r.q, q.q
+//   K2m k_cgm_update  alpha, beta; x += alpha p; r -= alpha q; p = r + beta p;
+//                     partials of r.r
+template <class V, class X>
+__global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c) {
+    Scalars* S = c.S;
+    if (S->done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) S->halt = 1;
+        return;
+    }
+    const int g = gridDim.x;
+    if (blockIdx.x == 0) {
+        const double rr = block_sum_partials(c.red, kSlotRR, S->rr_blocks);
+        if (threadIdx.x == 0) {
+            S->rr = rr;
+            S->rr_last = rr;
+            const int it = S->iter;
+            const double scale = S->bb > 0.0 ? S->bb : 1.0;
+            if (c.hist && it < S->hist_cap) c.hist[it] = sqrt(rr / scale);
+            if (!(rr > S->tol2 * scale) || it >= S->max_iter) S->done = 1;
+            if (!isfinite(rr)) { S->nonfinite = 1; S->done = 1; }
+            if (sqrt(rr) < S->replace_ratio * S->r_ref) S->need_update = 1;
+        }
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int lb = xcd_block(blockIdx.x, g);
+    const int spb = (c.A.n_slices + g - 1) / g;
+    const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
+        const int row = s * kWave + lane;
+        const X pi = row < c.A.n ? c.p[row] : (X)0;
+        const X qv = (X)sellp_row<V, X, sizeof(V) == 4>(c.A, s, lane, row, c.p, (double)pi);
+        if (row < c.A.n) {
+            const double rv = (double)c.r[row];
+            c.q[row] = qv;
+            acc[0] += (double)pi * (double)qv;
+            acc[1] += rv * (double)qv;
+            acc[2] += (double)qv * (double)qv;
+        }
+    }
+    const double pq1[1] = {acc[0]};
+    block_partials<1>(pq1, c.red, kSlotPQ);
+    // r.q and q.q go to their own slots (kSlotRQ, kSlotQQ follow kSlotBB)
+    const double rq_qq[2] = {acc[1], acc[2]};
+    block_partials<2>(rq_qq, c.red, kSlotRQ);
+}
+
+template <class V, class X>
+__global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int n) {
+    constexpr int N = Vec16<X>::N;
+    constexpr int P = 1;  // prefetch depth: seven streams per element already
+    Scalars* S = c.S;
+    if (S->done) return;
+    const int g = gridDim.x;
+    const int nv = n / N;
+    const int stride = g * blockDim.x;
+    const int v0 = blockIdx.x * blockDim.x + threadIdx.x;
+    double xv[P][N];
+    X pv[P][N], rv[P][N], qv[P][N];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const int v = v0 + k * stride;
+        if (v < nv) {
+            dload<N>(c.x, v, xv[k]);
+            vload(c.p, v, pv[k]);
+            vload(c.r, v, rv[k]);
+            vload(c.q, v, qv[k]);
+        }
+    }
+    constexpr int kSlots[3] = {kSlotPQ, kSlotRQ, kSlotQQ};
+    double d3[3];
+    block_sum_partials_k<3>(c.red, kSlots, c.g_spmv, d3);
+    const double pq = d3[0], rq = d3[1], qq = d3[2];
+    if (!cg_pq_ok(pq)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            S->breakdown = (pq == 0.0);
+            S->nonfinite = !isfinite(pq);
+            S->done = 1;
+        }
+        return;
+    }
+    const double rr = S->rr;
+    const double alpha = rr / pq;
+    double rr_next = rr - 2.0 * alpha * rq + alpha * alpha * qq;
+    if (!(rr_next > 0.0)) rr_next = 0.0;  // rounding: restart the direction
+    const double beta = rr_next / rr;
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const int v = v0 + k * stride;
+        if (v < nv) {
+            X rn[N], pn[N];
+#pragma unroll
+            for (int e = 0; e < N; ++e) {
+                const double p0 = (double)pv[k][e];
+                xv[k][e] += alpha * p0;
+                rn[e] = (X)((double)rv[k][e] - alpha * (double)qv[k][e]);
+                pn[e] = (X)((double)rn[e] + beta * p0);
+                acc[0] += (double)rn[e] * (double)rn[e];
+            }
+            dstore<N>(c.x, v, xv[k]);
+            vstore(c.r, v, rn);
+            vstore(c.p, v, pn);
+        }
+    }
+    for (int v = v0 + P * stride; v < nv; v += stride) {
+        double xe[N];
+        X pe[N], re[N], qe[N], rn[N], pn[N];
+        dload<N>(c.x, v, xe);
+        vload(c.p, v, pe);
+        vload(c.r, v, re);
+        vload(c.q, v, qe);
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
+            const double p0 = (double)pe[e];
+            xe[e] += alpha * p0;
+            rn[e] = (X)((double)re[e] - alpha * (double)qe[e]);
+            pn[e] = (X)((double)rn[e] + beta * p0);
+            acc[0] += (double)rn[e] * (double)rn[e];
+        }
+        dstore<N>(c.x, v, xe);
+        vstore(c.r, v, rn);
+        vstore(c.p, v, pn);
+    }
+    if (blockIdx.x == g - 1) {
+        const int i = nv * N + (int)threadIdx.x;
+        if (i < n) {
+            const double p0 = (double)c.p[i];
+            c.x[i] += alpha * p0;
+            const X r2 = (X)((double)c.r[i] - alpha * (double)c.q[i]);
+            c.r[i] = r2;
+            c.p[i] = (X)((double)r2 + beta * p0);
+            acc[0] += (double)r2 * (double)r2;
+        }
+    }
+    block_partials<1>(acc, c.red, kSlotRR);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->alpha = alpha;
+        S->beta = beta;
+        S->pq = pq;
+        S->rr_blocks = gridDim.x;
+        ++S->iter;
     }
 }
 

@@ -217,19 +217,46 @@ void build_sellp(const HostCSR& a, const std::vector<int>* order, SellP& s, hipS
                     ++k;
                 }
             }
+            // padding: value 0; column = the row, or for lanes past the last
+            // row the last row (a valid index within 16 bits of the lane)
             for (; k < w; ++k) {
                 const int64_t q = sp[sl] + (int64_t)(k / 2) * 2 * kWave + 2 * l + (k & 1);
-                col[q] = r < n ? r : 0;
+                col[q] = r < n ? r : n - 1;
                 src[q] = -1;
             }
         }
     }
+    // 16-bit column offsets when every entry allows it
+    bool compact = true;
+    for (int sl = 0; sl < ns && compact; ++sl)
+        for (int64_t q = sp[sl]; q < sp[sl + 1]; ++q) {
+            const int l = (int)((q - sp[sl]) % (2 * kWave)) / 2;
+            const int d = col[q] - (sl * kWave + l);
+            if (d < -32768 || d > 32767) { compact = false; break; }
+        }
     s.n = n;
     s.n_slices = ns;
     s.total = total;
     s.identity = order == nullptr;
+    s.compact = compact;
     s.slice_ptr.upload(sp.data(), sp.size(), st);
-    s.col.upload(col.data(), col.size(), st);
+    if (compact) {
+        std::vector<unsigned> colw(total / 2);
+        for (int sl = 0; sl < ns; ++sl) {
+            const int np = (sp[sl + 1] - sp[sl]) / (2 * kWave);
+            for (int t = 0; t < np; ++t)
+                for (int l = 0; l < kWave; ++l) {
+                    const int64_t q = sp[sl] + (int64_t)t * 2 * kWave + 2 * l;
+                    const int row = sl * kWave + l;
+                    const unsigned lo = (unsigned)(uint16_t)(int16_t)(col[q] - row);
+                    const unsigned hi = (unsigned)(uint16_t)(int16_t)(col[q + 1] - row);
+                    colw[sp[sl] / 2 + (int64_t)t * kWave + l] = lo | (hi << 16);
+                }
+        }
+        s.colw.upload(colw.data(), colw.size(), st);
+    } else {
+        s.col.upload(col.data(), col.size(), st);
+    }
     s.src.upload(src.data(), src.size(), st);
     s.diag_src.upload(dsrc.data(), dsrc.size(), st);
     if (order) s.perm.upload(order->data(), n, st);
