@@ -174,6 +174,26 @@ ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reorder, int rep
  * pointers produced on another stream. */
 void* ocsl_get_stream(ocsl_handle h);
 
+/* ---- batches of independent systems (no counterpart in the reference, which
+ * solves one system at a time through CMatrix_solver::solve) ----
+ * m systems with their own CSR structures (the reference layout), values and
+ * right-hand sides, solved by TOLERANCE-mode CG in the same launches: every
+ * system keeps its own scalars and convergence, blocks of converged systems
+ * return at once, iteration chunks replay as hipGraphs.  MIXED: FP32
+ * iterations with FP64 solution and per-system FP64 residual replacement. */
+typedef struct ocsl_batch_s* ocsl_batch;
+ocsl_status ocsl_batch_create(ocsl_batch* bt, int m, const int64_t* n_rows,
+                              const uint64_t* const* row_ptr, const uint64_t* const* column_index);
+ocsl_status ocsl_batch_destroy(ocsl_batch bt);
+ocsl_status ocsl_batch_size(ocsl_batch bt, int* m, int64_t* rows_total);
+ocsl_status ocsl_batch_set_values(ocsl_batch bt, const double* const* values, ocsl_where where);
+ocsl_status ocsl_batch_set_rhs(ocsl_batch bt, const double* const* b, ocsl_where where);
+/* x[k]: initial guess on entry, solution on return.  reps (optional): m
+ * reports; latency_ms (optional): per system, the wall time from the start
+ * of the solve until its convergence was observed (at chunk granularity). */
+ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* opt, double* const* x,
+                             ocsl_where where, ocsl_report* reps, double* latency_ms);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ocsl_last_error(void);
 

@@ -94,6 +94,15 @@ SIGNATURES = {
     "ocsl_time_cg_spmv": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _f64p, _f64p]),
     "ocsl_get_stream": (ctypes.c_void_p, [c_handle]),
+    "ocsl_batch_create": (ctypes.c_int, [ctypes.POINTER(c_handle), ctypes.c_int, _i64p,
+                                         ctypes.POINTER(_u64p), ctypes.POINTER(_u64p)]),
+    "ocsl_batch_destroy": (ctypes.c_int, [c_handle]),
+    "ocsl_batch_size": (ctypes.c_int, [c_handle, _i32p, _i64p]),
+    "ocsl_batch_set_values": (ctypes.c_int, [c_handle, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
+    "ocsl_batch_set_rhs": (ctypes.c_int, [c_handle, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
+    "ocsl_batch_solve": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.POINTER(Options),
+                                        ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                        ctypes.POINTER(Report), _f64p]),
     "ocsl_last_error": (ctypes.c_char_p, []),
 }
 

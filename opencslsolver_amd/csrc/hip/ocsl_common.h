@@ -91,6 +91,12 @@ struct RedScratch {
     double* partial;  // [kMaxRed][kMaxRedBlocks]
 };
 
+// Partial sums of batched kernels: slot r, block b at partial[r * stride + b].
+struct RedScratchBig {
+    double* partial;
+    size_t stride;
+};
+
 // ---------------------------------------------------------------- device helpers
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -222,3 +222,66 @@ class Solver:
         _check(self._lib.ocsl_time_cg_spmv(self._h, PRECISIONS[precision], reorder, reps,
                                            ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
+
+
+class Batch:
+    """Many independent CSR systems solved by CG in the same launches."""
+
+    def __init__(self, systems):
+        """systems: list of (row_ptr, column_index) in the reference layout."""
+        self._lib = hip_lib()
+        self.m = len(systems)
+        self._rp = [np.ascontiguousarray(rp, dtype=np.uint64) for rp, _ in systems]
+        self._col = [np.ascontiguousarray(c, dtype=np.uint64) for _, c in systems]
+        self.n = [int(r.shape[0] - 1) for r in self._rp]
+        n_arr = np.array(self.n, np.int64)
+        rps = (_u64p * self.m)(*[r.ctypes.data_as(_u64p) for r in self._rp])
+        cols = (_u64p * self.m)(*[c.ctypes.data_as(_u64p) for c in self._col])
+        h = _abi.c_handle()
+        _check(self._lib.ocsl_batch_create(ctypes.byref(h), self.m,
+                                           n_arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                           rps, cols))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ocsl_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def _ptrs(arrs):
+        keep = [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+        ptrs = (ctypes.c_void_p * len(keep))(*[a.ctypes.data_as(ctypes.c_void_p) for a in keep])
+        return ptrs, keep
+
+    def set_values(self, values):
+        ptrs, keep = self._ptrs(values)
+        _check(self._lib.ocsl_batch_set_values(self._h, ptrs, _abi.HOST))
+
+    def set_rhs(self, rhs):
+        ptrs, keep = self._ptrs(rhs)
+        _check(self._lib.ocsl_batch_set_rhs(self._h, ptrs, _abi.HOST))
+
+    def solve(self, precision="fp64", x0=None, **opts):
+        """Returns (solutions, reports, latency_ms)."""
+        xs = [np.zeros(n) if x0 is None else np.array(x0[k], dtype=np.float64) for k, n in enumerate(self.n)]
+        ptrs = (ctypes.c_void_p * self.m)(*[x.ctypes.data_as(ctypes.c_void_p) for x in xs])
+        o = default_options(**opts)
+        reps = (_abi.Report * self.m)()
+        lat = np.zeros(self.m)
+        st = self._lib.ocsl_batch_solve(self._h, PRECISIONS[precision], ctypes.byref(o), ptrs, _abi.HOST,
+                                        reps, lat.ctypes.data_as(_f64p))
+        _check(st, allow_warn=True)
+        return xs, [r.as_dict() for r in reps], lat
