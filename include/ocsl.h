@@ -194,6 +194,28 @@ ocsl_status ocsl_batch_set_rhs(ocsl_batch bt, const double* const* b, ocsl_where
 ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* opt, double* const* x,
                              ocsl_where where, ocsl_report* reps, double* latency_ms);
 
+/* ---- differentiation (no counterpart in the reference, which has no
+ * derivative path; for gradient-based callers of CMatrix_solver::solve,
+ * cmatrix_solver.cpp:764-780) ----
+ * Adjoint of the handle's last solve x = A^{-1} b for a loss L(x) with
+ * g = dL/dx: solves A^T lambda = g to opt->tol (TOLERANCE mode; method CG
+ * for symmetric positive definite A, else ILU(0)-BiCGSTAB; FP64 or MIXED),
+ * then dL/db = lambda and dL/dA_ij = -lambda_i x_j on every stored entry,
+ * in the caller's slot order (0 in sentinel slots).  x is the handle's
+ * current solution (ocsl_get_x).  dLdb or dLdA may be NULL. */
+ocsl_status ocsl_adjoint(ocsl_handle h, int method, int precision, const double* g, double* dLdb,
+                         double* dLdA, ocsl_where where, const ocsl_options* opt, ocsl_report* rep);
+
+/* out[p] = alpha * u[i] * v[j] for every stored entry p = (i, j), in the
+ * caller's slot order (0 in sentinel slots). */
+ocsl_status ocsl_sampled_outer(ocsl_handle h, double alpha, const double* u, const double* v,
+                               double* out, ocsl_where where);
+
+/* Measurement hook: average duration of the sampled outer-product kernel
+ * over `reps` launches, and its algorithmic bytes nnz*(8+4) + 4*(n+1) + 2*n*8
+ * (values written, columns read, row offsets, u and v once). */
+ocsl_status ocsl_time_sampled_outer(ocsl_handle h, int reps, double* avg_ms, double* bytes_per_launch);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ocsl_last_error(void);
 

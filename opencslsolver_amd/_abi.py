@@ -94,6 +94,12 @@ SIGNATURES = {
     "ocsl_time_cg_spmv": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _f64p, _f64p]),
     "ocsl_get_stream": (ctypes.c_void_p, [c_handle]),
+    "ocsl_adjoint": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.POINTER(Options), ctypes.POINTER(Report)]),
+    "ocsl_sampled_outer": (ctypes.c_int, [c_handle, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_int]),
+    "ocsl_time_sampled_outer": (ctypes.c_int, [c_handle, ctypes.c_int, _f64p, _f64p]),
     "ocsl_batch_create": (ctypes.c_int, [ctypes.POINTER(c_handle), ctypes.c_int, _i64p,
                                          ctypes.POINTER(_u64p), ctypes.POINTER(_u64p)]),
     "ocsl_batch_destroy": (ctypes.c_int, [c_handle]),

@@ -262,6 +262,7 @@ ocsl_status ocsl_destroy(ocsl_handle h) {
     if (!h) return OCSL_ERR_ARG;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
+    if (h->adj) ocsl_destroy(h->adj);
     if (h->S_host) (void)hipHostFree(h->S_host);
     if (h->dscal_host) (void)hipHostFree(h->dscal_host);
     if (h->ev0) (void)hipEventDestroy(h->ev0);

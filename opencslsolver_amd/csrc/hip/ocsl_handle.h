@@ -126,6 +126,12 @@ struct ocsl_solver {
     std::vector<double> hist_host;
     int hist_len = 0;
 
+    // adjoint: handle of the transposed pattern, slot map into the caller's values
+    ocsl_solver* adj = nullptr;
+    ocsl::DBuf<int> adj_map;
+    uint64_t adj_epoch = ~0ull;
+    ocsl::DBuf<double> grad_own;
+
     // hipGraph cache of iteration chunks, keyed by (kind, chunk)
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;
 

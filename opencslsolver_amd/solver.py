@@ -223,6 +223,50 @@ class Solver:
                                            ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
 
+    # -- differentiation ---------------------------------------------------
+    def adjoint(self, g, method="cg", precision="fp64", dLdb=None, dLdA=None, **opts):
+        """Adjoint of the last solve for a loss with dL/dx = g.
+
+        Solves A^T lambda = g; returns (dL/db = lambda, dL/dA on the stored
+        pattern, report).  Torch CUDA tensors (float64) stay on the device;
+        pass ``dLdb``/``dLdA`` tensors to receive the results in place."""
+        o = default_options(**opts)
+        rep = _abi.Report()
+        pg, where, _kg = _ptr(g)
+        if where == _abi.DEVICE:
+            import torch
+
+            if dLdb is None:
+                dLdb = torch.empty(self.n, dtype=torch.float64, device=g.device)
+            if dLdA is None:
+                dLdA = torch.empty(self.nnz, dtype=torch.float64, device=g.device)
+            pb, pa = ctypes.c_void_p(dLdb.data_ptr()), ctypes.c_void_p(dLdA.data_ptr())
+        else:
+            dLdb = np.empty(self.n) if dLdb is None else dLdb
+            dLdA = np.empty(self.nnz) if dLdA is None else dLdA
+            pb, pa = dLdb.ctypes.data_as(ctypes.c_void_p), dLdA.ctypes.data_as(ctypes.c_void_p)
+        st = self._lib.ocsl_adjoint(self._h, METHODS[method], PRECISIONS[precision], pg, pb, pa, where,
+                                    ctypes.byref(o), ctypes.byref(rep))
+        _check(st, allow_warn=True)
+        d = rep.as_dict()
+        d["status"] = st
+        return dLdb, dLdA, d
+
+    def sampled_outer(self, alpha, u, v) -> np.ndarray:
+        """out[p] = alpha * u[i] * v[j] on the stored pattern (caller slot order)."""
+        pu, _w, _ku = _ptr(u)
+        pv, _w2, _kv = _ptr(v)
+        out = np.empty(self.nnz)
+        _check(self._lib.ocsl_sampled_outer(self._h, alpha, pu, pv, out.ctypes.data_as(ctypes.c_void_p),
+                                            _abi.HOST))
+        return out
+
+    def time_sampled_outer(self, reps=50):
+        """(average ms, algorithmic bytes) of one sampled outer-product launch."""
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.ocsl_time_sampled_outer(self._h, reps, ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
 
 class Batch:
     """Many independent CSR systems solved by CG in the same launches."""

@@ -225,3 +225,36 @@ def test_transpose_and_sampled_outer():
         for p in range(int(a.row_ptr[i]), int(a.row_ptr[i + 1])):
             c = int(a.col[p])
             assert d[p] == (dense[i, c] if c < a.n else 0.0)
+
+
+def test_adjoint_gradient_vs_dense_finite_differences():
+    """The oracle's gradient pipeline (transpose solve + sampled outer product)
+    against central differences of L = 0.5 ||A^{-1} b - x_t||^2 with dense
+    perturbed solves, on a nonsymmetric system (a transposition error fails)."""
+    a, _ = gen.convdiff3d(4, seed_perm=3, dt=10.0)  # weak time term: ||A - A^T|| / ||A|| = 0.55
+    b = gen.uniform(4, a.n)
+    xt = gen.normal(9, a.n)
+    _, x, _, _ = oracle.bicgstab_tol(a, b, tol=1e-14)
+    g = x - xt
+    _, lam, _, _ = oracle.bicgstab_tol(oracle.transpose(a), g, tol=1e-14)
+    dA = oracle.sampled_outer(a, lam, x)
+    dense = a.to_dense()
+
+    def loss(m, rhs):
+        y = np.linalg.solve(m, rhs) - xt
+        return 0.5 * float(y @ y)
+
+    h = 1e-6
+    for p in gen.sample_slots(9, a, 12):
+        i = int(np.searchsorted(a.row_ptr, p, side="right") - 1)
+        j = int(a.col[p])
+        mp, mm = dense.copy(), dense.copy()
+        mp[i, j] += h
+        mm[i, j] -= h
+        fd = (loss(mp, b) - loss(mm, b)) / (2 * h)
+        assert abs(fd - dA[p]) <= 1e-6 * max(abs(fd), 1e-2), (p, fd, dA[p])
+        bp, bm = b.copy(), b.copy()
+        bp[i] += h
+        bm[i] -= h
+        fdb = (loss(dense, bp) - loss(dense, bm)) / (2 * h)
+        assert abs(fdb - lam[i]) <= 1e-6 * max(abs(fdb), 1e-2), (i, fdb, lam[i])
