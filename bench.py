@@ -5,9 +5,18 @@ pressure-correction system (128^3 hex mesh, cells randomly permuted from seed
 next to the plain FP64 solve.
 
 One "step" is one complete solve from x0 = 0 with the matrix, b and x0
-already resident in HBM.  Prints one JSON line.
+already resident in HBM.  The JSON line also carries the side workloads:
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+* multi_rhs -- 96^3 Taylor-Green convection-diffusion system (seed 3), three
+  right-hand sides (seeds 4-6) sharing A, ILU(0)-BiCGSTAB to 1e-10, MIXED and
+  FP64, with the error of each MIXED solution against the FP64 one;
+* batch -- 64 independent pressure systems of n^3 cells, n log-uniform in
+  [22, 80] from seed 7, MIXED CG to 1e-8 in shared launches: total time,
+  per-system latency p50/p99, and the same systems solved one by one;
+* adjoint -- forward CG solve plus the adjoint (transpose CG solve and the
+  sampled outer product dL/dA on the CSR pattern) on the 128^3 system.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--sections all]
 """
 from __future__ import annotations
 
@@ -23,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def _log(msg):
@@ -54,19 +64,18 @@ def cpu_baseline(a, b, iters_full, budget_s=12.0):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the solve is single-device; see the sharding note in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel factors)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+        return d[kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
+
+def headline(args, out):
     import torch
 
     from opencslsolver_amd import gen
@@ -113,8 +122,8 @@ def main():
     relm = float(np.linalg.norm(b - a.matvec(xm)) / np.linalg.norm(b))
     err = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
 
-    # dominant kernel: the fused CG SpMV (q = A p, p.q), timed with HIP events
-    # on the handle's stream
+    # dominant kernel: the fused CG SpMV (q = A p and its dot products), timed
+    # with HIP events on the handle's stream
     spmv_ms_m, spmv_bytes_m = s.time_cg_spmv("mixed", reps=50)
     spmv_ms_64, spmv_bytes_64 = s.time_cg_spmv("fp64", reps=50)
     ach_m = spmv_bytes_m / (spmv_ms_m * 1e-3) / 1e9
@@ -129,19 +138,10 @@ def main():
         _log("bench: CPU baseline")
         cpu = cpu_baseline(a, b, r64["iterations"])
 
-    out = {
-        "metric": "Krylov solve ms + HBM GB/s at 2.1M cells, FP32-refined vs FP64, to 1e-8 rel res",
+    kname = "k_cgm_spmv<float, float>"
+    out.update({
         "value": round(msm, 3),
-        "unit": "ms",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": round(msm, 3),
-        "higher_is_better": False,
-        "scaling": "strong",
-        "vs_baseline": None,
-        "dtype": "f32+f64",
-        "data": "synthetic (seeded generator, regenerated on the box)",
         "config": {"workload": f"SPD pressure-correction CG, {n}^3 hex mesh permuted (seed 1), "
                                f"N={a.n}, nnz={a.nnz}, tol=1e-8, x0=0",
                    "n_rows": a.n, "nnz": a.nnz},
@@ -154,16 +154,189 @@ def main():
                  "fp64_relres": rel64},
         "speedup_mixed_vs_fp64": round(ms64 / msm, 3),
         "solve_GBps": round(gbs_m, 1),
-        "roofline": {"bound": "hbm", "kernel": "k_cg_spmv<float,float>",
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(ach_m, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach_m / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(ach_m / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname),
                      "avg_launch_us": round(spmv_ms_m * 1e3, 2),
                      "bytes_per_launch": spmv_bytes_m,
-                     "fp64_kernel": {"achieved": round(ach_64, 1), "avg_launch_us": round(spmv_ms_64 * 1e3, 2),
+                     "fp64_kernel": {"kernel": "k_cg_spmv<double, double>", "achieved": round(ach_64, 1),
+                                     "avg_launch_us": round(spmv_ms_64 * 1e3, 2),
                                      "bytes_per_launch": spmv_bytes_64}},
         "cpu_baseline": cpu,
-    }
+    })
+
+    if "adjoint" in args.sections:
+        _log("bench: adjoint")
+        xt_d = torch.from_numpy(gen.normal(9, a.n)).to(dev)
+
+        def fwd_adj():
+            s.set_x(zero_d)
+            rf = s.solve("cg", "tolerance", "fp64", tol=1e-8)
+            x = torch.empty(a.n, dtype=torch.float64, device=dev)
+            s.get_x(out=x)
+            _, _, ra = s.adjoint(x - xt_d, "cg", "fp64", tol=1e-8)
+            return rf, ra
+
+        fwd_adj()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rf, ra = fwd_adj()
+        torch.cuda.synchronize()
+        ms_pair = (time.perf_counter() - t0) * 1e3 / args.steps
+        o_ms, o_bytes = s.time_sampled_outer(reps=50)
+        out["adjoint"] = {"workload": f"forward + adjoint FP64 CG to 1e-8 on the {n}^3 system, "
+                                      "L = 0.5||x - x_t||^2, x_t ~ N(0,1) seed 9",
+                          "ms_forward_plus_adjoint": round(ms_pair, 3),
+                          "forward_device_ms": round(rf["solve_ms"], 3),
+                          "transpose_solve_device_ms": round(ra["solve_ms"], 3),
+                          "transpose_iterations": ra["iterations"],
+                          "outer_product": {"kernel": "k_sampled_outer<false>", "avg_launch_us": round(o_ms * 1e3, 2),
+                                            "bytes_per_launch": o_bytes,
+                                            "achieved_GBps": round(o_bytes / (o_ms * 1e-3) / 1e9, 1)}}
     s.close()
+
+
+def multi_rhs(args, out):
+    import torch
+
+    from opencslsolver_amd import gen
+    from opencslsolver_amd.solver import Solver
+
+    _log("bench: 3-RHS convection-diffusion")
+    a, _ = gen.convdiff3d(96, seed_perm=3)
+    B = np.stack([gen.uniform(sd, a.n) for sd in (4, 5, 6)])
+    dev = torch.device("cuda:0")
+    B_d = torch.from_numpy(B).to(dev)
+    with Solver(a.row_ptr, a.col) as s:
+        s.set_values(torch.from_numpy(a.val).to(dev))
+        res = {}
+        for prec in ("fp64", "mixed"):
+            s.solve_multi(B_d, precision=prec, tol=1e-10)  # warm (setup, graphs)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            X, reps = s.solve_multi(B_d, precision=prec, tol=1e-10)
+            torch.cuda.synchronize()
+            res[prec] = ((time.perf_counter() - t0) * 1e3, X.cpu().numpy(), reps)
+    X64, Xm = res["fp64"][1], res["mixed"][1]
+    out["multi_rhs"] = {
+        "workload": f"ILU(0)-BiCGSTAB, 96^3 Taylor-Green convection-diffusion (seed 3), N={a.n}, nnz={a.nnz}, "
+                    "3 RHS (seeds 4-6), tol 1e-10",
+        "mixed_ms": round(res["mixed"][0], 3), "fp64_ms": round(res["fp64"][0], 3),
+        "mixed_iterations": [r["iterations"] for r in res["mixed"][2]],
+        "fp64_iterations": [r["iterations"] for r in res["fp64"][2]],
+        "mixed_fp64_relres": [float(np.linalg.norm(B[k] - a.matvec(Xm[k])) / np.linalg.norm(B[k]))
+                              for k in range(3)],
+        "error_vs_fp64": [float(np.linalg.norm(Xm[k] - X64[k]) / np.linalg.norm(X64[k])) for k in range(3)],
+    }
+
+
+def batch(args, out):
+    from opencslsolver_amd import gen
+    from opencslsolver_amd.solver import Batch, Solver
+
+    _log("bench: batch of 64 systems")
+    ns, sm, sr = gen.batch_plan(7, 64, 22, 80)
+    systems = [gen.poisson3d(int(ns[k]), seed_mesh=int(sm[k]), seed_rhs=int(sr[k]))[:2] for k in range(64)]
+    cells = int(sum(a.n for a, _ in systems))
+    with Batch([(a.row_ptr, a.col) for a, _ in systems]) as bt:
+        bt.set_values([a.val for a, _ in systems])
+        bt.set_rhs([b for _, b in systems])
+        bt.solve("mixed", tol=1e-8)  # warm (graph capture)
+        t0 = time.perf_counter()
+        xs, reps, lat = bt.solve("mixed", tol=1e-8)
+        total = (time.perf_counter() - t0) * 1e3
+    rel = [float(np.linalg.norm(b - a.matvec(x)) / np.linalg.norm(b)) for (a, b), x in zip(systems, xs)]
+    seq = 0.0
+    for a, b in systems:
+        with Solver(a.row_ptr, a.col) as s:
+            s.set_values(a.val)
+            s.set_rhs(b)
+            s.set_x(np.zeros(a.n))
+            s.solve("cg", "tolerance", "mixed", tol=1e-8)
+            s.set_x(np.zeros(a.n))
+            seq += s.solve("cg", "tolerance", "mixed", tol=1e-8)["solve_ms"]
+    out["batch"] = {
+        "workload": f"64 SPD pressure systems, n^3 cells with n log-uniform in [22, 80] (seed 7), "
+                    f"{cells} cells in total, MIXED CG to 1e-8",
+        "total_ms": round(total, 3), "device_ms": round(reps[0]["solve_ms"], 3),
+        "latency_p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "latency_p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "per_system_ms": [round(float(v), 3) for v in lat],
+        "iterations": [r["iterations"] for r in reps],
+        "max_fp64_relres": max(rel),
+        "sequential_sum_ms": round(seq, 3),
+    }
+
+
+def profile_kernels(args):
+    """Launch each measured kernel alone (for rocprofv3 kernel traces and PMC
+    passes): a 256 MiB device copy (the counters' calibration), the CG SpMV
+    in MIXED and FP64, and the sampled outer product, on the headline system."""
+    import torch
+
+    from opencslsolver_amd import gen
+    from opencslsolver_amd.solver import Solver
+
+    dev = torch.device("cuda:0")
+    src = torch.ones(32 << 20, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b, _ = gen.poisson3d(args.n, seed_mesh=1, seed_rhs=2, sigma=0.5, leak=1e-6)
+    res = {"copy_bytes": 2 * src.numel() * 8}
+    with Solver(a.row_ptr, a.col) as s:
+        s.set_values(a.val)
+        s.set_rhs(b)
+        for prec in ("mixed", "fp64"):
+            ms, by = s.time_cg_spmv(prec, reps=args.steps)
+            res[f"cg_spmv_{prec}"] = {"avg_us": ms * 1e3, "bytes": by, "GBps": by / (ms * 1e-3) / 1e9}
+        ms, by = s.time_sampled_outer(reps=args.steps)
+        res["sampled_outer"] = {"avg_us": ms * 1e3, "bytes": by, "GBps": by / (ms * 1e-3) / 1e9}
+    print(json.dumps(res), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--sections", default="adjoint,multi,batch",
+                    help="side workloads to add to the headline: adjoint, multi, batch (or none)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only launch the measured kernels, --steps times each (for rocprofv3)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the solvers are single-device)", file=sys.stderr)
+        sys.exit(2)
+    args.sections = set(x.strip() for x in args.sections.split(",")) if args.sections != "none" else set()
+    if args.profile_kernels:
+        profile_kernels(args)
+        return
+
+    out = {
+        "metric": "Krylov solve ms + HBM GB/s at 2.1M cells, FP32-refined vs FP64, to 1e-8 rel res",
+        "value": None,
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded generator, regenerated on the box)",
+    }
+    headline(args, out)
+    if "multi" in args.sections:
+        multi_rhs(args, out)
+    if "batch" in args.sections:
+        batch(args, out)
     print(json.dumps(out), flush=True)
 
 

@@ -145,22 +145,26 @@ class Solver:
 
     def solve_multi(self, B, X0=None, method="ilu_bicgstab", precision="fp64", **opts):
         """k right-hand sides (rows of B, shape (k, n)) sharing the matrix.
-        Returns (X of shape (k, n), list of k reports)."""
-        B = np.ascontiguousarray(B, dtype=np.float64)
-        k = B.shape[0]
-        X = np.zeros_like(B) if X0 is None else np.array(X0, dtype=np.float64, order="C")
+        Returns (X of shape (k, n), list of k reports).  Torch CUDA tensors
+        (float64, contiguous) are used in place on the device."""
         o = default_options(**opts)
+        if _is_torch_cuda(B):
+            import torch
+
+            k = B.shape[0]
+            X = torch.zeros_like(B) if X0 is None else X0
+            _ptr(B)  # orders torch's stream before the call
+            pB, pX, where = ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(X.data_ptr()), _abi.DEVICE
+        else:
+            B = np.ascontiguousarray(B, dtype=np.float64)
+            k = B.shape[0]
+            X = np.zeros_like(B) if X0 is None else np.array(X0, dtype=np.float64, order="C")
+            pB, pX, where = B.ctypes.data_as(ctypes.c_void_p), X.ctypes.data_as(ctypes.c_void_p), _abi.HOST
         reps = (_abi.Report * k)()
-        st = self._lib.ocsl_solve_multi(self._h, METHODS[method], PRECISIONS[precision], k,
-                                        B.ctypes.data_as(ctypes.c_void_p),
-                                        X.ctypes.data_as(ctypes.c_void_p), _abi.HOST,
+        st = self._lib.ocsl_solve_multi(self._h, METHODS[method], PRECISIONS[precision], k, pB, pX, where,
                                         ctypes.byref(o), reps)
         _check(st, allow_warn=True)
-        out = []
-        for r in reps:
-            d = r.as_dict()
-            out.append(d)
-        return X, out
+        return X, [r.as_dict() for r in reps]
 
     def CG_solve(self) -> float:
         """CMatrix_solver::CG_solve (cmatrix_solver.cpp:635-680)."""
