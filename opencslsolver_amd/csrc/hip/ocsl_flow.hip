@@ -1,0 +1,1210 @@
+// Flow solver around the GPU coefficient assembly.
+//
+// Assembly kernels: one thread per cell gathers its row of the system from
+// the cell's faces (ascending face index, then its boundary faces, then the
+// cell terms) and writes the whole row of the caller's CSR value array, the
+// right-hand side and the initial guess.  Every coefficient of a face is
+// computed by both cells that share it, so there are no atomics and the sums
+// run in the reference's order (the sequential face loop, then the boundary
+// loop, then the cell loop: cflow_solver.cpp:268-283, 323-342, 429-446), which
+// keeps the assembly deterministic.
+//
+// Host part: the rest of CFlow_solver::iterate (boundary updates, face mass
+// fluxes, gradients, corrections, cflow_solver.cpp:229-476,
+// cflow_solver_supply.cpp) restated in C++ on host arrays, uploading what each
+// equation's kernel reads.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "ocsl_flow.h"
+#include "ocsl_handle.h"
+#include "ocsl_kernels.h"
+
+namespace ocsl {
+
+constexpr int kMaxRow = 16;           // stored entries per row (sentinels included)
+constexpr double kAlpha = 0.7;        // blended-central weight (solvers_common.h:30)
+constexpr double kUpwind = 1.0 - kAlpha;
+constexpr double kRelaxP = 0.5;       // explicit_underrelaxation_P (cflow_solver.h:178)
+constexpr double kRelaxU = 0.8;       // explicit_underrelaxation_u (cflow_solver.h:179)
+
+struct BcDev {
+    int type, no_slip, pad0, pad1;
+};
+
+// Gas model (cfluid_properties.cpp), fixed heat capacity.
+struct Gas {
+    double R, gamma, cp, t_ref, h_f, s0, s1, pr;
+    __host__ __device__ double density(double p, double T) const { return p / R / T; }
+    __host__ __device__ double dp_drho(double T) const {
+        double r = R * T;
+        r = r * gamma;
+        return 1.0 / r;
+    }
+    __host__ __device__ double sound(double T) const {
+        double r = R * T;
+        return sqrt(r * gamma);
+    }
+    __host__ __device__ double viscosity(double T) const {
+        double r = s0 * pow(T, 3.0 / 2.0);
+        return r / (T + s1);
+    }
+    __host__ __device__ double enthalpy(double T, const double* v) const {
+        const double vm = mag3(v);
+        double r = h_f;
+        r += (T - t_ref) * cp;
+        r += vm * vm * 0.5;
+        return r;
+    }
+    __host__ __device__ double temperature(double h, const double* v) const {
+        const double vm = mag3(v);
+        double r = t_ref + (h - h_f - vm * vm * 0.5) / cp;
+        if (r <= 200.0) r = 200.0;
+        if (r >= 6000.0) r = 6000.0;
+        return r;
+    }
+    __host__ __device__ static double dot3(const double* a, const double* b) {
+        double r = 0.0;
+        for (int i = 0; i < 3; ++i) r += a[i] * b[i];
+        return r;
+    }
+    __host__ __device__ static double mag3(const double* a) { return sqrt(dot3(a, a)); }
+};
+
+struct FlowPhys {
+    Gas gas;
+    double urf;  // implicit under-relaxation (steady)
+    int n_bc;
+    BcDev bc[OCSL_FLOW_MAX_BC];
+};
+
+struct MeshDev {
+    int n, F, B;
+    const double* vol;
+    const int *fL, *fR;
+    const double *fS, *fe, *fdist, *fksi, *fgeo;
+    const int *bL, *bmark;
+    const double *bS, *barea, *bksi, *bperp;
+    const int *cf_ptr, *cf_code, *cf_pos;  // per cell: faces (f << 1 | side), neighbour slot position
+    const int *cb_ptr, *cb_face;           // per cell: boundary faces
+    const int *row_ptr, *dpos;             // caller layout (int32), diagonal position in the row
+};
+
+struct StateDev {
+    double *u, *u_old, *rho, *rho_old, *p, *p_old, *pc, *gp, *T, *h, *h_old, *ap, *ap_vec, *dt;
+    double *f_mdot, *f_rho, *f_qy, *f_stress;
+    double *b_T, *b_rho, *b_mdot, *b_h, *b_qy, *b_u;
+    int* b_super;
+};
+
+enum { kMomentum = 0, kPressure = 1, kEnthalpy = 2 };
+
+template <class T>
+__device__ __forceinline__ void add_at(T (&a)[kMaxRow], int pos, T v) {
+#pragma unroll
+    for (int k = 0; k < kMaxRow; ++k)
+        if (k == pos) a[k] += v;
+}
+
+__device__ __forceinline__ double face_avg(double geo, double l, double r) {
+    double v = geo * l;
+    v += (1.0 - geo) * r;
+    return v;
+}
+
+template <bool kBlend>
+__device__ __forceinline__ double conv_interp(double geo, double mdot, double l, double r) {
+    if (mdot >= 0.0) return kBlend ? l * kUpwind + kAlpha * face_avg(geo, l, r) : l;
+    return kBlend ? r * kUpwind + kAlpha * face_avg(geo, l, r) : r;
+}
+
+// Convective_flux (solvers_common.h:90-135) seen from one cell of the face:
+// side 0 = left (owner), 1 = right; dp / op: positions of the diagonal and of
+// the other cell's column in this row.
+template <bool kBlend>
+__device__ __forceinline__ void conv_row(double (&a)[kMaxRow], double& rhs, int side, int dp, int op,
+                                         double mdot, double l, double r, double geo) {
+    if (mdot >= 0.0) {
+        if (side == 0) add_at(a, dp, mdot);
+        else add_at(a, op, -mdot);
+        if constexpr (kBlend) {
+            const double val = kAlpha * (face_avg(geo, l, r) - l);
+            rhs += side == 0 ? -mdot * val : mdot * val;
+        }
+    } else if (mdot < 0.0) {
+        if (side == 0) add_at(a, op, mdot);
+        else add_at(a, dp, -mdot);
+        if constexpr (kBlend) {
+            const double val = kAlpha * (face_avg(geo, l, r) - r);
+            rhs += side == 0 ? -mdot * val : mdot * val;
+        }
+    }
+}
+
+// Diffusive_flux (solvers_common.h:42-53): +D on the diagonal, -D off it.
+__device__ __forceinline__ void diff_row(double (&a)[kMaxRow], int dp, int op, double D) {
+    add_at(a, dp, D);
+    add_at(a, op, -D);
+}
+
+template <int kEq, bool kVisc, bool kBlend, bool kTransient>
+__global__ void __launch_bounds__(kBlock) k_assemble(MeshDev m, StateDev s, FlowPhys ph, int d,
+                                                     double* __restrict__ val, double* __restrict__ b,
+                                                     double* __restrict__ x0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n) return;
+    const Gas& gas = ph.gas;
+    double a[kMaxRow];
+#pragma unroll
+    for (int k = 0; k < kMaxRow; ++k) a[k] = 0.0;
+    double rhs = 0.0;
+    const int dp = m.dpos[i];
+    // interior faces (cflow_solver_cell_face_loops.cpp:38-58, 186-218, 334-346)
+    for (int e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
+        const int code = m.cf_code[e];
+        const int f = code >> 1, side = code & 1, op = m.cf_pos[e];
+        const int L = m.fL[f], R = m.fR[f];
+        const double geo = m.fgeo[f];
+        const double mdot = s.f_mdot[f];
+        if constexpr (kEq == kMomentum) {
+            conv_row<kBlend>(a, rhs, side, dp, op, mdot, s.u[3 * L + d], s.u[3 * R + d], geo);
+            if constexpr (kVisc) {
+                const double Tf = face_avg(geo, s.T[L], s.T[R]);
+                double D = gas.viscosity(Tf);
+                D = D * m.fksi[f];
+                diff_row(a, dp, op, D);
+                const double st = s.f_stress[3 * f + d];
+                rhs += side == 0 ? st : -st;
+            }
+        } else if constexpr (kEq == kPressure) {
+            rhs += side == 0 ? -mdot : mdot;
+            const double Tf = conv_interp<kBlend>(geo, mdot, s.T[L], s.T[R]);
+            double dum = gas.dp_drho(Tf);
+            dum = dum / s.f_rho[f];
+            dum *= mdot;
+            conv_row<kBlend>(a, rhs, side, dp, op, dum, s.pc[L], s.pc[R], geo);
+            const double avgD = s.f_qy[f] * s.f_rho[f];
+            diff_row(a, dp, op, avgD);
+        } else {
+            conv_row<kBlend>(a, rhs, side, dp, op, mdot, s.h[L], s.h[R], geo);
+            const double Tf = face_avg(geo, s.T[L], s.T[R]);
+            double avgD = gas.viscosity(Tf);
+            avgD = avgD / gas.pr * m.fksi[f];
+            diff_row(a, dp, op, avgD);
+        }
+    }
+    // boundary faces (cflow_solver_cell_face_loops.cpp:77-132, 229-297, 357-421)
+    for (int e = m.cb_ptr[i]; e < m.cb_ptr[i + 1]; ++e) {
+        const int bf = m.cb_face[e];
+        const BcDev bc = ph.bc[m.bmark[bf]];
+        const double bm = s.b_mdot[bf];
+        if constexpr (kEq == kMomentum) {
+            if (bc.type == OCSL_BC_WALL) {
+                if constexpr (kVisc) {
+                    if (bc.no_slip) {
+                        const double inv = 1.0 / m.barea[bf];
+                        double nrm[3];
+                        for (int k = 0; k < 3; ++k) nrm[k] = inv * m.bS[3 * bf + k];
+                        double D = gas.viscosity(s.T[i]);
+                        D *= m.barea[bf] / m.bperp[bf];
+                        double dum = D * (1.0 - (nrm[d] * nrm[d]));
+                        add_at(a, dp, dum);
+                        dum = 0.0;
+                        for (int k = d + 1; k < d + 3; ++k) {
+                            const int k_ = k % 3;
+                            dum += s.u[3 * i + k_] * nrm[k_] * nrm[d];
+                        }
+                        dum = D * dum;
+                        rhs += dum;
+                    }
+                }
+            }
+            if (bc.type == OCSL_BC_VELOCITY_INLET || bc.type == OCSL_BC_SUPERSONIC_INLET) {
+                const double v = bm * s.b_u[3 * bf + d];
+                rhs += -v;
+            }
+            if (bc.type == OCSL_BC_PRESSURE_OUTLET) add_at(a, dp, bm);
+        } else if constexpr (kEq == kPressure) {
+            if (bc.type == OCSL_BC_VELOCITY_INLET) {
+                rhs += -bm;
+                double dum = gas.dp_drho(s.b_T[bf]) / s.rho[i];
+                dum *= bm;
+                if (!isnan(dum)) add_at(a, dp, dum);  // Convective_flux_neumann_zero_BC adds any non-NaN flux
+            }
+            if (bc.type == OCSL_BC_SUPERSONIC_INLET) {
+                rhs += -bm;
+                const double avgD = s.b_qy[bf] * s.b_rho[bf];
+                add_at(a, dp, avgD);
+                rhs += avgD * 0.0;
+            }
+            if (bc.type == OCSL_BC_PRESSURE_OUTLET) {
+                rhs += -bm;
+                if (s.b_super[bf]) {
+                    double dum = gas.dp_drho(s.T[i]) / s.rho[i];
+                    dum *= bm;
+                    if (!isnan(dum)) add_at(a, dp, dum);
+                } else {
+                    const double avgD = s.b_qy[bf] * s.rho[i];
+                    add_at(a, dp, avgD);
+                    rhs += avgD * 0.0;
+                }
+            }
+        } else {
+            if (bc.type == OCSL_BC_VELOCITY_INLET || bc.type == OCSL_BC_SUPERSONIC_INLET) {
+                const double v = bm * s.b_h[bf];
+                rhs += -v;
+                double avgD = gas.viscosity(s.b_T[bf]);
+                avgD *= gas.pr * m.bksi[bf];  // mu * Pr: the reference's boundary form
+                add_at(a, dp, avgD);
+                rhs += avgD * s.b_h[bf];
+            }
+            if (bc.type == OCSL_BC_PRESSURE_OUTLET) {
+                if (!isnan(bm)) add_at(a, dp, bm);
+            }
+        }
+    }
+    // cell terms (cflow_solver_cell_face_loops.cpp:143-175, 299-322, 423-446)
+    const double V = m.vol[i];
+    double diag = 0.0;
+#pragma unroll
+    for (int k = 0; k < kMaxRow; ++k)
+        if (k == dp) diag = a[k];
+    if constexpr (kEq == kMomentum) {
+        double dum = 0.0, dum2 = 0.0;
+        if constexpr (kTransient) {
+            dum = V / s.dt[i];
+            dum2 = dum * s.rho[i];
+            diag += dum2;
+            dum2 = dum * (s.u_old[3 * i + d] * s.rho_old[i]);
+        }
+        dum2 = dum2 - s.gp[3 * i + d] * V;
+        rhs += dum2;
+        if constexpr (!kTransient) {
+            const double dd = diag;
+            diag = dd / ph.urf;
+            rhs += (1.0 - ph.urf) / ph.urf * dd * s.u_old[3 * i + d];
+        }
+        s.ap[i] += diag / 3.0;
+        s.ap_vec[3 * i + d] = diag;
+        x0[i] = s.u[3 * i + d];
+    } else if constexpr (kEq == kPressure) {
+        if constexpr (kTransient) {
+            const double dum = V / s.dt[i];
+            double dum2 = dum * (s.rho[i] - s.rho_old[i]);
+            rhs += -dum2;
+            dum2 = dum * gas.dp_drho(s.T[i]);
+            diag += dum2;
+        } else {
+            diag = diag / ph.urf;
+        }
+        x0[i] = s.pc[i];
+    } else {
+        if constexpr (kTransient) {
+            const double dum = V / s.dt[i];
+            double dum2 = dum * s.rho[i];
+            diag += dum2;
+            dum2 = dum * (s.h_old[i] * s.rho_old[i]);
+            dum2 = dum2 + (s.p[i] - s.p_old[i]) / s.dt[i] * V;
+            rhs += dum2;
+        } else {
+            const double dd = diag;
+            diag = dd / ph.urf;
+            rhs += (1.0 - ph.urf) / ph.urf * dd * s.h_old[i];
+        }
+        x0[i] = s.h[i];
+    }
+    const int p0 = m.row_ptr[i], len = m.row_ptr[i + 1] - p0;
+#pragma unroll
+    for (int k = 0; k < kMaxRow; ++k)
+        if (k < len) val[p0 + k] = (k == dp) ? diag : a[k];
+    b[i] = rhs;
+}
+
+// calc_flux_explicit_visc_part (cflow_solver_cell_face_loops.cpp:25-35):
+// mu(T_f) ((grad u)^T - 2/3 (div u) I) S_f with the two-point gradient
+// (tensor.h:194-227).
+__global__ void __launch_bounds__(kBlock) k_visc_stress(MeshDev m, StateDev s, Gas gas) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    const int L = m.fL[f], R = m.fR[f];
+    const double dist = m.fdist[f];
+    double du[3], e[3];
+    for (int k = 0; k < 3; ++k) {
+        du[k] = s.u[3 * R + k] - s.u[3 * L + k];
+        e[k] = m.fe[3 * f + k];
+    }
+    double M[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) M[r][c] = du[c] * e[r] / dist;  // transpose of du_i e_j / d
+    double divd = 0.0;
+    for (int k = 0; k < 3; ++k) divd += du[k] * e[k] / dist;
+    const double two3 = 2.0 / 3.0;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) M[r][c] = M[r][c] - two3 * (r == c ? divd : 0.0);
+    const double mu = gas.viscosity(face_avg(m.fgeo[f], s.T[L], s.T[R]));
+    for (int r = 0; r < 3; ++r) {
+        double acc = 0.0;
+        for (int c = 0; c < 3; ++c) acc += (mu * M[r][c]) * m.fS[3 * f + c];
+        s.f_stress[3 * f + r] = acc;
+    }
+}
+
+}  // namespace ocsl
+
+using namespace ocsl;
+
+// Host mirror of the flow state (CFlow_cell / CFlow_face / CFlow_BC_face).
+struct HostFlow {
+    std::vector<double> u, u_old, rho, rho_old, p, p_old, pc, gp, gpc, T, h, h_old, ap, ap_vec, dt, mach;
+    std::vector<double> f_mdot, f_rho, f_qy, f_stress;
+    std::vector<double> b_T, b_p, b_p_out, b_rho, b_mdot, b_h, b_qy, b_u;
+    std::vector<int> b_super;
+};
+
+struct HostMesh {
+    int n = 0, F = 0, B = 0;
+    std::vector<double> vol, fS, fe, falpha, farea, fdist, fksi, fgeo, bS, be, balpha, barea, bdist, bksi, bperp;
+    std::vector<int> fL, fR, bL, bmark;
+};
+
+struct ocsl_flow_s {
+    ocsl_solver* solver = nullptr;
+    ocsl_flow_params prm{};
+    Gas gas{};
+    double urf = 0.5;  // implicit_underrelaxation (cflow_solver.h:177)
+    HostMesh mesh;
+    HostFlow st;
+    // device mesh
+    DBuf<double> d_vol, d_fS, d_fe, d_fdist, d_fksi, d_fgeo, d_bS, d_barea, d_bksi, d_bperp;
+    DBuf<int> d_fL, d_fR, d_bL, d_bmark, d_cf_ptr, d_cf_code, d_cf_pos, d_cb_ptr, d_cb_face, d_row_ptr, d_dpos;
+    // device state
+    DBuf<double> d_u, d_u_old, d_rho, d_rho_old, d_p, d_p_old, d_pc, d_gp, d_T, d_h, d_h_old, d_ap, d_ap_vec, d_dt;
+    DBuf<double> d_f_mdot, d_f_rho, d_f_qy, d_f_stress;
+    DBuf<double> d_b_T, d_b_rho, d_b_mdot, d_b_h, d_b_qy, d_b_u;
+    DBuf<int> d_b_super;
+    // assembled system
+    DBuf<double> d_val, d_b, d_x0;
+    int64_t nnz_in = 0;
+    std::vector<double> x_host, hist;
+    bool initialized = false;
+
+    MeshDev mesh_view() const {
+        return {mesh.n, mesh.F, mesh.B, d_vol.p, d_fL.p, d_fR.p, d_fS.p, d_fe.p, d_fdist.p, d_fksi.p, d_fgeo.p,
+                d_bL.p, d_bmark.p, d_bS.p, d_barea.p, d_bksi.p, d_bperp.p, d_cf_ptr.p, d_cf_code.p, d_cf_pos.p,
+                d_cb_ptr.p, d_cb_face.p, d_row_ptr.p, d_dpos.p};
+    }
+    StateDev state_view() const {
+        return {d_u.p, d_u_old.p, d_rho.p, d_rho_old.p, d_p.p, d_p_old.p, d_pc.p, d_gp.p, d_T.p, d_h.p,
+                d_h_old.p, d_ap.p, d_ap_vec.p, d_dt.p, d_f_mdot.p, d_f_rho.p, d_f_qy.p, d_f_stress.p,
+                d_b_T.p, d_b_rho.p, d_b_mdot.p, d_b_h.p, d_b_qy.p, d_b_u.p, d_b_super.p};
+    }
+    FlowPhys phys() const {
+        FlowPhys ph{};
+        ph.gas = gas;
+        ph.urf = urf;
+        ph.n_bc = prm.n_bc;
+        for (int k = 0; k < prm.n_bc; ++k) ph.bc[k] = {prm.bc[k].type, prm.bc[k].no_slip, 0, 0};
+        return ph;
+    }
+    int bc_type(int bf) const { return prm.bc[mesh.bmark[bf]].type; }
+};
+
+namespace {
+
+template <class F>
+ocsl_status guarded_f(F&& f) {
+    try {
+        f();
+        return OCSL_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what);
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return OCSL_ERR_NOMEM;
+    }
+}
+
+template <class T>
+void up(DBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
+    d.upload(h.data(), h.size(), s);
+}
+
+template <class T>
+void down(std::vector<T>& h, const DBuf<T>& d, hipStream_t s) {
+    if (!h.empty()) OCSL_HIP(hipMemcpyAsync(h.data(), d.p, sizeof(T) * h.size(), hipMemcpyDeviceToHost, s));
+}
+
+inline const double* v3(const std::vector<double>& a, int i) { return a.data() + 3 * i; }
+
+// ---- host restatement of the SIMPLE-step support (cflow_solver_supply.cpp) --
+
+template <bool kBlend>
+double conv_interp_h(double geo, double mdot, double l, double r) {
+    auto avg = [&](double x, double y) {
+        double v = geo * x;
+        v += (1.0 - geo) * y;
+        return v;
+    };
+    if (mdot >= 0.0) return kBlend ? l * kUpwind + kAlpha * avg(l, r) : l;
+    return kBlend ? r * kUpwind + kAlpha * avg(l, r) : r;
+}
+
+double conv_interp_rt(bool blend, double geo, double mdot, double l, double r) {
+    return blend ? conv_interp_h<true>(geo, mdot, l, r) : conv_interp_h<false>(geo, mdot, l, r);
+}
+
+// calc_face_mdot, interior faces (cflow_solver_supply.cpp:25-46)
+void face_mdot(ocsl_flow_s* fl) {
+    const HostMesh& m = fl->mesh;
+    HostFlow& s = fl->st;
+    for (int f = 0; f < m.F; ++f) {
+        const int L = m.fL[f], R = m.fR[f];
+        const double geo = m.fgeo[f];
+        s.f_qy[f] = ((m.vol[L] / s.ap[L]) + (m.vol[R] / s.ap[R])) * Gas::dot3(v3(m.falpha, f), v3(m.fS, f));
+        double pgf[3], vf[3], dvec[3];
+        for (int k = 0; k < 3; ++k) {
+            pgf[k] = geo * s.gp[3 * L + k];
+            pgf[k] += (1.0 - geo) * s.gp[3 * R + k];
+            vf[k] = geo * s.u[3 * L + k];
+            vf[k] += (1.0 - geo) * s.u[3 * R + k];
+            dvec[k] = m.fdist[f] * m.fe[3 * f + k];
+        }
+        const double YF = s.f_qy[f] * (s.p[R] - s.p[L] - Gas::dot3(pgf, dvec));
+        double mdot = Gas::dot3(vf, v3(m.fS, f)) - YF;
+        s.f_rho[f] = conv_interp_rt(fl->prm.blended, geo, mdot, s.rho[L], s.rho[R]);
+        mdot *= s.f_rho[f];
+        s.f_mdot[f] = mdot;
+    }
+}
+
+// calc_face_mdot, boundary faces (cflow_solver_supply.cpp:54-93)
+void bc_mdot(ocsl_flow_s* fl) {
+    const HostMesh& m = fl->mesh;
+    HostFlow& s = fl->st;
+    const Gas& g = fl->gas;
+    for (int b = 0; b < m.B; ++b) {
+        const int L = m.bL[b], t = fl->bc_type(b);
+        if (t == OCSL_BC_VELOCITY_INLET)
+            s.b_mdot[b] = g.density(s.p[L], s.b_T[b]) * Gas::dot3(v3(s.b_u, b), v3(m.bS, b));
+        if (t == OCSL_BC_SUPERSONIC_INLET)
+            s.b_qy[b] = (m.vol[L]) / (s.ap[L]) * Gas::dot3(v3(m.balpha, b), v3(m.bS, b));
+        if (t == OCSL_BC_PRESSURE_OUTLET) {
+            if (s.b_super[b]) {
+                s.b_mdot[b] = g.density(s.p[L], s.T[L]) * Gas::dot3(v3(s.u, L), v3(m.bS, b));
+                s.b_qy[b] = 1.0;
+            } else {
+                s.b_qy[b] = (m.vol[L]) / (s.ap[L]) * Gas::dot3(v3(m.balpha, b), v3(m.bS, b));
+                double dvec[3];
+                for (int k = 0; k < 3; ++k) dvec[k] = m.bdist[b] * m.be[3 * b + k];
+                const double YF = s.b_qy[b] * (s.b_p[b] - s.p[L] - 0.5 * Gas::dot3(v3(s.gp, L), dvec));
+                double mdot = Gas::dot3(v3(s.u, L), v3(m.bS, b)) - YF;
+                mdot *= s.b_rho[b];
+                s.b_mdot[b] = mdot;
+            }
+        }
+    }
+}
+
+// correct_face_mdot (cflow_solver_supply.cpp:96-130)
+void correct_mdot(ocsl_flow_s* fl) {
+    const HostMesh& m = fl->mesh;
+    HostFlow& s = fl->st;
+    const Gas& g = fl->gas;
+    const bool blend = fl->prm.blended;
+    for (int f = 0; f < m.F; ++f) {
+        const int L = m.fL[f], R = m.fR[f];
+        const double geo = m.fgeo[f];
+        const double Tf = conv_interp_rt(blend, geo, s.f_mdot[f], s.T[L], s.T[R]);
+        const double pcf = conv_interp_rt(blend, geo, s.f_mdot[f], s.pc[L], s.pc[R]);
+        s.f_mdot[f] += s.f_mdot[f] / s.f_rho[f] * g.dp_drho(Tf) * pcf;
+        s.f_mdot[f] += s.f_qy[f] * s.f_rho[f] * (s.pc[L] - s.pc[R]);
+    }
+    for (int b = 0; b < m.B; ++b)
+        if (fl->bc_type(b) == OCSL_BC_PRESSURE_OUTLET && !s.b_super[b])
+            s.b_mdot[b] += s.b_qy[b] * s.b_rho[b] * (s.pc[m.bL[b]]);
+}
+
+// calc_press_grad (cflow_solver_supply.cpp:132-175)
+void press_grad(ocsl_flow_s* fl) {
+    const HostMesh& m = fl->mesh;
+    HostFlow& s = fl->st;
+    std::fill(s.gp.begin(), s.gp.end(), 0.0);
+    for (int f = 0; f < m.F; ++f) {
+        const int L = m.fL[f], R = m.fR[f];
+        double fp = (s.ap[L] + s.ap[R]);
+        fp = (s.ap[L] * s.p[L] + s.ap[R] * s.p[R]) / fp;
+        for (int k = 0; k < 3; ++k) {
+            s.gp[3 * L + k] = s.gp[3 * L + k] + fp * m.fS[3 * f + k];
+            s.gp[3 * R + k] = s.gp[3 * R + k] - fp * m.fS[3 * f + k];
+        }
+    }
+    for (int b = 0; b < m.B; ++b) {
+        const int L = m.bL[b], t = fl->bc_type(b);
+        double pv;
+        if (t == OCSL_BC_WALL || t == OCSL_BC_SYMMETRY || t == OCSL_BC_EMPTY || t == OCSL_BC_VELOCITY_INLET)
+            pv = s.p[L];
+        else if (t == OCSL_BC_PRESSURE_OUTLET || t == OCSL_BC_SUPERSONIC_INLET)
+            pv = s.b_p[b];
+        else
+            continue;
+        for (int k = 0; k < 3; ++k) s.gp[3 * L + k] = s.gp[3 * L + k] + pv * m.bS[3 * b + k];
+    }
+    for (int i = 0; i < m.n; ++i)
+        for (int k = 0; k < 3; ++k) s.gp[3 * i + k] = (1.0 / m.vol[i]) * s.gp[3 * i + k];
+}
+
+// calc_press_corr_grad (cflow_solver_supply.cpp:177-214)
+void press_corr_grad(ocsl_flow_s* fl) {
+    const HostMesh& m = fl->mesh;
+    HostFlow& s = fl->st;
+    std::fill(s.gpc.begin(), s.gpc.end(), 0.0);
+    for (int f = 0; f < m.F; ++f) {
+        const int L = m.fL[f], R = m.fR[f];
+        double fp = m.fgeo[f] * s.pc[L];
+        fp += (1.0 - m.fgeo[f]) * s.pc[R];
+        for (int k = 0; k < 3; ++k) {
+            s.gpc[3 * L + k] = s.gpc[3 * L + k] + fp * m.fS[3 * f + k];
+            s.gpc[3 * R + k] = s.gpc[3 * R + k] - fp * m.fS[3 * f + k];
+        }
+    }
+    for (int b = 0; b < m.B; ++b) {
+        const int L = m.bL[b], t = fl->bc_type(b);
+        if (t == OCSL_BC_WALL || t == OCSL_BC_SYMMETRY || t == OCSL_BC_EMPTY || t == OCSL_BC_VELOCITY_INLET ||
+            (s.b_super[b] && t == OCSL_BC_PRESSURE_OUTLET))
+            for (int k = 0; k < 3; ++k) s.gpc[3 * L + k] = s.gpc[3 * L + k] + s.pc[L] * m.bS[3 * b + k];
+    }
+    for (int i = 0; i < m.n; ++i)
+        for (int k = 0; k < 3; ++k) s.gpc[3 * i + k] = (1.0 / m.vol[i]) * s.gpc[3 * i + k];
+}
+
+// set_boundary_conditions (cflow_solver.cpp:192-227)
+void set_bcs(ocsl_flow_s* fl) {
+    const HostMesh& m = fl->mesh;
+    HostFlow& s = fl->st;
+    const Gas& g = fl->gas;
+    for (int b = 0; b < m.B; ++b) {
+        const int L = m.bL[b], t = fl->bc_type(b);
+        if (t == OCSL_BC_VELOCITY_INLET) {
+            s.b_p[b] = s.p[L];
+            s.b_rho[b] = g.density(s.p[L], s.b_T[b]);
+            s.b_mdot[b] = s.b_rho[b] * Gas::dot3(v3(s.b_u, b), v3(m.bS, b));
+            s.b_h[b] = g.enthalpy(s.b_T[b], v3(s.b_u, b));
+        }
+        if (t == OCSL_BC_PRESSURE_OUTLET) {
+            s.b_T[b] = s.T[L];
+            for (int k = 0; k < 3; ++k) s.b_u[3 * b + k] = s.u[3 * L + k];
+            s.b_rho[b] = g.density(s.b_p[b], s.T[L]);
+            s.b_h[b] = g.enthalpy(s.T[L], v3(s.u, L));
+            // both branches mark the face supersonic (cflow_solver.cpp:208-217)
+            if ((s.b_mdot[b] / s.rho[L] / m.barea[b]) >= g.sound(s.T[L])) {
+                s.b_super[b] = 1;
+                s.b_p[b] = s.p[L];
+            } else {
+                s.b_super[b] = 1;
+                s.b_p[b] = s.b_p_out[b];
+            }
+        }
+        if (t == OCSL_BC_WALL)
+            s.b_p[b] = s.p[L] + Gas::dot3(v3(s.gp, L), v3(m.be, b)) * m.bdist[b];
+    }
+}
+
+void upload_cells(ocsl_flow_s* fl, int eq) {
+    HostFlow& s = fl->st;
+    hipStream_t st = fl->solver->stream;
+    up(fl->d_T, s.T, st);
+    up(fl->d_rho, s.rho, st);
+    up(fl->d_dt, s.dt, st);
+    up(fl->d_f_mdot, s.f_mdot, st);
+    up(fl->d_b_mdot, s.b_mdot, st);
+    if (eq <= OCSL_EQ_MOMENTUM_Z) {
+        up(fl->d_u, s.u, st);
+        up(fl->d_u_old, s.u_old, st);
+        up(fl->d_rho_old, s.rho_old, st);
+        up(fl->d_gp, s.gp, st);
+        up(fl->d_ap, s.ap, st);
+        up(fl->d_ap_vec, s.ap_vec, st);
+        up(fl->d_f_stress, s.f_stress, st);
+        up(fl->d_b_u, s.b_u, st);
+    } else if (eq == OCSL_EQ_PRESSURE) {
+        up(fl->d_rho_old, s.rho_old, st);
+        up(fl->d_pc, s.pc, st);
+        up(fl->d_f_rho, s.f_rho, st);
+        up(fl->d_f_qy, s.f_qy, st);
+        up(fl->d_b_T, s.b_T, st);
+        up(fl->d_b_rho, s.b_rho, st);
+        up(fl->d_b_qy, s.b_qy, st);
+        up(fl->d_b_super, s.b_super, st);
+    } else {
+        up(fl->d_h, s.h, st);
+        up(fl->d_h_old, s.h_old, st);
+        up(fl->d_rho_old, s.rho_old, st);
+        up(fl->d_p, s.p, st);
+        up(fl->d_p_old, s.p_old, st);
+        up(fl->d_b_h, s.b_h, st);
+        up(fl->d_b_T, s.b_T, st);
+    }
+}
+
+template <int kEq>
+void launch_assemble(ocsl_flow_s* fl, int d) {
+    const ocsl_flow_params& p = fl->prm;
+    const int n = fl->mesh.n, grid = (n + kBlock - 1) / kBlock;
+    hipStream_t st = fl->solver->stream;
+    const MeshDev m = fl->mesh_view();
+    const StateDev s = fl->state_view();
+    const FlowPhys ph = fl->phys();
+    double *val = fl->d_val.p, *b = fl->d_b.p, *x0 = fl->d_x0.p;
+#define OCSL_ASM(V, BL, TR) k_assemble<kEq, V, BL, TR><<<grid, kBlock, 0, st>>>(m, s, ph, d, val, b, x0)
+    const int key = (p.viscous ? 4 : 0) + (p.blended ? 2 : 0) + (p.transient ? 1 : 0);
+    switch (key) {
+        case 0: OCSL_ASM(false, false, false); break;
+        case 1: OCSL_ASM(false, false, true); break;
+        case 2: OCSL_ASM(false, true, false); break;
+        case 3: OCSL_ASM(false, true, true); break;
+        case 4: OCSL_ASM(true, false, false); break;
+        case 5: OCSL_ASM(true, false, true); break;
+        case 6: OCSL_ASM(true, true, false); break;
+        default: OCSL_ASM(true, true, true); break;
+    }
+#undef OCSL_ASM
+    OCSL_HIP(hipGetLastError());
+}
+
+// Assembles equation eq into d_val / d_b / d_x0 and hands them to the solver.
+void assemble(ocsl_flow_s* fl, int eq) {
+    OCSL_REQUIRE(eq >= OCSL_EQ_MOMENTUM_X && eq <= OCSL_EQ_ENTHALPY, OCSL_ERR_ARG, "bad equation");
+    hipStream_t st = fl->solver->stream;
+    upload_cells(fl, eq);
+    if (eq <= OCSL_EQ_MOMENTUM_Z) launch_assemble<kMomentum>(fl, eq);
+    else if (eq == OCSL_EQ_PRESSURE) launch_assemble<kPressure>(fl, 0);
+    else launch_assemble<kEnthalpy>(fl, 0);
+    if (eq <= OCSL_EQ_MOMENTUM_Z) {
+        down(fl->st.ap, fl->d_ap, st);
+        down(fl->st.ap_vec, fl->d_ap_vec, st);
+    }
+    OCSL_HIP(hipStreamSynchronize(st));
+    ocsl_solver* h = fl->solver;
+    h->val_in = fl->d_val.p;
+    h->have_vals = true;
+    h->val_epoch++;
+    h->csr_val_valid = h->csr_val32_valid = false;
+    h->b_dev = fl->d_b.p;
+    h->have_b = true;
+    h->x_own.alloc(fl->mesh.n);
+    OCSL_HIP(hipMemcpyAsync(h->x_own.p, fl->d_x0.p, sizeof(double) * fl->mesh.n, hipMemcpyDeviceToDevice, st));
+}
+
+void viscous_stress(ocsl_flow_s* fl) {
+    hipStream_t st = fl->solver->stream;
+    up(fl->d_u, fl->st.u, st);
+    up(fl->d_T, fl->st.T, st);
+    const int F = fl->mesh.F;
+    if (F > 0)
+        k_visc_stress<<<(F + kBlock - 1) / kBlock, kBlock, 0, st>>>(fl->mesh_view(), fl->state_view(), fl->gas);
+    OCSL_HIP(hipGetLastError());
+    down(fl->st.f_stress, fl->d_f_stress, st);
+    OCSL_HIP(hipStreamSynchronize(st));
+}
+
+// Solves the assembled system with the configured method; the solution lands
+// in fl->x_host.
+void solve(ocsl_flow_s* fl, int method) {
+    ocsl_solver* h = fl->solver;
+    double res = 0.0;
+    if (fl->prm.mode == OCSL_MODE_REFERENCE) {
+        const ocsl_status st = method == OCSL_CG ? ocsl_CG_solve(h, &res) : ocsl_ILU_BICGSTAB_solve(h, &res);
+        if (st < 0) throw Error{st, std::string("flow solve: ") + ocsl_last_error()};
+    } else {
+        ocsl_report rep{};
+        const ocsl_status st = ocsl_solve(h, method, OCSL_MODE_TOLERANCE, fl->prm.precision, &fl->prm.solve_options,
+                                          &rep);
+        if (st < 0) throw Error{st, std::string("flow solve: ") + ocsl_last_error()};
+        res = rep.residual;
+    }
+    fl->hist.push_back(res);
+    fl->x_host.resize(fl->mesh.n);
+    const ocsl_status st = ocsl_get_x(h, fl->x_host.data(), OCSL_HOST);
+    if (st < 0) throw Error{st, std::string("flow solve: ") + ocsl_last_error()};
+}
+
+// CFlow_solver::iterate (cflow_solver.cpp:229-476)
+void iterate(ocsl_flow_s* fl) {
+    const HostMesh& m = fl->mesh;
+    HostFlow& s = fl->st;
+    const Gas& g = fl->gas;
+    const int n = m.n;
+    set_bcs(fl);
+    press_grad(fl);
+    std::fill(s.ap.begin(), s.ap.end(), 0.0);
+    if (fl->prm.viscous) viscous_stress(fl);
+    for (int d = 0; d < 3; ++d) {
+        assemble(fl, OCSL_EQ_MOMENTUM_X + d);
+        solve(fl, fl->prm.method_momentum);
+        for (int i = 0; i < n; ++i) s.u[3 * i + d] = fl->x_host[i];
+    }
+    face_mdot(fl);
+    bc_mdot(fl);
+    assemble(fl, OCSL_EQ_PRESSURE);
+    solve(fl, fl->prm.method_pressure);
+    for (int i = 0; i < n; ++i) {
+        s.pc[i] = kRelaxP * fl->x_host[i];
+        s.p[i] += s.pc[i];
+    }
+    for (int b = 0; b < m.B; ++b) {
+        const int t = fl->bc_type(b);
+        if (t == OCSL_BC_VELOCITY_INLET) s.b_p[b] = s.p[m.bL[b]];
+        if (t == OCSL_BC_PRESSURE_OUTLET && s.b_super[b]) s.b_p[b] = s.p[m.bL[b]];
+    }
+    press_corr_grad(fl);
+    correct_mdot(fl);
+    for (int i = 0; i < n; ++i) s.rho[i] = g.density(s.p[i], s.T[i]);
+    for (int b = 0; b < m.B; ++b) {
+        const int L = m.bL[b], t = fl->bc_type(b);
+        if (t == OCSL_BC_VELOCITY_INLET) s.b_rho[b] = g.density(s.p[L], s.b_T[b]);
+        if (t == OCSL_BC_PRESSURE_OUTLET)
+            s.b_rho[b] = s.b_super[b] ? g.density(s.p[L], s.T[L]) : g.density(s.b_p[b], s.T[L]);
+    }
+    for (int i = 0; i < n; ++i) {
+        const double a = kRelaxU * (m.vol[i]);
+        for (int k = 0; k < 3; ++k) {
+            double den = s.ap_vec[3 * i + k];
+            if (den == 0.0) den = std::numeric_limits<double>::min();
+            s.u[3 * i + k] -= a * (s.gpc[3 * i + k] / den);
+        }
+    }
+    assemble(fl, OCSL_EQ_ENTHALPY);
+    solve(fl, fl->prm.method_enthalpy);
+    for (int i = 0; i < n; ++i) {
+        s.h[i] = fl->x_host[i];
+        s.T[i] = g.temperature(s.h[i], v3(s.u, i));
+        s.mach[i] = Gas::mag3(v3(s.u, i)) / g.sound(s.T[i]);
+    }
+}
+
+// update_timestep (cflow_solver.cpp:497-519)
+void update_timestep(ocsl_flow_s* fl) {
+    HostFlow& s = fl->st;
+    s.u_old = s.u;
+    s.rho_old = s.rho;
+    s.p_old = s.p;
+    s.h_old = s.h;
+    const ocsl_flow_params& p = fl->prm;
+    if (!p.transient && fl->urf < p.implicit_underrelax_max)
+        fl->urf += (p.implicit_underrelax_max - p.implicit_underrelax) / p.ramp_iterations;
+}
+
+// CFlow_solver::initialize (cflow_solver.cpp:34-190)
+void initialize(ocsl_flow_s* fl) {
+    const HostMesh& m = fl->mesh;
+    HostFlow& s = fl->st;
+    const Gas& g = fl->gas;
+    const ocsl_flow_params& p = fl->prm;
+    const int n = m.n;
+    for (int i = 0; i < n; ++i) {
+        s.T[i] = p.t0;
+        s.p[i] = p.p0;
+        for (int k = 0; k < 3; ++k) s.u[3 * i + k] = p.u0[k];
+        s.rho[i] = g.density(s.p[i], s.T[i]);
+        s.h[i] = g.enthalpy(s.T[i], v3(s.u, i));
+        s.rho_old[i] = s.rho[i];
+        s.p_old[i] = s.p[i];
+        for (int k = 0; k < 3; ++k) s.u_old[3 * i + k] = s.u[3 * i + k];
+        s.h_old[i] = s.h[i];
+    }
+    if (p.shock_tube) {
+        for (int i = n / 2 - 1; i < n; ++i) {
+            s.p[i] = p.p2;
+            s.T[i] = p.t2;
+            s.rho[i] = g.density(s.p[i], s.T[i]);
+            s.h[i] = g.enthalpy(s.T[i], v3(s.u, i));
+            s.rho_old[i] = s.rho[i];
+            s.p_old[i] = s.p[i];
+            s.h_old[i] = s.h[i];
+        }
+    }
+    for (int b = 0; b < m.B; ++b) {
+        const int L = m.bL[b];
+        const ocsl_bc& bc = p.bc[m.bmark[b]];
+        const int t = bc.type;
+        if (t == OCSL_BC_VELOCITY_INLET || t == OCSL_BC_SUPERSONIC_INLET) {
+            for (int k = 0; k < 3; ++k) s.b_u[3 * b + k] = bc.velocity[k];
+            s.b_p[b] = t == OCSL_BC_VELOCITY_INLET ? s.p[L] : bc.pressure;
+            s.b_T[b] = bc.temperature;
+            s.b_rho[b] = g.density(s.b_p[b], s.b_T[b]);
+            s.b_h[b] = g.enthalpy(s.b_T[b], v3(s.b_u, b));
+            s.b_mdot[b] = s.b_rho[b] * Gas::dot3(v3(s.b_u, b), v3(m.bS, b));
+        }
+        if (t == OCSL_BC_PRESSURE_OUTLET) {
+            s.b_T[b] = s.T[L];
+            s.b_p[b] = bc.pressure;
+            s.b_p_out[b] = bc.pressure;
+            s.b_rho[b] = g.density(s.b_p[b], s.b_T[b]);
+            for (int k = 0; k < 3; ++k) s.b_u[3 * b + k] = s.u[3 * L + k];
+            s.b_mdot[b] = s.rho[L] * Gas::dot3(v3(s.u, L), v3(m.bS, b));
+            s.b_h[b] = g.enthalpy(s.T[L], v3(s.u, L));
+            if ((s.b_mdot[b] / s.rho[L] / m.barea[b]) >= g.sound(s.T[L])) {
+                s.b_super[b] = 1;
+                s.b_p[b] = s.p[L];
+                s.b_rho[b] = g.density(s.b_p[b], s.b_T[b]);
+                s.b_mdot[b] = s.rho[L] * Gas::dot3(v3(s.u, L), v3(m.bS, b));
+            }
+        }
+        if (t == OCSL_BC_WALL) s.b_p[b] = s.p[L];
+    }
+    std::fill(s.ap.begin(), s.ap.end(), 1.0);
+    face_mdot(fl);
+    bc_mdot(fl);
+    if (!p.transient) fl->urf = p.implicit_underrelax;
+    else std::fill(s.dt.begin(), s.dt.end(), p.dt);
+    fl->initialized = true;
+}
+
+void alloc_state(ocsl_flow_s* fl) {
+    const int n = fl->mesh.n, F = fl->mesh.F, B = fl->mesh.B;
+    HostFlow& s = fl->st;
+    for (auto* v : {&s.u, &s.u_old, &s.gp, &s.gpc, &s.ap_vec}) v->assign(3 * (size_t)n, 0.0);
+    for (auto* v : {&s.rho, &s.rho_old, &s.p, &s.p_old, &s.pc, &s.T, &s.h, &s.h_old, &s.mach}) v->assign(n, 0.0);
+    s.ap.assign(n, 1.0);
+    s.dt.assign(n, 0.001);
+    s.f_mdot.assign(F, 0.0);
+    s.f_rho.assign(F, 0.0);
+    s.f_qy.assign(F, 1.0);
+    s.f_stress.assign(3 * (size_t)F, 0.0);
+    for (auto* v : {&s.b_T, &s.b_p, &s.b_p_out, &s.b_rho, &s.b_mdot, &s.b_h}) v->assign(B, 0.0);
+    s.b_qy.assign(B, 1.0);
+    s.b_u.assign(3 * (size_t)B, 0.0);
+    s.b_super.assign(B, 0);
+    hipStream_t st = fl->solver->stream;
+    up(fl->d_f_stress, s.f_stress, st);
+    up(fl->d_ap, s.ap, st);
+    up(fl->d_ap_vec, s.ap_vec, st);
+    for (auto* d : {&fl->d_u, &fl->d_u_old, &fl->d_gp}) d->alloc(3 * (size_t)n);
+    for (auto* d : {&fl->d_rho, &fl->d_rho_old, &fl->d_p, &fl->d_p_old, &fl->d_pc, &fl->d_T, &fl->d_h, &fl->d_h_old,
+                    &fl->d_dt})
+        d->alloc(n);
+    for (auto* d : {&fl->d_f_mdot, &fl->d_f_rho, &fl->d_f_qy}) d->alloc(F);
+    for (auto* d : {&fl->d_b_T, &fl->d_b_rho, &fl->d_b_mdot, &fl->d_b_h, &fl->d_b_qy}) d->alloc(B);
+    fl->d_b_u.alloc(3 * (size_t)B);
+    fl->d_b_super.alloc(B);
+}
+
+template <class T>
+void copy_in(std::vector<T>& dst, const T* src) {
+    if (src) std::memcpy(dst.data(), src, sizeof(T) * dst.size());
+}
+
+template <class T>
+void copy_out(T* dst, const std::vector<T>& src) {
+    if (dst) std::memcpy(dst, src.data(), sizeof(T) * src.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+void ocsl_flow_default_params(ocsl_flow_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->viscous = 0;
+    p->blended = 1;
+    p->transient = 1;
+    p->dt = 1e-3;
+    p->implicit_underrelax = 0.4;
+    p->implicit_underrelax_max = 0.8;
+    p->ramp_iterations = 100;
+    p->gamma = 1.4;
+    p->molar_mass = 28.9651159 / 1000.0;
+    p->t_ref = 298.15;
+    p->h_formation = 0.0;
+    p->sutherland[0] = 1.458e-6;
+    p->sutherland[1] = 110.4;
+    p->prandtl = 0.713;
+    p->method_momentum = p->method_pressure = p->method_enthalpy = OCSL_ILU_BICGSTAB;
+    p->mode = OCSL_MODE_REFERENCE;
+    p->precision = OCSL_FP64;
+    ocsl_default_options(&p->solve_options);
+}
+
+ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64_t* row_ptr,
+                             const uint64_t* column_index, const ocsl_flow_params* prm) {
+    if (!out || !mesh || !row_ptr || !column_index || !prm) return OCSL_ERR_ARG;
+    *out = nullptr;
+    ocsl_flow_s* fl = new (std::nothrow) ocsl_flow_s();
+    if (!fl) return OCSL_ERR_NOMEM;
+    ocsl_status st = guarded_f([&] {
+        OCSL_REQUIRE(mesh->n_cells > 0 && mesh->n_cells < (1 << 30) && mesh->n_faces >= 0 && mesh->n_bc >= 0,
+                     OCSL_ERR_ARG, "bad mesh sizes");
+        OCSL_REQUIRE(prm->n_bc >= 0 && prm->n_bc <= OCSL_FLOW_MAX_BC, OCSL_ERR_ARG, "bad boundary-condition count");
+        OCSL_REQUIRE(prm->transient || prm->implicit_underrelax > 0.0, OCSL_ERR_ARG, "bad under-relaxation");
+        fl->prm = *prm;
+        const ocsl_status cs = ocsl_create(&fl->solver, mesh->n_cells, row_ptr, column_index);
+        if (cs != OCSL_OK) throw Error{cs, std::string("flow solver handle: ") + ocsl_last_error()};
+        HostMesh& m = fl->mesh;
+        m.n = (int)mesh->n_cells;
+        m.F = (int)mesh->n_faces;
+        m.B = (int)mesh->n_bc;
+        auto cp = [](std::vector<double>& v, const double* src, size_t cnt) {
+            OCSL_REQUIRE(src || cnt == 0, OCSL_ERR_ARG, "null mesh array");
+            v.assign(src, src + cnt);
+        };
+        auto cpi = [](std::vector<int>& v, const int32_t* src, size_t cnt) {
+            OCSL_REQUIRE(src || cnt == 0, OCSL_ERR_ARG, "null mesh array");
+            v.assign(src, src + cnt);
+        };
+        const size_t n = m.n, F = m.F, B = m.B;
+        cp(m.vol, mesh->cell_volume, n);
+        cpi(m.fL, mesh->face_left, F);
+        cpi(m.fR, mesh->face_right, F);
+        cp(m.fS, mesh->face_area_vector, 3 * F);
+        cp(m.fe, mesh->face_unit, 3 * F);
+        cp(m.falpha, mesh->face_alpha, 3 * F);
+        cp(m.farea, mesh->face_area, F);
+        cp(m.fdist, mesh->face_dist, F);
+        cp(m.fksi, mesh->face_ksi, F);
+        cp(m.fgeo, mesh->face_geo_aver, F);
+        cpi(m.bL, mesh->bc_left, B);
+        cpi(m.bmark, mesh->bc_marker, B);
+        cp(m.bS, mesh->bc_area_vector, 3 * B);
+        cp(m.be, mesh->bc_unit, 3 * B);
+        cp(m.balpha, mesh->bc_alpha, 3 * B);
+        cp(m.barea, mesh->bc_area, B);
+        cp(m.bdist, mesh->bc_dist, B);
+        cp(m.bksi, mesh->bc_ksi, B);
+        cp(m.bperp, mesh->bc_perp, B);
+        for (size_t f = 0; f < F; ++f)
+            OCSL_REQUIRE(m.fL[f] >= 0 && m.fL[f] < m.n && m.fR[f] >= 0 && m.fR[f] < m.n && m.fL[f] != m.fR[f],
+                         OCSL_ERR_STRUCTURE, "face cell index out of range");
+        for (size_t b = 0; b < B; ++b) {
+            OCSL_REQUIRE(m.bL[b] >= 0 && m.bL[b] < m.n, OCSL_ERR_STRUCTURE, "boundary face cell out of range");
+            OCSL_REQUIRE(m.bmark[b] >= 0 && m.bmark[b] < prm->n_bc, OCSL_ERR_ARG, "boundary marker without condition");
+        }
+        // row layout: diagonal and neighbour positions in the caller's rows
+        const uint64_t nnz = row_ptr[n];
+        OCSL_REQUIRE(nnz < (1ull << 31), OCSL_ERR_ARG, "too many entries");
+        fl->nnz_in = (int64_t)nnz;
+        std::vector<int> rp32(n + 1), dpos(n, -1);
+        auto find = [&](int row, int c) -> int {
+            for (uint64_t q = row_ptr[row]; q < row_ptr[row + 1]; ++q)
+                if (column_index[q] == (uint64_t)c) return (int)(q - row_ptr[row]);
+            return -1;
+        };
+        for (size_t i = 0; i <= n; ++i) rp32[i] = (int)row_ptr[i];
+        for (size_t i = 0; i < n; ++i) {
+            OCSL_REQUIRE(row_ptr[i + 1] - row_ptr[i] <= (uint64_t)kMaxRow, OCSL_ERR_STRUCTURE,
+                         "row longer than the assembly supports (16 entries)");
+            dpos[i] = find((int)i, (int)i);
+            OCSL_REQUIRE(dpos[i] >= 0, OCSL_ERR_STRUCTURE, "row without diagonal");
+        }
+        std::vector<std::vector<int>> cf(n), cb(n);
+        for (size_t f = 0; f < F; ++f) {
+            cf[m.fL[f]].push_back((int)(f << 1));
+            cf[m.fR[f]].push_back((int)((f << 1) | 1));
+        }
+        for (size_t b = 0; b < B; ++b) cb[m.bL[b]].push_back((int)b);
+        std::vector<int> cf_ptr(n + 1, 0), cf_code, cf_pos, cb_ptr(n + 1, 0), cb_face;
+        for (size_t i = 0; i < n; ++i) {
+            for (int code : cf[i]) {
+                const int f = code >> 1;
+                const int other = (code & 1) ? m.fL[f] : m.fR[f];
+                const int pos = find((int)i, other);
+                OCSL_REQUIRE(pos >= 0, OCSL_ERR_STRUCTURE, "face neighbour missing from the matrix row");
+                cf_code.push_back(code);
+                cf_pos.push_back(pos);
+            }
+            cf_ptr[i + 1] = (int)cf_code.size();
+            for (int b : cb[i]) cb_face.push_back(b);
+            cb_ptr[i + 1] = (int)cb_face.size();
+        }
+        hipStream_t s = fl->solver->stream;
+        up(fl->d_vol, m.vol, s);
+        up(fl->d_fL, m.fL, s);
+        up(fl->d_fR, m.fR, s);
+        up(fl->d_fS, m.fS, s);
+        up(fl->d_fe, m.fe, s);
+        up(fl->d_fdist, m.fdist, s);
+        up(fl->d_fksi, m.fksi, s);
+        up(fl->d_fgeo, m.fgeo, s);
+        up(fl->d_bL, m.bL, s);
+        up(fl->d_bmark, m.bmark, s);
+        up(fl->d_bS, m.bS, s);
+        up(fl->d_barea, m.barea, s);
+        up(fl->d_bksi, m.bksi, s);
+        up(fl->d_bperp, m.bperp, s);
+        up(fl->d_cf_ptr, cf_ptr, s);
+        up(fl->d_cf_code, cf_code, s);
+        up(fl->d_cf_pos, cf_pos, s);
+        up(fl->d_cb_ptr, cb_ptr, s);
+        up(fl->d_cb_face, cb_face, s);
+        up(fl->d_row_ptr, rp32, s);
+        up(fl->d_dpos, dpos, s);
+        fl->d_val.alloc(nnz);
+        fl->d_b.alloc(n);
+        fl->d_x0.alloc(n);
+        // gas constants (Cfluid_properties::calc_R_specific / calc_CP_fixed)
+        Gas& g = fl->gas;
+        g.R = 8.31446261815324 / prm->molar_mass;
+        g.gamma = prm->gamma;
+        g.cp = (g.gamma * g.R) / (g.gamma - 1.0);
+        g.t_ref = prm->t_ref;
+        g.h_f = prm->h_formation;
+        g.s0 = prm->sutherland[0];
+        g.s1 = prm->sutherland[1];
+        g.pr = prm->prandtl;
+        fl->urf = prm->transient ? 0.5 : prm->implicit_underrelax;
+        alloc_state(fl);
+        OCSL_HIP(hipStreamSynchronize(s));
+    });
+    if (st != OCSL_OK) {
+        if (fl->solver) ocsl_destroy(fl->solver);
+        fl->solver = nullptr;
+        delete fl;
+        return st;
+    }
+    *out = fl;
+    return OCSL_OK;
+}
+
+ocsl_status ocsl_flow_destroy(ocsl_flow fl) {
+    if (!fl) return OCSL_ERR_ARG;
+    ocsl_solver* h = fl->solver;
+    if (h) (void)hipStreamSynchronize(h->stream);
+    fl->solver = nullptr;
+    delete fl;  // device buffers first, then the handle (its stream)
+    if (h) ocsl_destroy(h);
+    return OCSL_OK;
+}
+
+ocsl_handle ocsl_flow_solver(ocsl_flow fl) { return fl ? fl->solver : nullptr; }
+
+ocsl_status ocsl_flow_initialize(ocsl_flow fl) {
+    if (!fl) return OCSL_ERR_ARG;
+    return guarded_f([&] { initialize(fl); });
+}
+
+ocsl_status ocsl_flow_set_state(ocsl_flow fl, const ocsl_flow_state* st) {
+    if (!fl || !st) return OCSL_ERR_ARG;
+    return guarded_f([&] {
+        HostFlow& s = fl->st;
+        copy_in(s.u, st->u);
+        copy_in(s.u_old, st->u_old);
+        copy_in(s.rho, st->rho);
+        copy_in(s.rho_old, st->rho_old);
+        copy_in(s.p, st->p);
+        copy_in(s.p_old, st->p_old);
+        copy_in(s.pc, st->p_corr);
+        copy_in(s.gp, st->grad_p);
+        copy_in(s.gpc, st->grad_pc);
+        copy_in(s.T, st->T);
+        copy_in(s.h, st->h);
+        copy_in(s.h_old, st->h_old);
+        copy_in(s.ap, st->ap);
+        copy_in(s.ap_vec, st->ap_vec);
+        copy_in(s.dt, st->dt);
+        copy_in(s.mach, st->mach);
+        copy_in(s.f_mdot, st->f_mdot);
+        copy_in(s.f_rho, st->f_rho);
+        copy_in(s.f_qy, st->f_qy);
+        copy_in(s.f_stress, st->f_stress);
+        copy_in(s.b_T, st->b_T);
+        copy_in(s.b_p, st->b_p);
+        copy_in(s.b_p_out, st->b_p_out);
+        copy_in(s.b_rho, st->b_rho);
+        copy_in(s.b_mdot, st->b_mdot);
+        copy_in(s.b_h, st->b_h);
+        copy_in(s.b_qy, st->b_qy);
+        copy_in(s.b_u, st->b_u);
+        copy_in(s.b_super, st->b_supersonic);
+        if (st->f_stress) up(fl->d_f_stress, s.f_stress, fl->solver->stream);
+        OCSL_HIP(hipStreamSynchronize(fl->solver->stream));
+        fl->initialized = true;
+    });
+}
+
+ocsl_status ocsl_flow_get_state(ocsl_flow fl, ocsl_flow_state* st) {
+    if (!fl || !st) return OCSL_ERR_ARG;
+    const HostFlow& s = fl->st;
+    copy_out(st->u, s.u);
+    copy_out(st->u_old, s.u_old);
+    copy_out(st->rho, s.rho);
+    copy_out(st->rho_old, s.rho_old);
+    copy_out(st->p, s.p);
+    copy_out(st->p_old, s.p_old);
+    copy_out(st->p_corr, s.pc);
+    copy_out(st->grad_p, s.gp);
+    copy_out(st->grad_pc, s.gpc);
+    copy_out(st->T, s.T);
+    copy_out(st->h, s.h);
+    copy_out(st->h_old, s.h_old);
+    copy_out(st->ap, s.ap);
+    copy_out(st->ap_vec, s.ap_vec);
+    copy_out(st->dt, s.dt);
+    copy_out(st->mach, s.mach);
+    copy_out(st->f_mdot, s.f_mdot);
+    copy_out(st->f_rho, s.f_rho);
+    copy_out(st->f_qy, s.f_qy);
+    copy_out(st->f_stress, s.f_stress);
+    copy_out(st->b_T, s.b_T);
+    copy_out(st->b_p, s.b_p);
+    copy_out(st->b_p_out, s.b_p_out);
+    copy_out(st->b_rho, s.b_rho);
+    copy_out(st->b_mdot, s.b_mdot);
+    copy_out(st->b_h, s.b_h);
+    copy_out(st->b_qy, s.b_qy);
+    copy_out(st->b_u, s.b_u);
+    copy_out(st->b_supersonic, s.b_super);
+    return OCSL_OK;
+}
+
+ocsl_status ocsl_flow_assemble(ocsl_flow fl, int equation, double* val, double* b, double* x0, ocsl_where where) {
+    if (!fl) return OCSL_ERR_ARG;
+    return guarded_f([&] {
+        OCSL_REQUIRE(where == OCSL_HOST || where == OCSL_DEVICE, OCSL_ERR_ARG, "bad ocsl_where");
+        assemble(fl, equation);
+        hipStream_t st = fl->solver->stream;
+        const auto kind = where == OCSL_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        if (val) OCSL_HIP(hipMemcpyAsync(val, fl->d_val.p, sizeof(double) * fl->nnz_in, kind, st));
+        if (b) OCSL_HIP(hipMemcpyAsync(b, fl->d_b.p, sizeof(double) * fl->mesh.n, kind, st));
+        if (x0) OCSL_HIP(hipMemcpyAsync(x0, fl->d_x0.p, sizeof(double) * fl->mesh.n, kind, st));
+        OCSL_HIP(hipStreamSynchronize(st));
+    });
+}
+
+ocsl_status ocsl_flow_viscous_stress(ocsl_flow fl) {
+    if (!fl) return OCSL_ERR_ARG;
+    return guarded_f([&] { viscous_stress(fl); });
+}
+
+ocsl_status ocsl_flow_iterate(ocsl_flow fl, int end_of_step) {
+    if (!fl) return OCSL_ERR_ARG;
+    return guarded_f([&] {
+        OCSL_REQUIRE(fl->initialized, OCSL_ERR_STATE, "flow not initialized");
+        iterate(fl);
+        if (end_of_step) update_timestep(fl);
+    });
+}
+
+ocsl_status ocsl_flow_run(ocsl_flow fl, int steps, int inner) {
+    if (!fl || steps < 0 || inner < 1) return OCSL_ERR_ARG;
+    return guarded_f([&] {
+        OCSL_REQUIRE(fl->initialized, OCSL_ERR_STATE, "flow not initialized");
+        const int in = fl->prm.transient ? inner : 1;  // Croot::initialize_solver (croot.cpp:92-93)
+        for (int s = 0; s < steps; ++s) {
+            for (int k = 0; k < in; ++k) iterate(fl);
+            update_timestep(fl);
+        }
+    });
+}
+
+ocsl_status ocsl_flow_solve_history(ocsl_flow fl, double* out, int64_t cap, int64_t* len) {
+    if (!fl || !len) return OCSL_ERR_ARG;
+    *len = (int64_t)fl->hist.size();
+    if (out)
+        for (int64_t k = 0; k < std::min<int64_t>(cap, *len); ++k) out[k] = fl->hist[k];
+    return OCSL_OK;
+}
+
+}  // extern "C"

@@ -126,8 +126,170 @@ def sod_cases():
         json.dump(meta, f, indent=1)
 
 
+class RefFlowConfig(ctypes.Structure):
+    _fields_ = [("n_cells", ctypes.c_int), ("lx", ctypes.c_double), ("ly", ctypes.c_double),
+                ("lz", ctypes.c_double), ("steps", ctypes.c_int), ("inner", ctypes.c_int),
+                ("dt", ctypes.c_double), ("steady", ctypes.c_int), ("implicit_underrelax", ctypes.c_double),
+                ("implicit_underrelax_max", ctypes.c_double), ("ramp_iterations", ctypes.c_int),
+                ("solver_type", ctypes.c_int), ("solver_mode", ctypes.c_int), ("conv_type", ctypes.c_int),
+                ("visc_type", ctypes.c_int), ("init_shock_tube", ctypes.c_int), ("p0", ctypes.c_double),
+                ("T0", ctypes.c_double), ("u0", ctypes.c_double * 3), ("p2", ctypes.c_double),
+                ("T2", ctypes.c_double), ("bc_type", ctypes.c_int * 3), ("bc_no_slip", ctypes.c_int * 3),
+                ("bc_const_T", ctypes.c_int * 3), ("bc_T", ctypes.c_double * 3), ("bc_p", ctypes.c_double * 3),
+                ("bc_u", (ctypes.c_double * 3) * 3)]
+
+
+STATE_CELL = ("u", "u_old", "rho", "rho_old", "p", "p_old", "p_corr", "grad_p", "T", "h", "h_old", "ap",
+              "ap_vec", "dt")
+STATE_FACE = ("f_mdot", "f_rho", "f_qy", "f_stress")
+STATE_BC = ("b_T", "b_p", "b_rho", "b_mdot", "b_h", "b_qy", "b_u")
+VEC3 = ("u", "u_old", "grad_p", "ap_vec", "f_stress", "b_u")
+MESH_F = ("face_area_vector", "face_unit", "face_alpha", "face_area", "face_dist", "face_ksi", "face_geo_aver")
+MESH_B = ("bc_area_vector", "bc_unit", "bc_alpha", "bc_area", "bc_dist", "bc_ksi", "bc_perp")
+
+
+class RefFlowState(ctypes.Structure):
+    _fields_ = ([(k, _f) for k in STATE_CELL + STATE_FACE + STATE_BC] + [("b_super", _ip)])
+
+
+class RefFlowMesh(ctypes.Structure):
+    _fields_ = ([("cell_volume", _f), ("face_left", _ip), ("face_right", _ip)] + [(k, _f) for k in MESH_F]
+                + [("bc_left", _ip), ("bc_marker", _ip)] + [(k, _f) for k in MESH_B])
+
+
+# Eflow_BC_type order (enums.h:38)
+EMPTY, SYMMETRY, WALL, VELOCITY_INLET, SUPERSONIC_INLET, PRESSURE_OUTLET = range(6)
+
+
+def flow_run(cfg, max_dump, max_state):
+    """Runs the reference flow solver on its 1D mesh; returns the mesh, the
+    systems and flow states of the first solves, and the final fields."""
+    c = RefFlowConfig()
+    for k, v in cfg.items():
+        if k in ("u0",):
+            for d in range(3):
+                c.u0[d] = v[d]
+        elif k in ("bc_type", "bc_no_slip", "bc_const_T", "bc_T", "bc_p"):
+            for m in range(3):
+                getattr(c, k)[m] = v[m]
+        elif k == "bc_u":
+            for m in range(3):
+                for d in range(3):
+                    c.bc_u[m][d] = v[m][d]
+        else:
+            setattr(c, k, v)
+    n = c.n_cells
+    F, B, nnz0 = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    oracle.ref().ref_flow_sizes(n, ctypes.c_double(c.lx), ctypes.c_double(c.ly), ctypes.c_double(c.lz),
+                                ctypes.byref(F), ctypes.byref(B), ctypes.byref(nnz0))
+    F, B = F.value, B.value
+    nnz_cap = 5 * n
+    steps = c.steps
+    inner = 1 if c.steady else c.inner
+    hist_cap = steps * inner * 5
+    hist = np.zeros(hist_cap)
+    rp = np.zeros(n + 1, np.uint64)
+    col = np.zeros(nnz_cap, np.uint64)
+    nnz = ctypes.c_int64()
+    md = max(max_dump, 1)
+    dv, db, dx0, dx = np.zeros((md, nnz_cap)), np.zeros((md, n)), np.zeros((md, n)), np.zeros((md, n))
+    dm = np.zeros(md, np.int32)
+    ms = max(max_state, 1)
+    st_arr = {}
+    for k in STATE_CELL:
+        st_arr[k] = np.zeros((ms, n, 3) if k in VEC3 else (ms, n))
+    for k in STATE_FACE:
+        st_arr[k] = np.zeros((ms, F, 3) if k in VEC3 else (ms, F))
+    for k in STATE_BC:
+        st_arr[k] = np.zeros((ms, B, 3) if k in VEC3 else (ms, B))
+    st_arr["b_super"] = np.zeros((ms, B), np.int32)
+    st = RefFlowState(**{k: v.ctypes.data_as(_ip if k == "b_super" else _f) for k, v in st_arr.items()})
+    mesh_arr = {"cell_volume": np.zeros(n), "face_left": np.zeros(F, np.int32), "face_right": np.zeros(F, np.int32),
+                "bc_left": np.zeros(B, np.int32), "bc_marker": np.zeros(B, np.int32)}
+    for k in MESH_F:
+        mesh_arr[k] = np.zeros((F, 3) if k.endswith(("vector", "unit", "alpha")) else F)
+    for k in MESH_B:
+        mesh_arr[k] = np.zeros((B, 3) if k.endswith(("vector", "unit", "alpha")) else B)
+    mesh = RefFlowMesh(**{k: v.ctypes.data_as(_ip if v.dtype == np.int32 else _f) for k, v in mesh_arr.items()})
+    p, T, rho, h = (np.zeros(n) for _ in range(4))
+    u = np.zeros(3 * n)
+    cwd = os.getcwd()
+    with tempfile.TemporaryDirectory() as tmp:
+        os.chdir(tmp)
+        try:
+            cnt = oracle.ref().ref_flow_run(
+                ctypes.byref(c), ctypes.c_int64(hist_cap), hist.ctypes.data_as(_f), max_dump,
+                ctypes.c_int64(nnz_cap), rp.ctypes.data_as(_u), col.ctypes.data_as(_u), ctypes.byref(nnz),
+                dv.ctypes.data_as(_f), db.ctypes.data_as(_f), dx0.ctypes.data_as(_f), dx.ctypes.data_as(_f),
+                dm.ctypes.data_as(_ip), max_state, ctypes.byref(st), ctypes.byref(mesh),
+                p.ctypes.data_as(_f), u.ctypes.data_as(_f), T.ctypes.data_as(_f), rho.ctypes.data_as(_f),
+                h.ctypes.data_as(_f))
+        finally:
+            os.chdir(cwd)
+    assert cnt > 0, cnt
+    k = int(nnz.value)
+    out = dict(count=cnt, res_hist=hist[:cnt], row_ptr=rp, col=col[:k], dump_val=dv[:max_dump, :k],
+               dump_b=db[:max_dump], dump_x0=dx0[:max_dump], dump_x=dx[:max_dump], dump_method=dm[:max_dump],
+               p=p, u=u.reshape(n, 3), T=T, rho=rho, h=h)
+    out.update({"state_" + k: v[:max_state] for k, v in st_arr.items()})
+    out.update({"mesh_" + k: v for k, v in mesh_arr.items()})
+    return out
+
+
+_AIR_SLIP = dict(bc_type=(WALL, EMPTY, WALL), bc_no_slip=(0, 0, 0), bc_const_T=(0, 0, 0), bc_T=(0, 0, 0),
+                 bc_p=(0, 0, 0), bc_u=((0, 0, 0),) * 3)
+
+FLOW_CASES = {
+    # the shock tube on a coarser mesh: inviscid, blended, slip walls
+    "sod64": dict(n_cells=64, lx=1.0, ly=0.1, lz=0.1, steps=3, inner=10, dt=1e-5, steady=0,
+                  implicit_underrelax=0.4, implicit_underrelax_max=0.8, ramp_iterations=100, solver_type=1,
+                  solver_mode=0, conv_type=1, visc_type=1, init_shock_tube=1, p0=1e5, T0=348.4,
+                  u0=(0, 0, 0), p2=1e4, T2=278.7, **_AIR_SLIP),
+    # duct: velocity inlet, no-slip side walls, subsonic pressure outlet; viscous, upwind
+    "duct64": dict(n_cells=64, lx=1.0, ly=0.1, lz=0.1, steps=3, inner=5, dt=1e-4, steady=0,
+                   implicit_underrelax=0.4, implicit_underrelax_max=0.8, ramp_iterations=100, solver_type=1,
+                   solver_mode=0, conv_type=0, visc_type=0, init_shock_tube=0, p0=1e5, T0=300.0,
+                   u0=(5.0, 0.5, 0.2), p2=0.0, T2=0.0, bc_type=(VELOCITY_INLET, WALL, PRESSURE_OUTLET),
+                   bc_no_slip=(0, 1, 0), bc_const_T=(0, 0, 0), bc_T=(320.0, 0, 0), bc_p=(0, 0, 99990.0),
+                   bc_u=((10.0, 0.3, -0.2), (0, 0, 0), (0, 0, 0))),
+    # supersonic inlet, symmetry sides, outlet; viscous, blended
+    "jet64": dict(n_cells=64, lx=1.0, ly=0.1, lz=0.1, steps=3, inner=5, dt=1e-5, steady=0,
+                  implicit_underrelax=0.4, implicit_underrelax_max=0.8, ramp_iterations=100, solver_type=1,
+                  solver_mode=0, conv_type=1, visc_type=0, init_shock_tube=0, p0=1e5, T0=300.0,
+                  u0=(500.0, 0.0, 0.0), p2=0.0, T2=0.0, bc_type=(SUPERSONIC_INLET, SYMMETRY, PRESSURE_OUTLET),
+                  bc_no_slip=(0, 0, 0), bc_const_T=(0, 0, 0), bc_T=(300.0, 0, 0), bc_p=(1e5, 0, 1e5),
+                  bc_u=((520.0, 0.0, 0.0), (0, 0, 0), (0, 0, 0))),
+    # steady, implicit under-relaxation ramp; viscous, upwind, inlet / no-slip / outlet
+    "steady64": dict(n_cells=64, lx=1.0, ly=0.1, lz=0.1, steps=6, inner=1, dt=1e-3, steady=1,
+                     implicit_underrelax=0.4, implicit_underrelax_max=0.8, ramp_iterations=4, solver_type=1,
+                     solver_mode=1, conv_type=0, visc_type=0, init_shock_tube=0, p0=1e5, T0=300.0,
+                     u0=(2.0, 0.0, 0.0), p2=0.0, T2=0.0, bc_type=(VELOCITY_INLET, WALL, PRESSURE_OUTLET),
+                     bc_no_slip=(0, 1, 0), bc_const_T=(0, 0, 0), bc_T=(300.0, 0, 0), bc_p=(0, 0, 1e5),
+                     bc_u=((2.0, 0.0, 0.0), (0, 0, 0), (0, 0, 0))),
+}
+
+
+def flow_cases():
+    meta = {}
+    for tag, cfg in FLOW_CASES.items():
+        r = flow_run(cfg, max_dump=10, max_state=10)
+        np.savez_compressed(os.path.join(HERE, f"flow_{tag}.npz"), **r)
+        meta[tag] = cfg
+    # the 1000-cell shock tube mesh of the sod_* fixtures
+    cfg = dict(FLOW_CASES["sod64"], n_cells=1000, dt=1e-6, steps=1, inner=1)
+    r = flow_run(cfg, max_dump=0, max_state=0)
+    np.savez_compressed(os.path.join(HERE, "flow_sod1000_mesh.npz"),
+                        **{k: v for k, v in r.items() if k.startswith("mesh_") or k in ("row_ptr", "col")})
+    with open(os.path.join(HERE, "flow_meta.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
 if __name__ == "__main__":
     if not oracle.ref_available():
         sys.exit("oracle/_ref/libocsl_ref.so missing: make -C oracle ref (needs the reference sources)")
+    if len(sys.argv) > 1 and sys.argv[1] == "flow":
+        flow_cases()
+        sys.exit(0)
     solver_cases()
     sod_cases()
+    flow_cases()

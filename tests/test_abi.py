@@ -16,7 +16,8 @@ def _declared(header):
     return sorted(set(re.findall(r"\b(ocsl_[A-Za-z0-9_]+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header,lib", [("ocsl.h", "libocsl_hip.so"), ("ocsl_gen.h", "libocsl_gen.so")])
+@pytest.mark.parametrize("header,lib", [("ocsl.h", "libocsl_hip.so"), ("ocsl_flow.h", "libocsl_hip.so"),
+                                        ("ocsl_gen.h", "libocsl_gen.so")])
 def test_exports(header, lib):
     path = os.path.join(PKG, lib)
     assert os.path.exists(path), f"{lib} not built"
@@ -31,6 +32,26 @@ def test_python_signatures_cover_header():
     from opencslsolver_amd import _abi
 
     assert sorted(_abi.SIGNATURES) == _declared("ocsl.h")
+
+
+def test_flow_signatures_cover_header():
+    from opencslsolver_amd import flow
+
+    assert sorted(flow.SIGNATURES) == _declared("ocsl_flow.h")
+
+
+def test_flow_struct_layouts():
+    """ctypes mirrors of the flow structs have the C sizes (checked against
+    offsets computed from the header's field list)."""
+    import ctypes
+
+    from opencslsolver_amd import _abi, flow
+
+    assert ctypes.sizeof(flow.BC) == 3 * 4 + 4 + 5 * 8
+    assert ctypes.sizeof(flow.Mesh) == 3 * 8 + 19 * 8
+    assert ctypes.sizeof(flow.State) == 29 * 8
+    assert flow.Params.solve_options.offset % 8 == 0
+    assert ctypes.sizeof(flow.Params) == flow.Params.solve_options.offset + ctypes.sizeof(_abi.Options)
 
 
 def test_no_device_is_reported_not_fatal():
