@@ -138,7 +138,7 @@ def headline(args, out):
         _log("bench: CPU baseline")
         cpu = cpu_baseline(a, b, r64["iterations"])
 
-    kname = "k_cgm_spmv<float, float>"
+    kname = "k_cgm_spmv<float, float, 2>"
     out.update({
         "value": round(msm, 3),
         "ms_per_step": round(msm, 3),
@@ -159,7 +159,7 @@ def headline(args, out):
                      "frac": round(ach_m / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname),
                      "avg_launch_us": round(spmv_ms_m * 1e3, 2),
                      "bytes_per_launch": spmv_bytes_m,
-                     "fp64_kernel": {"kernel": "k_cg_spmv<double, double>", "achieved": round(ach_64, 1),
+                     "fp64_kernel": {"kernel": "k_cg_spmv<double, double, 2>", "achieved": round(ach_64, 1),
                                      "avg_launch_us": round(spmv_ms_64 * 1e3, 2),
                                      "bytes_per_launch": spmv_bytes_64}},
         "cpu_baseline": cpu,

@@ -47,6 +47,7 @@ class Options(ctypes.Structure):
         ("block_size", ctypes.c_int),
         ("grid_blocks", ctypes.c_int),
         ("vec_blocks", ctypes.c_int),
+        ("rows_per_lane", ctypes.c_int),
     ]
 
 
@@ -64,6 +65,11 @@ class Report(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("rows_per_lane", ctypes.c_int), ("block_size", ctypes.c_int), ("grid_blocks", ctypes.c_int),
+                ("vec_blocks", ctypes.c_int), ("us_per_iteration", ctypes.c_double)]
 
 
 # name -> (restype, argtypes)
@@ -91,9 +97,12 @@ SIGNATURES = {
     "ocsl_gemv": (ctypes.c_int, [c_handle, ctypes.c_double, ctypes.c_void_p, ctypes.c_double,
                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "ocsl_ILU_decompose": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
-    "ocsl_time_cg_spmv": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+    "ocsl_time_cg_spmv": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _f64p, _f64p]),
     "ocsl_get_stream": (ctypes.c_void_p, [c_handle]),
+    "ocsl_tune_cg": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Options), _f64p,
+                                    ctypes.POINTER(TuneEntry), ctypes.c_int, _i32p]),
+    "ocsl_set_tuned_cg": (ctypes.c_int, [c_handle, ctypes.POINTER(Options)]),
     "ocsl_adjoint": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.POINTER(Options), ctypes.POINTER(Report)]),

@@ -82,15 +82,16 @@ void ensure_sell_values(ocsl_solver* h, Sell& s, bool need64, bool need32) {
     if (want32) e32 = h->val_epoch;
 }
 
-SellP& sellp_for(ocsl_solver* h, bool reorder) {
-    SellP& s = reorder ? h->sellp_rcm : h->sellp_id;
+SellP& sellp_for(ocsl_solver* h, bool reorder, int rpl) {
+    OCSL_REQUIRE(rpl == 1 || rpl == 2, OCSL_ERR_ARG, "rows_per_lane must be 0, 1 or 2");
+    SellP& s = h->sellp[reorder ? 1 : 0][rpl - 1];
     if (s.n == 0) {
         drop_graphs(h);
         if (reorder) {
             std::vector<int> order = rcm_order(h->host);
-            build_sellp(h->host, &order, s, h->stream);
+            build_sellp(h->host, &order, rpl, s, h->stream);
         } else {
-            build_sellp(h->host, nullptr, s, h->stream);
+            build_sellp(h->host, nullptr, rpl, s, h->stream);
         }
     }
     return s;
@@ -101,8 +102,8 @@ SellP& sellp_for(ocsl_solver* h, bool reorder) {
 // sum rounded once.
 void ensure_sellp_values(ocsl_solver* h, SellP& s, bool need64, bool need32) {
     OCSL_REQUIRE(h->have_vals, OCSL_ERR_STATE, "values not set");
-    uint64_t& e64 = (&s == &h->sellp_rcm) ? h->sellp_rcm_epoch64 : h->sellp_id_epoch64;
-    uint64_t& e32 = (&s == &h->sellp_rcm) ? h->sellp_rcm_epoch32 : h->sellp_id_epoch32;
+    uint64_t& e64 = s.epoch64;
+    uint64_t& e32 = s.epoch32;
     const bool want64 = (need64 || need32) && e64 != h->val_epoch;
     const bool want32 = need32 && e32 != h->val_epoch;
     if (!want64 && !want32) return;
@@ -124,8 +125,9 @@ void ensure_sellp_values(ocsl_solver* h, SellP& s, bool need64, bool need32) {
     }
     if (want32) {
         k_gather_vals<float><<<g, kBlock, 0, h->stream>>>(s.total, s.src.p, h->val_in, s.val32.p);
-        k_sellp_rowsum<<<grid_for((int64_t)s.n_slices * kWave), kBlock, 0, h->stream>>>(s.view<double>(),
-                                                                                       s.diag32.p);
+        const int gr = grid_for((int64_t)s.n_slices * kWave);
+        if (s.rpl == 2) k_sellp_rowsum<2><<<gr, kBlock, 0, h->stream>>>(s.view<double>(), s.diag32.p);
+        else k_sellp_rowsum<1><<<gr, kBlock, 0, h->stream>>>(s.view<double>(), s.diag32.p);
         e32 = h->val_epoch;
     }
     OCSL_HIP(hipGetLastError());
@@ -213,6 +215,7 @@ void ocsl_default_options(ocsl_options* o) {
     o->block_size = 0;
     o->grid_blocks = 0;
     o->vec_blocks = 0;
+    o->rows_per_lane = 0;
 }
 
 int ocsl_device_count(void) {

@@ -35,11 +35,14 @@ static void launch_spmv(ocsl_solver* h, const Sell& s, double a, const double* x
 static double true_relres_p(ocsl_solver* h, const SellP& s, const double* b, const double* x) {
     const int n = s.n;
     double* r = work_buf(h, h->work.d[11], n);
-    const int g = std::min(kMaxRedBlocks, (s.n_slices + 15) / 16 * 1);
-    k_cg_residual<false><<<std::max(1, g), 1024, 0, h->stream>>>(s.view<double>(), b, x, r, nullptr, nullptr,
-                                                                 nullptr, h->red(), 0);
-    k_finalize<2, FinStore<2>><<<1, kBlock, 0, h->stream>>>(h->red(), std::max(1, g), FinStore<2>{h->dscal.p},
-                                                            kSlotRR);
+    const int g = std::max(1, std::min(kMaxRedBlocks, (s.n_slices + 15) / 16 * 1));
+    if (s.rpl == 2)
+        k_cg_residual<false, 2><<<g, 1024, 0, h->stream>>>(s.view<double>(), b, x, r, nullptr, nullptr, nullptr,
+                                                           h->red(), 0);
+    else
+        k_cg_residual<false, 1><<<g, 1024, 0, h->stream>>>(s.view<double>(), b, x, r, nullptr, nullptr, nullptr,
+                                                           h->red(), 0);
+    k_finalize<2, FinStore<2>><<<1, kBlock, 0, h->stream>>>(h->red(), g, FinStore<2>{h->dscal.p}, kSlotRR);
     OCSL_HIP(hipGetLastError());
     OCSL_HIP(hipMemcpyAsync(h->dscal_host, h->dscal.p, 2 * sizeof(double), hipMemcpyDeviceToHost,
                             h->stream));
@@ -141,13 +144,13 @@ CgLaunch cg_launch(int n_slices, int n, const ocsl_options& o) {
 
 // FP64: the textbook three-kernel iteration; MIXED: the two-kernel iteration
 // with merged reductions.
-template <class V, class X>
+template <int R, class V, class X>
 void launch_iter(ocsl_solver* h, const CgArgs<V, X>& c, int n, CgLaunch L) {
     if constexpr (sizeof(X) == 4) {
-        k_cgm_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
+        k_cgm_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(c);
         k_cgm_update<V, X><<<L.vgrid, L.block, 0, h->stream>>>(c, n);
     } else {
-        k_cg_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
+        k_cg_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(c);
         k_cg_update_r<V, X><<<L.vgrid, L.block, 0, h->stream>>>(c, n);
         k_cg_update_xp<V, X><<<L.vgrid, L.block, 0, h->stream>>>(c, n);
     }
@@ -156,20 +159,21 @@ void launch_iter(ocsl_solver* h, const CgArgs<V, X>& c, int n, CgLaunch L) {
 // `chunk` iterations, replayed as one hipGraph when use_graph.  Kernel arguments are fixed per handle: every buffer
 // involved is allocated once at the handle's size (work_buf drops the cached
 // graphs if one is reallocated).
-template <class V, class X>
+template <int R, class V, class X>
 void run_chunk(ocsl_solver* h, const CgArgs<V, X>& c, int n, int chunk, bool use_graph, int kind,
                CgLaunch L) {
     if (!use_graph) {
-        for (int k = 0; k < chunk; ++k) launch_iter(h, c, n, L);
+        for (int k = 0; k < chunk; ++k) launch_iter<R>(h, c, n, L);
         OCSL_HIP(hipGetLastError());
         return;
     }
-    auto key = std::make_pair(kind + 4 * L.grid + (1 << 13) * L.vgrid + (1 << 25) * (L.block / kWave), chunk);
+    auto key = std::make_pair(
+        kind + 4 * L.grid + (1 << 13) * L.vgrid + (1 << 25) * (L.block / kWave) + (1 << 30) * (R - 1), chunk);
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
         hipGraph_t gr;
         OCSL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < chunk; ++k) launch_iter(h, c, n, L);
+        for (int k = 0; k < chunk; ++k) launch_iter<R>(h, c, n, L);
         OCSL_HIP(hipStreamEndCapture(h->stream, &gr));
         hipGraphExec_t ex;
         OCSL_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
@@ -186,8 +190,25 @@ void read_scalars(ocsl_solver* h) {
 
 }  // namespace
 
-void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report* rep) {
-    SellP& s = sellp_for(h, o.reorder != 0);
+// Rows per lane of the SELL-P layout: the option, else the handle's tuned
+// choice, else 2 (two rows per lane: 16-byte value and 8-byte vector accesses).
+int cg_rows_per_lane(const ocsl_solver* h, const ocsl_options& o) {
+    if (o.rows_per_lane > 0) return o.rows_per_lane;
+    return h->tuned.rows_per_lane > 0 ? h->tuned.rows_per_lane : 2;
+}
+
+// Launch options left at 0 take the handle's tuned values.
+ocsl_options with_tuned(const ocsl_solver* h, const ocsl_options& o) {
+    ocsl_options t = o;
+    if (t.block_size <= 0) t.block_size = h->tuned.block_size;
+    if (t.grid_blocks <= 0) t.grid_blocks = h->tuned.grid_blocks;
+    if (t.vec_blocks <= 0) t.vec_blocks = h->tuned.vec_blocks;
+    return t;
+}
+
+void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_report* rep) {
+    const ocsl_options o = with_tuned(h, o_in);
+    SellP& s = sellp_for(h, o.reorder != 0, cg_rows_per_lane(h, o));
     ensure_sellp_values(h, s, true, mixed);
     const int n = s.n;
     const int* perm = s.identity ? nullptr : s.perm.p;
@@ -226,12 +247,19 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report
     c32.g_spmv = c64.g_spmv = L.grid;
     c32.g_vec = c64.g_vec = L.vgrid;
     auto residual = [&](int first) {
-        if (mixed)
-            k_cg_residual<true><<<L.grid, L.block, 0, h->stream>>>(s.view<double>(), b, x, nullptr, c32.r,
-                                                                  nullptr, c32.p, h->red(), first);
+        const SellPView<double> A = s.view<double>();
+        if (mixed && s.rpl == 2)
+            k_cg_residual<true, 2><<<L.grid, L.block, 0, h->stream>>>(A, b, x, nullptr, c32.r, nullptr, c32.p,
+                                                                     h->red(), first);
+        else if (mixed)
+            k_cg_residual<true, 1><<<L.grid, L.block, 0, h->stream>>>(A, b, x, nullptr, c32.r, nullptr, c32.p,
+                                                                     h->red(), first);
+        else if (s.rpl == 2)
+            k_cg_residual<false, 2><<<L.grid, L.block, 0, h->stream>>>(A, b, x, c64.r, nullptr, c64.p, nullptr,
+                                                                      h->red(), first);
         else
-            k_cg_residual<false><<<L.grid, L.block, 0, h->stream>>>(s.view<double>(), b, x, c64.r, nullptr,
-                                                                   c64.p, nullptr, h->red(), first);
+            k_cg_residual<false, 1><<<L.grid, L.block, 0, h->stream>>>(A, b, x, c64.r, nullptr, c64.p, nullptr,
+                                                                      h->red(), first);
         k_finalize<2, FinCgResidual><<<1, kBlock, 0, h->stream>>>(
             h->red(), L.grid, FinCgResidual{h->S.p, h->hist.p, first, L.grid}, kSlotRR);
         OCSL_HIP(hipGetLastError());
@@ -244,8 +272,10 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report
         const int left = o.max_iter - h->S_host->iter;
         const bool graph = o.use_graph && left >= o.check_every;
         const int chunk = graph ? o.check_every : std::max(1, std::min(o.check_every, left));
-        if (mixed) run_chunk(h, c32, n, chunk, graph, kind, L);
-        else run_chunk(h, c64, n, chunk, graph, kind, L);
+        if (mixed && s.rpl == 2) run_chunk<2>(h, c32, n, chunk, graph, kind, L);
+        else if (mixed) run_chunk<1>(h, c32, n, chunk, graph, kind, L);
+        else if (s.rpl == 2) run_chunk<2>(h, c64, n, chunk, graph, kind, L);
+        else run_chunk<1>(h, c64, n, chunk, graph, kind, L);
         read_scalars(h);
         const Scalars& S = *h->S_host;
         if (S.nonfinite || S.breakdown) break;
@@ -290,13 +320,13 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report
     const double sv = mixed ? 4.0 : 8.0;
     const double cb = s.compact ? 2.0 : 4.0;
     const double spmv_bytes = off * (sv + cb) + n * sv + 4.0 * (s.n_slices + 1) + 2.0 * n * sv;
-    const double it_bytes = mixed ? spmv_bytes + n * sv + (16.0 + 3.0 * sv) * n + (16.0 + 2.0 * sv) * n
+    const double it_bytes = mixed ? spmv_bytes + n * sv + (16.0 + 5.0 * sv) * n
                                   : spmv_bytes + 3.0 * n * sv + (16.0 + 3.0 * sv) * n;
     const double repl_bytes = off * (8.0 + cb) + 8.0 * n + 4.0 * (s.n_slices + 1) + n * (8.0 + 8.0 + sv);
     rep->bytes = iters * it_bytes + replacements * repl_bytes;
 }
 
-template <class V, class X>
+template <int R, class V, class X>
 static double time_spmv(ocsl_solver* h, SellP& s, X* p, X* q, int reps, const ocsl_options& o) {
     const CgLaunch L = cg_launch(s.n_slices, s.n, o);
     CgArgs<V, X> c{s.view<V>(), p, p, q, nullptr, h->S.p, nullptr, h->red(), L.grid, L.vgrid};
@@ -306,8 +336,8 @@ static double time_spmv(ocsl_solver* h, SellP& s, X* p, X* q, int reps, const oc
     init.rr_blocks = 1;
     OCSL_HIP(hipMemcpyAsync(h->S.p, &init, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     auto launch = [&] {
-        if constexpr (sizeof(X) == 4) k_cgm_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
-        else k_cg_spmv<V, X><<<L.grid, L.block, 0, h->stream>>>(c);
+        if constexpr (sizeof(X) == 4) k_cgm_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(c);
+        else k_cg_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(c);
     };
     launch();  // warm
     OCSL_HIP(hipEventRecord(h->ev0, h->stream));
@@ -322,15 +352,17 @@ static double time_spmv(ocsl_solver* h, SellP& s, X* p, X* q, int reps, const oc
 
 }  // namespace ocsl
 
-extern "C" ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reorder, int reps,
+extern "C" ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reorder, int rows_per_lane, int reps,
                                          double* avg_ms, double* bytes_per_launch) {
     using namespace ocsl;
     if (!h || !avg_ms || !bytes_per_launch || reps <= 0) return OCSL_ERR_ARG;
     try {
         ocsl_options o;
         ocsl_default_options(&o);
+        o = with_tuned(h, o);
         const bool mixed = precision == OCSL_MIXED;
-        SellP& s = sellp_for(h, reorder != 0);
+        o.rows_per_lane = rows_per_lane;
+        SellP& s = sellp_for(h, reorder != 0, cg_rows_per_lane(h, o));
         ensure_sellp_values(h, s, true, mixed);
         const int n = s.n;
         DBuf<double> p64, q64;
@@ -340,12 +372,14 @@ extern "C" ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reord
             p32.alloc(n);
             q32.alloc(n);
             k_fill<float><<<grid_for(n), kBlock, 0, h->stream>>>(n, 1.0f, p32.p);
-            ms = time_spmv<float, float>(h, s, p32.p, q32.p, reps, o);
+            ms = s.rpl == 2 ? time_spmv<2, float, float>(h, s, p32.p, q32.p, reps, o)
+                            : time_spmv<1, float, float>(h, s, p32.p, q32.p, reps, o);
         } else {
             p64.alloc(n);
             q64.alloc(n);
             k_fill<double><<<grid_for(n), kBlock, 0, h->stream>>>(n, 1.0, p64.p);
-            ms = time_spmv<double, double>(h, s, p64.p, q64.p, reps, o);
+            ms = s.rpl == 2 ? time_spmv<2, double, double>(h, s, p64.p, q64.p, reps, o)
+                            : time_spmv<1, double, double>(h, s, p64.p, q64.p, reps, o);
         }
         const double sv = mixed ? 4.0 : 8.0;
         *avg_ms = ms;
@@ -360,4 +394,93 @@ extern "C" ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reord
         set_last_error(e.what);
         return e.status;
     }
+}
+
+// Autotuner of the tolerance-CG launch: every candidate (rows per lane x
+// threads per workgroup x SpMV workgroups x vector workgroups) runs
+// `iterations` iterations of the real solve on the handle's system, timed
+// with HIP events; the fastest per iteration is kept in the handle and used by
+// later solves whose options leave these fields at 0.  The caller's x is
+// restored afterwards.
+extern "C" ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations, ocsl_options* best,
+                                    double* best_us_per_iter, ocsl_tune_entry* table, int table_cap, int* table_len) {
+    using namespace ocsl;
+    if (!h || iterations < 4 || (precision != OCSL_FP64 && precision != OCSL_MIXED)) return OCSL_ERR_ARG;
+    try {
+        OCSL_REQUIRE(h->have_vals && h->have_b, OCSL_ERR_STATE, "values or rhs not set");
+        const int n = h->host.n;
+        DBuf<double> saved;
+        const bool had_x = h->x_own.p != nullptr;
+        if (had_x) {
+            saved.alloc(n);
+            OCSL_HIP(hipMemcpyAsync(saved.p, h->x_own.p, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+        } else {
+            h->x_own.alloc(n);
+        }
+        const int rpls[] = {1, 2};
+        const int blocks[] = {256, 512, 1024};
+        const int grids[] = {256, 512, 1024, 2048};
+        const int vgrids[] = {0, 1024, 2048};
+        ocsl_options bo{};
+        double best_us = 1e300;
+        int count = 0;
+        h->tuned = ocsl_solver_tuned{};
+        for (int rpl : rpls)
+            for (int blk : blocks)
+                for (int gr : grids)
+                    for (int vg : vgrids) {
+                        if (vg != 0 && vg <= gr) continue;
+                        ocsl_options o;
+                        ocsl_default_options(&o);
+                        o.tol = 0.0;  // run every iteration
+                        o.max_iter = iterations;
+                        o.check_every = std::min(iterations, 32);
+                        o.rows_per_lane = rpl;
+                        o.block_size = blk;
+                        o.grid_blocks = gr;
+                        o.vec_blocks = vg;
+                        OCSL_HIP(hipMemsetAsync(h->x_own.p, 0, sizeof(double) * n, h->stream));
+                        ocsl_report r{};
+                        cg_tolerance(h, precision == OCSL_MIXED, o, &r);  // warm: layout, graphs
+                        OCSL_HIP(hipMemsetAsync(h->x_own.p, 0, sizeof(double) * n, h->stream));
+                        cg_tolerance(h, precision == OCSL_MIXED, o, &r);
+                        const double us = r.iterations > 0 ? r.solve_ms * 1e3 / r.iterations : 1e300;
+                        if (table && count < table_cap)
+                            table[count] = ocsl_tune_entry{rpl, blk, gr, vg, us};
+                        ++count;
+                        if (us < best_us) {
+                            best_us = us;
+                            bo = o;
+                        }
+                    }
+        h->tuned = ocsl_solver_tuned{bo.rows_per_lane, bo.block_size, bo.grid_blocks, bo.vec_blocks};
+        drop_graphs(h);
+        if (had_x)
+            OCSL_HIP(hipMemcpyAsync(h->x_own.p, saved.p, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+        sync(h);
+        if (best) {
+            ocsl_default_options(best);
+            best->rows_per_lane = bo.rows_per_lane;
+            best->block_size = bo.block_size;
+            best->grid_blocks = bo.grid_blocks;
+            best->vec_blocks = bo.vec_blocks;
+        }
+        if (best_us_per_iter) *best_us_per_iter = best_us;
+        if (table_len) *table_len = count;
+        return OCSL_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what);
+        return e.status;
+    }
+}
+
+// Sets (or, with all zeros, clears) the handle's tuned launch.
+extern "C" ocsl_status ocsl_set_tuned_cg(ocsl_handle h, const ocsl_options* o) {
+    if (!h || !o) return OCSL_ERR_ARG;
+    if (o->rows_per_lane < 0 || o->rows_per_lane > 2 || o->block_size < 0 || o->block_size > 1024 ||
+        o->grid_blocks < 0 || o->vec_blocks < 0)
+        return OCSL_ERR_ARG;
+    h->tuned = ocsl_solver_tuned{o->rows_per_lane, o->block_size, o->grid_blocks, o->vec_blocks};
+    ocsl::drop_graphs(h);
+    return OCSL_OK;
 }

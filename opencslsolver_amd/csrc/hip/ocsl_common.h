@@ -284,52 +284,114 @@ struct Pair2<double> {
     using T = double2;
 };
 
-// Row of A x over SELL-P.  kDiff: difference form, diag = row sum s_i:
-// s_i x_i + sum_off a_ij (x_j - x_i); otherwise a_ii x_i + sum_off a_ij x_j.
-// Arithmetic in FP64 whatever the storage.
+// K consecutive elements of T moved as one aligned vector access.
+template <class T, int K>
+struct alignas(sizeof(T) * K) Pack {
+    T v[K];
+};
+template <class T, int K>
+__device__ __forceinline__ Pack<T, K> ldp(const T* __restrict__ p) {
+    return *reinterpret_cast<const Pack<T, K>*>(p);
+}
+
+// R consecutive rows of a length-n vector as doubles (0 past the end); one
+// vector load unless the group straddles n.
+template <int R, class T>
+__device__ __forceinline__ void load_rows(const T* __restrict__ a, int row0, int n, double (&out)[R]) {
+    if (row0 + R <= n) {
+        const Pack<T, R> v = ldp<T, R>(a + row0);
+#pragma unroll
+        for (int j = 0; j < R; ++j) out[j] = (double)v.v[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) out[j] = row0 + j < n ? (double)a[row0 + j] : 0.0;
+    }
+}
+template <int R, class T>
+__device__ __forceinline__ void store_rows(T* __restrict__ a, int row0, int n, const T (&v)[R]) {
+    if (row0 + R <= n) {
+        Pack<T, R> pk;
+#pragma unroll
+        for (int j = 0; j < R; ++j) pk.v[j] = v[j];
+        *reinterpret_cast<Pack<T, R>*>(a + row0) = pk;
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+            if (row0 + j < n) a[row0 + j] = v[j];
+    }
+}
+
+// R rows of A x over SELL-P with R rows per lane: slices of 64 R rows, lane
+// l owning rows s*64R + l*R + j.  Pair t of row j of lane l is at element
+// base + 128 R t + 2 R l + 2 j, so a lane fetches 2R values (and R packed
+// column words) per access; R = 1 is the one-row-per-lane layout.  kDiff:
+// difference form, diag = row sum s_i: s_i x_i + sum_off a_ij (x_j - x_i);
+// otherwise a_ii x_i + sum_off a_ij x_j.  Arithmetic in FP64 whatever the
+// storage; per row the terms are added in the same order for every R.
+template <class V, class X, bool kDiff, int R>
+__device__ __forceinline__ void sellp_rows(const SellPView<V>& A, int s, int lane, int row0,
+                                           const X* __restrict__ x, const double (&xi)[R], double (&acc)[R]) {
+    const int base = A.slice_ptr[s];
+    const int np = (A.slice_ptr[s + 1] - base) / (128 * R);  // pairs per row
+    double dg[R];
+    load_rows<R>(A.diag, row0, A.n, dg);
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc[j] = dg[j] * xi[j];
+    constexpr int kPairs = R == 1 ? 4 : 2;
+    for (int t0 = 0; t0 < np; t0 += kPairs) {
+        int c[kPairs][2 * R];
+        V v[kPairs][2 * R];
+#pragma unroll
+        for (int jj = 0; jj < kPairs; ++jj) {
+            const int t = t0 + jj;
+            if (t < np) {
+                if (A.colw) {
+                    const Pack<unsigned, R> w = ldp<unsigned, R>(A.colw + (base >> 1) + t * 64 * R + lane * R);
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        c[jj][2 * j] = row0 + j + (int)(short)(w.v[j] & 0xffffu);
+                        c[jj][2 * j + 1] = row0 + j + (int)(short)(w.v[j] >> 16);
+                    }
+                } else {
+                    const Pack<int, 2 * R> cc = ldp<int, 2 * R>(A.col + base + t * 128 * R + lane * 2 * R);
+#pragma unroll
+                    for (int e = 0; e < 2 * R; ++e) c[jj][e] = cc.v[e];
+                }
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < kPairs; ++jj) {
+            const int t = t0 + jj;
+            if (t < np) {
+                const Pack<V, 2 * R> vv = ldp<V, 2 * R>(A.val + base + t * 128 * R + lane * 2 * R);
+#pragma unroll
+                for (int e = 0; e < 2 * R; ++e) v[jj][e] = vv.v[e];
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < kPairs; ++jj)
+            if (t0 + jj < np) {
+#pragma unroll
+                for (int j = 0; j < R; ++j)
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const double a = (double)v[jj][2 * j + e];
+                        const double xj = (double)x[c[jj][2 * j + e]];
+                        if constexpr (kDiff) acc[j] += a * (xj - xi[j]);
+                        else acc[j] += a * xj;
+                    }
+            }
+    }
+}
+
+// One row per lane (R = 1).
 template <class V, class X, bool kDiff>
 __device__ __forceinline__ double sellp_row(const SellPView<V>& A, int s, int lane, int row,
                                             const X* __restrict__ x, double xi) {
-    using V2 = typename Pair2<V>::T;
-    const int base = A.slice_ptr[s];
-    const int np = (A.slice_ptr[s + 1] - base) >> 7;  // pairs per lane
-    const V2* vp = reinterpret_cast<const V2*>(A.val + base) + lane;
-    double acc = row < A.n ? (double)A.diag[row] * xi : 0.0;
-    constexpr int kPairs = 4;
-    for (int t0 = 0; t0 < np; t0 += kPairs) {
-        int2 c[kPairs];
-        V2 v[kPairs];
-        if (A.colw) {
-            const unsigned* wp = A.colw + (base >> 1) + lane;
-#pragma unroll
-            for (int j = 0; j < kPairs; ++j)
-                if (t0 + j < np) {
-                    const unsigned wd = wp[(t0 + j) * kWave];
-                    c[j].x = row + (int)(short)(wd & 0xffffu);
-                    c[j].y = row + (int)(short)(wd >> 16);
-                }
-        } else {
-            const int2* cp = reinterpret_cast<const int2*>(A.col + base) + lane;
-#pragma unroll
-            for (int j = 0; j < kPairs; ++j)
-                if (t0 + j < np) c[j] = cp[(t0 + j) * kWave];
-        }
-#pragma unroll
-        for (int j = 0; j < kPairs; ++j)
-            if (t0 + j < np) v[j] = vp[(t0 + j) * kWave];
-#pragma unroll
-        for (int j = 0; j < kPairs; ++j)
-            if (t0 + j < np) {
-                if constexpr (kDiff) {
-                    acc += (double)v[j].x * ((double)x[c[j].x] - xi);
-                    acc += (double)v[j].y * ((double)x[c[j].y] - xi);
-                } else {
-                    acc += (double)v[j].x * (double)x[c[j].x];
-                    acc += (double)v[j].y * (double)x[c[j].y];
-                }
-            }
-    }
-    return acc;
+    const double xr[1] = {xi};
+    double acc[1];
+    sellp_rows<V, X, kDiff, 1>(A, s, lane, row, x, xr, acc);
+    return acc[0];
 }
 
 // NR interleaved vectors (x[row * NR + k]) against one SELL row: one pass

@@ -38,9 +38,11 @@ inline SellView<float> Sell::view<float>() const {
     return {n, n_slices, slice_ptr.p, col.p, val32.p, max_w};
 }
 
-// SELL-P copy of the matrix (tolerance-mode CG), RCM or input row order.
+// SELL-P copy of the matrix (tolerance-mode CG), RCM or input row order,
+// rpl rows per lane (slices of 64 rpl rows).
 struct SellP {
-    int n = 0, n_slices = 0;
+    int n = 0, n_slices = 0, rpl = 1;
+    uint64_t epoch64 = ~0ull, epoch32 = ~0ull;  // value epochs of val64 / val32
     int64_t total = 0;                      // padded off-diagonal entries
     DBuf<int> slice_ptr, col, src, diag_src, perm;  // perm: new -> old (empty for identity)
     DBuf<unsigned> colw;                    // packed 16-bit column offsets (compact)
@@ -83,6 +85,11 @@ struct Work {
 
 }  // namespace ocsl
 
+// Tuned tolerance-CG launch of a handle (0: not tuned).
+struct ocsl_solver_tuned {
+    int rows_per_lane = 0, block_size = 0, grid_blocks = 0, vec_blocks = 0;
+};
+
 struct ocsl_solver {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -103,9 +110,8 @@ struct ocsl_solver {
     uint64_t val_epoch = 0;          // bumped on every ocsl_set_values
 
     ocsl::Sell sell_id, sell_rcm;
-    ocsl::SellP sellp_id, sellp_rcm;
-    uint64_t sellp_id_epoch64 = ~0ull, sellp_id_epoch32 = ~0ull;
-    uint64_t sellp_rcm_epoch64 = ~0ull, sellp_rcm_epoch32 = ~0ull;
+    ocsl::SellP sellp[2][2];  // [reorder][rows per lane - 1]
+    ocsl_solver_tuned tuned;   // launch picked by ocsl_tune_cg (zeros: none)
     uint64_t sell_id_epoch64 = ~0ull, sell_id_epoch32 = ~0ull;
     uint64_t sell_rcm_epoch64 = ~0ull, sell_rcm_epoch32 = ~0ull;
 
@@ -144,7 +150,7 @@ namespace ocsl {
 void analyse_structure(int64_t n, const uint64_t* rp, const uint64_t* col, HostCSR& out);
 std::vector<int> rcm_order(const HostCSR& a);
 void build_sell(const HostCSR& a, const std::vector<int>* order, Sell& s, hipStream_t st);
-void build_sellp(const HostCSR& a, const std::vector<int>* order, SellP& s, hipStream_t st);
+void build_sellp(const HostCSR& a, const std::vector<int>* order, int rpl, SellP& s, hipStream_t st);
 void build_levels(const HostCSR& a, bool lower, Levels& L, hipStream_t st);
 void build_ilu_tables(const HostCSR& a, IluTables& t, hipStream_t st);
 
@@ -152,7 +158,7 @@ void build_ilu_tables(const HostCSR& a, IluTables& t, hipStream_t st);
 void ensure_csr_values(ocsl_solver* h, bool need32);
 void ensure_sell_values(ocsl_solver* h, Sell& s, bool need64, bool need32);
 Sell& sell_for(ocsl_solver* h, bool reorder);
-SellP& sellp_for(ocsl_solver* h, bool reorder);
+SellP& sellp_for(ocsl_solver* h, bool reorder, int rpl);
 void ensure_sellp_values(ocsl_solver* h, SellP& s, bool need64, bool need32);
 void sync(ocsl_solver* h);
 void drop_graphs(ocsl_solver* h);

@@ -236,7 +236,7 @@ __device__ __forceinline__ void dstore(double* __restrict__ p, int v, const doub
 
 // r = b - A x (FP64), stored as r64 or r32; p = r on the first call; partials
 // of r.r (kSlotRR) and b.b (kSlotBB).
-template <bool kMixed>
+template <bool kMixed, int R>
 __global__ void __launch_bounds__(kCgMaxBlock) k_cg_residual(SellPView<double> A, const double* __restrict__ b,
                                                             const double* __restrict__ x,
                                                             double* __restrict__ r64, float* __restrict__ r32,
@@ -249,21 +249,29 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_residual(SellPView<double> A
     const int s0 = lb * spb, s1 = min(A.n_slices, s0 + spb);
     double acc[2] = {0.0, 0.0};
     for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
-        const int row = s * kWave + lane;
-        const double xi = row < A.n ? x[row] : 0.0;
-        const double ax = sellp_row<double, double, false>(A, s, lane, row, x, xi);
-        if (row < A.n) {
-            const double bv = b[row];
-            const double rv = bv - ax;
-            acc[0] += rv * rv;
-            acc[1] += bv * bv;
-            if (kMixed) {
-                r32[row] = (float)rv;
-                if (first) p32[row] = (float)rv;
-            } else {
-                r64[row] = rv;
-                if (first) p64[row] = rv;
+        const int row0 = s * kWave * R + lane * R;
+        double xi[R], ax[R], bv[R];
+        load_rows<R>(x, row0, A.n, xi);
+        sellp_rows<double, double, false, R>(A, s, lane, row0, x, xi, ax);
+        load_rows<R>(b, row0, A.n, bv);
+        double rv[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            rv[j] = bv[j] - ax[j];
+            if (row0 + j < A.n) {
+                acc[0] += rv[j] * rv[j];
+                acc[1] += bv[j] * bv[j];
             }
+        }
+        if constexpr (kMixed) {
+            float rf[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) rf[j] = (float)rv[j];
+            store_rows<R>(r32, row0, A.n, rf);
+            if (first) store_rows<R>(p32, row0, A.n, rf);
+        } else {
+            store_rows<R>(r64, row0, A.n, rv);
+            if (first) store_rows<R>(p64, row0, A.n, rv);
         }
     }
     block_partials<2>(acc, red, kSlotRR);
@@ -297,7 +305,7 @@ struct FinCgResidual {
 };
 
 // K1: q = A p, partials of p.q.  FP32 storage uses the difference form.
-template <class V, class X>
+template <class V, class X, int R>
 __global__ void __launch_bounds__(kCgMaxBlock) k_cg_spmv(CgArgs<V, X> c) {
     Scalars* S = c.S;
     if (S->done) {
@@ -315,13 +323,17 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_spmv(CgArgs<V, X> c) {
     const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
     double acc[1] = {0.0};
     for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
-        const int row = s * kWave + lane;
-        const X pi = row < c.A.n ? c.p[row] : (X)0;
-        const X qv = (X)sellp_row<V, X, sizeof(V) == 4>(c.A, s, lane, row, c.p, (double)pi);
-        if (row < c.A.n) {
-            c.q[row] = qv;
-            acc[0] += (double)pi * (double)qv;
+        const int row0 = s * kWave * R + lane * R;
+        double pi[R], qd[R];
+        load_rows<R>(c.p, row0, c.A.n, pi);
+        sellp_rows<V, X, sizeof(V) == 4, R>(c.A, s, lane, row0, c.p, pi, qd);
+        X qv[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            qv[j] = (X)qd[j];
+            if (row0 + j < c.A.n) acc[0] += pi[j] * (double)qv[j];
         }
+        store_rows<R>(c.q, row0, c.A.n, qv);
     }
     block_partials<1>(acc, c.red, kSlotPQ);
 }
@@ -488,7 +500,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_xp(CgArgs<V, X> c, in
 //                     q = A p; partials of p.q, r.q, q.q
 //   K2m k_cgm_update  alpha, beta; x += alpha p; r -= alpha q; p = r + beta p;
 //                     partials of r.r
-template <class V, class X>
+template <class V, class X, int R>
 __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c) {
     Scalars* S = c.S;
     if (S->done) {
@@ -515,16 +527,23 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c) {
     const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
     double acc[3] = {0.0, 0.0, 0.0};
     for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
-        const int row = s * kWave + lane;
-        const X pi = row < c.A.n ? c.p[row] : (X)0;
-        const X qv = (X)sellp_row<V, X, sizeof(V) == 4>(c.A, s, lane, row, c.p, (double)pi);
-        if (row < c.A.n) {
-            const double rv = (double)c.r[row];
-            c.q[row] = qv;
-            acc[0] += (double)pi * (double)qv;
-            acc[1] += rv * (double)qv;
-            acc[2] += (double)qv * (double)qv;
+        const int row0 = s * kWave * R + lane * R;
+        double pi[R], qd[R], rv[R];
+        load_rows<R>(c.p, row0, c.A.n, pi);
+        sellp_rows<V, X, sizeof(V) == 4, R>(c.A, s, lane, row0, c.p, pi, qd);
+        load_rows<R>(c.r, row0, c.A.n, rv);
+        X qv[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            qv[j] = (X)qd[j];
+            if (row0 + j < c.A.n) {
+                const double qj = (double)qv[j];
+                acc[0] += pi[j] * qj;
+                acc[1] += rv[j] * qj;
+                acc[2] += qj * qj;
+            }
         }
+        store_rows<R>(c.q, row0, c.A.n, qv);
     }
     const double pq1[1] = {acc[0]};
     block_partials<1>(pq1, c.red, kSlotPQ);
@@ -631,20 +650,23 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
     }
 }
 
-// SELL-P FP32 difference form: diag32 = row sum of the FP64 values.
-[[maybe_unused]] static __global__ void __launch_bounds__(kBlock) k_sellp_rowsum(SellPView<double> A,
-                                                                                 float* __restrict__ diag32) {
+// SELL-P FP32 difference form: diag32 = row sum of the FP64 values, the
+// off-diagonals added in their storage order (the same for every R).
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_sellp_rowsum(SellPView<double> A, float* __restrict__ diag32) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
     const int nwaves = gridDim.x * kWavesPerBlock;
     for (int s = wave; s < A.n_slices; s += nwaves) {
-        const int row = s * kWave + lane;
-        if (row >= A.n) continue;
         const int base = A.slice_ptr[s];
-        const int w = (A.slice_ptr[s + 1] - base) >> 6;
-        double sum = A.diag[row];
-        for (int k = 0; k < w; ++k) sum += A.val[base + (k / 2) * 2 * kWave + 2 * lane + (k & 1)];
-        diag32[row] = (float)sum;
+        const int w = (A.slice_ptr[s + 1] - base) / (kWave * R);
+        for (int j = 0; j < R; ++j) {
+            const int row = s * kWave * R + lane * R + j;
+            if (row >= A.n) continue;
+            double sum = A.diag[row];
+            for (int k = 0; k < w; ++k) sum += A.val[base + (k / 2) * 128 * R + lane * 2 * R + 2 * j + (k & 1)];
+            diag32[row] = (float)sum;
+        }
     }
 }
 

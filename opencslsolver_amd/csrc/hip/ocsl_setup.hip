@@ -178,80 +178,92 @@ void build_sell(const HostCSR& a, const std::vector<int>* order, Sell& s, hipStr
 
 // SELL-P (see SellPView): diagonal apart, off-diagonals of each row in the
 // reference's ascending column order, slices padded to an even width.
-void build_sellp(const HostCSR& a, const std::vector<int>* order, SellP& s, hipStream_t st) {
+void build_sellp(const HostCSR& a, const std::vector<int>* order, int rpl, SellP& s, hipStream_t st) {
     const int n = a.n;
-    const int ns = (n + kWave - 1) / kWave;
+    const int H = kWave * rpl;  // rows per slice
+    const int ns = (n + H - 1) / H;
     std::vector<int> iperm;
     if (order) {
         iperm.assign(n, 0);
         for (int i = 0; i < n; ++i) iperm[(*order)[i]] = i;
     }
     auto old_of = [&](int r) { return order ? (*order)[r] : r; };
+    // element of off-diagonal k of row j of lane l in a slice starting at base
+    auto slot = [&](int64_t base, int l, int j, int k) {
+        return base + (int64_t)(k / 2) * 2 * H + 2 * rpl * l + 2 * j + (k & 1);
+    };
     std::vector<int> sp(ns + 1, 0);
     for (int sl = 0; sl < ns; ++sl) {
         int w = 0;
-        for (int l = 0; l < kWave; ++l) {
-            const int r = sl * kWave + l;
+        for (int q = 0; q < H; ++q) {
+            const int r = sl * H + q;
             if (r >= n) break;
             const int o = old_of(r);
             w = std::max(w, a.rp[o + 1] - a.rp[o] - 1);
         }
         w += w & 1;
-        sp[sl + 1] = sp[sl] + w * kWave;
+        sp[sl + 1] = sp[sl] + w * H;
     }
     const int64_t total = sp[ns];
     std::vector<int> col(total), src(total), dsrc(n);
     for (int sl = 0; sl < ns; ++sl) {
-        const int w = (sp[sl + 1] - sp[sl]) / kWave;
-        for (int l = 0; l < kWave; ++l) {
-            const int r = sl * kWave + l;
-            int k = 0;
-            if (r < n) {
-                const int o = old_of(r);
-                dsrc[r] = a.src[a.diag[o]];
-                for (int p = a.rp[o]; p < a.rp[o + 1]; ++p) {
-                    if (p == a.diag[o]) continue;
-                    const int64_t q = sp[sl] + (int64_t)(k / 2) * 2 * kWave + 2 * l + (k & 1);
-                    col[q] = order ? iperm[a.col[p]] : a.col[p];
-                    src[q] = a.src[p];
-                    ++k;
+        const int w = (sp[sl + 1] - sp[sl]) / H;
+        for (int l = 0; l < kWave; ++l)
+            for (int j = 0; j < rpl; ++j) {
+                const int r = sl * H + l * rpl + j;
+                int k = 0;
+                if (r < n) {
+                    const int o = old_of(r);
+                    dsrc[r] = a.src[a.diag[o]];
+                    for (int p = a.rp[o]; p < a.rp[o + 1]; ++p) {
+                        if (p == a.diag[o]) continue;
+                        const int64_t q = slot(sp[sl], l, j, k);
+                        col[q] = order ? iperm[a.col[p]] : a.col[p];
+                        src[q] = a.src[p];
+                        ++k;
+                    }
+                }
+                // padding: value 0; column = the row, or for rows past the
+                // last one the last row (a valid index within 16 bits)
+                for (; k < w; ++k) {
+                    const int64_t q = slot(sp[sl], l, j, k);
+                    col[q] = r < n ? r : n - 1;
+                    src[q] = -1;
                 }
             }
-            // padding: value 0; column = the row, or for lanes past the last
-            // row the last row (a valid index within 16 bits of the lane)
-            for (; k < w; ++k) {
-                const int64_t q = sp[sl] + (int64_t)(k / 2) * 2 * kWave + 2 * l + (k & 1);
-                col[q] = r < n ? r : n - 1;
-                src[q] = -1;
-            }
-        }
     }
     // 16-bit column offsets when every entry allows it
     bool compact = true;
-    for (int sl = 0; sl < ns && compact; ++sl)
-        for (int64_t q = sp[sl]; q < sp[sl + 1]; ++q) {
-            const int l = (int)((q - sp[sl]) % (2 * kWave)) / 2;
-            const int d = col[q] - (sl * kWave + l);
-            if (d < -32768 || d > 32767) { compact = false; break; }
-        }
+    for (int sl = 0; sl < ns && compact; ++sl) {
+        const int w = (sp[sl + 1] - sp[sl]) / H;
+        for (int l = 0; l < kWave && compact; ++l)
+            for (int j = 0; j < rpl && compact; ++j)
+                for (int k = 0; k < w; ++k) {
+                    const int d = col[slot(sp[sl], l, j, k)] - (sl * H + l * rpl + j);
+                    if (d < -32768 || d > 32767) { compact = false; break; }
+                }
+    }
     s.n = n;
     s.n_slices = ns;
+    s.rpl = rpl;
     s.total = total;
     s.identity = order == nullptr;
     s.compact = compact;
     s.slice_ptr.upload(sp.data(), sp.size(), st);
     if (compact) {
+        // word of pair t of row j of lane l: base / 2 + t H + l rpl + j
         std::vector<unsigned> colw(total / 2);
         for (int sl = 0; sl < ns; ++sl) {
-            const int np = (sp[sl + 1] - sp[sl]) / (2 * kWave);
+            const int np = (sp[sl + 1] - sp[sl]) / (2 * H);
             for (int t = 0; t < np; ++t)
-                for (int l = 0; l < kWave; ++l) {
-                    const int64_t q = sp[sl] + (int64_t)t * 2 * kWave + 2 * l;
-                    const int row = sl * kWave + l;
-                    const unsigned lo = (unsigned)(uint16_t)(int16_t)(col[q] - row);
-                    const unsigned hi = (unsigned)(uint16_t)(int16_t)(col[q + 1] - row);
-                    colw[sp[sl] / 2 + (int64_t)t * kWave + l] = lo | (hi << 16);
-                }
+                for (int l = 0; l < kWave; ++l)
+                    for (int j = 0; j < rpl; ++j) {
+                        const int64_t q = slot(sp[sl], l, j, 2 * t);
+                        const int row = sl * H + l * rpl + j;
+                        const unsigned lo = (unsigned)(uint16_t)(int16_t)(col[q] - row);
+                        const unsigned hi = (unsigned)(uint16_t)(int16_t)(col[q + 1] - row);
+                        colw[sp[sl] / 2 + (int64_t)t * H + l * rpl + j] = lo | (hi << 16);
+                    }
         }
         s.colw.upload(colw.data(), colw.size(), st);
     } else {

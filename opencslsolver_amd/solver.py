@@ -220,12 +220,32 @@ class Solver:
                                             _abi.HOST))
         return out
 
-    def time_cg_spmv(self, precision="fp64", reorder=1, reps=50):
+    def time_cg_spmv(self, precision="fp64", reorder=1, reps=50, rows_per_lane=0):
         """(average ms, algorithmic bytes) of one CG SpMV launch on the handle's stream."""
         ms, by = ctypes.c_double(), ctypes.c_double()
-        _check(self._lib.ocsl_time_cg_spmv(self._h, PRECISIONS[precision], reorder, reps,
+        _check(self._lib.ocsl_time_cg_spmv(self._h, PRECISIONS[precision], reorder, rows_per_lane, reps,
                                            ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
+
+    def tune_cg(self, precision="mixed", iterations=64):
+        """Times every CG launch candidate on this system; keeps the fastest in
+        the handle.  Returns (best options dict, best us/iteration, table)."""
+        best = _abi.Options()
+        us = ctypes.c_double()
+        cap = 256
+        table = (_abi.TuneEntry * cap)()
+        ln = ctypes.c_int()
+        _check(self._lib.ocsl_tune_cg(self._h, PRECISIONS[precision], iterations, ctypes.byref(best),
+                                      ctypes.byref(us), table, cap, ctypes.byref(ln)))
+        keys = ("rows_per_lane", "block_size", "grid_blocks", "vec_blocks")
+        rows = [dict({k: getattr(table[i], k) for k in keys}, us_per_iteration=table[i].us_per_iteration)
+                for i in range(min(ln.value, cap))]
+        return {k: getattr(best, k) for k in keys}, us.value, rows
+
+    def set_tuned_cg(self, rows_per_lane=0, block_size=0, grid_blocks=0, vec_blocks=0):
+        o = default_options(rows_per_lane=rows_per_lane, block_size=block_size, grid_blocks=grid_blocks,
+                            vec_blocks=vec_blocks)
+        _check(self._lib.ocsl_set_tuned_cg(self._h, ctypes.byref(o)))
 
     # -- differentiation ---------------------------------------------------
     def adjoint(self, g, method="cg", precision="fp64", dLdb=None, dLdA=None, **opts):

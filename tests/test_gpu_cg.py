@@ -71,13 +71,13 @@ def test_blas_helpers_match_reference():
         assert s.sum(d["b"]) == pytest.approx(float(np.sum(d["b"])), rel=1e-12, abs=1e-12)
 
 
-@pytest.mark.parametrize("reorder", [1, 0])
-def test_cg_tolerance_fp64_matches_oracle(reorder):
+@pytest.mark.parametrize("reorder,rpl", [(1, 1), (1, 2), (0, 1), (0, 2)])
+def test_cg_tolerance_fp64_matches_oracle(reorder, rpl):
     a, b, _ = gen.poisson3d(16, seed_mesh=1, seed_rhs=2)
     oracle.set_threads(1)
     it_o, x_o, h_o, tr_o = oracle.cg_tol(a, b, tol=1e-8, maxit=5000)
     with _solver(a, b) as s:
-        rep = s.solve("cg", "tolerance", "fp64", tol=1e-8, max_iter=5000, reorder=reorder)
+        rep = s.solve("cg", "tolerance", "fp64", tol=1e-8, max_iter=5000, reorder=reorder, rows_per_lane=rpl)
         x = s.get_x()
         h = s.history()
     assert rep["status"] == 0, rep
@@ -172,3 +172,44 @@ def test_cg_full_size_headline_system():
         assert rel <= 1.05e-8, (rel, rep)
     # the oracle takes 1993 iterations on this system
     assert abs(rep64["iterations"] - 1993) <= 20
+
+
+@pytest.mark.parametrize("n", [5, 7, 13])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cg_rows_per_lane_layouts_agree(n, precision):
+    """One and two rows per lane: the same per-row SpMV sums, so the solves
+    agree to the dot-product rounding; odd row counts end inside a row pair."""
+    a, b, _ = gen.poisson3d(n, seed_mesh=3, seed_rhs=4, sentinels=(n == 7))
+    tol = 1e-10 if precision == "fp64" else 1e-8
+    out = {}
+    with _solver(a, b) as s:
+        for rpl in (1, 2):
+            s.set_x(np.zeros(a.n))
+            rep = s.solve("cg", "tolerance", precision, tol=tol, rows_per_lane=rpl)
+            assert rep["status"] == 0, (rpl, rep)
+            out[rpl] = (rep, s.get_x())
+    # FP64 stops on the recursive residual: the true one may sit a few times
+    # above the tolerance at 1e-10 on these near-singular systems
+    for rpl in (1, 2):
+        assert np.linalg.norm(b - a.matvec(out[rpl][1])) / np.linalg.norm(b) <= 10 * tol
+    if precision == "fp64":
+        assert abs(out[1][0]["iterations"] - out[2][0]["iterations"]) <= 2
+    assert _rel(out[1][1], out[2][1]) <= (1e-6 if precision == "fp64" else 1e-4)
+
+
+def test_cg_autotuner_keeps_x_and_tuned_solves_converge():
+    a, b, _ = gen.poisson3d(24, seed_mesh=1, seed_rhs=2)
+    x0 = gen.uniform(9, a.n)
+    with _solver(a, b, x0) as s:
+        best, us, table = s.tune_cg("mixed", iterations=8)
+        assert np.array_equal(s.get_x(), x0)
+        assert best["rows_per_lane"] in (1, 2) and best["block_size"] in (256, 512, 1024)
+        assert len(table) >= 24 and all(t["us_per_iteration"] > 0 for t in table)
+        assert us == min(t["us_per_iteration"] for t in table)
+        s.set_x(np.zeros(a.n))
+        rep = s.solve("cg", "tolerance", "mixed", tol=1e-8)
+        assert rep["status"] == 0, rep
+        assert np.linalg.norm(b - a.matvec(s.get_x())) / np.linalg.norm(b) <= 1e-8
+        s.set_tuned_cg()  # cleared: defaults again
+        s.set_x(np.zeros(a.n))
+        assert s.solve("cg", "tolerance", "mixed", tol=1e-8)["status"] == 0
