@@ -74,7 +74,7 @@ typedef struct {
     int block_size;      /* threads per workgroup of the CG kernels, 64..1024; 0 = default (1024) */
     int grid_blocks;     /* workgroups of the CG SpMV kernels; 0 = default (512) */
     int vec_blocks;      /* workgroups of the CG vector-update kernels; 0 = as grid_blocks */
-    int rows_per_lane;   /* rows per lane of the CG SELL-P layout, 1 or 2; 0 = tuned, else 2 */
+    int rows_per_lane;   /* rows per lane of the CG SELL-P layout, 1, 2 or 4; 0 = tuned, else 2 */
 } ocsl_options;
 
 typedef struct {
@@ -173,10 +173,10 @@ ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reorder, int row
 
 /* Autotuner (no counterpart in the reference): runs `iterations` iterations
  * of the TOLERANCE-mode CG solve of the handle's system for every launch
- * candidate -- rows per lane {1, 2} x threads per workgroup {256, 512, 1024}
+ * candidate -- rows per lane {1, 2, 4} x threads per workgroup {256, 512, 1024}
  * x SpMV workgroups {256 .. 2048} x vector-update workgroups -- timed with
- * HIP events, and keeps the fastest in the handle for later solves whose
- * options leave those fields at 0.  best (optional) receives it; table
+ * HIP events, and keeps the fastest in the handle for later solves of that
+ * precision whose options leave those fields at 0.  best (optional) receives it; table
  * (optional) every candidate's time per iteration.  x is preserved. */
 typedef struct {
     int rows_per_lane, block_size, grid_blocks, vec_blocks;
@@ -185,9 +185,9 @@ typedef struct {
 ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations, ocsl_options* best,
                          double* best_us_per_iter, ocsl_tune_entry* table, int table_cap, int* table_len);
 
-/* Sets the handle's tuned launch (e.g. from a cache of ocsl_tune_cg
- * results); all-zero fields clear it. */
-ocsl_status ocsl_set_tuned_cg(ocsl_handle h, const ocsl_options* tuned);
+/* Sets the handle's tuned launch of a precision (e.g. from a cache of
+ * ocsl_tune_cg results); all-zero fields clear it. */
+ocsl_status ocsl_set_tuned_cg(ocsl_handle h, int precision, const ocsl_options* tuned);
 
 /* HIP stream of the handle (a hipStream_t), for callers that pass device
  * pointers produced on another stream. */

@@ -102,7 +102,7 @@ SIGNATURES = {
     "ocsl_get_stream": (ctypes.c_void_p, [c_handle]),
     "ocsl_tune_cg": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Options), _f64p,
                                     ctypes.POINTER(TuneEntry), ctypes.c_int, _i32p]),
-    "ocsl_set_tuned_cg": (ctypes.c_int, [c_handle, ctypes.POINTER(Options)]),
+    "ocsl_set_tuned_cg": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.POINTER(Options)]),
     "ocsl_adjoint": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.POINTER(Options), ctypes.POINTER(Report)]),

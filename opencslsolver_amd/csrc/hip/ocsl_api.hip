@@ -83,8 +83,8 @@ void ensure_sell_values(ocsl_solver* h, Sell& s, bool need64, bool need32) {
 }
 
 SellP& sellp_for(ocsl_solver* h, bool reorder, int rpl) {
-    OCSL_REQUIRE(rpl == 1 || rpl == 2, OCSL_ERR_ARG, "rows_per_lane must be 0, 1 or 2");
-    SellP& s = h->sellp[reorder ? 1 : 0][rpl - 1];
+    OCSL_REQUIRE(rpl == 1 || rpl == 2 || rpl == 4, OCSL_ERR_ARG, "rows_per_lane must be 0, 1, 2 or 4");
+    SellP& s = h->sellp[reorder ? 1 : 0][rpl == 4 ? 2 : rpl - 1];
     if (s.n == 0) {
         drop_graphs(h);
         if (reorder) {
@@ -126,8 +126,9 @@ void ensure_sellp_values(ocsl_solver* h, SellP& s, bool need64, bool need32) {
     if (want32) {
         k_gather_vals<float><<<g, kBlock, 0, h->stream>>>(s.total, s.src.p, h->val_in, s.val32.p);
         const int gr = grid_for((int64_t)s.n_slices * kWave);
-        if (s.rpl == 2) k_sellp_rowsum<2><<<gr, kBlock, 0, h->stream>>>(s.view<double>(), s.diag32.p);
-        else k_sellp_rowsum<1><<<gr, kBlock, 0, h->stream>>>(s.view<double>(), s.diag32.p);
+        with_rpl(s.rpl, [&](auto RC) {
+            k_sellp_rowsum<decltype(RC)::value><<<gr, kBlock, 0, h->stream>>>(s.view<double>(), s.diag32.p);
+        });
         e32 = h->val_epoch;
     }
     OCSL_HIP(hipGetLastError());

@@ -36,12 +36,10 @@ static double true_relres_p(ocsl_solver* h, const SellP& s, const double* b, con
     const int n = s.n;
     double* r = work_buf(h, h->work.d[11], n);
     const int g = std::max(1, std::min(kMaxRedBlocks, (s.n_slices + 15) / 16 * 1));
-    if (s.rpl == 2)
-        k_cg_residual<false, 2><<<g, 1024, 0, h->stream>>>(s.view<double>(), b, x, r, nullptr, nullptr, nullptr,
-                                                           h->red(), 0);
-    else
-        k_cg_residual<false, 1><<<g, 1024, 0, h->stream>>>(s.view<double>(), b, x, r, nullptr, nullptr, nullptr,
-                                                           h->red(), 0);
+    with_rpl(s.rpl, [&](auto RC) {
+        k_cg_residual<false, decltype(RC)::value><<<g, 1024, 0, h->stream>>>(s.view<double>(), b, x, r, nullptr,
+                                                                            nullptr, nullptr, h->red(), 0);
+    });
     k_finalize<2, FinStore<2>><<<1, kBlock, 0, h->stream>>>(h->red(), g, FinStore<2>{h->dscal.p}, kSlotRR);
     OCSL_HIP(hipGetLastError());
     OCSL_HIP(hipMemcpyAsync(h->dscal_host, h->dscal.p, 2 * sizeof(double), hipMemcpyDeviceToHost,
@@ -144,11 +142,16 @@ CgLaunch cg_launch(int n_slices, int n, const ocsl_options& o) {
 
 // FP64: the textbook three-kernel iteration; MIXED: the two-kernel iteration
 // with merged reductions.
+// Iteration k of a chunk; MIXED iterations run in (even, odd) pairs with the
+// direction buffers swapped in the odd one.
 template <int R, class V, class X>
-void launch_iter(ocsl_solver* h, const CgArgs<V, X>& c, int n, CgLaunch L) {
+void launch_iter(ocsl_solver* h, const CgArgs<V, X>& c, int n, CgLaunch L, int k) {
     if constexpr (sizeof(X) == 4) {
-        k_cgm_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(c);
-        k_cgm_update<V, X><<<L.vgrid, L.block, 0, h->stream>>>(c, n);
+        CgArgs<V, X> ck = c;
+        if (k & 1) std::swap(ck.p, ck.p_alt);
+        k_cgm_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(ck, (k & 1) ? 0 : 1);
+        if (k & 1) k_cgm_update<V, X, 1><<<L.vgrid, L.block, 0, h->stream>>>(ck, n);
+        else k_cgm_update<V, X, 0><<<L.vgrid, L.block, 0, h->stream>>>(ck, n);
     } else {
         k_cg_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(c);
         k_cg_update_r<V, X><<<L.vgrid, L.block, 0, h->stream>>>(c, n);
@@ -163,17 +166,16 @@ template <int R, class V, class X>
 void run_chunk(ocsl_solver* h, const CgArgs<V, X>& c, int n, int chunk, bool use_graph, int kind,
                CgLaunch L) {
     if (!use_graph) {
-        for (int k = 0; k < chunk; ++k) launch_iter<R>(h, c, n, L);
+        for (int k = 0; k < chunk; ++k) launch_iter<R>(h, c, n, L, k);
         OCSL_HIP(hipGetLastError());
         return;
     }
-    auto key = std::make_pair(
-        kind + 4 * L.grid + (1 << 13) * L.vgrid + (1 << 25) * (L.block / kWave) + (1 << 30) * (R - 1), chunk);
+    auto key = std::make_pair(kind + 4 * L.grid + (1 << 13) * L.vgrid + (1 << 25) * (L.block / kWave), chunk * 8 + R);
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
         hipGraph_t gr;
         OCSL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < chunk; ++k) launch_iter<R>(h, c, n, L);
+        for (int k = 0; k < chunk; ++k) launch_iter<R>(h, c, n, L, k);
         OCSL_HIP(hipStreamEndCapture(h->stream, &gr));
         hipGraphExec_t ex;
         OCSL_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
@@ -192,23 +194,25 @@ void read_scalars(ocsl_solver* h) {
 
 // Rows per lane of the SELL-P layout: the option, else the handle's tuned
 // choice, else 2 (two rows per lane: 16-byte value and 8-byte vector accesses).
-int cg_rows_per_lane(const ocsl_solver* h, const ocsl_options& o) {
+int cg_rows_per_lane(const ocsl_solver* h, bool mixed, const ocsl_options& o) {
     if (o.rows_per_lane > 0) return o.rows_per_lane;
-    return h->tuned.rows_per_lane > 0 ? h->tuned.rows_per_lane : 2;
+    const ocsl_solver_tuned& t = h->tuned[mixed ? 1 : 0];
+    return t.rows_per_lane > 0 ? t.rows_per_lane : 2;
 }
 
-// Launch options left at 0 take the handle's tuned values.
-ocsl_options with_tuned(const ocsl_solver* h, const ocsl_options& o) {
+// Launch options left at 0 take the handle's tuned values for the precision.
+ocsl_options with_tuned(const ocsl_solver* h, bool mixed, const ocsl_options& o) {
+    const ocsl_solver_tuned& tu = h->tuned[mixed ? 1 : 0];
     ocsl_options t = o;
-    if (t.block_size <= 0) t.block_size = h->tuned.block_size;
-    if (t.grid_blocks <= 0) t.grid_blocks = h->tuned.grid_blocks;
-    if (t.vec_blocks <= 0) t.vec_blocks = h->tuned.vec_blocks;
+    if (t.block_size <= 0) t.block_size = tu.block_size;
+    if (t.grid_blocks <= 0) t.grid_blocks = tu.grid_blocks;
+    if (t.vec_blocks <= 0) t.vec_blocks = tu.vec_blocks;
     return t;
 }
 
 void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_report* rep) {
-    const ocsl_options o = with_tuned(h, o_in);
-    SellP& s = sellp_for(h, o.reorder != 0, cg_rows_per_lane(h, o));
+    const ocsl_options o = with_tuned(h, mixed, o_in);
+    SellP& s = sellp_for(h, o.reorder != 0, cg_rows_per_lane(h, mixed, o));
     ensure_sellp_values(h, s, true, mixed);
     const int n = s.n;
     const int* perm = s.identity ? nullptr : s.perm.p;
@@ -219,10 +223,10 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     CgArgs<double, double> c64{};
     CgArgs<float, float> c32{};
     if (mixed) {
-        c32 = {s.view<float>(), work_buf(h, w.f[0], n), work_buf(h, w.f[1], n), work_buf(h, w.f[2], n), x,
-               h->S.p, nullptr, h->red()};
+        c32 = {s.view<float>(), work_buf(h, w.f[0], n), work_buf(h, w.f[1], n), work_buf(h, w.f[2], n),
+               work_buf(h, w.f[3], n), x, h->S.p, nullptr, h->red()};
     } else {
-        c64 = {s.view<double>(), work_buf(h, w.d[7], n), work_buf(h, w.d[8], n), work_buf(h, w.d[9], n), x,
+        c64 = {s.view<double>(), work_buf(h, w.d[7], n), work_buf(h, w.d[8], n), work_buf(h, w.d[9], n), nullptr, x,
                h->S.p, nullptr, h->red()};
     }
     work_buf(h, h->hist, (size_t)o.max_iter + 2);
@@ -248,18 +252,15 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     c32.g_vec = c64.g_vec = L.vgrid;
     auto residual = [&](int first) {
         const SellPView<double> A = s.view<double>();
-        if (mixed && s.rpl == 2)
-            k_cg_residual<true, 2><<<L.grid, L.block, 0, h->stream>>>(A, b, x, nullptr, c32.r, nullptr, c32.p,
-                                                                     h->red(), first);
-        else if (mixed)
-            k_cg_residual<true, 1><<<L.grid, L.block, 0, h->stream>>>(A, b, x, nullptr, c32.r, nullptr, c32.p,
-                                                                     h->red(), first);
-        else if (s.rpl == 2)
-            k_cg_residual<false, 2><<<L.grid, L.block, 0, h->stream>>>(A, b, x, c64.r, nullptr, c64.p, nullptr,
-                                                                      h->red(), first);
-        else
-            k_cg_residual<false, 1><<<L.grid, L.block, 0, h->stream>>>(A, b, x, c64.r, nullptr, c64.p, nullptr,
-                                                                      h->red(), first);
+        with_rpl(s.rpl, [&](auto RC) {
+            constexpr int R = decltype(RC)::value;
+            if (mixed)
+                k_cg_residual<true, R><<<L.grid, L.block, 0, h->stream>>>(A, b, x, nullptr, c32.r, nullptr, c32.p,
+                                                                         h->red(), first);
+            else
+                k_cg_residual<false, R><<<L.grid, L.block, 0, h->stream>>>(A, b, x, c64.r, nullptr, c64.p, nullptr,
+                                                                          h->red(), first);
+        });
         k_finalize<2, FinCgResidual><<<1, kBlock, 0, h->stream>>>(
             h->red(), L.grid, FinCgResidual{h->S.p, h->hist.p, first, L.grid}, kSlotRR);
         OCSL_HIP(hipGetLastError());
@@ -268,18 +269,27 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     read_scalars(h);
     int replacements = 0;
     const int kind = (mixed ? 1 : 0) + (s.identity ? 0 : 2);
+    // MIXED chunks hold whole (even, odd) pairs
+    const int every = mixed ? (o.check_every + 1) / 2 * 2 : o.check_every;
     while (!h->S_host->done) {
         const int left = o.max_iter - h->S_host->iter;
-        const bool graph = o.use_graph && left >= o.check_every;
-        const int chunk = graph ? o.check_every : std::max(1, std::min(o.check_every, left));
-        if (mixed && s.rpl == 2) run_chunk<2>(h, c32, n, chunk, graph, kind, L);
-        else if (mixed) run_chunk<1>(h, c32, n, chunk, graph, kind, L);
-        else if (s.rpl == 2) run_chunk<2>(h, c64, n, chunk, graph, kind, L);
-        else run_chunk<1>(h, c64, n, chunk, graph, kind, L);
+        const bool graph = o.use_graph && left >= every;
+        int chunk = graph ? every : std::max(1, std::min(every, left));
+        if (mixed) chunk = (chunk + 1) / 2 * 2;  // iterations past max_iter stop on their own
+        with_rpl(s.rpl, [&](auto RC) {
+            constexpr int R = decltype(RC)::value;
+            if (mixed) run_chunk<R>(h, c32, n, chunk, graph, kind, L);
+            else run_chunk<R>(h, c64, n, chunk, graph, kind, L);
+        });
         read_scalars(h);
         const Scalars& S = *h->S_host;
         if (S.nonfinite || S.breakdown) break;
         if (mixed && (S.need_update || S.done)) {
+            if (S.iter & 1) {
+                // a pair was cut after its even update: x already holds that
+                // step, the current direction sits in the second buffer
+                OCSL_HIP(hipMemcpyAsync(c32.p, c32.p_alt, sizeof(float) * n, hipMemcpyDeviceToDevice, h->stream));
+            }
             residual(0);
             ++replacements;
             read_scalars(h);
@@ -315,12 +325,14 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     // columns -- 2 or 4 bytes each --, the diagonal, slice pointers, p
     // gathered, q written) and the vector updates.  FP64 (three kernels):
     // r, q read, r written; x read/written, p, r read, p written.  MIXED (two
-    // kernels): r read in the SpMV; x (FP64), r, q, p read, x, r, p written.
+    // kernels): r read in the SpMV, then the paired update below.
     const double off = (double)h->host.col.size() - n;
     const double sv = mixed ? 4.0 : 8.0;
     const double cb = s.compact ? 2.0 : 4.0;
     const double spmv_bytes = off * (sv + cb) + n * sv + 4.0 * (s.n_slices + 1) + 2.0 * n * sv;
-    const double it_bytes = mixed ? spmv_bytes + n * sv + (16.0 + 5.0 * sv) * n
+    // MIXED update, per pair: even r, q, p read, r, p' written (5 sv); odd
+    // the same plus x (FP64) read and written and p_prev read (16 + 6 sv)
+    const double it_bytes = mixed ? spmv_bytes + n * sv + (8.0 + 5.5 * sv) * n
                                   : spmv_bytes + 3.0 * n * sv + (16.0 + 3.0 * sv) * n;
     const double repl_bytes = off * (8.0 + cb) + 8.0 * n + 4.0 * (s.n_slices + 1) + n * (8.0 + 8.0 + sv);
     rep->bytes = iters * it_bytes + replacements * repl_bytes;
@@ -329,14 +341,14 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
 template <int R, class V, class X>
 static double time_spmv(ocsl_solver* h, SellP& s, X* p, X* q, int reps, const ocsl_options& o) {
     const CgLaunch L = cg_launch(s.n_slices, s.n, o);
-    CgArgs<V, X> c{s.view<V>(), p, p, q, nullptr, h->S.p, nullptr, h->red(), L.grid, L.vgrid};
+    CgArgs<V, X> c{s.view<V>(), p, p, q, nullptr, nullptr, h->S.p, nullptr, h->red(), L.grid, L.vgrid};
     Scalars init{};
     init.max_iter = 1 << 30;
     init.tol2 = -1.0;
     init.rr_blocks = 1;
     OCSL_HIP(hipMemcpyAsync(h->S.p, &init, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     auto launch = [&] {
-        if constexpr (sizeof(X) == 4) k_cgm_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(c);
+        if constexpr (sizeof(X) == 4) k_cgm_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(c, 0);
         else k_cg_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(c);
     };
     launch();  // warm
@@ -359,27 +371,29 @@ extern "C" ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reord
     try {
         ocsl_options o;
         ocsl_default_options(&o);
-        o = with_tuned(h, o);
         const bool mixed = precision == OCSL_MIXED;
+        o = with_tuned(h, mixed, o);
         o.rows_per_lane = rows_per_lane;
-        SellP& s = sellp_for(h, reorder != 0, cg_rows_per_lane(h, o));
+        SellP& s = sellp_for(h, reorder != 0, cg_rows_per_lane(h, mixed, o));
         ensure_sellp_values(h, s, true, mixed);
         const int n = s.n;
         DBuf<double> p64, q64;
         DBuf<float> p32, q32;
-        double ms;
+        double ms = 0.0;
         if (mixed) {
             p32.alloc(n);
             q32.alloc(n);
             k_fill<float><<<grid_for(n), kBlock, 0, h->stream>>>(n, 1.0f, p32.p);
-            ms = s.rpl == 2 ? time_spmv<2, float, float>(h, s, p32.p, q32.p, reps, o)
-                            : time_spmv<1, float, float>(h, s, p32.p, q32.p, reps, o);
+            with_rpl(s.rpl, [&](auto RC) {
+                ms = time_spmv<decltype(RC)::value, float, float>(h, s, p32.p, q32.p, reps, o);
+            });
         } else {
             p64.alloc(n);
             q64.alloc(n);
             k_fill<double><<<grid_for(n), kBlock, 0, h->stream>>>(n, 1.0, p64.p);
-            ms = s.rpl == 2 ? time_spmv<2, double, double>(h, s, p64.p, q64.p, reps, o)
-                            : time_spmv<1, double, double>(h, s, p64.p, q64.p, reps, o);
+            with_rpl(s.rpl, [&](auto RC) {
+                ms = time_spmv<decltype(RC)::value, double, double>(h, s, p64.p, q64.p, reps, o);
+            });
         }
         const double sv = mixed ? 4.0 : 8.0;
         *avg_ms = ms;
@@ -417,19 +431,20 @@ extern "C" ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations
         } else {
             h->x_own.alloc(n);
         }
-        const int rpls[] = {1, 2};
+        const int rpls[] = {1, 2, 4};
         const int blocks[] = {256, 512, 1024};
         const int grids[] = {256, 512, 1024, 2048};
-        const int vgrids[] = {0, 1024, 2048};
+        const int vgrids[] = {0, 128, 256, 1024, 2048};
         ocsl_options bo{};
         double best_us = 1e300;
         int count = 0;
-        h->tuned = ocsl_solver_tuned{};
+        const int pi = precision == OCSL_MIXED ? 1 : 0;
+        h->tuned[pi] = ocsl_solver_tuned{};
         for (int rpl : rpls)
             for (int blk : blocks)
                 for (int gr : grids)
                     for (int vg : vgrids) {
-                        if (vg != 0 && vg <= gr) continue;
+                        if (vg != 0 && (vg == gr || (vg > 256 && vg < gr))) continue;
                         ocsl_options o;
                         ocsl_default_options(&o);
                         o.tol = 0.0;  // run every iteration
@@ -453,7 +468,7 @@ extern "C" ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations
                             bo = o;
                         }
                     }
-        h->tuned = ocsl_solver_tuned{bo.rows_per_lane, bo.block_size, bo.grid_blocks, bo.vec_blocks};
+        h->tuned[pi] = ocsl_solver_tuned{bo.rows_per_lane, bo.block_size, bo.grid_blocks, bo.vec_blocks};
         drop_graphs(h);
         if (had_x)
             OCSL_HIP(hipMemcpyAsync(h->x_own.p, saved.p, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
@@ -474,13 +489,15 @@ extern "C" ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations
     }
 }
 
-// Sets (or, with all zeros, clears) the handle's tuned launch.
-extern "C" ocsl_status ocsl_set_tuned_cg(ocsl_handle h, const ocsl_options* o) {
-    if (!h || !o) return OCSL_ERR_ARG;
-    if (o->rows_per_lane < 0 || o->rows_per_lane > 2 || o->block_size < 0 || o->block_size > 1024 ||
+// Sets (or, with all zeros, clears) the handle's tuned launch of a precision.
+extern "C" ocsl_status ocsl_set_tuned_cg(ocsl_handle h, int precision, const ocsl_options* o) {
+    if (!h || !o || (precision != OCSL_FP64 && precision != OCSL_MIXED)) return OCSL_ERR_ARG;
+    if (o->rows_per_lane < 0 || o->rows_per_lane == 3 || o->rows_per_lane > 4 || o->block_size < 0 ||
+        o->block_size > 1024 ||
         o->grid_blocks < 0 || o->vec_blocks < 0)
         return OCSL_ERR_ARG;
-    h->tuned = ocsl_solver_tuned{o->rows_per_lane, o->block_size, o->grid_blocks, o->vec_blocks};
+    h->tuned[precision == OCSL_MIXED ? 1 : 0] =
+        ocsl_solver_tuned{o->rows_per_lane, o->block_size, o->grid_blocks, o->vec_blocks};
     ocsl::drop_graphs(h);
     return OCSL_OK;
 }

@@ -82,7 +82,8 @@ struct Scalars {
     int donek[4], earlyk[4];
     double replace_ratio;
     int hist_cap;
-    int pad[3];
+    int lag;            // MIXED CG: x lags the last even iteration's alpha p
+    double alpha_prev;  // MIXED CG: alpha of that even iteration
 };
 
 // Scratch of the deterministic grid reductions: one partial sum per block

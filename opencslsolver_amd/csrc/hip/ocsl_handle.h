@@ -3,6 +3,7 @@
 
 #include <map>
 #include <memory>
+#include <type_traits>
 
 #include "ocsl_common.h"
 
@@ -110,8 +111,8 @@ struct ocsl_solver {
     uint64_t val_epoch = 0;          // bumped on every ocsl_set_values
 
     ocsl::Sell sell_id, sell_rcm;
-    ocsl::SellP sellp[2][2];  // [reorder][rows per lane - 1]
-    ocsl_solver_tuned tuned;   // launch picked by ocsl_tune_cg (zeros: none)
+    ocsl::SellP sellp[2][3];  // [reorder][rows per lane 1, 2, 4]
+    ocsl_solver_tuned tuned[2];  // launch picked by ocsl_tune_cg per precision (zeros: none)
     uint64_t sell_id_epoch64 = ~0ull, sell_id_epoch32 = ~0ull;
     uint64_t sell_rcm_epoch64 = ~0ull, sell_rcm_epoch32 = ~0ull;
 
@@ -175,6 +176,17 @@ inline T* work_buf(ocsl_solver* h, DBuf<T>& b, size_t count) {
         b.alloc(count);
     }
     return b.p;
+}
+
+// Calls f(std::integral_constant<int, R>) for the SELL-P rows per lane R.
+template <class F>
+inline void with_rpl(int rpl, F&& f) {
+    switch (rpl) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        default: throw Error{OCSL_ERR_ARG, "rows_per_lane must be 0, 1, 2 or 4"};
+    }
 }
 
 inline int grid_for(int64_t n, int per_block = kBlock, int cap = 4096) {

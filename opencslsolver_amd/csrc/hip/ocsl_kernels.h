@@ -143,6 +143,7 @@ struct CgArgs {
     X* r;
     X* p;
     X* q;
+    X* p_alt;       // MIXED: the other direction buffer (p is double-buffered)
     double* x;      // solution, FP64 in both precisions
     Scalars* S;
     double* hist;
@@ -498,19 +499,30 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_xp(CgArgs<V, X> c, in
 // replacement trigger; only beta comes from the recurrence.
 //   K1m k_cgm_spmv    block 0: rr = sum r.r, convergence, history;
 //                     q = A p; partials of p.q, r.q, q.q
-//   K2m k_cgm_update  alpha, beta; x += alpha p; r -= alpha q; p = r + beta p;
-//                     partials of r.r
+//   K2m k_cgm_update  alpha, beta; r -= alpha q; p' = r + beta p; partials of r.r
+// Iterations run in pairs with p double-buffered (p -> p_alt -> p): the even
+// update writes p' to the other buffer and leaves x alone; the odd one, with
+// both directions at hand, adds alpha_prev p_prev + alpha p to x.  The FP64 x
+// is then read and written every other iteration (30 instead of 36 bytes per
+// row and iteration); the sums are the same two FP64 additions in the same
+// order.  When a pair is cut by convergence after its even update, the odd
+// update still adds the pending alpha_prev p_prev (S->lag, cleared by the even
+// SpMV of the next pair, so it is read and written in different kernels).
 template <class V, class X, int R>
-__global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c) {
+__global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int even) {
     Scalars* S = c.S;
     if (S->done) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) S->halt = 1;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            S->halt = 1;
+            if (even) S->lag = 0;
+        }
         return;
     }
     const int g = gridDim.x;
     if (blockIdx.x == 0) {
         const double rr = block_sum_partials(c.red, kSlotRR, S->rr_blocks);
         if (threadIdx.x == 0) {
+            if (even) S->lag = 0;
             S->rr = rr;
             S->rr_last = rr;
             const int it = S->iter;
@@ -552,37 +564,72 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c) {
     block_partials<2>(rq_qq, c.red, kSlotRQ);
 }
 
-template <class V, class X>
+template <class V, class X, int kOdd>
 __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int n) {
     constexpr int N = Vec16<X>::N;
-    constexpr int P = 1;  // prefetch depth: seven streams per element already
     Scalars* S = c.S;
-    if (S->done) return;
     const int g = gridDim.x;
     const int nv = n / N;
     const int stride = g * blockDim.x;
     const int v0 = blockIdx.x * blockDim.x + threadIdx.x;
-    double xv[P][N];
-    X pv[P][N], rv[P][N], qv[P][N];
+    if (S->done) {
+        // a pair cut after its even update: add the pending alpha_prev p_prev
+        if (kOdd && S->lag) {
+            const double ap = S->alpha_prev;
+            for (int v = v0; v < nv; v += stride) {
+                double xe[N];
+                X pp[N];
+                dload<N>(c.x, v, xe);
+                vload(c.p_alt, v, pp);
 #pragma unroll
-    for (int k = 0; k < P; ++k) {
-        const int v = v0 + k * stride;
-        if (v < nv) {
-            dload<N>(c.x, v, xv[k]);
-            vload(c.p, v, pv[k]);
-            vload(c.r, v, rv[k]);
-            vload(c.q, v, qv[k]);
+                for (int e = 0; e < N; ++e) xe[e] += ap * (double)pp[e];
+                dstore<N>(c.x, v, xe);
+            }
+            if (blockIdx.x == g - 1) {
+                const int i = nv * N + (int)threadIdx.x;
+                if (i < n) c.x[i] += ap * (double)c.p_alt[i];
+            }
         }
+        return;
+    }
+    double xv[N];
+    X pv[N], rv[N], qv[N], ppv[N];
+    const bool have = v0 < nv;
+    if (have) {  // first vector loaded before the partial sums are re-added
+        if (kOdd) {
+            dload<N>(c.x, v0, xv);
+            vload(c.p_alt, v0, ppv);
+        }
+        vload(c.p, v0, pv);
+        vload(c.r, v0, rv);
+        vload(c.q, v0, qv);
     }
     constexpr int kSlots[3] = {kSlotPQ, kSlotRQ, kSlotQQ};
     double d3[3];
     block_sum_partials_k<3>(c.red, kSlots, c.g_spmv, d3);
     const double pq = d3[0], rq = d3[1], qq = d3[2];
     if (!cg_pq_ok(pq)) {
+        if (kOdd) {  // keep the even iteration's step
+            const double ap = S->alpha_prev;
+            for (int v = v0; v < nv; v += stride) {
+                double xe[N];
+                X pp[N];
+                dload<N>(c.x, v, xe);
+                vload(c.p_alt, v, pp);
+#pragma unroll
+                for (int e = 0; e < N; ++e) xe[e] += ap * (double)pp[e];
+                dstore<N>(c.x, v, xe);
+            }
+            if (blockIdx.x == g - 1) {
+                const int i = nv * N + (int)threadIdx.x;
+                if (i < n) c.x[i] += ap * (double)c.p_alt[i];
+            }
+        }
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             S->breakdown = (pq == 0.0);
             S->nonfinite = !isfinite(pq);
             S->done = 1;
+            S->lag = 0;
         }
         return;
     }
@@ -591,52 +638,52 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
     double rr_next = rr - 2.0 * alpha * rq + alpha * alpha * qq;
     if (!(rr_next > 0.0)) rr_next = 0.0;  // rounding: restart the direction
     const double beta = rr_next / rr;
+    const double ap = kOdd ? S->alpha_prev : 0.0;
     double acc[1] = {0.0};
-#pragma unroll
-    for (int k = 0; k < P; ++k) {
-        const int v = v0 + k * stride;
-        if (v < nv) {
-            X rn[N], pn[N];
-#pragma unroll
-            for (int e = 0; e < N; ++e) {
-                const double p0 = (double)pv[k][e];
-                xv[k][e] += alpha * p0;
-                rn[e] = (X)((double)rv[k][e] - alpha * (double)qv[k][e]);
-                pn[e] = (X)((double)rn[e] + beta * p0);
-                acc[0] += (double)rn[e] * (double)rn[e];
-            }
-            dstore<N>(c.x, v, xv[k]);
-            vstore(c.r, v, rn);
-            vstore(c.p, v, pn);
-        }
-    }
-    for (int v = v0 + P * stride; v < nv; v += stride) {
-        double xe[N];
-        X pe[N], re[N], qe[N], rn[N], pn[N];
-        dload<N>(c.x, v, xe);
-        vload(c.p, v, pe);
-        vload(c.r, v, re);
-        vload(c.q, v, qe);
+    auto body = [&](int v, double (&xe)[N], const X (&pe)[N], const X (&re)[N], const X (&qe)[N],
+                    const X (&ppe)[N]) {
+        X rn[N], pn[N];
 #pragma unroll
         for (int e = 0; e < N; ++e) {
             const double p0 = (double)pe[e];
-            xe[e] += alpha * p0;
+            if (kOdd) {
+                xe[e] += ap * (double)ppe[e];
+                xe[e] += alpha * p0;
+            }
             rn[e] = (X)((double)re[e] - alpha * (double)qe[e]);
             pn[e] = (X)((double)rn[e] + beta * p0);
             acc[0] += (double)rn[e] * (double)rn[e];
         }
-        dstore<N>(c.x, v, xe);
+        if (kOdd) dstore<N>(c.x, v, xe);
         vstore(c.r, v, rn);
-        vstore(c.p, v, pn);
+        vstore(c.p_alt, v, pn);
+    };
+    if (have) body(v0, xv, pv, rv, qv, ppv);
+    for (int v = v0 + stride; v < nv; v += stride) {
+        double xe[N];
+        X pe[N], re[N], qe[N], ppe[N];
+        if (kOdd) {
+            dload<N>(c.x, v, xe);
+            vload(c.p_alt, v, ppe);
+        }
+        vload(c.p, v, pe);
+        vload(c.r, v, re);
+        vload(c.q, v, qe);
+        body(v, xe, pe, re, qe, ppe);
     }
     if (blockIdx.x == g - 1) {
         const int i = nv * N + (int)threadIdx.x;
         if (i < n) {
             const double p0 = (double)c.p[i];
-            c.x[i] += alpha * p0;
+            if (kOdd) {
+                double xi = c.x[i];
+                xi += ap * (double)c.p_alt[i];
+                xi += alpha * p0;
+                c.x[i] = xi;
+            }
             const X r2 = (X)((double)c.r[i] - alpha * (double)c.q[i]);
             c.r[i] = r2;
-            c.p[i] = (X)((double)r2 + beta * p0);
+            c.p_alt[i] = (X)((double)r2 + beta * p0);
             acc[0] += (double)r2 * (double)r2;
         }
     }
@@ -646,6 +693,10 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
         S->beta = beta;
         S->pq = pq;
         S->rr_blocks = gridDim.x;
+        if (!kOdd) {
+            S->alpha_prev = alpha;
+            S->lag = 1;
+        }
         ++S->iter;
     }
 }

@@ -242,10 +242,10 @@ class Solver:
                 for i in range(min(ln.value, cap))]
         return {k: getattr(best, k) for k in keys}, us.value, rows
 
-    def set_tuned_cg(self, rows_per_lane=0, block_size=0, grid_blocks=0, vec_blocks=0):
+    def set_tuned_cg(self, precision="mixed", rows_per_lane=0, block_size=0, grid_blocks=0, vec_blocks=0):
         o = default_options(rows_per_lane=rows_per_lane, block_size=block_size, grid_blocks=grid_blocks,
                             vec_blocks=vec_blocks)
-        _check(self._lib.ocsl_set_tuned_cg(self._h, ctypes.byref(o)))
+        _check(self._lib.ocsl_set_tuned_cg(self._h, PRECISIONS[precision], ctypes.byref(o)))
 
     # -- differentiation ---------------------------------------------------
     def adjoint(self, g, method="cg", precision="fp64", dLdb=None, dLdA=None, **opts):

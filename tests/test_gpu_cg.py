@@ -71,7 +71,7 @@ def test_blas_helpers_match_reference():
         assert s.sum(d["b"]) == pytest.approx(float(np.sum(d["b"])), rel=1e-12, abs=1e-12)
 
 
-@pytest.mark.parametrize("reorder,rpl", [(1, 1), (1, 2), (0, 1), (0, 2)])
+@pytest.mark.parametrize("reorder,rpl", [(1, 1), (1, 2), (1, 4), (0, 1), (0, 2), (0, 4)])
 def test_cg_tolerance_fp64_matches_oracle(reorder, rpl):
     a, b, _ = gen.poisson3d(16, seed_mesh=1, seed_rhs=2)
     oracle.set_threads(1)
@@ -177,24 +177,25 @@ def test_cg_full_size_headline_system():
 @pytest.mark.parametrize("n", [5, 7, 13])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_cg_rows_per_lane_layouts_agree(n, precision):
-    """One and two rows per lane: the same per-row SpMV sums, so the solves
-    agree to the dot-product rounding; odd row counts end inside a row pair."""
+    """One, two and four rows per lane: the same per-row SpMV sums, so the
+    solves agree to the dot-product rounding; odd row counts end inside a row
+    group."""
     a, b, _ = gen.poisson3d(n, seed_mesh=3, seed_rhs=4, sentinels=(n == 7))
     tol = 1e-10 if precision == "fp64" else 1e-8
     out = {}
     with _solver(a, b) as s:
-        for rpl in (1, 2):
+        for rpl in (1, 2, 4):
             s.set_x(np.zeros(a.n))
             rep = s.solve("cg", "tolerance", precision, tol=tol, rows_per_lane=rpl)
             assert rep["status"] == 0, (rpl, rep)
             out[rpl] = (rep, s.get_x())
     # FP64 stops on the recursive residual: the true one may sit a few times
     # above the tolerance at 1e-10 on these near-singular systems
-    for rpl in (1, 2):
+    for rpl in (1, 2, 4):
         assert np.linalg.norm(b - a.matvec(out[rpl][1])) / np.linalg.norm(b) <= 10 * tol
-    if precision == "fp64":
-        assert abs(out[1][0]["iterations"] - out[2][0]["iterations"]) <= 2
-    assert _rel(out[1][1], out[2][1]) <= (1e-6 if precision == "fp64" else 1e-4)
+        if precision == "fp64":
+            assert abs(out[1][0]["iterations"] - out[rpl][0]["iterations"]) <= 2
+        assert _rel(out[rpl][1], out[1][1]) <= (1e-6 if precision == "fp64" else 1e-4)
 
 
 def test_cg_autotuner_keeps_x_and_tuned_solves_converge():
@@ -203,13 +204,13 @@ def test_cg_autotuner_keeps_x_and_tuned_solves_converge():
     with _solver(a, b, x0) as s:
         best, us, table = s.tune_cg("mixed", iterations=8)
         assert np.array_equal(s.get_x(), x0)
-        assert best["rows_per_lane"] in (1, 2) and best["block_size"] in (256, 512, 1024)
-        assert len(table) >= 24 and all(t["us_per_iteration"] > 0 for t in table)
+        assert best["rows_per_lane"] in (1, 2, 4) and best["block_size"] in (256, 512, 1024)
+        assert len(table) >= 36 and all(t["us_per_iteration"] > 0 for t in table)
         assert us == min(t["us_per_iteration"] for t in table)
         s.set_x(np.zeros(a.n))
         rep = s.solve("cg", "tolerance", "mixed", tol=1e-8)
         assert rep["status"] == 0, rep
         assert np.linalg.norm(b - a.matvec(s.get_x())) / np.linalg.norm(b) <= 1e-8
-        s.set_tuned_cg()  # cleared: defaults again
+        s.set_tuned_cg("mixed")  # cleared: defaults again
         s.set_x(np.zeros(a.n))
         assert s.solve("cg", "tolerance", "mixed", tol=1e-8)["status"] == 0
