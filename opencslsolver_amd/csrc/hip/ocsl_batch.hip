@@ -22,6 +22,10 @@
 
 namespace ocsl {
 
+// Threads per workgroup of the update kernel: one 16-byte FP32 vector per
+// thread over a block's 16 slices of 64 rows.
+constexpr int kBtUpdateBlock = 256;
+
 struct SysScalars {
     double rr, bb, r_ref, tol2, alpha, beta, replace_ratio;
     int iter, max_iter, done, halt, need_update, breakdown, nonfinite, replacements;
@@ -34,6 +38,8 @@ struct BatchMap {
     const int* bs0;     // block -> first slice (global)
     const int* bs1;     // block -> one past its last slice
     const int* bptr;    // system -> first block (m + 1)
+    const int* active;  // launched workgroup -> block (null: all blocks, in order)
+    __device__ int block() const { return active ? active[blockIdx.x] : (int)blockIdx.x; }
 };
 
 template <class V, class X>
@@ -65,7 +71,7 @@ __device__ __forceinline__ double sys_sum(const RedScratchBig& red, int slot, in
 }
 
 template <int NR>
-__device__ __forceinline__ void bt_partials(const double (&v)[NR], const RedScratchBig& red, int slot0) {
+__device__ __forceinline__ void bt_partials(const double (&v)[NR], const RedScratchBig& red, int slot0, int gb) {
     __shared__ double sh[NR][16];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int nw = blockDim.x / kWave;
@@ -79,7 +85,7 @@ __device__ __forceinline__ void bt_partials(const double (&v)[NR], const RedScra
         for (int r = 0; r < NR; ++r) {
             double s = 0.0;
             for (int w = 0; w < nw; ++w) s += sh[r][w];
-            red.partial[(size_t)(slot0 + r) * red.stride + blockIdx.x] = s;
+            red.partial[(size_t)(slot0 + r) * red.stride + gb] = s;
         }
 }
 
@@ -95,11 +101,12 @@ template <class X>
 __global__ void __launch_bounds__(1024) k_bt_residual(SellPView<double> A, BatchMap map, const double* __restrict__ b,
                                                     const double* __restrict__ x, X* r, X* p, SysScalars* sys,
                                                     RedScratchBig red, int first) {
-    const int k = map.bsys[blockIdx.x];
+    const int gb = map.block();
+    const int k = map.bsys[gb];
     if (!first && !bt_wants_residual(sys[k])) return;
     const int lane = threadIdx.x & (kWave - 1);
     double acc[2] = {0.0, 0.0};
-    for (int s = map.bs0[blockIdx.x] + (int)(threadIdx.x / kWave); s < map.bs1[blockIdx.x];
+    for (int s = map.bs0[gb] + (int)(threadIdx.x / kWave); s < map.bs1[gb];
          s += (int)(blockDim.x / kWave)) {
         const int row = s * kWave + lane;
         const double ax = sellp_row<double, double, false>(A, s, lane, row, x, row < A.n ? x[row] : 0.0);
@@ -112,7 +119,7 @@ __global__ void __launch_bounds__(1024) k_bt_residual(SellPView<double> A, Batch
             if (first) p[row] = (X)rv;
         }
     }
-    bt_partials<2>(acc, red, kSlotRR);
+    bt_partials<2>(acc, red, kSlotRR, gb);
 }
 
 // One block per system: bb, rr, the convergence test on the true residual.
@@ -142,13 +149,14 @@ __global__ void k_bt_fin_residual(BatchMap map, SysScalars* sys, RedScratchBig r
 // K1: per-system rr (first block of the system), q = A p, partials p.q, r.q, q.q
 template <class V, class X>
 __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
-    const int k = c.map.bsys[blockIdx.x];
+    const int gb = c.map.block();
+    const int k = c.map.bsys[gb];
     SysScalars* S = c.sys + k;
     if (S->done) {
-        if (blockIdx.x == c.map.bptr[k] && threadIdx.x == 0) S->halt = 1;
+        if (gb == c.map.bptr[k] && threadIdx.x == 0) S->halt = 1;
         return;
     }
-    if (blockIdx.x == c.map.bptr[k]) {
+    if (gb == c.map.bptr[k]) {
         const double rr = sys_sum(c.red, kSlotRR, c.map.bptr[k], c.map.bptr[k + 1]);
         if (threadIdx.x == 0) {
             S->rr = rr;
@@ -160,8 +168,7 @@ __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
     }
     const int lane = threadIdx.x & (kWave - 1);
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int s = c.map.bs0[blockIdx.x] + (int)(threadIdx.x / kWave); s < c.map.bs1[blockIdx.x];
-         s += (int)(blockDim.x / kWave)) {
+    for (int s = c.map.bs0[gb] + (int)(threadIdx.x / kWave); s < c.map.bs1[gb]; s += (int)(blockDim.x / kWave)) {
         const int row = s * kWave + lane;
         const X pi = row < c.A.n ? c.p[row] : (X)0;
         const X qv = (X)sellp_row<V, X, sizeof(V) == 4>(c.A, s, lane, row, c.p, (double)pi);
@@ -176,23 +183,37 @@ __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
     // p.q, then r.q and q.q in slots of their own: the r.r slot is being read
     // by the first block of each system during this kernel
     const double pq1[1] = {acc[0]};
-    bt_partials<1>(pq1, c.red, kSlotPQ);
+    bt_partials<1>(pq1, c.red, kSlotPQ, gb);
     const double rq_qq[2] = {acc[1], acc[2]};
-    bt_partials<2>(rq_qq, c.red, kSlotRQ);
+    bt_partials<2>(rq_qq, c.red, kSlotRQ, gb);
 }
 
 // K2: alpha, beta per system; x += alpha p; r -= alpha q; p = r + beta p; r.r
 template <class V, class X>
 __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
-    const int k = c.map.bsys[blockIdx.x];
+    constexpr int N = Vec16<X>::N;
+    const int gb = c.map.block();
+    const int k = c.map.bsys[gb];
     SysScalars* S = c.sys + k;
     if (S->done) return;
     const int b0 = c.map.bptr[k], b1 = c.map.bptr[k + 1];
+    // the block's rows are whole 64-row slices: 16-byte vectors, the first
+    // one loaded before the partial sums are re-added
+    const int v0 = c.map.bs0[gb] * (kWave / N), v1 = c.map.bs1[gb] * (kWave / N);
+    const int va = v0 + (int)threadIdx.x;
+    double xa[N];
+    X pa[N], ra[N], qa[N];
+    if (va < v1) {
+        dload<N>(c.x, va, xa);
+        vload(c.p, va, pa);
+        vload(c.r, va, ra);
+        vload(c.q, va, qa);
+    }
     const double pq = sys_sum(c.red, kSlotPQ, b0, b1);
     const double rq = sys_sum(c.red, kSlotRQ, b0, b1);
     const double qq = sys_sum(c.red, kSlotQQ, b0, b1);
     if (!(pq != 0.0 && isfinite(pq))) {
-        if (blockIdx.x == b0 && threadIdx.x == 0) {
+        if (gb == b0 && threadIdx.x == 0) {
             S->breakdown = (pq == 0.0);
             S->nonfinite = !isfinite(pq);
             S->done = 1;
@@ -205,17 +226,32 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
     if (!(rr_next > 0.0)) rr_next = 0.0;
     const double beta = rr_next / rr;
     double acc[1] = {0.0};
-    const int i0 = c.map.bs0[blockIdx.x] * kWave, i1 = c.map.bs1[blockIdx.x] * kWave;
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += (int)blockDim.x) {
-        const double p0 = (double)c.p[i];
-        c.x[i] += alpha * p0;
-        const X r2 = (X)((double)c.r[i] - alpha * (double)c.q[i]);
-        c.r[i] = r2;
-        c.p[i] = (X)((double)r2 + beta * p0);
-        acc[0] += (double)r2 * (double)r2;
+    auto body = [&](int v, double (&xe)[N], const X (&pe)[N], const X (&re)[N], const X (&qe)[N]) {
+        X rn[N], pn[N];
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
+            const double p0 = (double)pe[e];
+            xe[e] += alpha * p0;
+            rn[e] = (X)((double)re[e] - alpha * (double)qe[e]);
+            pn[e] = (X)((double)rn[e] + beta * p0);
+            acc[0] += (double)rn[e] * (double)rn[e];
+        }
+        dstore<N>(c.x, v, xe);
+        vstore(c.r, v, rn);
+        vstore(c.p, v, pn);
+    };
+    if (va < v1) body(va, xa, pa, ra, qa);
+    for (int v = va + (int)blockDim.x; v < v1; v += (int)blockDim.x) {
+        double xe[N];
+        X pe[N], re[N], qe[N];
+        dload<N>(c.x, v, xe);
+        vload(c.p, v, pe);
+        vload(c.r, v, re);
+        vload(c.q, v, qe);
+        body(v, xe, pe, re, qe);
     }
-    bt_partials<1>(acc, c.red, kSlotRR);
-    if (blockIdx.x == b0 && threadIdx.x == 0) {
+    bt_partials<1>(acc, c.red, kSlotRR, gb);
+    if (gb == b0 && threadIdx.x == 0) {
         S->alpha = alpha;
         S->beta = beta;
         ++S->iter;
@@ -238,6 +274,7 @@ struct ocsl_batch_s {
     std::vector<int> bptr_host;
     // concatenated SELL-P (global numbering)
     DBuf<int> slice_ptr, src, diag_src, bsys, bs0, bs1, bptr, perm;  // perm: padded row -> (flat input index)
+    DBuf<int> active;                    // compacted list of the blocks of unfinished systems
     DBuf<unsigned> colw;
     DBuf<int> col;
     bool compact = true;
@@ -252,7 +289,7 @@ struct ocsl_batch_s {
     DBuf<double> red_partial;
     DBuf<SysScalars> sys;
     SysScalars* sys_host = nullptr;
-    std::map<int, hipGraphExec_t> graphs;
+    std::map<int64_t, hipGraphExec_t> graphs;  // iteration chunks by (precision, chunk, launch size, compacted)
     ~ocsl_batch_s() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         if (sys_host) (void)hipHostFree(sys_host);
@@ -271,7 +308,7 @@ struct ocsl_batch_s {
     SellPView<float> view32() const {
         return {n_tot, ns_tot, slice_ptr.p, compact ? nullptr : col.p, compact ? colw.p : nullptr, val32.p, diag32.p};
     }
-    BatchMap map() const { return {bsys.p, bs0.p, bs1.p, bptr.p}; }
+    BatchMap map(bool compact = false) const { return {bsys.p, bs0.p, bs1.p, bptr.p, compact ? active.p : nullptr}; }
 };
 
 namespace {
@@ -445,6 +482,7 @@ ocsl_status ocsl_batch_create(ocsl_batch* out, int m, const int64_t* n_rows, con
         bt->bs1.upload(bs1.data(), bs1.size(), s);
         bt->bptr.upload(bptr.data(), bptr.size(), s);
         bt->red_partial.alloc((size_t)bt->nblocks * 8);
+        bt->active.alloc(bt->nblocks);
         bt->sys.alloc(m);
         OCSL_HIP(hipHostMalloc((void**)&bt->sys_host, sizeof(SysScalars) * m, hipHostMallocDefault));
         OCSL_HIP(hipStreamSynchronize(s));
@@ -562,27 +600,32 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
             if (mixed) OCSL_HIP(hipMemsetAsync(bt->f[i].p, 0, sizeof(float) * N, s));
             else OCSL_HIP(hipMemsetAsync(bt->d[i].p, 0, sizeof(double) * N, s));
         }
+        // launched blocks: all, or (compacted) the blocks of unfinished systems
+        int G_cur = G;
+        bool comp = false;
         BtArgs<float, float> c32{bt->view32(), bt->map(), bt->f[0].p, bt->f[1].p, bt->f[2].p, bt->x_glob.p,
                                  bt->sys.p, bt->red()};
         BtArgs<double, double> c64{bt->view64(), bt->map(), bt->d[0].p, bt->d[1].p, bt->d[2].p, bt->x_glob.p,
                                    bt->sys.p, bt->red()};
-        auto residual = [&](int first) {
+        auto residual = [&](int first, bool all) {
+            const BatchMap mp = bt->map(comp && !all);
+            const int gl = (comp && !all) ? G_cur : G;
             if (mixed)
-                k_bt_residual<float><<<G, B, 0, s>>>(bt->view64(), bt->map(), bt->b_glob.p, bt->x_glob.p, c32.r, c32.p,
-                                                     bt->sys.p, bt->red(), first);
+                k_bt_residual<float><<<gl, B, 0, s>>>(bt->view64(), mp, bt->b_glob.p, bt->x_glob.p, c32.r, c32.p,
+                                                      bt->sys.p, bt->red(), first);
             else
-                k_bt_residual<double><<<G, B, 0, s>>>(bt->view64(), bt->map(), bt->b_glob.p, bt->x_glob.p, c64.r,
-                                                      c64.p, bt->sys.p, bt->red(), first);
+                k_bt_residual<double><<<gl, B, 0, s>>>(bt->view64(), mp, bt->b_glob.p, bt->x_glob.p, c64.r, c64.p,
+                                                       bt->sys.p, bt->red(), first);
             k_bt_fin_residual<<<m, 256, 0, s>>>(bt->map(), bt->sys.p, bt->red(), first, m);
             OCSL_HIP(hipGetLastError());
         };
         auto iter = [&] {
             if (mixed) {
-                k_bt_spmv<float, float><<<G, B, 0, s>>>(c32);
-                k_bt_update<float, float><<<G, B, 0, s>>>(c32);
+                k_bt_spmv<float, float><<<G_cur, B, 0, s>>>(c32);
+                k_bt_update<float, float><<<G_cur, kBtUpdateBlock, 0, s>>>(c32);
             } else {
-                k_bt_spmv<double, double><<<G, B, 0, s>>>(c64);
-                k_bt_update<double, double><<<G, B, 0, s>>>(c64);
+                k_bt_spmv<double, double><<<G_cur, B, 0, s>>>(c64);
+                k_bt_update<double, double><<<G_cur, kBtUpdateBlock, 0, s>>>(c64);
             }
         };
         auto read_sys = [&] {
@@ -600,17 +643,40 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
             for (int k = 0; k < m; ++k)
                 if (lat[k] < 0.0 && finished(bt->sys_host[k])) lat[k] = ms;
         };
+        // Compaction: once the blocks of unfinished systems are at most 3/4
+        // of those launched, launch only them (a workgroup reads its block
+        // from the active list; partial sums stay indexed by block).
+        std::vector<int> act;
+        auto compact = [&] {
+            act.clear();
+            for (int k = 0; k < m; ++k)
+                if (!finished(bt->sys_host[k]))
+                    for (int q = bt->bptr_host[k]; q < bt->bptr_host[k + 1]; ++q) act.push_back(q);
+            if (act.empty() || 4 * (int64_t)act.size() > 3 * (int64_t)G_cur) return;
+            OCSL_HIP(hipMemcpyAsync(bt->active.p, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, s));
+            OCSL_HIP(hipStreamSynchronize(s));
+            G_cur = (int)act.size();
+            comp = true;
+            c32.map = c64.map = bt->map(true);
+        };
         OCSL_HIP(hipEventRecord(bt->ev0, s));
-        residual(1);
+        residual(1, true);
         read_sys();
         stamp();
         const int chunk = o.check_every;
         int launched = 0;
+        if (bt->graphs.size() > 64) {
+            for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
+            bt->graphs.clear();
+        }
         while (true) {
             bool all = true;
             for (int k = 0; k < m; ++k) all = all && finished(bt->sys_host[k]);
             if (all || launched >= o.max_iter) break;
-            const int key = (mixed ? 1 : 0) + 2 * chunk;
+            compact();
+            // graphs hold the launch size and map; the active list is read
+            // through its (fixed) device address when the graph runs
+            const int64_t key = (mixed ? 1 : 0) + 2 * (int64_t)chunk + ((int64_t)G_cur << 24) + (comp ? (1ll << 56) : 0);
             if (o.use_graph) {
                 auto it = bt->graphs.find(key);
                 if (it == bt->graphs.end()) {
@@ -637,7 +703,7 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
                     any = any || S.need_update || (S.done && !S.verified && !S.nonfinite && !S.breakdown);
                 }
                 if (any) {
-                    residual(0);
+                    residual(0, false);
                     read_sys();
                 }
             }
@@ -661,7 +727,7 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
             bt->sys_host[k].breakdown = 0;
         }
         OCSL_HIP(hipMemcpyAsync(bt->sys.p, bt->sys_host, sizeof(SysScalars) * m, hipMemcpyHostToDevice, s));
-        residual(0);
+        residual(0, true);
         read_sys();
         for (int k = 0; k < m; ++k) {
             const SysScalars& S = bt->sys_host[k];
