@@ -20,8 +20,10 @@ def _row_of(a, p):
     return int(np.searchsorted(a.row_ptr, p, side="right") - 1)
 
 
-def test_sampled_outer_is_exact():
-    a, _ = gen.convdiff3d(7, sentinels=True)
+@pytest.mark.parametrize("n,sentinels", [(7, True), (33, True), (40, False)])
+def test_sampled_outer_is_exact(n, sentinels):
+    """Small and multi-workgroup systems, sentinel slots included: bit-exact."""
+    a, _ = gen.convdiff3d(n, sentinels=sentinels)
     u, v = gen.normal(1, a.n), gen.normal(2, a.n)
     with Solver(a.row_ptr, a.col) as s:
         out = s.sampled_outer(-1.0, u, v)
