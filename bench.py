@@ -103,7 +103,8 @@ def headline(args, out):
     _log("bench: autotuning the CG launch")
     from opencslsolver_amd import tuning
 
-    tuned = {prec: tuning.tuned_cg(s, prec, iterations=64, retune=True) for prec in ("fp64", "mixed")}
+    tuned = {prec: tuning.tuned_cg(s, prec, iterations=64, retune=not args.reuse_tuning)
+             for prec in ("fp64", "mixed")}
 
     _log("bench: warmup")
     for _ in range(args.warmup):
@@ -319,6 +320,8 @@ def main():
     ap.add_argument("--sections", default="adjoint,multi,batch",
                     help="side workloads to add to the headline: adjoint, multi, batch (or none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--reuse-tuning", action="store_true",
+                    help="apply the cached tuned launch when present instead of tuning again")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the measured kernels, --steps times each (for rocprofv3)")
     args = ap.parse_args()
