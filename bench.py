@@ -14,7 +14,9 @@ already resident in HBM.  The JSON line also carries the side workloads:
   [22, 80] from seed 7, MIXED CG to 1e-8 in shared launches: total time,
   per-system latency p50/p99, and the same systems solved one by one;
 * adjoint -- forward CG solve plus the adjoint (transpose CG solve and the
-  sampled outer product dL/dA on the CSR pattern) on the 128^3 system.
+  sampled outer product dL/dA on the CSR pattern) on the 128^3 system;
+* assembly -- the GPU coefficient assembly of each equation on a 128^3 hex
+  mesh, beside the oracle's host restatement of the reference loops.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--sections all]
 """
@@ -311,14 +313,69 @@ def profile_kernels(args):
     print(json.dumps(res), flush=True)
 
 
+def assembly(args, out):
+    """Coefficient assembly (the face, boundary-face and cell loops of each
+    equation) on an n^3 hex mesh with the headline's permuted cell numbering,
+    every boundary kind on its sides, viscous, blended, transient; a seeded
+    random flow state.  GPU kernel time per equation (HIP events) beside the
+    oracle's OpenMP scatter loops (the reference's structure) on the host."""
+    import oracle
+    from opencslsolver_amd import flow, gen
+
+    n = args.n
+    _log(f"bench: assembly on a {n}^3 hex mesh")
+    _, _, perm = gen.poisson3d(n, seed_mesh=1, seed_rhs=2)
+    mesh, rp, col = gen.hex_mesh(n, perm=perm)
+    bcs = [dict(type=flow.VELOCITY_INLET, velocity=(3.0, 0.5, -0.2), temperature=300.0),
+           dict(type=flow.PRESSURE_OUTLET, pressure=1e5), dict(type=flow.WALL, no_slip=1),
+           dict(type=flow.WALL, no_slip=0), dict(type=flow.SYMMETRY),
+           dict(type=flow.SUPERSONIC_INLET, velocity=(400.0, 0.0, 0.0), pressure=1e5, temperature=300.0)]
+    prm = flow.default_params(viscous=1, blended=1, transient=1, bcs=bcs)
+    N, F, B = mesh["cell_volume"].shape[0], mesh["face_left"].shape[0], mesh["bc_left"].shape[0]
+    rng = np.random.default_rng(21)
+    st = dict(u=rng.normal(size=(N, 3)), u_old=rng.normal(size=(N, 3)), rho=1.0 + rng.random(N),
+              rho_old=1.0 + rng.random(N), p=1e5 + 1e3 * rng.random(N), p_old=1e5 + 1e3 * rng.random(N),
+              p_corr=rng.normal(size=N), grad_p=rng.normal(size=(N, 3)), T=280.0 + 40.0 * rng.random(N),
+              h=3e5 + 1e4 * rng.random(N), h_old=3e5 + 1e4 * rng.random(N), dt=np.full(N, 1e-4),
+              ap=np.zeros(N), ap_vec=np.zeros((N, 3)), f_mdot=rng.normal(size=F) * 1e-3,
+              f_rho=1.0 + rng.random(F), f_qy=rng.random(F) * 1e-3, f_stress=rng.normal(size=(F, 3)) * 1e-5,
+              b_T=280.0 + 40.0 * rng.random(B), b_p=np.full(B, 1e5), b_rho=1.0 + rng.random(B),
+              b_mdot=rng.normal(size=B) * 1e-3, b_h=np.full(B, 3e5), b_qy=rng.random(B) * 1e-3,
+              b_u=rng.normal(size=(B, 3)), b_supersonic=np.zeros(B, np.int32))
+    names = ["momentum_x", "momentum_y", "momentum_z", "pressure", "enthalpy"]
+    res = {}
+    with flow.Flow(mesh, rp, col, prm) as fl:
+        fl.set_state(**st)
+        for eq, name in enumerate(names):
+            ms, by = fl.time_assemble(eq, reps=10)
+            res[name] = {"avg_us": round(ms * 1e3, 2), "bytes_per_launch": by,
+                         "achieved_GBps": round(by / (ms * 1e-3) / 1e9, 1),
+                         "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        oracle.set_threads(threads)
+        t0 = time.perf_counter()
+        for eq in range(5):
+            oracle.assemble(eq, mesh, rp, col, prm, st, fast=True)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms for the five equations", "cores": threads, "kind": "port",
+               "sample": "one assembly of each equation on the same mesh and state (the oracle's restatement of "
+                         "the reference loops, OpenMP with atomic adds into linearly searched CSR slots)"}
+    out["assembly"] = {"workload": f"face/boundary/cell coefficient loops, {n}^3 hex mesh (N={N}, faces={F}, "
+                                   f"boundary faces={B}), viscous, blended, transient, permuted cells",
+                       "kernel": "k_assemble", "gpu": res,
+                       "gpu_total_us": round(sum(v["avg_us"] for v in res.values()), 2), "cpu_baseline": cpu}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--sections", default="adjoint,multi,batch",
-                    help="side workloads to add to the headline: adjoint, multi, batch (or none)")
+    ap.add_argument("--sections", default="adjoint,multi,batch,assembly",
+                    help="side workloads to add to the headline: adjoint, multi, batch, assembly (or none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reuse-tuning", action="store_true",
                     help="apply the cached tuned launch when present instead of tuning again")
@@ -353,6 +410,8 @@ def main():
         multi_rhs(args, out)
     if "batch" in args.sections:
         batch(args, out)
+    if "assembly" in args.sections:
+        assembly(args, out)
     print(json.dumps(out), flush=True)
 
 

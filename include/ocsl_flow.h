@@ -137,6 +137,12 @@ ocsl_status ocsl_flow_get_state(ocsl_flow f, ocsl_flow_state* st);
 ocsl_status ocsl_flow_assemble(ocsl_flow f, int equation, double* val, double* b, double* x0,
                                ocsl_where where);
 
+/* Measurement hook: launches one equation's assembly kernel `reps` times on
+ * the solver handle's stream between two HIP events (state as last set);
+ * returns the average duration and the algorithmic bytes of one launch (every
+ * array the kernel needs, read or written once).  Leaves the state as set. */
+ocsl_status ocsl_flow_time_assemble(ocsl_flow f, int equation, int reps, double* avg_ms, double* bytes);
+
 /* Explicit viscous stress of every interior face (calc_flux_explicit_visc_part,
  * cflow_solver_cell_face_loops.cpp:25-35) from the current state. */
 ocsl_status ocsl_flow_viscous_stress(ocsl_flow f);

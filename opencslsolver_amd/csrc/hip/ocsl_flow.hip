@@ -1173,6 +1173,53 @@ ocsl_status ocsl_flow_assemble(ocsl_flow fl, int equation, double* val, double* 
     });
 }
 
+// Algorithmic bytes of one assembly launch: every array the kernel needs,
+// read or written once (face data once per face, not per side).
+static double assemble_bytes(const ocsl_flow_s* fl, int eq) {
+    const double n = fl->mesh.n, F = fl->mesh.F, B = fl->mesh.B, nnz = (double)fl->nnz_in;
+    const ocsl_flow_params& p = fl->prm;
+    double by = nnz * 8.0 + n * (8.0 + 8.0 + 4.0 * 4.0 + 8.0);  // values, b, x0, row maps, volume
+    by += 2.0 * F * 8.0 + F * (8.0 + 8.0 + 8.0);                 // cell-face lists; L, R, geo, mass flux
+    by += B * (4.0 + 4.0 + 8.0);                                  // boundary list, marker, mass flux
+    if (p.transient) by += n * 8.0;                               // dt
+    if (eq <= OCSL_EQ_MOMENTUM_Z) {
+        by += n * (8.0 + 8.0 + 8.0 + 8.0 + 8.0 + 16.0 + 8.0);     // u_d, u_old_d, rho, rho_old, dp/dx_d, ap rw, ap_vec
+        by += B * 24.0;                                           // boundary velocity
+        if (p.viscous) by += n * 8.0 + F * (8.0 + 24.0) + B * (24.0 + 16.0);  // T, ksi, stress; walls
+    } else if (eq == OCSL_EQ_PRESSURE) {
+        by += n * (8.0 * 4.0) + F * 16.0 + B * 28.0;              // T, rho, rho_old, p'; rho_f, Qy; boundary
+    } else {
+        by += n * (8.0 * 7.0) + F * 8.0 + B * 16.0;               // h, h_old, T, rho, rho_old, p, p_old; ksi
+    }
+    return by;
+}
+
+ocsl_status ocsl_flow_time_assemble(ocsl_flow fl, int equation, int reps, double* avg_ms, double* bytes) {
+    if (!fl || !avg_ms || !bytes || reps <= 0) return OCSL_ERR_ARG;
+    return guarded_f([&] {
+        OCSL_REQUIRE(equation >= OCSL_EQ_MOMENTUM_X && equation <= OCSL_EQ_ENTHALPY, OCSL_ERR_ARG, "bad equation");
+        ocsl_solver* h = fl->solver;
+        upload_cells(fl, equation);
+        auto launch = [&] {
+            if (equation <= OCSL_EQ_MOMENTUM_Z) launch_assemble<kMomentum>(fl, equation);
+            else if (equation == OCSL_EQ_PRESSURE) launch_assemble<kPressure>(fl, 0);
+            else launch_assemble<kEnthalpy>(fl, 0);
+        };
+        launch();  // warm
+        OCSL_HIP(hipEventRecord(h->ev0, h->stream));
+        for (int r = 0; r < reps; ++r) launch();
+        OCSL_HIP(hipEventRecord(h->ev1, h->stream));
+        OCSL_HIP(hipStreamSynchronize(h->stream));
+        float ms = 0.f;
+        OCSL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *avg_ms = ms / reps;
+        *bytes = assemble_bytes(fl, equation);
+        // the repeated momentum launches accumulated into ap: restore it
+        up(fl->d_ap, fl->st.ap, h->stream);
+        OCSL_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
 ocsl_status ocsl_flow_viscous_stress(ocsl_flow fl) {
     if (!fl) return OCSL_ERR_ARG;
     return guarded_f([&] { viscous_stress(fl); });

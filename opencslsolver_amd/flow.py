@@ -78,6 +78,7 @@ SIGNATURES = {
     "ocsl_flow_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_int]),
     "ocsl_flow_viscous_stress": (ctypes.c_int, [ctypes.c_void_p]),
+    "ocsl_flow_time_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]),
     "ocsl_flow_iterate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ocsl_flow_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "ocsl_flow_solve_history": (ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_int64,
@@ -124,6 +125,44 @@ def default_params(**kw) -> Params:
     return p
 
 
+def mesh_struct(mesh: dict):
+    """(Mesh, arrays kept alive) of a mesh dict."""
+    n = int(np.asarray(mesh["cell_volume"]).shape[0])
+    F = int(np.asarray(mesh["face_left"]).shape[0])
+    B = int(np.asarray(mesh["bc_left"]).shape[0])
+    keep = {}
+    m = Mesh(n_cells=n, n_faces=F, n_bc=B)
+    for k in ("cell_volume",) + MESH_FLOAT + MESH_BC_FLOAT:
+        a = np.ascontiguousarray(mesh[k], dtype=np.float64)
+        keep[k] = a
+        setattr(m, k, a.ctypes.data_as(_f64p))
+    for k in ("face_left", "face_right", "bc_left", "bc_marker"):
+        a = np.ascontiguousarray(mesh[k], dtype=np.int32)
+        keep[k] = a
+        setattr(m, k, a.ctypes.data_as(_i32p))
+    return m, keep
+
+
+def state_struct(fields: dict, n: int, F: int, B: int):
+    """(State, arrays) of a dict of state fields; missing fields are NULL.
+    The arrays are private float64/int32 copies the C side may write."""
+    st = State()
+    arrays = {}
+    sizes = {**{k: n * c for k, c in CELL_FIELDS.items()}, **{k: F * c for k, c in FACE_FIELDS.items()},
+             **{k: B * c for k, c in BC_FIELDS.items()}}
+    for k, v in fields.items():
+        if k == "b_supersonic":
+            a = np.array(v, dtype=np.int32).reshape(-1)
+            assert a.shape[0] == B
+            st.b_supersonic = a.ctypes.data_as(_i32p)
+        else:
+            a = np.array(v, dtype=np.float64).reshape(-1)
+            assert a.shape[0] == sizes[k], (k, a.shape, sizes[k])
+            setattr(st, k, a.ctypes.data_as(_f64p))
+        arrays[k] = a
+    return st, arrays
+
+
 class Flow:
     """CFlow_solver on one mesh: GPU assembly + GPU linear solves."""
 
@@ -132,16 +171,7 @@ class Flow:
         self.n = int(np.asarray(mesh["cell_volume"]).shape[0])
         self.F = int(np.asarray(mesh["face_left"]).shape[0])
         self.B = int(np.asarray(mesh["bc_left"]).shape[0])
-        self._mesh_keep = {}
-        m = Mesh(n_cells=self.n, n_faces=self.F, n_bc=self.B)
-        for k in ("cell_volume",) + MESH_FLOAT + MESH_BC_FLOAT:
-            a = np.ascontiguousarray(mesh[k], dtype=np.float64)
-            self._mesh_keep[k] = a
-            setattr(m, k, a.ctypes.data_as(_f64p))
-        for k in ("face_left", "face_right", "bc_left", "bc_marker"):
-            a = np.ascontiguousarray(mesh[k], dtype=np.int32)
-            self._mesh_keep[k] = a
-            setattr(m, k, a.ctypes.data_as(_i32p))
+        m, self._mesh_keep = mesh_struct(mesh)
         self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint64)
         self.col = np.ascontiguousarray(col, dtype=np.uint64)
         self.nnz = int(self.col.shape[0])
@@ -181,20 +211,7 @@ class Flow:
         _check(self._lib.ocsl_flow_initialize(self._h))
 
     def set_state(self, **fields):
-        st = State()
-        keep = []
-        sizes = self._sizes()
-        for k, v in fields.items():
-            if k == "b_supersonic":
-                a = np.ascontiguousarray(v, dtype=np.int32).reshape(-1)
-                assert a.shape[0] == self.B
-                st.b_supersonic = a.ctypes.data_as(_i32p)
-            else:
-                n, c = sizes[k]
-                a = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
-                assert a.shape[0] == n * c, (k, a.shape, n, c)
-                setattr(st, k, a.ctypes.data_as(_f64p))
-            keep.append(a)
+        st, _keep = state_struct(fields, self.n, self.F, self.B)
         _check(self._lib.ocsl_flow_set_state(self._h, ctypes.byref(st)))
 
     def get_state(self) -> dict:
@@ -217,6 +234,12 @@ class Flow:
                                             b.ctypes.data_as(ctypes.c_void_p), x0.ctypes.data_as(ctypes.c_void_p),
                                             _abi.HOST))
         return val, b, x0
+
+    def time_assemble(self, equation: int, reps: int = 20):
+        """(average ms, algorithmic bytes) of one assembly launch of an equation."""
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.ocsl_flow_time_assemble(self._h, equation, reps, ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
 
     def viscous_stress(self):
         _check(self._lib.ocsl_flow_viscous_stress(self._h))

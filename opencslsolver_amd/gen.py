@@ -171,3 +171,78 @@ def tridiag_unit_test_system():
     a = CSR(np.array(rp, np.uint64), np.array(cols, np.uint64), np.array(vals, np.float64))
     return a, np.full(n, 2.0), np.full(n, 1.0)
 
+
+
+def hex_mesh(n: int, h: float | None = None, perm: np.ndarray | None = None):
+    """Uniform n^3 hexahedral finite-volume mesh of the unit cube (cell size
+    h = 1/n unless given) in the CFVM form (cfvm.cpp:100-170): interior faces
+    x-, then y-, then z-oriented in natural order, boundary faces of the six
+    sides (markers 0..5: x-, x+, y-, y+, z-, z+).  perm[c] renumbers natural
+    cell c = i + n (j + n k).  Returns (mesh dict, row_ptr, col) with the
+    CSR structure of CFVM::define_matrix_structure (cfvm.cpp:74-98): per row
+    the sorted cell and neighbour columns, and one sentinel column
+    (2^64 - 1) when the cell has boundary faces."""
+    h = 1.0 / n if h is None else float(h)
+    N = n ** 3
+    nat = np.arange(N, dtype=np.int64).reshape(n, n, n)  # [k, j, i]
+    ids = nat if perm is None else np.asarray(perm, dtype=np.int64)[nat]
+    A = h * h
+    faces_L, faces_R, normals = [], [], []
+    for axis in range(3):  # 0: x (i), 1: y (j), 2: z (k)
+        ax = 2 - axis
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[ax] = slice(0, n - 1)
+        hi[ax] = slice(1, n)
+        faces_L.append(ids[tuple(lo)].reshape(-1))
+        faces_R.append(ids[tuple(hi)].reshape(-1))
+        e = np.zeros(3)
+        e[axis] = 1.0
+        normals.append(np.tile(e, (faces_L[-1].shape[0], 1)))
+    fL = np.concatenate(faces_L).astype(np.int32)
+    fR = np.concatenate(faces_R).astype(np.int32)
+    fe = np.concatenate(normals)
+    F = fL.shape[0]
+    bL, bm, bn = [], [], []
+    for axis in range(3):
+        ax = 2 - axis
+        for side, idx in ((0, 0), (1, n - 1)):
+            sl = [slice(None)] * 3
+            sl[ax] = idx
+            cells = ids[tuple(sl)].reshape(-1)
+            e = np.zeros(3)
+            e[axis] = -1.0 if side == 0 else 1.0
+            bL.append(cells)
+            bm.append(np.full(cells.shape[0], 2 * axis + side, np.int32))
+            bn.append(np.tile(e, (cells.shape[0], 1)))
+    bL = np.concatenate(bL).astype(np.int32)
+    bm = np.concatenate(bm)
+    be = np.concatenate(bn)
+    B = bL.shape[0]
+    mesh = {
+        "cell_volume": np.full(N, h ** 3),
+        "face_left": fL, "face_right": fR,
+        "face_area_vector": A * fe, "face_unit": fe.copy(), "face_alpha": fe / h,
+        "face_area": np.full(F, A), "face_dist": np.full(F, h), "face_ksi": np.full(F, h),
+        "face_geo_aver": np.full(F, 0.5),
+        "bc_left": bL, "bc_marker": bm,
+        "bc_area_vector": A * be, "bc_unit": be.copy(), "bc_alpha": 2.0 * be / h,
+        "bc_area": np.full(B, A), "bc_dist": np.full(B, 0.5 * h), "bc_ksi": np.full(B, 2.0 * h),
+        "bc_perp": np.full(B, 0.5 * h),
+    }
+    # matrix structure: cell, neighbours, one sentinel for boundary cells
+    sentinel = np.uint64(0xFFFFFFFFFFFFFFFF)
+    rows = np.concatenate([np.arange(N), fL.astype(np.int64), fR.astype(np.int64)])
+    cols = np.concatenate([np.arange(N), fR.astype(np.int64), fL.astype(np.int64)]).astype(np.uint64)
+    bcells = np.unique(bL.astype(np.int64))
+    rows = np.concatenate([rows, bcells])
+    cols = np.concatenate([cols, np.full(bcells.shape[0], sentinel, np.uint64)])
+    order = np.lexsort((cols, rows))
+    rows, cols = rows[order], cols[order]
+    keep = np.ones(rows.shape[0], bool)
+    keep[1:] = (rows[1:] != rows[:-1]) | (cols[1:] != cols[:-1])
+    rows, cols = rows[keep], cols[keep]
+    row_ptr = np.zeros(N + 1, np.uint64)
+    np.add.at(row_ptr, rows + 1, 1)
+    row_ptr = np.cumsum(row_ptr).astype(np.uint64)
+    return mesh, row_ptr, cols

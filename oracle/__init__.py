@@ -192,6 +192,31 @@ def sampled_outer(A, lam, x):
     return out
 
 
+def assemble(eq, mesh, row_ptr, col, params, state, urf=0.5, fast=False):
+    """The reference's face/boundary/cell loops for one equation (0..2
+    momentum, 3 pressure correction, 4 enthalpy) on a mesh dict and state
+    dict; returns (val, b, x0, ap, ap_vec) -- ap / ap_vec updated as the
+    momentum cell loop does."""
+    from opencslsolver_amd import flow
+
+    m, _keep = flow.mesh_struct(mesh)
+    n, F, B = m.n_cells, m.n_faces, m.n_bc
+    fields = dict(state)
+    fields.setdefault("ap", np.zeros(n))
+    fields.setdefault("ap_vec", np.zeros((n, 3)))
+    st, arrays = flow.state_struct(fields, n, F, B)
+    rp = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+    cl = np.ascontiguousarray(col, dtype=np.uint64)
+    val = np.zeros(cl.shape[0])
+    b = np.zeros(n)
+    x0 = np.zeros(n)
+    rc = lib(fast).or_assemble(int(eq), ctypes.byref(m), ctypes.byref(st), ctypes.byref(params), _d(urf),
+                               _p(rp, _u64p), _p(cl, _u64p), _p(val), _p(b), _p(x0))
+    if rc != 0:
+        raise ValueError(f"or_assemble failed: {rc}")
+    return val, b, x0, arrays["ap"], arrays["ap_vec"].reshape(n, 3)
+
+
 # --- reference build (_ref) --------------------------------------------------
 
 def ref_cg_solve(A, b, x0):

@@ -2,7 +2,6 @@
 reference's own runs (golden fixtures from tests/golden/make_golden.py): the
 system assembled from each recorded flow state, and whole runs (fields and
 per-solve residuals)."""
-import json
 import os
 
 import numpy as np
@@ -10,31 +9,11 @@ import pytest
 
 from opencslsolver_amd import flow
 from opencslsolver_amd.gen import CSR
+from opencslsolver_amd.gen import hex_mesh as gen_hex
+
+from flowcases import CASES, FLOW_META, GOLDEN, SOD_META, params_for, records, state_at
 
 pytestmark = pytest.mark.gpu
-GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-FLOW_META = json.load(open(os.path.join(GOLDEN, "flow_meta.json")))
-SOD_META = json.load(open(os.path.join(GOLDEN, "sod_meta.json")))
-CASES = sorted(FLOW_META)
-
-
-def params_for(cfg):
-    """ocsl_flow_params of a reference configuration (Cconfiguration fields)."""
-    bcs = []
-    for m in range(3):
-        bcs.append(dict(type=cfg["bc_type"][m], no_slip=cfg["bc_no_slip"][m], const_temperature=cfg["bc_const_T"][m],
-                        temperature=cfg["bc_T"][m], pressure=cfg["bc_p"][m], velocity=tuple(cfg["bc_u"][m])))
-    # solver_mode 1: CG for the pressure correction, ILU(0)-BiCGSTAB for the
-    # others (the reference driver's split dispatch); else solver_type for all
-    split = cfg.get("solver_mode", 0) == 1
-    method = 1 if (split or cfg["solver_type"] == 1) else 0
-    return flow.default_params(
-        viscous=0 if cfg["visc_type"] else 1, blended=cfg["conv_type"], transient=0 if cfg["steady"] else 1,
-        dt=cfg["dt"], implicit_underrelax=cfg["implicit_underrelax"],
-        implicit_underrelax_max=cfg["implicit_underrelax_max"], ramp_iterations=cfg["ramp_iterations"],
-        shock_tube=cfg["init_shock_tube"], p0=cfg["p0"], t0=cfg["T0"], u0=tuple(cfg["u0"]), p2=cfg["p2"],
-        t2=cfg["T2"], bcs=bcs, method_momentum=method, method_pressure=0 if split else method,
-        method_enthalpy=method)
 
 
 def _check_history(hist, ref, rtol=1e-6):
@@ -64,21 +43,10 @@ def _close(a, b, rtol):
 def test_assembly_matches_reference_states(case):
     d = np.load(os.path.join(GOLDEN, f"flow_{case}.npz"))
     cfg = FLOW_META[case]
-    records = 5 if cfg["steady"] else d["dump_val"].shape[0]  # steady: under-relaxation ramps after step 1
     with flow.Flow(flow.mesh_from_npz(d), d["row_ptr"], d["col"], params_for(cfg)) as fl:
         fl.initialize()
-        n = fl.n
-        for k in range(records):
-            eq = k % 5
-            st = {name: d["state_" + name][k] for name in ("u", "u_old", "rho", "rho_old", "p", "p_old", "p_corr",
-                                                            "grad_p", "T", "h", "h_old", "ap_vec", "dt")}
-            st.update({name: d["state_" + name][k] for name in ("f_mdot", "f_rho", "f_qy", "f_stress", "b_T", "b_p",
-                                                                 "b_rho", "b_mdot", "b_h", "b_qy", "b_u")})
-            st["b_supersonic"] = d["state_b_super"][k]
-            if eq <= flow.MOMENTUM_Z:  # ap before this dimension's cell loop
-                st["ap"] = np.zeros(n) if eq == 0 else d["state_ap"][k - 1]
-            else:
-                st["ap"] = d["state_ap"][k]
+        for k in range(records(case, d)):
+            eq, st = state_at(d, k, fl.n)
             fl.set_state(**st)
             val, b, x0 = fl.assemble(eq)
             assert _close(val, d["dump_val"][k], 1e-12), (case, k)
@@ -168,3 +136,51 @@ def test_sod_200_steps_split():
     assert all(e <= 2e-9 for e in errs.values()), errs
     assert len(hist) == int(d["count"])
     _check_history(hist, d["res_hist"], rtol=1e-4)
+
+
+def _random_state(rng, n, F, B):
+    return dict(
+        u=rng.normal(size=(n, 3)), u_old=rng.normal(size=(n, 3)), rho=1.0 + rng.random(n),
+        rho_old=1.0 + rng.random(n), p=1e5 + 1e3 * rng.random(n), p_old=1e5 + 1e3 * rng.random(n),
+        p_corr=rng.normal(size=n), grad_p=rng.normal(size=(n, 3)), T=280.0 + 40.0 * rng.random(n),
+        h=3e5 + 1e4 * rng.random(n), h_old=3e5 + 1e4 * rng.random(n), dt=np.full(n, 1e-4),
+        ap=rng.random(n), ap_vec=rng.random((n, 3)),
+        f_mdot=rng.normal(size=F) * 1e-3, f_rho=1.0 + rng.random(F), f_qy=rng.random(F) * 1e-3,
+        f_stress=rng.normal(size=(F, 3)) * 1e-5,
+        b_T=280.0 + 40.0 * rng.random(B), b_p=1e5 + 1e3 * rng.random(B), b_rho=1.0 + rng.random(B),
+        b_mdot=rng.normal(size=B) * 1e-3, b_h=3e5 + 1e4 * rng.random(B), b_qy=rng.random(B) * 1e-3,
+        b_u=rng.normal(size=(B, 3)), b_supersonic=(rng.random(B) < 0.5).astype(np.int32))
+
+
+@pytest.mark.parametrize("viscous,blended,transient", [(1, 1, 1), (1, 0, 0), (0, 1, 0), (0, 0, 1)])
+def test_assembly_3d_matches_oracle(viscous, blended, transient):
+    """A permuted 10^3 hex mesh with every boundary kind (velocity inlet,
+    pressure outlet, no-slip and slip walls, symmetry, supersonic inlet) and a
+    random state: the GPU rows against the oracle's scatter loops."""
+    import oracle
+
+    oracle.set_threads(1)
+    rng = np.random.default_rng(11)
+    n = 10
+    perm = rng.permutation(n ** 3)
+    mesh, rp, col = gen_hex(n, perm=perm)
+    bcs = [dict(type=flow.VELOCITY_INLET, velocity=(3.0, 0.5, -0.2), temperature=300.0),
+           dict(type=flow.PRESSURE_OUTLET, pressure=1e5),
+           dict(type=flow.WALL, no_slip=1), dict(type=flow.WALL, no_slip=0),
+           dict(type=flow.SYMMETRY), dict(type=flow.SUPERSONIC_INLET, velocity=(400.0, 0.0, 0.0), pressure=1e5,
+                                          temperature=300.0)]
+    prm = flow.default_params(viscous=viscous, blended=blended, transient=transient, implicit_underrelax=0.6,
+                              bcs=bcs)
+    with flow.Flow(mesh, rp, col, prm) as fl:
+        st = _random_state(rng, fl.n, fl.F, fl.B)
+        for eq in range(5):
+            fl.set_state(**st)
+            val, b, x0 = fl.assemble(eq)
+            v_o, b_o, x_o, ap_o, apv_o = oracle.assemble(eq, mesh, rp, col, prm, st, urf=0.6)
+            assert _close(val, v_o, 1e-12), eq
+            assert _close(b, b_o, 1e-12), eq
+            assert np.array_equal(x0, x_o), eq
+            if eq < 3:
+                out = fl.get_state()
+                assert _close(out["ap"], ap_o, 1e-12), eq
+                assert _close(out["ap_vec"][:, eq], apv_o[:, eq], 1e-12), eq
