@@ -132,12 +132,27 @@ def headline(args, out):
     relm = float(np.linalg.norm(b - a.matvec(xm)) / np.linalg.norm(b))
     err = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
 
-    # dominant kernel: the fused CG SpMV (q = A p and its dot products), timed
-    # with HIP events on the handle's stream
-    spmv_ms_m, spmv_bytes_m = s.time_cg_spmv("mixed", reps=50)
-    spmv_ms_64, spmv_bytes_64 = s.time_cg_spmv("fp64", reps=50)
-    ach_m = spmv_bytes_m / (spmv_ms_m * 1e-3) / 1e9
-    ach_64 = spmv_bytes_64 / (spmv_ms_64 * 1e-3) / 1e9
+    # dominant kernel, timed with HIP events on the handle's stream: with one
+    # kernel per iteration, the odd iteration kernel (the even one, which
+    # leaves x alone, is reported beside it); with the split form, the CG
+    # SpMV (q = A p and its dot products)
+    def dominant(prec):
+        t = tuned[prec]
+        rpl = t["rows_per_lane"]
+        ty = "float, float" if prec == "mixed" else "double, double"
+        if t["cg_kernels"] == 1:
+            ms, by = s.time_cg_iteration(prec, pairs=50)
+            return {"kernel": f"k_cgf_iter<{ty}, {rpl}, 1>", "ms": ms[1], "bytes": by[1],
+                    "even": {"kernel": f"k_cgf_iter<{ty}, {rpl}, 0>", "avg_launch_us": round(ms[0] * 1e3, 2),
+                             "bytes_per_launch": by[0],
+                             "achieved": round(by[0] / (ms[0] * 1e-3) / 1e9, 1)}}
+        ms, by = s.time_cg_spmv(prec, reps=50)
+        name = f"k_cgm_spmv<{ty}, {rpl}>" if prec == "mixed" else f"k_cg_spmv<{ty}, {rpl}>"
+        return {"kernel": name, "ms": ms, "bytes": by}
+
+    dom_m, dom_64 = dominant("mixed"), dominant("fp64")
+    ach_m = dom_m["bytes"] / (dom_m["ms"] * 1e-3) / 1e9
+    ach_64 = dom_64["bytes"] / (dom_64["ms"] * 1e-3) / 1e9
 
     rm, r64 = repsm[-1], reps64[-1]
     gbs_m = rm["bytes"] / (rm["solve_ms"] * 1e-3) / 1e9
@@ -148,8 +163,7 @@ def headline(args, out):
         _log("bench: CPU baseline")
         cpu = cpu_baseline(a, b, r64["iterations"])
 
-    kname = f"k_cgm_spmv<float, float, {tuned['mixed']['rows_per_lane']}>"
-    kname64 = f"k_cg_spmv<double, double, {tuned['fp64']['rows_per_lane']}>"
+    kname, kname64 = dom_m["kernel"], dom_64["kernel"]
     out.update({
         "value": round(msm, 3),
         "ms_per_step": round(msm, 3),
@@ -169,11 +183,13 @@ def headline(args, out):
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(ach_m, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach_m / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname),
-                     "avg_launch_us": round(spmv_ms_m * 1e3, 2),
-                     "bytes_per_launch": spmv_bytes_m,
+                     "avg_launch_us": round(dom_m["ms"] * 1e3, 2),
+                     "bytes_per_launch": dom_m["bytes"],
+                     **({"even_kernel": dom_m["even"]} if "even" in dom_m else {}),
                      "fp64_kernel": {"kernel": kname64, "achieved": round(ach_64, 1),
-                                     "avg_launch_us": round(spmv_ms_64 * 1e3, 2),
-                                     "bytes_per_launch": spmv_bytes_64}},
+                                     "avg_launch_us": round(dom_64["ms"] * 1e3, 2),
+                                     "bytes_per_launch": dom_64["bytes"],
+                                     "traffic": measured_traffic(kname64)}},
         "cpu_baseline": cpu,
     })
 
@@ -284,7 +300,8 @@ def batch(args, out):
 def profile_kernels(args):
     """Launch each measured kernel alone (for rocprofv3 kernel traces and PMC
     passes): a 256 MiB device copy (the counters' calibration), the CG SpMV
-    in MIXED and FP64, and the sampled outer product, on the headline system."""
+    and the one-kernel CG iteration (even and odd) in MIXED and FP64, and the
+    sampled outer product, on the headline system."""
     import torch
 
     from opencslsolver_amd import gen
@@ -308,6 +325,10 @@ def profile_kernels(args):
         for prec in ("mixed", "fp64"):
             ms, by = s.time_cg_spmv(prec, reps=args.steps)
             res[f"cg_spmv_{prec}"] = {"avg_us": ms * 1e3, "bytes": by, "GBps": by / (ms * 1e-3) / 1e9}
+            ms2, by2 = s.time_cg_iteration(prec, pairs=args.steps)
+            for par, name in ((0, "even"), (1, "odd")):
+                res[f"cg_iter_{prec}_{name}"] = {"avg_us": ms2[par] * 1e3, "bytes": by2[par],
+                                                 "GBps": by2[par] / (ms2[par] * 1e-3) / 1e9}
         ms, by = s.time_sampled_outer(reps=args.steps)
         res["sampled_outer"] = {"avg_us": ms * 1e3, "bytes": by, "GBps": by / (ms * 1e-3) / 1e9}
     print(json.dumps(res), flush=True)

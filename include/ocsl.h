@@ -75,6 +75,10 @@ typedef struct {
     int grid_blocks;     /* workgroups of the CG SpMV kernels; 0 = default (512) */
     int vec_blocks;      /* workgroups of the CG vector-update kernels; 0 = as grid_blocks */
     int rows_per_lane;   /* rows per lane of the CG SELL-P layout, 1, 2 or 4; 0 = tuned, else 2 */
+    int cg_kernels;      /* tolerance-CG iteration form: 1 = one fused kernel per iteration (MIXED:
+                            with each block's row window of the direction staged in LDS);
+                            2 = split (SpMV + update for MIXED, three kernels for FP64);
+                            3 = one kernel, gathering from memory; 0 = tuned, else 2 */
 } ocsl_options;
 
 typedef struct {
@@ -171,15 +175,27 @@ ocsl_status ocsl_ILU_decompose(ocsl_handle h, int sweeps, double* m_out, ocsl_wh
 ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reorder, int rows_per_lane, int reps,
                               double* avg_ms, double* bytes_per_launch);
 
+/* Measurement hook (no counterpart in the reference): the one-kernel CG
+ * iteration (cg_kernels 1, or 3 when tuned so) of the handle's tuned launch.  After a true
+ * residual from x = 0 and the first q = A p, runs `pairs` (even, odd) pairs of
+ * iteration kernels with a HIP event between every two launches; avg_ms[0] /
+ * avg_ms[1] receive the average duration of the even / odd kernel and bytes[0]
+ * / bytes[1] the algorithmic bytes of one launch: the matrix as for
+ * ocsl_time_cg_spmv, r, q, p read once and written once (6*n*sv), and for the
+ * odd kernel x read and written and the previous direction read (16*n +
+ * n*sv).  Values and b must be set; the handle's x is left unchanged. */
+ocsl_status ocsl_time_cg_iteration(ocsl_handle h, int precision, int pairs, double* avg_ms, double* bytes);
+
 /* Autotuner (no counterpart in the reference): runs `iterations` iterations
  * of the TOLERANCE-mode CG solve of the handle's system for every launch
- * candidate -- rows per lane {1, 2, 4} x threads per workgroup {256, 512, 1024}
- * x SpMV workgroups {256 .. 2048} x vector-update workgroups -- timed with
- * HIP events, and keeps the fastest in the handle for later solves of that
- * precision whose options leave those fields at 0.  best (optional) receives it; table
- * (optional) every candidate's time per iteration.  x is preserved. */
+ * candidate -- iteration form {1, 2, 3} x rows per lane {1, 2, 4} x threads
+ * per workgroup {256, 512, 1024} x SpMV workgroups {256 .. 2048} x (split
+ * form) vector-update workgroups -- timed with HIP events, and keeps the
+ * fastest in the handle for later solves of that precision whose options
+ * leave those fields at 0.  best (optional) receives it; table (optional)
+ * every candidate's time per iteration.  x is preserved. */
 typedef struct {
-    int rows_per_lane, block_size, grid_blocks, vec_blocks;
+    int rows_per_lane, block_size, grid_blocks, vec_blocks, cg_kernels;
     double us_per_iteration;
 } ocsl_tune_entry;
 ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations, ocsl_options* best,

@@ -48,6 +48,7 @@ class Options(ctypes.Structure):
         ("grid_blocks", ctypes.c_int),
         ("vec_blocks", ctypes.c_int),
         ("rows_per_lane", ctypes.c_int),
+        ("cg_kernels", ctypes.c_int),
     ]
 
 
@@ -69,7 +70,7 @@ class Report(ctypes.Structure):
 
 class TuneEntry(ctypes.Structure):
     _fields_ = [("rows_per_lane", ctypes.c_int), ("block_size", ctypes.c_int), ("grid_blocks", ctypes.c_int),
-                ("vec_blocks", ctypes.c_int), ("us_per_iteration", ctypes.c_double)]
+                ("vec_blocks", ctypes.c_int), ("cg_kernels", ctypes.c_int), ("us_per_iteration", ctypes.c_double)]
 
 
 # name -> (restype, argtypes)
@@ -99,6 +100,7 @@ SIGNATURES = {
     "ocsl_ILU_decompose": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
     "ocsl_time_cg_spmv": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _f64p, _f64p]),
+    "ocsl_time_cg_iteration": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, _f64p, _f64p]),
     "ocsl_get_stream": (ctypes.c_void_p, [c_handle]),
     "ocsl_tune_cg": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Options), _f64p,
                                     ctypes.POINTER(TuneEntry), ctypes.c_int, _i32p]),

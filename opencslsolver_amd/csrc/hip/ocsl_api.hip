@@ -217,6 +217,7 @@ void ocsl_default_options(ocsl_options* o) {
     o->grid_blocks = 0;
     o->vec_blocks = 0;
     o->rows_per_lane = 0;
+    o->cg_kernels = 0;
 }
 
 int ocsl_device_count(void) {

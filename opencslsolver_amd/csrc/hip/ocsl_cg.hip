@@ -2,15 +2,17 @@
 //   cg_reference : CMatrix_solver::CG_solve (cmatrix_solver.cpp:635-680) with
 //                  its 4-iteration cap, absolute 1e-6 stop and two SpMVs per
 //                  iteration, FP64, input row order.
-//   cg_tolerance : textbook CG to a relative residual, three fused kernels per
-//                  iteration with device-resident scalars, RCM row order,
-//                  iteration chunks replayed as hipGraphs.  MIXED runs the
+//   cg_tolerance : textbook CG to a relative residual with device-resident
+//                  scalars, RCM row order, iteration chunks replayed as
+//                  hipGraphs; one fused kernel per iteration, or the split
+//                  form (three kernels, two for MIXED).  MIXED runs the
 //                  iterations in FP32 (matrix in difference form, r, p, q)
 //                  with FP64 dot accumulation, the solution accumulated in
 //                  FP64 every iteration and the FP64 true residual b - A x
 //                  replacing the FP32 recurrence residual whenever the latter
 //                  has dropped by `replace_ratio` (reliable updates).
 #include <cmath>
+#include <type_traits>
 #include <cstdio>
 
 #include "ocsl_handle.h"
@@ -112,6 +114,16 @@ void cg_reference(ocsl_solver* h, ocsl_report* rep) {
     rep->bytes = it * 2.0 * (nnz * 12.0 + 16.0 * n);
 }
 
+// Algorithmic bytes per launch of the tolerance-CG kernels (see cg_bytes).
+struct CgBytes {
+    double split;       // one iteration of the split form
+    double fused_even;  // one-kernel iteration without the x update
+    double fused_odd;   // with it
+    double first;       // k_cgf_first
+    double residual;    // k_cg_residual
+};
+CgBytes cg_bytes(const ocsl_solver* h, const SellP& s, bool mixed);
+
 namespace {
 
 struct CgLaunch {
@@ -159,23 +171,56 @@ void launch_iter(ocsl_solver* h, const CgArgs<V, X>& c, int n, CgLaunch L, int k
     }
 }
 
-// `chunk` iterations, replayed as one hipGraph when use_graph.  Kernel arguments are fixed per handle: every buffer
+// Slices per wave of the one-kernel launch over a SELL-P layout.
+int slices_per_wave(int n_slices, CgLaunch L) {
+    const int spb = (n_slices + L.grid - 1) / L.grid;
+    const int waves = L.block / kWave;
+    return (spb + waves - 1) / waves;
+}
+
+// One-kernel iteration k of a chunk (parity = k & 1).  kLds (FP32): the
+// LDS-window kernel, instantiated for up to 4 slices per wave; a launch with
+// more takes the gathering kernel.
+template <int R, bool kLds, class V, class X>
+void launch_iter_f(ocsl_solver* h, const CgfArgs<V, X>& c, CgLaunch L, int spw, int k, int kstop) {
+    const bool odd = k & 1;
+    if constexpr (kLds && sizeof(X) == 4) {
+        if (spw <= 4) {
+            auto go = [&](auto SPW) {
+                constexpr int kSpw = decltype(SPW)::value;
+                if (odd) k_cgl_iter<R, 1, kSpw><<<L.grid, L.block, 0, h->stream>>>(c, k, kstop);
+                else k_cgl_iter<R, 0, kSpw><<<L.grid, L.block, 0, h->stream>>>(c, k, kstop);
+            };
+            if (spw <= 1) go(std::integral_constant<int, 1>{});
+            else if (spw <= 2) go(std::integral_constant<int, 2>{});
+            else go(std::integral_constant<int, 4>{});
+            return;
+        }
+    }
+    if (odd) k_cgf_iter<V, X, R, 1><<<L.grid, L.block, 0, h->stream>>>(c, k, kstop);
+    else k_cgf_iter<V, X, R, 0><<<L.grid, L.block, 0, h->stream>>>(c, k, kstop);
+}
+
+// `chunk` iterations (launch(k) issues iteration k), replayed as one hipGraph
+// when use_graph.  Kernel arguments are fixed per handle: every buffer
 // involved is allocated once at the handle's size (work_buf drops the cached
 // graphs if one is reallocated).
-template <int R, class V, class X>
-void run_chunk(ocsl_solver* h, const CgArgs<V, X>& c, int n, int chunk, bool use_graph, int kind,
-               CgLaunch L) {
+template <class F>
+void run_chunk(ocsl_solver* h, int chunk, bool use_graph, int kind, int R, CgLaunch L, F&& launch) {
     if (!use_graph) {
-        for (int k = 0; k < chunk; ++k) launch_iter<R>(h, c, n, L, k);
+        for (int k = 0; k < chunk; ++k) launch(k);
         OCSL_HIP(hipGetLastError());
         return;
     }
-    auto key = std::make_pair(kind + 4 * L.grid + (1 << 13) * L.vgrid + (1 << 25) * (L.block / kWave), chunk * 8 + R);
+    // kind < 64, grid and vgrid <= 2048, block / 64 <= 16
+    const int64_t k1 = (int64_t)1 << 48 | (int64_t)kind << 40 | (int64_t)L.grid << 24 | (int64_t)L.vgrid << 8 |
+                       (L.block / kWave);
+    auto key = std::make_pair(k1, chunk * 8 + R);
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
         hipGraph_t gr;
         OCSL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < chunk; ++k) launch_iter<R>(h, c, n, L, k);
+        for (int k = 0; k < chunk; ++k) launch(k);
         OCSL_HIP(hipStreamEndCapture(h->stream, &gr));
         hipGraphExec_t ex;
         OCSL_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
@@ -200,37 +245,50 @@ int cg_rows_per_lane(const ocsl_solver* h, bool mixed, const ocsl_options& o) {
     return t.rows_per_lane > 0 ? t.rows_per_lane : 2;
 }
 
-// Launch options left at 0 take the handle's tuned values for the precision.
+// Launch options left at 0 take the handle's tuned values for the precision;
+// the iteration form defaults to the split kernels.
 ocsl_options with_tuned(const ocsl_solver* h, bool mixed, const ocsl_options& o) {
     const ocsl_solver_tuned& tu = h->tuned[mixed ? 1 : 0];
     ocsl_options t = o;
     if (t.block_size <= 0) t.block_size = tu.block_size;
     if (t.grid_blocks <= 0) t.grid_blocks = tu.grid_blocks;
     if (t.vec_blocks <= 0) t.vec_blocks = tu.vec_blocks;
+    if (t.cg_kernels <= 0) t.cg_kernels = tu.cg_kernels > 0 ? tu.cg_kernels : 2;
     return t;
 }
 
 void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_report* rep) {
     const ocsl_options o = with_tuned(h, mixed, o_in);
+    OCSL_REQUIRE(o.cg_kernels >= 1 && o.cg_kernels <= 3, OCSL_ERR_ARG, "cg_kernels must be 0, 1, 2 or 3");
+    const bool fused = o.cg_kernels != 2;
+    const bool lds = mixed && o.cg_kernels == 1;  // FP32 directions fit the LDS window
     SellP& s = sellp_for(h, o.reorder != 0, cg_rows_per_lane(h, mixed, o));
     ensure_sellp_values(h, s, true, mixed);
     const int n = s.n;
     const int* perm = s.identity ? nullptr : s.perm.p;
     Work& w = h->work;
-    // b, x in FP64 (d5, d6); r, p, q in the Krylov precision
+    // b, x in FP64 (d5, d6); r, p, q in the Krylov precision, and for the
+    // one-kernel form their second buffers (MIXED f3..f5, FP64 d10, d12, d13)
     double* b = work_buf(h, w.d[5], n);
     double* x = work_buf(h, w.d[6], n);
+    work_buf(h, h->hist, (size_t)o.max_iter + 2);
     CgArgs<double, double> c64{};
     CgArgs<float, float> c32{};
+    CgfArgs<double, double> f64{};
+    CgfArgs<float, float> f32{};
     if (mixed) {
         c32 = {s.view<float>(), work_buf(h, w.f[0], n), work_buf(h, w.f[1], n), work_buf(h, w.f[2], n),
-               work_buf(h, w.f[3], n), x, h->S.p, nullptr, h->red()};
+               work_buf(h, w.f[3], n), x, h->S.p, h->hist.p, h->red()};
+        if (fused)
+            f32 = {s.view<float>(), {c32.r, work_buf(h, w.f[4], n)}, {c32.q, work_buf(h, w.f[5], n)},
+                   {c32.p, c32.p_alt}, x, h->S.p, h->hist.p, h->red()};
     } else {
         c64 = {s.view<double>(), work_buf(h, w.d[7], n), work_buf(h, w.d[8], n), work_buf(h, w.d[9], n), nullptr, x,
-               h->S.p, nullptr, h->red()};
+               h->S.p, h->hist.p, h->red()};
+        if (fused)
+            f64 = {s.view<double>(), {c64.r, work_buf(h, w.d[12], n)}, {c64.q, work_buf(h, w.d[13], n)},
+                   {c64.p, work_buf(h, w.d[10], n)}, x, h->S.p, h->hist.p, h->red()};
     }
-    work_buf(h, h->hist, (size_t)o.max_iter + 2);
-    c32.hist = c64.hist = h->hist.p;
 
     Scalars init{};
     init.tol2 = o.tol * o.tol;
@@ -267,31 +325,62 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     };
     residual(1);
     read_scalars(h);
-    int replacements = 0;
-    const int kind = (mixed ? 1 : 0) + (s.identity ? 0 : 2);
-    // MIXED chunks hold whole (even, odd) pairs
-    const int every = mixed ? (o.check_every + 1) / 2 * 2 : o.check_every;
+    int replacements = 0, firsts = 0;
+    bool need_first = fused;  // q = A p and the sums of buffer 0 after every true residual
+    const int kind = (mixed ? 1 : 0) + (s.identity ? 0 : 2) + (fused ? 4 : 0) + (lds ? 8 : 0);
+    // MIXED chunks, and every one-kernel chunk, hold whole (even, odd) pairs
+    const bool pairs = mixed || fused;
+    const int every = pairs ? (o.check_every + 1) / 2 * 2 : o.check_every;
     while (!h->S_host->done) {
-        const int left = o.max_iter - h->S_host->iter;
-        const bool graph = o.use_graph && left >= every;
+        const int iter0 = h->S_host->iter;
+        const int left = o.max_iter - iter0;
+        const bool graph = o.use_graph && left > every;
         int chunk = graph ? every : std::max(1, std::min(every, left));
-        if (mixed) chunk = (chunk + 1) / 2 * 2;  // iterations past max_iter stop on their own
+        if (pairs) {
+            // the kernel after the last allowed iteration records its residual
+            // and stops; iterations past max_iter stop on their own
+            if (!graph && chunk == left) ++chunk;
+            chunk = (chunk + 1) / 2 * 2;
+        }
+        const int kstop = std::min(chunk, left);
         with_rpl(s.rpl, [&](auto RC) {
             constexpr int R = decltype(RC)::value;
-            if (mixed) run_chunk<R>(h, c32, n, chunk, graph, kind, L);
-            else run_chunk<R>(h, c64, n, chunk, graph, kind, L);
+            if (fused) {
+                if (need_first) {
+                    if (mixed) k_cgf_first<float, float, R><<<L.grid, L.block, 0, h->stream>>>(f32);
+                    else k_cgf_first<double, double, R><<<L.grid, L.block, 0, h->stream>>>(f64);
+                    need_first = false;
+                    ++firsts;
+                }
+                const int spw = slices_per_wave(s.n_slices, L);
+                if (lds)
+                    run_chunk(h, chunk, graph, kind, R, L,
+                              [&](int k) { launch_iter_f<R, true>(h, f32, L, spw, k, kstop); });
+                else if (mixed)
+                    run_chunk(h, chunk, graph, kind, R, L,
+                              [&](int k) { launch_iter_f<R, false>(h, f32, L, spw, k, kstop); });
+                else
+                    run_chunk(h, chunk, graph, kind, R, L,
+                              [&](int k) { launch_iter_f<R, false>(h, f64, L, spw, k, kstop); });
+            } else if (mixed) {
+                run_chunk(h, chunk, graph, kind, R, L, [&](int k) { launch_iter<R>(h, c32, n, L, k); });
+            } else {
+                run_chunk(h, chunk, graph, kind, R, L, [&](int k) { launch_iter<R>(h, c64, n, L, k); });
+            }
         });
         read_scalars(h);
         const Scalars& S = *h->S_host;
         if (S.nonfinite || S.breakdown) break;
         if (mixed && (S.need_update || S.done)) {
-            if (S.iter & 1) {
-                // a pair was cut after its even update: x already holds that
-                // step, the current direction sits in the second buffer
+            if ((S.iter - iter0) & 1) {
+                // the chunk stopped after an odd number of iterations: x
+                // holds every step, the current direction sits in the second
+                // buffer
                 OCSL_HIP(hipMemcpyAsync(c32.p, c32.p_alt, sizeof(float) * n, hipMemcpyDeviceToDevice, h->stream));
             }
             residual(0);
             ++replacements;
+            need_first = fused;
             read_scalars(h);
         }
         if (h->S_host->iter >= o.max_iter) break;
@@ -321,21 +410,35 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     const bool converged = mixed ? (tr <= o.tol) : (S.rr_last <= S.tol2 * scale);
     if (!converged && st == OCSL_OK) st |= OCSL_WARN_MAXITER;
     rep->status = st;
-    // algorithmic bytes of an iteration: the SpMV (off-diagonal values and
-    // columns -- 2 or 4 bytes each --, the diagonal, slice pointers, p
-    // gathered, q written) and the vector updates.  FP64 (three kernels):
-    // r, q read, r written; x read/written, p, r read, p written.  MIXED (two
-    // kernels): r read in the SpMV, then the paired update below.
+    const CgBytes cb = cg_bytes(h, s, mixed);
+    const double it_bytes = fused ? 0.5 * (cb.fused_even + cb.fused_odd) : cb.split;
+    rep->bytes = iters * it_bytes + replacements * cb.residual + firsts * cb.first;
+}
+
+// Algorithmic bytes of the tolerance-CG kernels.  The SpMV part: off-diagonal
+// values and columns (2 or 4 bytes each), the diagonal, slice pointers, the
+// gathered vector once.  sv = bytes of a Krylov vector element.
+CgBytes cg_bytes(const ocsl_solver* h, const SellP& s, bool mixed) {
+    const double n = s.n;
     const double off = (double)h->host.col.size() - n;
     const double sv = mixed ? 4.0 : 8.0;
     const double cb = s.compact ? 2.0 : 4.0;
-    const double spmv_bytes = off * (sv + cb) + n * sv + 4.0 * (s.n_slices + 1) + 2.0 * n * sv;
-    // MIXED update, per pair: even r, q, p read, r, p' written (5 sv); odd
-    // the same plus x (FP64) read and written and p_prev read (16 + 6 sv)
-    const double it_bytes = mixed ? spmv_bytes + n * sv + (8.0 + 5.5 * sv) * n
-                                  : spmv_bytes + 3.0 * n * sv + (16.0 + 3.0 * sv) * n;
-    const double repl_bytes = off * (8.0 + cb) + 8.0 * n + 4.0 * (s.n_slices + 1) + n * (8.0 + 8.0 + sv);
-    rep->bytes = iters * it_bytes + replacements * repl_bytes;
+    const double mat = off * (sv + cb) + n * sv + 4.0 * (s.n_slices + 1);
+    CgBytes r;
+    // split: SpMV (p gathered, q written); FP64: K2 r, q read, r written; K3
+    // x read/written, p, r read, p written.  MIXED: r read in the SpMV, then
+    // per pair: even r, q, p read, r, p' written (5 sv); odd the same plus x
+    // (FP64) read and written and p_prev read (16 + 6 sv)
+    r.split = mixed ? mat + 2.0 * n * sv + n * sv + (8.0 + 5.5 * sv) * n
+                    : mat + 2.0 * n * sv + 3.0 * n * sv + (16.0 + 3.0 * sv) * n;
+    // one kernel: r, q, p read and written; odd adds x read/written and p_prev
+    r.fused_even = mat + 6.0 * n * sv;
+    r.fused_odd = r.fused_even + (16.0 + sv) * n;
+    // k_cgf_first: the matrix, p, r read, q written
+    r.first = mat + 3.0 * n * sv;
+    // true residual: FP64 matrix, x gathered, b read, r (and p) written
+    r.residual = off * (8.0 + cb) + 8.0 * n + 4.0 * (s.n_slices + 1) + n * (8.0 + 8.0 + sv);
+    return r;
 }
 
 template <int R, class V, class X>
@@ -410,6 +513,118 @@ extern "C" ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reord
     }
 }
 
+namespace ocsl {
+
+// Pairs of one-kernel iterations on the handle's system from x = 0, with an
+// event between every two launches; ms[0] / ms[1]: average even / odd kernel.
+template <int R, bool kLds, class V, class X>
+static void time_iteration_pairs(ocsl_solver* h, SellP& s, const ocsl_options& o, int pairs, double (&ms)[2]) {
+    const int n = s.n;
+    const int* perm = s.identity ? nullptr : s.perm.p;
+    const CgLaunch L = cg_launch(s.n_slices, n, o);
+    Work& w = h->work;
+    DBuf<double> b, x;
+    b.alloc(n);
+    x.alloc(n);
+    X* buf[6];
+    for (int i = 0; i < 6; ++i) {
+        if constexpr (sizeof(X) == 4) buf[i] = work_buf(h, w.f[i], n);
+        else buf[i] = work_buf(h, w.d[7 + i], n);
+    }
+    const CgfArgs<V, X> f{s.view<V>(), {buf[0], buf[3]}, {buf[2], buf[4]}, {buf[1], buf[5]}, x.p, h->S.p, nullptr,
+                          h->red()};
+    Scalars init{};
+    init.tol2 = 0.0;
+    init.max_iter = 1 << 30;
+    init.replace_ratio = 0.0;
+    OCSL_HIP(hipMemcpyAsync(h->S.p, &init, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
+    k_permute_in<double><<<grid_for(n), kBlock, 0, h->stream>>>(n, perm, h->b_dev, b.p);
+    k_fill<double><<<grid_for(n), kBlock, 0, h->stream>>>(n, 0.0, x.p);
+    float* r32 = nullptr;
+    float* p32 = nullptr;
+    double* r64 = nullptr;
+    double* p64 = nullptr;
+    if constexpr (sizeof(X) == 4) {
+        r32 = f.r[0];
+        p32 = f.p[0];
+    } else {
+        r64 = f.r[0];
+        p64 = f.p[0];
+    }
+    k_cg_residual<sizeof(X) == 4, R><<<L.grid, L.block, 0, h->stream>>>(s.view<double>(), b.p, x.p, r64, r32, p64,
+                                                                          p32, h->red(), 1);
+    k_finalize<2, FinCgResidual><<<1, kBlock, 0, h->stream>>>(h->red(), L.grid,
+                                                              FinCgResidual{h->S.p, nullptr, 1, L.grid}, kSlotRR);
+    k_cgf_first<V, X, R><<<L.grid, L.block, 0, h->stream>>>(f);
+    OCSL_HIP(hipGetLastError());
+    std::vector<hipEvent_t> ev(2 * pairs + 1, nullptr);
+    try {
+        for (auto& e : ev) OCSL_HIP(hipEventCreate(&e));
+        OCSL_HIP(hipEventRecord(ev[0], h->stream));
+        const int spw = slices_per_wave(s.n_slices, L);
+        for (int i = 0; i < pairs; ++i) {
+            launch_iter_f<R, kLds>(h, f, L, spw, 0, 2);
+            OCSL_HIP(hipEventRecord(ev[2 * i + 1], h->stream));
+            launch_iter_f<R, kLds>(h, f, L, spw, 1, 2);
+            OCSL_HIP(hipEventRecord(ev[2 * i + 2], h->stream));
+        }
+        OCSL_HIP(hipGetLastError());
+        sync(h);
+        double tot[2] = {0.0, 0.0};
+        for (int i = 0; i < 2 * pairs; ++i) {
+            float m = 0.f;
+            OCSL_HIP(hipEventElapsedTime(&m, ev[i], ev[i + 1]));
+            tot[i & 1] += m;
+        }
+        ms[0] = tot[0] / pairs;
+        ms[1] = tot[1] / pairs;
+        Scalars S{};
+        OCSL_HIP(hipMemcpy(&S, h->S.p, sizeof(Scalars), hipMemcpyDeviceToHost));
+        OCSL_REQUIRE(S.iter == 2 * pairs, OCSL_ERR_STATE,
+                     "timed iterations stopped early (" + std::to_string(S.iter) + " of " +
+                         std::to_string(2 * pairs) + ")");
+    } catch (...) {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+}
+
+}  // namespace ocsl
+
+extern "C" ocsl_status ocsl_time_cg_iteration(ocsl_handle h, int precision, int pairs, double* avg_ms,
+                                              double* bytes) {
+    using namespace ocsl;
+    if (!h || !avg_ms || !bytes || pairs <= 0 || (precision != OCSL_FP64 && precision != OCSL_MIXED))
+        return OCSL_ERR_ARG;
+    try {
+        OCSL_REQUIRE(h->have_vals && h->have_b, OCSL_ERR_STATE, "values or rhs not set");
+        ocsl_options o;
+        ocsl_default_options(&o);
+        const bool mixed = precision == OCSL_MIXED;
+        o = with_tuned(h, mixed, o);
+        SellP& s = sellp_for(h, o.reorder != 0, cg_rows_per_lane(h, mixed, o));
+        ensure_sellp_values(h, s, true, mixed);
+        double ms[2] = {0.0, 0.0};
+        with_rpl(s.rpl, [&](auto RC) {
+            constexpr int R = decltype(RC)::value;
+            if (mixed && o.cg_kernels != 3) time_iteration_pairs<R, true, float, float>(h, s, o, pairs, ms);
+            else if (mixed) time_iteration_pairs<R, false, float, float>(h, s, o, pairs, ms);
+            else time_iteration_pairs<R, false, double, double>(h, s, o, pairs, ms);
+        });
+        const CgBytes cb = cg_bytes(h, s, mixed);
+        avg_ms[0] = ms[0];
+        avg_ms[1] = ms[1];
+        bytes[0] = cb.fused_even;
+        bytes[1] = cb.fused_odd;
+        return OCSL_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what);
+        return e.status;
+    }
+}
+
 // Autotuner of the tolerance-CG launch: every candidate (rows per lane x
 // threads per workgroup x SpMV workgroups x vector workgroups) runs
 // `iterations` iterations of the real solve on the handle's system, timed
@@ -431,6 +646,7 @@ extern "C" ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations
         } else {
             h->x_own.alloc(n);
         }
+        const int kernels[] = {1, 2, 3};
         const int rpls[] = {1, 2, 4};
         const int blocks[] = {256, 512, 1024};
         const int grids[] = {256, 512, 1024, 2048};
@@ -440,35 +656,39 @@ extern "C" ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations
         int count = 0;
         const int pi = precision == OCSL_MIXED ? 1 : 0;
         h->tuned[pi] = ocsl_solver_tuned{};
-        for (int rpl : rpls)
-            for (int blk : blocks)
-                for (int gr : grids)
-                    for (int vg : vgrids) {
-                        if (vg != 0 && (vg == gr || (vg > 256 && vg < gr))) continue;
-                        ocsl_options o;
-                        ocsl_default_options(&o);
-                        o.tol = 0.0;  // run every iteration
-                        o.max_iter = iterations;
-                        o.check_every = std::min(iterations, 32);
-                        o.rows_per_lane = rpl;
-                        o.block_size = blk;
-                        o.grid_blocks = gr;
-                        o.vec_blocks = vg;
-                        OCSL_HIP(hipMemsetAsync(h->x_own.p, 0, sizeof(double) * n, h->stream));
-                        ocsl_report r{};
-                        cg_tolerance(h, precision == OCSL_MIXED, o, &r);  // warm: layout, graphs
-                        OCSL_HIP(hipMemsetAsync(h->x_own.p, 0, sizeof(double) * n, h->stream));
-                        cg_tolerance(h, precision == OCSL_MIXED, o, &r);
-                        const double us = r.iterations > 0 ? r.solve_ms * 1e3 / r.iterations : 1e300;
-                        if (table && count < table_cap)
-                            table[count] = ocsl_tune_entry{rpl, blk, gr, vg, us};
-                        ++count;
-                        if (us < best_us) {
-                            best_us = us;
-                            bo = o;
+        for (int kn : kernels)
+            for (int rpl : rpls)
+                for (int blk : blocks)
+                    for (int gr : grids)
+                        for (int vg : vgrids) {
+                            if (vg != 0 && (kn != 2 || vg == gr || (vg > 256 && vg < gr))) continue;
+                            if (kn == 3 && precision != OCSL_MIXED) continue;  // FP64: 1 and 3 coincide
+                            ocsl_options o;
+                            ocsl_default_options(&o);
+                            o.tol = 0.0;  // run every iteration
+                            o.max_iter = iterations;
+                            o.check_every = std::min(iterations, 32);
+                            o.cg_kernels = kn;
+                            o.rows_per_lane = rpl;
+                            o.block_size = blk;
+                            o.grid_blocks = gr;
+                            o.vec_blocks = vg;
+                            OCSL_HIP(hipMemsetAsync(h->x_own.p, 0, sizeof(double) * n, h->stream));
+                            ocsl_report r{};
+                            cg_tolerance(h, precision == OCSL_MIXED, o, &r);  // warm: layout, graphs
+                            OCSL_HIP(hipMemsetAsync(h->x_own.p, 0, sizeof(double) * n, h->stream));
+                            cg_tolerance(h, precision == OCSL_MIXED, o, &r);
+                            const double us = r.iterations > 0 ? r.solve_ms * 1e3 / r.iterations : 1e300;
+                            if (table && count < table_cap)
+                                table[count] = ocsl_tune_entry{rpl, blk, gr, vg, kn, us};
+                            ++count;
+                            if (us < best_us) {
+                                best_us = us;
+                                bo = o;
+                            }
                         }
-                    }
-        h->tuned[pi] = ocsl_solver_tuned{bo.rows_per_lane, bo.block_size, bo.grid_blocks, bo.vec_blocks};
+        h->tuned[pi] =
+            ocsl_solver_tuned{bo.rows_per_lane, bo.block_size, bo.grid_blocks, bo.vec_blocks, bo.cg_kernels};
         drop_graphs(h);
         if (had_x)
             OCSL_HIP(hipMemcpyAsync(h->x_own.p, saved.p, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
@@ -479,6 +699,7 @@ extern "C" ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations
             best->block_size = bo.block_size;
             best->grid_blocks = bo.grid_blocks;
             best->vec_blocks = bo.vec_blocks;
+            best->cg_kernels = bo.cg_kernels;
         }
         if (best_us_per_iter) *best_us_per_iter = best_us;
         if (table_len) *table_len = count;
@@ -493,11 +714,10 @@ extern "C" ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations
 extern "C" ocsl_status ocsl_set_tuned_cg(ocsl_handle h, int precision, const ocsl_options* o) {
     if (!h || !o || (precision != OCSL_FP64 && precision != OCSL_MIXED)) return OCSL_ERR_ARG;
     if (o->rows_per_lane < 0 || o->rows_per_lane == 3 || o->rows_per_lane > 4 || o->block_size < 0 ||
-        o->block_size > 1024 ||
-        o->grid_blocks < 0 || o->vec_blocks < 0)
+        o->block_size > 1024 || o->grid_blocks < 0 || o->vec_blocks < 0 || o->cg_kernels < 0 || o->cg_kernels > 3)
         return OCSL_ERR_ARG;
     h->tuned[precision == OCSL_MIXED ? 1 : 0] =
-        ocsl_solver_tuned{o->rows_per_lane, o->block_size, o->grid_blocks, o->vec_blocks};
+        ocsl_solver_tuned{o->rows_per_lane, o->block_size, o->grid_blocks, o->vec_blocks, o->cg_kernels};
     ocsl::drop_graphs(h);
     return OCSL_OK;
 }

@@ -84,6 +84,7 @@ struct Scalars {
     int hist_cap;
     int lag;            // MIXED CG: x lags the last even iteration's alpha p
     double alpha_prev;  // MIXED CG: alpha of that even iteration
+    int fstop[2];       // one-kernel CG: stop flag written by the kernels of each parity
 };
 
 // Scratch of the deterministic grid reductions: one partial sum per block
@@ -272,6 +273,8 @@ struct SellPView {
     const unsigned* colw;   // packed 16-bit column offsets, or null
     const V* val;
     const V* diag;          // a_ii, or the row sum in the FP32 difference form
+    const int* win_lo;      // per slice: smallest row the slice's rows touch (own rows, stored columns)
+    const int* win_hi;      // per slice: largest such row
 };
 
 template <class V>
@@ -309,6 +312,17 @@ __device__ __forceinline__ void load_rows(const T* __restrict__ a, int row0, int
     }
 }
 template <int R, class T>
+__device__ __forceinline__ void load_rows_t(const T* __restrict__ a, int row0, int n, T (&out)[R]) {
+    if (row0 + R <= n) {
+        const Pack<T, R> v = ldp<T, R>(a + row0);
+#pragma unroll
+        for (int j = 0; j < R; ++j) out[j] = v.v[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) out[j] = row0 + j < n ? a[row0 + j] : T(0);
+    }
+}
+template <int R, class T>
 __device__ __forceinline__ void store_rows(T* __restrict__ a, int row0, int n, const T (&v)[R]) {
     if (row0 + R <= n) {
         Pack<T, R> pk;
@@ -329,16 +343,17 @@ __device__ __forceinline__ void store_rows(T* __restrict__ a, int row0, int n, c
 // difference form, diag = row sum s_i: s_i x_i + sum_off a_ij (x_j - x_i);
 // otherwise a_ii x_i + sum_off a_ij x_j.  Arithmetic in FP64 whatever the
 // storage; per row the terms are added in the same order for every R.
-template <class V, class X, bool kDiff, int R>
-__device__ __forceinline__ void sellp_rows(const SellPView<V>& A, int s, int lane, int row0,
-                                           const X* __restrict__ x, const double (&xi)[R], double (&acc)[R]) {
+// gather(c) returns the vector's element c as a double (a plain load, or a
+// value formed from several vectors).
+template <class V, bool kDiff, int R, class G, int kPairs = (R == 1 ? 4 : 2)>
+__device__ __forceinline__ void sellp_rows_g(const SellPView<V>& A, int s, int lane, int row0, const G& gather,
+                                             const double (&xi)[R], double (&acc)[R]) {
     const int base = A.slice_ptr[s];
     const int np = (A.slice_ptr[s + 1] - base) / (128 * R);  // pairs per row
     double dg[R];
     load_rows<R>(A.diag, row0, A.n, dg);
 #pragma unroll
     for (int j = 0; j < R; ++j) acc[j] = dg[j] * xi[j];
-    constexpr int kPairs = R == 1 ? 4 : 2;
     for (int t0 = 0; t0 < np; t0 += kPairs) {
         int c[kPairs][2 * R];
         V v[kPairs][2 * R];
@@ -377,12 +392,18 @@ __device__ __forceinline__ void sellp_rows(const SellPView<V>& A, int s, int lan
 #pragma unroll
                     for (int e = 0; e < 2; ++e) {
                         const double a = (double)v[jj][2 * j + e];
-                        const double xj = (double)x[c[jj][2 * j + e]];
+                        const double xj = gather(c[jj][2 * j + e]);
                         if constexpr (kDiff) acc[j] += a * (xj - xi[j]);
                         else acc[j] += a * xj;
                     }
             }
     }
+}
+
+template <class V, class X, bool kDiff, int R>
+__device__ __forceinline__ void sellp_rows(const SellPView<V>& A, int s, int lane, int row0,
+                                           const X* __restrict__ x, const double (&xi)[R], double (&acc)[R]) {
+    sellp_rows_g<V, kDiff, R>(A, s, lane, row0, [x](int c) { return (double)x[c]; }, xi, acc);
 }
 
 // One row per lane (R = 1).

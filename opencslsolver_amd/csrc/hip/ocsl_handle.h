@@ -46,6 +46,7 @@ struct SellP {
     uint64_t epoch64 = ~0ull, epoch32 = ~0ull;  // value epochs of val64 / val32
     int64_t total = 0;                      // padded off-diagonal entries
     DBuf<int> slice_ptr, col, src, diag_src, perm;  // perm: new -> old (empty for identity)
+    DBuf<int> win_lo, win_hi;               // per slice: row window of its SpMV (own rows and columns)
     DBuf<unsigned> colw;                    // packed 16-bit column offsets (compact)
     bool compact = false;
     DBuf<double> val64, diag64;
@@ -56,11 +57,13 @@ struct SellP {
 };
 template <>
 inline SellPView<double> SellP::view<double>() const {
-    return {n, n_slices, slice_ptr.p, compact ? nullptr : col.p, compact ? colw.p : nullptr, val64.p, diag64.p};
+    return {n, n_slices, slice_ptr.p, compact ? nullptr : col.p, compact ? colw.p : nullptr, val64.p, diag64.p,
+            win_lo.p, win_hi.p};
 }
 template <>
 inline SellPView<float> SellP::view<float>() const {
-    return {n, n_slices, slice_ptr.p, compact ? nullptr : col.p, compact ? colw.p : nullptr, val32.p, diag32.p};
+    return {n, n_slices, slice_ptr.p, compact ? nullptr : col.p, compact ? colw.p : nullptr, val32.p, diag32.p,
+            win_lo.p, win_hi.p};
 }
 
 // Level schedule of a triangular solve: rows grouped by dependency depth.
@@ -88,7 +91,7 @@ struct Work {
 
 // Tuned tolerance-CG launch of a handle (0: not tuned).
 struct ocsl_solver_tuned {
-    int rows_per_lane = 0, block_size = 0, grid_blocks = 0, vec_blocks = 0;
+    int rows_per_lane = 0, block_size = 0, grid_blocks = 0, vec_blocks = 0, cg_kernels = 0;
 };
 
 struct ocsl_solver {
@@ -140,7 +143,7 @@ struct ocsl_solver {
     ocsl::DBuf<double> grad_own;
 
     // hipGraph cache of iteration chunks, keyed by (kind, chunk)
-    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+    std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
 
     ocsl::RedScratch red() const { return {red_partial.p}; }
 };

@@ -301,6 +301,7 @@ struct FinCgResidual {
         S->done = (rr <= S->tol2 * scale) ? 1 : 0;
         S->halt = S->done;
         S->need_update = 0;
+        S->fstop[0] = S->fstop[1] = 0;
         if (!isfinite(rr)) { S->nonfinite = 1; S->done = 1; S->halt = 1; }
     }
 };
@@ -699,6 +700,464 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
         }
         ++S->iter;
     }
+}
+
+// ------------------------------------------------------------------ CG, one kernel per iteration
+// The whole iteration k -> k+1 in one launch.  Every block re-adds the four
+// partial sums of the previous launch (p_k.q_k, r_k.q_k, q_k.q_k, r_k.r_k) in
+// the same fixed order, so all blocks hold bitwise identical alpha_k, beta_k
+// (beta from the merged recurrence as in the two-kernel form) and the same
+// stop decision.  Each block then, for its own rows,
+//   r_{k+1} = r_k - alpha q_k,  p_{k+1} = r_{k+1} + beta p_k,
+//   q_{k+1} = A p_{k+1},
+// where the SpMV forms every gathered p_{k+1}[j] from r_k[j], q_k[j] and
+// p_k[j] with the same two roundings as the owner of row j, so a gathered
+// element and the stored one are the same value.  r, q and p are
+// double-buffered (the kernels of parity pi read buffer pi and write buffer
+// 1 - pi); the partial sums likewise use the slot set of the parity.  x
+// follows in pairs as in the two-kernel form: the odd kernel adds
+// alpha_{k-1} p_{k-1} + alpha_k p_k, p_{k-1} being its own rows of the buffer
+// it is about to overwrite.
+// Stop (convergence on r_k.r_k, breakdown, non-finite, or k reaching kstop):
+// every block sees it at once; the odd kernel first adds the pending
+// alpha_{k-1} p_{k-1}.  Block 0 raises fstop[pi]; a kernel that finds the
+// flag of the previous parity raised only passes it on.  A chunk holds an
+// even number of kernels, so it starts and ends with everything in buffer 0.
+// k_cgf_first (after every true-residual computation) forms q = A p and the
+// four sums of buffer 0.
+constexpr int kSetF = 5;  // slot set pi: p.q, r.q, q.q, r.r at kSetF + 4 pi + 0..3
+static_assert(kSetF + 8 <= kMaxRed, "two slot sets of four");
+
+template <class V, class X>
+struct CgfArgs {
+    SellPView<V> A;
+    X* r[2];
+    X* q[2];
+    X* p[2];
+    double* x;
+    Scalars* S;
+    double* hist;
+    RedScratch red;
+};
+
+template <class V, class X, int R>
+__global__ void __launch_bounds__(kCgMaxBlock) k_cgf_first(CgfArgs<V, X> c) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int g = gridDim.x;
+    const int lb = xcd_block(blockIdx.x, g);
+    const int spb = (c.A.n_slices + g - 1) / g;
+    const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
+    const X* __restrict__ p = c.p[0];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
+        const int row0 = s * kWave * R + lane * R;
+        double pi[R], qd[R], rv[R];
+        load_rows<R>(p, row0, c.A.n, pi);
+        sellp_rows<V, X, sizeof(V) == 4, R>(c.A, s, lane, row0, p, pi, qd);
+        load_rows<R>(c.r[0], row0, c.A.n, rv);
+        X qv[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            qv[j] = (X)qd[j];
+            if (row0 + j < c.A.n) {
+                const double qj = (double)qv[j];
+                acc[0] += pi[j] * qj;
+                acc[1] += rv[j] * qj;
+                acc[2] += qj * qj;
+                acc[3] += rv[j] * rv[j];
+            }
+        }
+        store_rows<R>(c.q[0], row0, c.A.n, qv);
+    }
+    block_partials<4>(acc, c.red, kSetF);
+}
+
+// The two roundings of the direction update, shared by the owner of a row and
+// by every gather of it.
+template <class X>
+struct CgfStep {
+    double alpha, beta;
+    __device__ __forceinline__ X r_next(double r, double q) const { return (X)fma(-alpha, q, r); }
+    __device__ __forceinline__ X p_next(X rn, double p) const { return (X)fma(beta, p, (double)rn); }
+};
+
+// The stop test and bookkeeping shared by the one-kernel iterations: every
+// block evaluates the same sums; block 0 records the history entry, the
+// replacement trigger and, on a stop, the flags.
+__device__ __forceinline__ bool cgf_stop(Scalars* S, double* hist, double pq, double rr, int k, int kstop, int odd) {
+    const double scale = S->bb > 0.0 ? S->bb : 1.0;
+    const bool conv = !(rr > S->tol2 * scale);
+    const bool fin = isfinite(rr);
+    const bool pq_ok = cg_pq_ok(pq);
+    const bool stop = conv || !fin || !pq_ok || k >= kstop;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int it = S->iter;
+        if (hist && it < S->hist_cap) hist[it] = sqrt(rr / scale);
+        S->rr = rr;
+        S->rr_last = rr;
+        if (sqrt(rr) < S->replace_ratio * S->r_ref) S->need_update = 1;
+        if (stop) {
+            S->fstop[odd] = 1;
+            S->done = 1;
+            if (!conv) {
+                if (!fin) S->nonfinite = 1;
+                else if (!pq_ok) {
+                    S->breakdown = (pq == 0.0);
+                    S->nonfinite = !isfinite(pq);
+                }
+            }
+        }
+    }
+    return stop;
+}
+
+__device__ __forceinline__ void cgf_record(Scalars* S, double alpha, double beta, double pq, int odd) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->alpha = alpha;
+        S->beta = beta;
+        S->pq = pq;
+        if (!odd) S->alpha_prev = alpha;
+        ++S->iter;
+    }
+}
+
+template <class V, class X, int R, int kOdd>
+__global__ void __launch_bounds__(kCgMaxBlock) k_cgf_iter(CgfArgs<V, X> c, int k, int kstop) {
+    Scalars* S = c.S;
+    if (S->fstop[1 - kOdd]) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) S->fstop[kOdd] = 1;
+        return;
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int g = gridDim.x;
+    const int lb = xcd_block(blockIdx.x, g);
+    const int spb = (c.A.n_slices + g - 1) / g;
+    const int s0 = min(c.A.n_slices, lb * spb), s1 = min(c.A.n_slices, s0 + spb);
+    const int n = c.A.n;
+    const X* __restrict__ rin = c.r[kOdd];
+    const X* __restrict__ qin = c.q[kOdd];
+    const X* __restrict__ pin = c.p[kOdd];
+    X* __restrict__ rout = c.r[1 - kOdd];
+    X* __restrict__ qout = c.q[1 - kOdd];
+    X* __restrict__ pout = c.p[1 - kOdd];
+    const int sw = s0 + (int)(threadIdx.x / kWave);  // this wave's first slice
+    const int sstep = (int)(blockDim.x / kWave);
+    // the first slice's own rows are in flight while the sums are re-added
+    X rv[R], qv[R], pv[R];
+    if (sw < s1) {
+        const int row0 = sw * kWave * R + lane * R;
+        load_rows_t<R>(rin, row0, n, rv);
+        load_rows_t<R>(qin, row0, n, qv);
+        load_rows_t<R>(pin, row0, n, pv);
+    }
+    constexpr int kIn[4] = {kSetF + 4 * kOdd, kSetF + 4 * kOdd + 1, kSetF + 4 * kOdd + 2, kSetF + 4 * kOdd + 3};
+    double d4[4];
+    block_sum_partials_k<4>(c.red, kIn, g, d4);
+    const double pq = d4[0], rq = d4[1], qq = d4[2], rr = d4[3];
+    if (cgf_stop(S, c.hist, pq, rr, k, kstop, kOdd)) {
+        if constexpr (kOdd) {  // the pending alpha_{k-1} p_{k-1} of the pair
+            const double ap = S->alpha_prev;
+            for (int s = sw; s < s1; s += sstep) {
+                const int row0 = s * kWave * R + lane * R;
+                double xv[R], pp[R];
+                load_rows<R>(c.x, row0, n, xv);
+                load_rows<R>(pout, row0, n, pp);
+#pragma unroll
+                for (int j = 0; j < R; ++j) xv[j] += ap * pp[j];
+                store_rows<R>(c.x, row0, n, xv);
+            }
+        }
+        return;
+    }
+    const double alpha = rr / pq;
+    double rr_next = rr - 2.0 * alpha * rq + alpha * alpha * qq;
+    if (!(rr_next > 0.0)) rr_next = 0.0;  // rounding: restart the direction
+    const CgfStep<X> st{alpha, rr_next / rr};
+    const double ap = kOdd ? S->alpha_prev : 0.0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    auto pnext = [&](int j) -> double {
+        return (double)st.p_next(st.r_next((double)rin[j], (double)qin[j]), (double)pin[j]);
+    };
+    for (int s = sw; s < s1; s += sstep) {
+        const int row0 = s * kWave * R + lane * R;
+        if (s != sw) {
+            load_rows_t<R>(rin, row0, n, rv);
+            load_rows_t<R>(qin, row0, n, qv);
+            load_rows_t<R>(pin, row0, n, pv);
+        }
+        X rn[R], pn[R];
+        double pnd[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            rn[j] = st.r_next((double)rv[j], (double)qv[j]);
+            pn[j] = st.p_next(rn[j], (double)pv[j]);
+            pnd[j] = (double)pn[j];
+        }
+        if constexpr (kOdd) {
+            double xv[R], pp[R];
+            load_rows<R>(c.x, row0, n, xv);
+            load_rows<R>(pout, row0, n, pp);
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                xv[j] += ap * pp[j];
+                xv[j] += alpha * (double)pv[j];
+            }
+            store_rows<R>(c.x, row0, n, xv);
+        }
+        double qd[R];
+        sellp_rows_g<V, sizeof(V) == 4, R, decltype(pnext), R == 4 ? 1 : 2>(c.A, s, lane, row0, pnext, pnd, qd);
+        X qn[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            qn[j] = (X)qd[j];
+            if (row0 + j < n) {
+                const double qj = (double)qn[j], rj = (double)rn[j];
+                acc[0] += pnd[j] * qj;
+                acc[1] += rj * qj;
+                acc[2] += qj * qj;
+                acc[3] += rj * rj;
+            }
+        }
+        store_rows<R>(rout, row0, n, rn);
+        store_rows<R>(pout, row0, n, pn);
+        store_rows<R>(qout, row0, n, qn);
+    }
+    block_partials<4>(acc, c.red, kSetF + 4 * (1 - kOdd));
+    cgf_record(S, alpha, st.beta, pq, kOdd);
+}
+
+// ------------------------------------------------------------------ CG, one kernel, LDS window (FP32)
+// The one-kernel iteration with the direction staged in LDS.  A block (one
+// per CU: it takes 158 KB of the 160 KB LDS of a gfx950 CU) owns a contiguous
+// range of rows; its row window is the union of the per-slice windows of the
+// layout (own rows and every column they touch -- with RCM on a 128^3 mesh a
+// block of 8192 rows spans at most ~33K rows).  The block
+//   1. loads its own rows' r_k, q_k, p_k (each wave up to kSpw slices, kept
+//      in registers) and, in float4 groups with coalesced loads, the rest of
+//      the window; the partial sums of the previous launch are re-added while
+//      these loads are in flight;
+//   2. forms r_{k+1}, p_{k+1} of its own rows (global stores, x update) and
+//      p_{k+1} over the window into LDS;
+//   3. after one barrier, runs the SpMV of its own rows with every gather
+//      served from LDS.
+// A block whose window exceeds the LDS gathers from memory as k_cgf_iter does
+// (same values).  The launch guarantees at most kSpw slices per wave.
+constexpr int kLdsFloats = 40448;
+constexpr int kHaloBatch = 2;  // float4 window groups per thread in flight
+
+template <int R, int kOdd, int kSpw>
+__global__ void __launch_bounds__(kCgMaxBlock) k_cgl_iter(CgfArgs<float, float> c, int k, int kstop) {
+    using X = float;
+    Scalars* S = c.S;
+    if (S->fstop[1 - kOdd]) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) S->fstop[kOdd] = 1;
+        return;
+    }
+    __shared__ float pw[kLdsFloats];  // p_{k+1} of window row j at pw[j - base]
+    __shared__ int wb[2];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int g = gridDim.x;
+    const int lb = xcd_block(blockIdx.x, g);
+    const int spb = (c.A.n_slices + g - 1) / g;
+    const int s0 = min(c.A.n_slices, lb * spb), s1 = min(c.A.n_slices, s0 + spb);
+    const int n = c.A.n;
+    const X* __restrict__ rin = c.r[kOdd];
+    const X* __restrict__ qin = c.q[kOdd];
+    const X* __restrict__ pin = c.p[kOdd];
+    X* __restrict__ rout = c.r[1 - kOdd];
+    X* __restrict__ qout = c.q[1 - kOdd];
+    X* __restrict__ pout = c.p[1 - kOdd];
+    const int sw = s0 + (int)(threadIdx.x / kWave);
+    const int sstep = (int)(blockDim.x / kWave);
+    // own rows of this wave's slices
+    X rv[kSpw][R], qv[kSpw][R], pv[kSpw][R];
+#pragma unroll
+    for (int t = 0; t < kSpw; ++t) {
+        const int s = sw + t * sstep;
+        if (s < s1) {
+            const int row0 = s * kWave * R + lane * R;
+            load_rows_t<R>(rin, row0, n, rv[t]);
+            load_rows_t<R>(qin, row0, n, qv[t]);
+            load_rows_t<R>(pin, row0, n, pv[t]);
+        }
+    }
+    // row window of the block
+    if (threadIdx.x == 0) {
+        wb[0] = 0x7fffffff;
+        wb[1] = -1;
+    }
+    __syncthreads();
+    {
+        int lo = 0x7fffffff, hi = -1;
+        for (int q = s0 + (int)threadIdx.x; q < s1; q += (int)blockDim.x) {
+            lo = min(lo, c.A.win_lo[q]);
+            hi = max(hi, c.A.win_hi[q]);
+        }
+        if (hi >= 0) {
+            atomicMin(&wb[0], lo);
+            atomicMax(&wb[1], hi);
+        }
+    }
+    __syncthreads();
+    const int ra = s0 * kWave * R, rb = min(s1 * kWave * R, n);  // own rows [ra, rb)
+    const int base = wb[0] & ~3;
+    const int nv4 = wb[1] >= base ? (wb[1] - base) / 4 + 1 : 0;   // float4 groups of the window
+    const bool fits = s0 < s1 && 4 * nv4 <= kLdsFloats;
+    const int vo0 = fits ? (ra - base) / 4 : 0;                     // own groups [vo0, vo1)
+    const int vo1 = fits ? vo0 + ((rb - ra + 3) >> 2) : 0;
+    const int nhalo = fits ? vo0 + (nv4 - vo1) : 0;
+    auto halo_group = [&](int h) { return h < vo0 ? h : h - vo0 + vo1; };
+    // the first halo groups in flight with the sums
+    Pack<float, 4> hr[kHaloBatch], hq[kHaloBatch], hp[kHaloBatch];
+    auto load_group = [&](int v, Pack<float, 4>& a, Pack<float, 4>& b, Pack<float, 4>& p) {
+        const int j0 = base + 4 * v;
+        if (j0 + 4 <= n) {
+            a = ldp<float, 4>(rin + j0);
+            b = ldp<float, 4>(qin + j0);
+            p = ldp<float, 4>(pin + j0);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const bool in = j0 + e < n;
+                a.v[e] = in ? rin[j0 + e] : 0.f;
+                b.v[e] = in ? qin[j0 + e] : 0.f;
+                p.v[e] = in ? pin[j0 + e] : 0.f;
+            }
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < kHaloBatch; ++i) {
+        const int h = (int)threadIdx.x + i * (int)blockDim.x;
+        if (h < nhalo) load_group(halo_group(h), hr[i], hq[i], hp[i]);
+    }
+    constexpr int kIn[4] = {kSetF + 4 * kOdd, kSetF + 4 * kOdd + 1, kSetF + 4 * kOdd + 2, kSetF + 4 * kOdd + 3};
+    double d4[4];
+    block_sum_partials_k<4>(c.red, kIn, g, d4);
+    const double pq = d4[0], rq = d4[1], qq = d4[2], rr = d4[3];
+    if (cgf_stop(S, c.hist, pq, rr, k, kstop, kOdd)) {
+        if constexpr (kOdd) {  // the pending alpha_{k-1} p_{k-1} of the pair
+            const double ap = S->alpha_prev;
+#pragma unroll
+            for (int t = 0; t < kSpw; ++t) {
+                const int s = sw + t * sstep;
+                if (s < s1) {
+                    const int row0 = s * kWave * R + lane * R;
+                    double xe[R], pe[R];
+                    load_rows<R>(c.x, row0, n, xe);
+                    load_rows<R>(pout, row0, n, pe);
+#pragma unroll
+                    for (int j = 0; j < R; ++j) xe[j] += ap * pe[j];
+                    store_rows<R>(c.x, row0, n, xe);
+                }
+            }
+        }
+        return;
+    }
+    const double alpha = rr / pq;
+    double rr_next = rr - 2.0 * alpha * rq + alpha * alpha * qq;
+    if (!(rr_next > 0.0)) rr_next = 0.0;  // rounding: restart the direction
+    const CgfStep<X> st{alpha, rr_next / rr};
+    const double ap = kOdd ? S->alpha_prev : 0.0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    // own rows: r_{k+1}, p_{k+1} (odd: x, which needs p_{k-1} from pout
+    // before p_{k+1} replaces it); p_{k+1} into the window
+    X rn[kSpw][R], pn[kSpw][R];
+#pragma unroll
+    for (int t = 0; t < kSpw; ++t) {
+        const int s = sw + t * sstep;
+        if (s < s1) {
+            const int row0 = s * kWave * R + lane * R;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                rn[t][j] = st.r_next((double)rv[t][j], (double)qv[t][j]);
+                pn[t][j] = st.p_next(rn[t][j], (double)pv[t][j]);
+                if (row0 + j < n) acc[3] += (double)rn[t][j] * (double)rn[t][j];
+            }
+            if constexpr (kOdd) {
+                double xe[R], pe[R];
+                load_rows<R>(c.x, row0, n, xe);
+                load_rows<R>(pout, row0, n, pe);
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    xe[j] += ap * pe[j];
+                    xe[j] += alpha * (double)pv[t][j];
+                }
+                store_rows<R>(c.x, row0, n, xe);
+            }
+            store_rows<R>(rout, row0, n, rn[t]);
+            store_rows<R>(pout, row0, n, pn[t]);
+            if (fits) {
+                if (row0 + R <= n) {
+                    Pack<float, R> pk;
+#pragma unroll
+                    for (int j = 0; j < R; ++j) pk.v[j] = pn[t][j];
+                    *reinterpret_cast<Pack<float, R>*>(pw + (row0 - base)) = pk;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < R; ++j)
+                        if (row0 + j < n) pw[row0 + j - base] = pn[t][j];
+                }
+            }
+        }
+    }
+    if (fits) {
+        // the rest of the window
+        auto put_group = [&](int v, const Pack<float, 4>& a, const Pack<float, 4>& b, const Pack<float, 4>& p) {
+            Pack<float, 4> o;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) o.v[e] = st.p_next(st.r_next((double)a.v[e], (double)b.v[e]), (double)p.v[e]);
+            *reinterpret_cast<Pack<float, 4>*>(pw + 4 * v) = o;
+        };
+#pragma unroll
+        for (int i = 0; i < kHaloBatch; ++i) {
+            const int h = (int)threadIdx.x + i * (int)blockDim.x;
+            if (h < nhalo) put_group(halo_group(h), hr[i], hq[i], hp[i]);
+        }
+        for (int h0 = (int)threadIdx.x + kHaloBatch * (int)blockDim.x; h0 < nhalo;
+             h0 += kHaloBatch * (int)blockDim.x) {
+#pragma unroll
+            for (int i = 0; i < kHaloBatch; ++i) {
+                const int h = h0 + i * (int)blockDim.x;
+                if (h < nhalo) load_group(halo_group(h), hr[i], hq[i], hp[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < kHaloBatch; ++i) {
+                const int h = h0 + i * (int)blockDim.x;
+                if (h < nhalo) put_group(halo_group(h), hr[i], hq[i], hp[i]);
+            }
+        }
+    }
+    __syncthreads();
+    auto pl = [&](int j) -> double { return (double)pw[j - base]; };
+    auto pnext = [&](int j) -> double {
+        return (double)st.p_next(st.r_next((double)rin[j], (double)qin[j]), (double)pin[j]);
+    };
+#pragma unroll
+    for (int t = 0; t < kSpw; ++t) {
+        const int s = sw + t * sstep;
+        if (s < s1) {
+            const int row0 = s * kWave * R + lane * R;
+            double pnd[R], qd[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) pnd[j] = (double)pn[t][j];
+            if (fits) sellp_rows_g<float, true, R, decltype(pl), R == 1 ? 4 : 2>(c.A, s, lane, row0, pl, pnd, qd);
+            else sellp_rows_g<float, true, R, decltype(pnext), R == 4 ? 1 : 2>(c.A, s, lane, row0, pnext, pnd, qd);
+            X qn[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                qn[j] = (X)qd[j];
+                if (row0 + j < n) {
+                    const double qj = (double)qn[j];
+                    acc[0] += pnd[j] * qj;
+                    acc[1] += (double)rn[t][j] * qj;
+                    acc[2] += qj * qj;
+                }
+            }
+            store_rows<R>(qout, row0, n, qn);
+        }
+    }
+    block_partials<4>(acc, c.red, kSetF + 4 * (1 - kOdd));
+    cgf_record(S, alpha, st.beta, pq, kOdd);
 }
 
 // SELL-P FP32 difference form: diag32 = row sum of the FP64 values, the

@@ -232,6 +232,19 @@ void build_sellp(const HostCSR& a, const std::vector<int>* order, int rpl, SellP
                 }
             }
     }
+    // row window of each slice: its real rows and every stored column
+    std::vector<int> wlo(ns), whi(ns);
+    for (int sl = 0; sl < ns; ++sl) {
+        int lo = sl * H, hi = std::min(n, (sl + 1) * H) - 1;
+        for (int64_t q = sp[sl]; q < sp[sl + 1]; ++q) {
+            lo = std::min(lo, col[q]);
+            hi = std::max(hi, col[q]);
+        }
+        wlo[sl] = lo;
+        whi[sl] = hi;
+    }
+    s.win_lo.upload(wlo.data(), wlo.size(), st);
+    s.win_hi.upload(whi.data(), whi.size(), st);
     // 16-bit column offsets when every entry allows it
     bool compact = true;
     for (int sl = 0; sl < ns && compact; ++sl) {

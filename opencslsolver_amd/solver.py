@@ -227,6 +227,14 @@ class Solver:
                                            ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
 
+    def time_cg_iteration(self, precision="mixed", pairs=50):
+        """One-kernel CG iterations of the tuned launch: ([even ms, odd ms],
+        [even bytes, odd bytes]) per launch, from `pairs` timed pairs."""
+        ms = (ctypes.c_double * 2)()
+        by = (ctypes.c_double * 2)()
+        _check(self._lib.ocsl_time_cg_iteration(self._h, PRECISIONS[precision], pairs, ms, by))
+        return [ms[0], ms[1]], [by[0], by[1]]
+
     def tune_cg(self, precision="mixed", iterations=64):
         """Times every CG launch candidate on this system; keeps the fastest in
         the handle.  Returns (best options dict, best us/iteration, table)."""
@@ -237,14 +245,15 @@ class Solver:
         ln = ctypes.c_int()
         _check(self._lib.ocsl_tune_cg(self._h, PRECISIONS[precision], iterations, ctypes.byref(best),
                                       ctypes.byref(us), table, cap, ctypes.byref(ln)))
-        keys = ("rows_per_lane", "block_size", "grid_blocks", "vec_blocks")
+        keys = ("cg_kernels", "rows_per_lane", "block_size", "grid_blocks", "vec_blocks")
         rows = [dict({k: getattr(table[i], k) for k in keys}, us_per_iteration=table[i].us_per_iteration)
                 for i in range(min(ln.value, cap))]
         return {k: getattr(best, k) for k in keys}, us.value, rows
 
-    def set_tuned_cg(self, precision="mixed", rows_per_lane=0, block_size=0, grid_blocks=0, vec_blocks=0):
+    def set_tuned_cg(self, precision="mixed", rows_per_lane=0, block_size=0, grid_blocks=0, vec_blocks=0,
+                     cg_kernels=0):
         o = default_options(rows_per_lane=rows_per_lane, block_size=block_size, grid_blocks=grid_blocks,
-                            vec_blocks=vec_blocks)
+                            vec_blocks=vec_blocks, cg_kernels=cg_kernels)
         _check(self._lib.ocsl_set_tuned_cg(self._h, PRECISIONS[precision], ctypes.byref(o)))
 
     # -- differentiation ---------------------------------------------------

@@ -1,7 +1,7 @@
 """Persistent cache of the CG launch autotuner (ocsl_tune_cg).
 
-A tuned launch (rows per lane, threads per workgroup, SpMV and vector
-workgroups) depends on the system's size and pattern and on the precision; it
+A tuned launch (kernels per iteration, rows per lane, threads per workgroup,
+SpMV and vector workgroups) depends on the system's size and pattern and on the precision; it
 is cached under a key of (rows, stored entries, pattern hash, precision) in a
 JSON file, so later processes skip the tuning runs.
 """
@@ -21,7 +21,7 @@ DEFAULT_PATH = os.path.join(os.path.expanduser("~"), ".cache", "opencslsolver_am
 def _key(solver: Solver, precision: str) -> str:
     pat = zlib.crc32(np.ascontiguousarray(solver.row_ptr).tobytes())
     pat = zlib.crc32(np.ascontiguousarray(solver.col).tobytes(), pat)
-    return f"n={solver.n}|nnz={solver.nnz}|pattern={pat:08x}|{precision}"
+    return f"v2|n={solver.n}|nnz={solver.nnz}|pattern={pat:08x}|{precision}"
 
 
 def tuned_cg(solver: Solver, precision: str = "mixed", iterations: int = 64, path: str | None = None,
@@ -39,8 +39,8 @@ def tuned_cg(solver: Solver, precision: str = "mixed", iterations: int = 64, pat
     key = _key(solver, precision)
     if key in cache and not retune:
         entry = cache[key]
-        solver.set_tuned_cg(precision, **{k: entry[k] for k in ("rows_per_lane", "block_size", "grid_blocks",
-                                                               "vec_blocks")})
+        solver.set_tuned_cg(precision, **{k: entry[k] for k in ("cg_kernels", "rows_per_lane", "block_size",
+                                                               "grid_blocks", "vec_blocks")})
         return entry
     best, us, _ = solver.tune_cg(precision, iterations)
     entry = dict(best, us_per_iteration=us)

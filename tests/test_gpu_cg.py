@@ -71,13 +71,15 @@ def test_blas_helpers_match_reference():
         assert s.sum(d["b"]) == pytest.approx(float(np.sum(d["b"])), rel=1e-12, abs=1e-12)
 
 
+@pytest.mark.parametrize("kernels", [1, 2])
 @pytest.mark.parametrize("reorder,rpl", [(1, 1), (1, 2), (1, 4), (0, 1), (0, 2), (0, 4)])
-def test_cg_tolerance_fp64_matches_oracle(reorder, rpl):
+def test_cg_tolerance_fp64_matches_oracle(reorder, rpl, kernels):
     a, b, _ = gen.poisson3d(16, seed_mesh=1, seed_rhs=2)
     oracle.set_threads(1)
     it_o, x_o, h_o, tr_o = oracle.cg_tol(a, b, tol=1e-8, maxit=5000)
     with _solver(a, b) as s:
-        rep = s.solve("cg", "tolerance", "fp64", tol=1e-8, max_iter=5000, reorder=reorder, rows_per_lane=rpl)
+        rep = s.solve("cg", "tolerance", "fp64", tol=1e-8, max_iter=5000, reorder=reorder, rows_per_lane=rpl,
+                      cg_kernels=kernels)
         x = s.get_x()
         h = s.history()
     assert rep["status"] == 0, rep
@@ -174,9 +176,10 @@ def test_cg_full_size_headline_system():
     assert abs(rep64["iterations"] - 1993) <= 20
 
 
+@pytest.mark.parametrize("kernels", [1, 2, 3])
 @pytest.mark.parametrize("n", [5, 7, 13])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_cg_rows_per_lane_layouts_agree(n, precision):
+def test_cg_rows_per_lane_layouts_agree(n, precision, kernels):
     """One, two and four rows per lane: the same per-row SpMV sums, so the
     solves agree to the dot-product rounding; odd row counts end inside a row
     group."""
@@ -186,7 +189,7 @@ def test_cg_rows_per_lane_layouts_agree(n, precision):
     with _solver(a, b) as s:
         for rpl in (1, 2, 4):
             s.set_x(np.zeros(a.n))
-            rep = s.solve("cg", "tolerance", precision, tol=tol, rows_per_lane=rpl)
+            rep = s.solve("cg", "tolerance", precision, tol=tol, rows_per_lane=rpl, cg_kernels=kernels)
             assert rep["status"] == 0, (rpl, rep)
             out[rpl] = (rep, s.get_x())
     # FP64 stops on the recursive residual: the true one may sit a few times
@@ -198,6 +201,79 @@ def test_cg_rows_per_lane_layouts_agree(n, precision):
         assert _rel(out[rpl][1], out[1][1]) <= (1e-6 if precision == "fp64" else 1e-4)
 
 
+@pytest.mark.parametrize("kernels", [1, 2])
+@pytest.mark.parametrize("max_iter", [1, 2, 7, 8, 32, 33])
+@pytest.mark.parametrize("graph", [1, 0])
+def test_cg_iteration_count_cut(max_iter, graph, kernels):
+    """FP64 CG stops at max_iter whatever its parity (the odd one-kernel
+    iteration then adds the pending step of its pair) and records the
+    residual of the last iterate: x and the history after k iterations are
+    the oracle's after k textbook CG iterations."""
+    a, b, _ = gen.poisson3d(12, seed_mesh=5, seed_rhs=6)
+    oracle.set_threads(1)
+    it_o, x_o, h_o, _ = oracle.cg_tol(a, b, tol=1e-30, maxit=max_iter)
+    assert it_o == max_iter
+    with _solver(a, b) as s:
+        rep = s.solve("cg", "tolerance", "fp64", tol=1e-30, max_iter=max_iter, cg_kernels=kernels,
+                      use_graph=graph, check_every=4)
+        x = s.get_x()
+        h = s.history()
+    assert rep["iterations"] == max_iter and rep["status"] == 1, rep  # OCSL_WARN_MAXITER
+    assert len(h) == max_iter + 1
+    assert np.allclose(h, h_o[:max_iter + 1], rtol=1e-9)
+    assert _rel(x, x_o) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("check_every", [2, 3, 8, 32])
+def test_cg_one_kernel_and_split_forms_agree(precision, check_every):
+    """Same system, every iteration form (one kernel with the LDS window, split,
+    one kernel gathering from memory), chunk lengths that put host checks (and
+    MIXED residual replacements) after even and odd iteration counts."""
+    a, b, _ = gen.poisson3d(20, seed_mesh=7, seed_rhs=8)
+    tol = 1e-10 if precision == "fp64" else 1e-9
+    res = {}
+    with _solver(a, b) as s:
+        for kernels in (1, 2, 3):
+            s.set_x(np.zeros(a.n))
+            rep = s.solve("cg", "tolerance", precision, tol=tol, cg_kernels=kernels, check_every=check_every)
+            assert rep["status"] == 0, (kernels, rep)
+            res[kernels] = (rep, s.get_x())
+    for kernels in (1, 2, 3):
+        rel = np.linalg.norm(b - a.matvec(res[kernels][1])) / np.linalg.norm(b)
+        assert rel <= (10 * tol if precision == "fp64" else tol), (kernels, rel)
+        assert abs(res[kernels][0]["iterations"] - res[2][0]["iterations"]) <= max(3, res[2][0]["iterations"] // 20)
+        assert _rel(res[kernels][1], res[2][1]) <= (1e-7 if precision == "fp64" else 1e-5)
+
+
+def test_cg_lds_window_overflow_takes_gather_path():
+    """Input row order on a permuted mesh: every block's row window spans
+    nearly all 64000 rows, more than the LDS holds, so the LDS form falls back
+    to gathering; it must agree with the gathering form."""
+    a, b, _ = gen.poisson3d(40, seed_mesh=1, seed_rhs=2)
+    res = {}
+    with _solver(a, b) as s:
+        for kernels in (1, 3):
+            s.set_x(np.zeros(a.n))
+            rep = s.solve("cg", "tolerance", "mixed", tol=1e-8, cg_kernels=kernels, reorder=0)
+            assert rep["status"] == 0, (kernels, rep)
+            res[kernels] = (rep, s.get_x())
+    for kernels in (1, 3):
+        assert np.linalg.norm(b - a.matvec(res[kernels][1])) / np.linalg.norm(b) <= 1e-8
+    assert abs(res[1][0]["iterations"] - res[3][0]["iterations"]) <= 5
+    assert _rel(res[1][1], res[3][1]) <= 1e-5
+
+
+def test_cg_one_kernel_timing_hook():
+    a, b, _ = gen.poisson3d(32, seed_mesh=1, seed_rhs=2)
+    with _solver(a, b) as s:
+        x0 = s.get_x()
+        for precision in ("fp64", "mixed"):
+            ms, by = s.time_cg_iteration(precision, pairs=5)
+            assert all(m > 0 for m in ms) and by[1] > by[0] > 0
+        assert np.array_equal(s.get_x(), x0)
+
+
 def test_cg_autotuner_keeps_x_and_tuned_solves_converge():
     a, b, _ = gen.poisson3d(24, seed_mesh=1, seed_rhs=2)
     x0 = gen.uniform(9, a.n)
@@ -205,6 +281,8 @@ def test_cg_autotuner_keeps_x_and_tuned_solves_converge():
         best, us, table = s.tune_cg("mixed", iterations=8)
         assert np.array_equal(s.get_x(), x0)
         assert best["rows_per_lane"] in (1, 2, 4) and best["block_size"] in (256, 512, 1024)
+        assert best["cg_kernels"] in (1, 2, 3)
+        assert {t["cg_kernels"] for t in table} == {1, 2, 3}
         assert len(table) >= 36 and all(t["us_per_iteration"] > 0 for t in table)
         assert us == min(t["us_per_iteration"] for t in table)
         s.set_x(np.zeros(a.n))
