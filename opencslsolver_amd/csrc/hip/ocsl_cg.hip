@@ -268,26 +268,29 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     const int* perm = s.identity ? nullptr : s.perm.p;
     Work& w = h->work;
     // b, x in FP64 (d5, d6); r, p, q in the Krylov precision, and for the
-    // one-kernel form their second buffers (MIXED f3..f5, FP64 d10, d12, d13)
+    // one-kernel form their second buffers (MIXED f3..f5, FP64 d10, d12, d13).
+    // The vectors hold n + 4 elements: the update kernels load at clamped
+    // indices past the last full 16-byte vector.
+    const size_t nb = (size_t)n + 4;
     double* b = work_buf(h, w.d[5], n);
-    double* x = work_buf(h, w.d[6], n);
+    double* x = work_buf(h, w.d[6], nb);
     work_buf(h, h->hist, (size_t)o.max_iter + 2);
     CgArgs<double, double> c64{};
     CgArgs<float, float> c32{};
     CgfArgs<double, double> f64{};
     CgfArgs<float, float> f32{};
     if (mixed) {
-        c32 = {s.view<float>(), work_buf(h, w.f[0], n), work_buf(h, w.f[1], n), work_buf(h, w.f[2], n),
-               work_buf(h, w.f[3], n), x, h->S.p, h->hist.p, h->red()};
+        c32 = {s.view<float>(), work_buf(h, w.f[0], nb), work_buf(h, w.f[1], nb), work_buf(h, w.f[2], nb),
+               work_buf(h, w.f[3], nb), x, h->S.p, h->hist.p, h->red()};
         if (fused)
-            f32 = {s.view<float>(), {c32.r, work_buf(h, w.f[4], n)}, {c32.q, work_buf(h, w.f[5], n)},
+            f32 = {s.view<float>(), {c32.r, work_buf(h, w.f[4], nb)}, {c32.q, work_buf(h, w.f[5], nb)},
                    {c32.p, c32.p_alt}, x, h->S.p, h->hist.p, h->red()};
     } else {
-        c64 = {s.view<double>(), work_buf(h, w.d[7], n), work_buf(h, w.d[8], n), work_buf(h, w.d[9], n), nullptr, x,
-               h->S.p, h->hist.p, h->red()};
+        c64 = {s.view<double>(), work_buf(h, w.d[7], nb), work_buf(h, w.d[8], nb), work_buf(h, w.d[9], nb), nullptr,
+               x, h->S.p, h->hist.p, h->red()};
         if (fused)
-            f64 = {s.view<double>(), {c64.r, work_buf(h, w.d[12], n)}, {c64.q, work_buf(h, w.d[13], n)},
-                   {c64.p, work_buf(h, w.d[10], n)}, x, h->S.p, h->hist.p, h->red()};
+            f64 = {s.view<double>(), {c64.r, work_buf(h, w.d[12], nb)}, {c64.q, work_buf(h, w.d[13], nb)},
+                   {c64.p, work_buf(h, w.d[10], nb)}, x, h->S.p, h->hist.p, h->red()};
     }
 
     Scalars init{};

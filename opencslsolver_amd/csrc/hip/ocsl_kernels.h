@@ -170,27 +170,70 @@ __device__ __forceinline__ double block_sum_partials(RedScratch red, int slot, i
 }
 
 // Several slots at once: one pass, one pair of barriers.
+
+// The same sums in two phases.  PartialSums::issue starts this thread's loads
+// of the first two rounds of partials (all slots, no wait); a kernel calls it
+// before its other global loads, because vector loads complete in issue
+// order: waiting for the partials then waits for nothing issued after them.
+// finish adds any further rounds and completes the block reduction.  Every
+// block runs the same code, so all get the same bits.
+template <int K>
+struct PartialSums {
+    double v[2][K];
+    __device__ __forceinline__ void issue(RedScratch red, const int (&slot)[K], const int (&G)[K]) {
+        const int bd = blockDim.x, i = threadIdx.x;
+        // unconditional loads (clamped index, result dropped), so that the
+        // compiler can count them
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const double t = red.partial[slot[k] * kMaxRedBlocks + min(i + u * bd, kMaxRedBlocks - 1)];
+                v[u][k] = i + u * bd < G[k] ? t : 0.0;
+            }
+    }
+    __device__ __forceinline__ void finish(RedScratch red, const int (&slot)[K], const int (&G)[K],
+                                           double (&out)[K]) {
+        const int bd = blockDim.x;
+        double a[K];
+        int gmax = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            a[k] = v[0][k] + v[1][k];
+            gmax = max(gmax, G[k]);
+        }
+        for (int i = threadIdx.x + 2 * bd; i < gmax; i += bd)
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (i < G[k]) a[k] += red.partial[slot[k] * kMaxRedBlocks + i];
+        __shared__ double sh[K][16];
+        const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+        const int nw = blockDim.x / kWave;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double s = wave_sum(a[k]);
+            if (lane == 0) sh[k][wid] = s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double t = 0.0;
+            for (int w = 0; w < nw; ++w) t += sh[k][w];
+            out[k] = t;
+        }
+        __syncthreads();
+    }
+};
+
 template <int K>
 __device__ __forceinline__ void block_sum_partials_k(RedScratch red, const int (&slot)[K], int G,
                                                      double (&out)[K]) {
-    __shared__ double sh[K][16];
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-    const int nw = blockDim.x / kWave;
+    int gs[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        double s = 0.0;
-        for (int i = tid; i < G; i += blockDim.x) s += red.partial[slot[k] * kMaxRedBlocks + i];
-        s = wave_sum(s);
-        if (lane == 0) sh[k][wid] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        double t = 0.0;
-        for (int w = 0; w < nw; ++w) t += sh[k][w];
-        out[k] = t;
-    }
-    __syncthreads();
+    for (int k = 0; k < K; ++k) gs[k] = G;
+    PartialSums<K> ps;
+    ps.issue(red, slot, gs);
+    ps.finish(red, slot, gs, out);
 }
 
 // 16-byte vector of X: each lane of the vector kernels moves kVec<X>
@@ -356,16 +399,21 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_r(CgArgs<V, X> c, int
     const int nv = n / N;
     const int stride = gridDim.x * blockDim.x;
     const int v0 = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int kSl[1] = {kSlotPQ};
+    const int gs[1] = {c.g_spmv};
+    double pqs[1];
+    PartialSums<1> ps;
+    ps.issue(c.red, kSl, gs);
+    // unconditional loads at a clamped index (the vectors hold n + 4 elements)
     X rv[kPre][N], qv[kPre][N];
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
-        const int v = v0 + k * stride;
-        if (v < nv) {
-            vload(c.r, v, rv[k]);
-            vload(c.q, v, qv[k]);
-        }
+        const int v = min(v0 + k * stride, nv);
+        vload(c.r, v, rv[k]);
+        vload(c.q, v, qv[k]);
     }
-    const double pq = block_sum_partials(c.red, kSlotPQ, c.g_spmv);
+    ps.finish(c.red, kSl, gs, pqs);
+    const double pq = pqs[0];
     if (!cg_pq_ok(pq)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             S->breakdown = (pq == 0.0);
@@ -423,20 +471,25 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_xp(CgArgs<V, X> c, in
     const int nv = n / N;
     const int stride = g * blockDim.x;
     const int v0 = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int kSl[2] = {kSlotPQ, kSlotRR};
+    const int gs[2] = {c.g_spmv, g};
+    double sums[2];
+    PartialSums<2> ps;
+    ps.issue(c.red, kSl, gs);
+    // unconditional loads at a clamped index (the vectors hold n + 4 elements)
     double xv[kPre][N];
     X pv[kPre][N], rv[kPre][N];
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
-        const int v = v0 + k * stride;
-        if (v < nv) {
-            dload<N>(c.x, v, xv[k]);
-            vload(c.p, v, pv[k]);
-            vload(c.r, v, rv[k]);
-        }
+        const int v = min(v0 + k * stride, nv);
+        dload<N>(c.x, v, xv[k]);
+        vload(c.p, v, pv[k]);
+        vload(c.r, v, rv[k]);
     }
-    const double pq = block_sum_partials(c.red, kSlotPQ, c.g_spmv);
+    ps.finish(c.red, kSl, gs, sums);
+    const double pq = sums[0];
     if (!cg_pq_ok(pq)) return;
-    const double rr_new = block_sum_partials(c.red, kSlotRR, g);
+    const double rr_new = sums[1];
     const double rr = S->rr;
     const double alpha = rr / pq, beta = rr_new / rr;
 #pragma unroll
@@ -593,21 +646,27 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
         }
         return;
     }
+    constexpr int kSlots[3] = {kSlotPQ, kSlotRQ, kSlotQQ};
+    const int gs[3] = {c.g_spmv, c.g_spmv, c.g_spmv};
+    double d3[3];
+    PartialSums<3> ps;
+    ps.issue(c.red, kSlots, gs);
+    // first vector in flight while the partial sums are re-added:
+    // unconditional loads at a clamped index (the vectors hold n + 4 elements)
     double xv[N];
     X pv[N], rv[N], qv[N], ppv[N];
     const bool have = v0 < nv;
-    if (have) {  // first vector loaded before the partial sums are re-added
+    {
+        const int vs = have ? v0 : nv;
         if (kOdd) {
-            dload<N>(c.x, v0, xv);
-            vload(c.p_alt, v0, ppv);
+            dload<N>(c.x, vs, xv);
+            vload(c.p_alt, vs, ppv);
         }
-        vload(c.p, v0, pv);
-        vload(c.r, v0, rv);
-        vload(c.q, v0, qv);
+        vload(c.p, vs, pv);
+        vload(c.r, vs, rv);
+        vload(c.q, vs, qv);
     }
-    constexpr int kSlots[3] = {kSlotPQ, kSlotRQ, kSlotQQ};
-    double d3[3];
-    block_sum_partials_k<3>(c.red, kSlots, c.g_spmv, d3);
+    ps.finish(c.red, kSlots, gs, d3);
     const double pq = d3[0], rq = d3[1], qq = d3[2];
     if (!cg_pq_ok(pq)) {
         if (kOdd) {  // keep the even iteration's step
@@ -842,6 +901,11 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgf_iter(CgfArgs<V, X> c, int k
     X* __restrict__ pout = c.p[1 - kOdd];
     const int sw = s0 + (int)(threadIdx.x / kWave);  // this wave's first slice
     const int sstep = (int)(blockDim.x / kWave);
+    constexpr int kIn[4] = {kSetF + 4 * kOdd, kSetF + 4 * kOdd + 1, kSetF + 4 * kOdd + 2, kSetF + 4 * kOdd + 3};
+    const int gs[4] = {g, g, g, g};
+    double d4[4];
+    PartialSums<4> ps;
+    ps.issue(c.red, kIn, gs);
     // the first slice's own rows are in flight while the sums are re-added
     X rv[R], qv[R], pv[R];
     if (sw < s1) {
@@ -850,9 +914,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgf_iter(CgfArgs<V, X> c, int k
         load_rows_t<R>(qin, row0, n, qv);
         load_rows_t<R>(pin, row0, n, pv);
     }
-    constexpr int kIn[4] = {kSetF + 4 * kOdd, kSetF + 4 * kOdd + 1, kSetF + 4 * kOdd + 2, kSetF + 4 * kOdd + 3};
-    double d4[4];
-    block_sum_partials_k<4>(c.red, kIn, g, d4);
+    ps.finish(c.red, kIn, gs, d4);
     const double pq = d4[0], rq = d4[1], qq = d4[2], rr = d4[3];
     if (cgf_stop(S, c.hist, pq, rr, k, kstop, kOdd)) {
         if constexpr (kOdd) {  // the pending alpha_{k-1} p_{k-1} of the pair
@@ -969,6 +1031,17 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgl_iter(CgfArgs<float, float> 
     X* __restrict__ pout = c.p[1 - kOdd];
     const int sw = s0 + (int)(threadIdx.x / kWave);
     const int sstep = (int)(blockDim.x / kWave);
+    constexpr int kIn[4] = {kSetF + 4 * kOdd, kSetF + 4 * kOdd + 1, kSetF + 4 * kOdd + 2, kSetF + 4 * kOdd + 3};
+    const int gs[4] = {g, g, g, g};
+    double d4[4];
+    PartialSums<4> ps;
+    ps.issue(c.red, kIn, gs);
+    // row window of the block (slice windows loaded before the own rows)
+    int wlo = 0x7fffffff, whi = -1;
+    for (int q = s0 + (int)threadIdx.x; q < s1; q += (int)blockDim.x) {
+        wlo = min(wlo, c.A.win_lo[q]);
+        whi = max(whi, c.A.win_hi[q]);
+    }
     // own rows of this wave's slices
     X rv[kSpw][R], qv[kSpw][R], pv[kSpw][R];
 #pragma unroll
@@ -981,22 +1054,14 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgl_iter(CgfArgs<float, float> 
             load_rows_t<R>(pin, row0, n, pv[t]);
         }
     }
-    // row window of the block
     if (threadIdx.x == 0) {
         wb[0] = 0x7fffffff;
         wb[1] = -1;
     }
     __syncthreads();
-    {
-        int lo = 0x7fffffff, hi = -1;
-        for (int q = s0 + (int)threadIdx.x; q < s1; q += (int)blockDim.x) {
-            lo = min(lo, c.A.win_lo[q]);
-            hi = max(hi, c.A.win_hi[q]);
-        }
-        if (hi >= 0) {
-            atomicMin(&wb[0], lo);
-            atomicMax(&wb[1], hi);
-        }
+    if (whi >= 0) {
+        atomicMin(&wb[0], wlo);
+        atomicMax(&wb[1], whi);
     }
     __syncthreads();
     const int ra = s0 * kWave * R, rb = min(s1 * kWave * R, n);  // own rows [ra, rb)
@@ -1030,9 +1095,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgl_iter(CgfArgs<float, float> 
         const int h = (int)threadIdx.x + i * (int)blockDim.x;
         if (h < nhalo) load_group(halo_group(h), hr[i], hq[i], hp[i]);
     }
-    constexpr int kIn[4] = {kSetF + 4 * kOdd, kSetF + 4 * kOdd + 1, kSetF + 4 * kOdd + 2, kSetF + 4 * kOdd + 3};
-    double d4[4];
-    block_sum_partials_k<4>(c.red, kIn, g, d4);
+    ps.finish(c.red, kIn, gs, d4);
     const double pq = d4[0], rq = d4[1], qq = d4[2], rr = d4[3];
     if (cgf_stop(S, c.hist, pq, rr, k, kstop, kOdd)) {
         if constexpr (kOdd) {  // the pending alpha_{k-1} p_{k-1} of the pair
