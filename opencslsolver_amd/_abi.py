@@ -49,6 +49,7 @@ class Options(ctypes.Structure):
         ("vec_blocks", ctypes.c_int),
         ("rows_per_lane", ctypes.c_int),
         ("cg_kernels", ctypes.c_int),
+        ("x_period", ctypes.c_int),
     ]
 
 

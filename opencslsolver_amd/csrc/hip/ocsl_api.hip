@@ -218,6 +218,7 @@ void ocsl_default_options(ocsl_options* o) {
     o->vec_blocks = 0;
     o->rows_per_lane = 0;
     o->cg_kernels = 0;
+    o->x_period = 0;
 }
 
 int ocsl_device_count(void) {

@@ -122,7 +122,10 @@ struct CgBytes {
     double first;       // k_cgf_first
     double residual;    // k_cg_residual
 };
-CgBytes cg_bytes(const ocsl_solver* h, const SellP& s, bool mixed);
+CgBytes cg_bytes(const ocsl_solver* h, const SellP& s, bool mixed, int M = 2);
+
+// Iterations per FP64 x update of the MIXED split form.
+int x_period(const ocsl_options& o) { return o.x_period == 2 || o.x_period == 4 ? o.x_period : 8; }
 
 namespace {
 
@@ -152,18 +155,30 @@ CgLaunch cg_launch(int n_slices, int n, const ocsl_options& o) {
     return {grid, block, round8(std::max(1, std::min(vcap, vneed)))};
 }
 
+// Update j of a period of M iterations.
+template <class V, class X, int M, int J = 0>
+void launch_cgm_update(ocsl_solver* h, const CgArgs<V, X>& ck, int n, CgLaunch L, int j) {
+    if constexpr (J < M) {
+        if (j == J) k_cgm_update<V, X, J, M><<<L.vgrid, L.block, 0, h->stream>>>(ck, n);
+        else launch_cgm_update<V, X, M, J + 1>(h, ck, n, L, j);
+    }
+}
+
 // FP64: the textbook three-kernel iteration; MIXED: the two-kernel iteration
-// with merged reductions.
-// Iteration k of a chunk; MIXED iterations run in (even, odd) pairs with the
-// direction buffers swapped in the odd one.
+// with merged reductions.  Iteration k of a chunk; MIXED iterations run in
+// periods of M (x updated by the last of each) over the rotating direction
+// buffers pb[0..M-1].
 template <int R, class V, class X>
-void launch_iter(ocsl_solver* h, const CgArgs<V, X>& c, int n, CgLaunch L, int k) {
+void launch_iter(ocsl_solver* h, const CgArgs<V, X>& c, int n, CgLaunch L, int k, int M) {
     if constexpr (sizeof(X) == 4) {
+        const int j = k % M;
         CgArgs<V, X> ck = c;
-        if (k & 1) std::swap(ck.p, ck.p_alt);
-        k_cgm_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(ck, (k & 1) ? 0 : 1);
-        if (k & 1) k_cgm_update<V, X, 1><<<L.vgrid, L.block, 0, h->stream>>>(ck, n);
-        else k_cgm_update<V, X, 0><<<L.vgrid, L.block, 0, h->stream>>>(ck, n);
+        ck.p = c.pb[j];
+        ck.p_alt = c.pb[(j + 1) % M];
+        k_cgm_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(ck, j == 0 ? 1 : 0);
+        if (M == 2) launch_cgm_update<V, X, 2>(h, ck, n, L, j);
+        else if (M == 4) launch_cgm_update<V, X, 4>(h, ck, n, L, j);
+        else launch_cgm_update<V, X, 8>(h, ck, n, L, j);
     } else {
         k_cg_spmv<V, X, R><<<L.grid, L.block, 0, h->stream>>>(c);
         k_cg_update_r<V, X><<<L.vgrid, L.block, 0, h->stream>>>(c, n);
@@ -279,9 +294,15 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     CgArgs<float, float> c32{};
     CgfArgs<double, double> f64{};
     CgfArgs<float, float> f32{};
+    // MIXED split form: x updated once per period of M iterations, over M
+    // direction buffers (f1, f3, then f6..f11)
+    const int M = mixed && !fused ? x_period(o) : 2;
     if (mixed) {
         c32 = {s.view<float>(), work_buf(h, w.f[0], nb), work_buf(h, w.f[1], nb), work_buf(h, w.f[2], nb),
                work_buf(h, w.f[3], nb), x, h->S.p, h->hist.p, h->red()};
+        c32.pb[0] = c32.p;
+        c32.pb[1] = c32.p_alt;
+        for (int i = 2; i < M; ++i) c32.pb[i] = work_buf(h, w.f[4 + i], nb);
         if (fused)
             f32 = {s.view<float>(), {c32.r, work_buf(h, w.f[4], nb)}, {c32.q, work_buf(h, w.f[5], nb)},
                    {c32.p, c32.p_alt}, x, h->S.p, h->hist.p, h->red()};
@@ -331,9 +352,10 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     int replacements = 0, firsts = 0;
     bool need_first = fused;  // q = A p and the sums of buffer 0 after every true residual
     const int kind = (mixed ? 1 : 0) + (s.identity ? 0 : 2) + (fused ? 4 : 0) + (lds ? 8 : 0);
-    // MIXED chunks, and every one-kernel chunk, hold whole (even, odd) pairs
-    const bool pairs = mixed || fused;
-    const int every = pairs ? (o.check_every + 1) / 2 * 2 : o.check_every;
+    // chunks hold whole periods: MIXED split M iterations, one-kernel pairs
+    const int per = fused ? 2 : (mixed ? M : 1);
+    const bool pairs = per > 1;
+    const int every = (o.check_every + per - 1) / per * per;
     while (!h->S_host->done) {
         const int iter0 = h->S_host->iter;
         const int left = o.max_iter - iter0;
@@ -343,7 +365,7 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
             // the kernel after the last allowed iteration records its residual
             // and stops; iterations past max_iter stop on their own
             if (!graph && chunk == left) ++chunk;
-            chunk = (chunk + 1) / 2 * 2;
+            chunk = (chunk + per - 1) / per * per;
         }
         const int kstop = std::min(chunk, left);
         with_rpl(s.rpl, [&](auto RC) {
@@ -366,20 +388,21 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
                     run_chunk(h, chunk, graph, kind, R, L,
                               [&](int k) { launch_iter_f<R, false>(h, f64, L, spw, k, kstop); });
             } else if (mixed) {
-                run_chunk(h, chunk, graph, kind, R, L, [&](int k) { launch_iter<R>(h, c32, n, L, k); });
+                run_chunk(h, chunk, graph, kind + 16 * M, R, L, [&](int k) { launch_iter<R>(h, c32, n, L, k, M); });
             } else {
-                run_chunk(h, chunk, graph, kind, R, L, [&](int k) { launch_iter<R>(h, c64, n, L, k); });
+                run_chunk(h, chunk, graph, kind, R, L, [&](int k) { launch_iter<R>(h, c64, n, L, k, 1); });
             }
         });
         read_scalars(h);
         const Scalars& S = *h->S_host;
         if (S.nonfinite || S.breakdown) break;
         if (mixed && (S.need_update || S.done)) {
-            if ((S.iter - iter0) & 1) {
-                // the chunk stopped after an odd number of iterations: x
-                // holds every step, the current direction sits in the second
-                // buffer
-                OCSL_HIP(hipMemcpyAsync(c32.p, c32.p_alt, sizeof(float) * n, hipMemcpyDeviceToDevice, h->stream));
+            const int j = (S.iter - iter0) % per;
+            if (j) {
+                // the chunk stopped inside a period: x holds every step, the
+                // current direction sits in buffer j
+                float* cur = fused ? c32.p_alt : c32.pb[j];
+                OCSL_HIP(hipMemcpyAsync(c32.p, cur, sizeof(float) * n, hipMemcpyDeviceToDevice, h->stream));
             }
             residual(0);
             ++replacements;
@@ -413,7 +436,7 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     const bool converged = mixed ? (tr <= o.tol) : (S.rr_last <= S.tol2 * scale);
     if (!converged && st == OCSL_OK) st |= OCSL_WARN_MAXITER;
     rep->status = st;
-    const CgBytes cb = cg_bytes(h, s, mixed);
+    const CgBytes cb = cg_bytes(h, s, mixed, M);
     const double it_bytes = fused ? 0.5 * (cb.fused_even + cb.fused_odd) : cb.split;
     rep->bytes = iters * it_bytes + replacements * cb.residual + firsts * cb.first;
 }
@@ -421,7 +444,7 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
 // Algorithmic bytes of the tolerance-CG kernels.  The SpMV part: off-diagonal
 // values and columns (2 or 4 bytes each), the diagonal, slice pointers, the
 // gathered vector once.  sv = bytes of a Krylov vector element.
-CgBytes cg_bytes(const ocsl_solver* h, const SellP& s, bool mixed) {
+CgBytes cg_bytes(const ocsl_solver* h, const SellP& s, bool mixed, int M) {
     const double n = s.n;
     const double off = (double)h->host.col.size() - n;
     const double sv = mixed ? 4.0 : 8.0;
@@ -430,9 +453,9 @@ CgBytes cg_bytes(const ocsl_solver* h, const SellP& s, bool mixed) {
     CgBytes r;
     // split: SpMV (p gathered, q written); FP64: K2 r, q read, r written; K3
     // x read/written, p, r read, p written.  MIXED: r read in the SpMV, then
-    // per pair: even r, q, p read, r, p' written (5 sv); odd the same plus x
-    // (FP64) read and written and p_prev read (16 + 6 sv)
-    r.split = mixed ? mat + 2.0 * n * sv + n * sv + (8.0 + 5.5 * sv) * n
+    // every update r, q, p read, r, p' written (5 sv), and once per period of
+    // M the FP64 x read and written and the M - 1 earlier directions read
+    r.split = mixed ? mat + 2.0 * n * sv + n * sv + 5.0 * sv * n + (16.0 + (M - 1) * sv) * n / M
                     : mat + 2.0 * n * sv + 3.0 * n * sv + (16.0 + 3.0 * sv) * n;
     // one kernel: r, q, p read and written; odd adds x read/written and p_prev
     r.fused_even = mat + 6.0 * n * sv;
@@ -616,7 +639,7 @@ extern "C" ocsl_status ocsl_time_cg_iteration(ocsl_handle h, int precision, int 
             else if (mixed) time_iteration_pairs<R, false, float, float>(h, s, o, pairs, ms);
             else time_iteration_pairs<R, false, double, double>(h, s, o, pairs, ms);
         });
-        const CgBytes cb = cg_bytes(h, s, mixed);
+        const CgBytes cb = cg_bytes(h, s, mixed, 2);
         avg_ms[0] = ms[0];
         avg_ms[1] = ms[1];
         bytes[0] = cb.fused_even;

@@ -82,9 +82,10 @@ struct Scalars {
     int donek[4], earlyk[4];
     double replace_ratio;
     int hist_cap;
-    int lag;            // MIXED CG: x lags the last even iteration's alpha p
+    int lag;            // MIXED CG: x lags steps of the current period (or pair)
     double alpha_prev;  // MIXED CG: alpha of that even iteration
     int fstop[2];       // one-kernel CG: stop flag written by the kernels of each parity
+    double alpha_q[8];  // MIXED CG: alpha of the iterations of the current x-update period
 };
 
 // Scratch of the deterministic grid reductions: one partial sum per block

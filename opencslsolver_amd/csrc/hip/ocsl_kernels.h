@@ -150,6 +150,7 @@ struct CgArgs {
     RedScratch red;
     int g_spmv;     // blocks of the SpMV kernels (and of the residual kernel)
     int g_vec;      // blocks of the vector-update kernels
+    X* pb[8];       // MIXED: the direction buffers of one x-update period
 };
 
 // Sum of the G partials of one slot, identical in every block (fixed order,
@@ -554,21 +555,26 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_xp(CgArgs<V, X> c, in
 //   K1m k_cgm_spmv    block 0: rr = sum r.r, convergence, history;
 //                     q = A p; partials of p.q, r.q, q.q
 //   K2m k_cgm_update  alpha, beta; r -= alpha q; p' = r + beta p; partials of r.r
-// Iterations run in pairs with p double-buffered (p -> p_alt -> p): the even
-// update writes p' to the other buffer and leaves x alone; the odd one, with
-// both directions at hand, adds alpha_prev p_prev + alpha p to x.  The FP64 x
-// is then read and written every other iteration (30 instead of 36 bytes per
-// row and iteration); the sums are the same two FP64 additions in the same
-// order.  When a pair is cut by convergence after its even update, the odd
-// update still adds the pending alpha_prev p_prev (S->lag, cleared by the even
-// SpMV of the next pair, so it is read and written in different kernels).
+// x is updated once per period of M iterations (M = 2, 4 or 8): iteration j
+// of a period (j = 0..M-1) reads its direction from buffer j and writes the
+// next one to buffer (j + 1) mod M, so the period's directions stay at hand;
+// the last update of the period adds alpha_0 P_0 + ... + alpha_{M-1} P_{M-1}
+// to the FP64 x, one addition per iteration in iteration order -- the same
+// sums as an update every iteration.  x is read and written once per period
+// (for FP32 vectors: 36 bytes per row and iteration when x is updated every
+// iteration, 5 sv + (16 + (M - 1) sv) / M in general: 30, 27, 25.5).
+// When a period is cut by a stop after its update j > 0, the update of the
+// stopping iteration adds the j pending steps (S->lag marks a period with
+// pending steps: set by update 0, cleared by the SpMV of the next period and
+// by any SpMV that finds the solve stopped, so it is read and written in
+// different kernels).
 template <class V, class X, int R>
-__global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int even) {
+__global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int first) {
     Scalars* S = c.S;
     if (S->done) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             S->halt = 1;
-            if (even) S->lag = 0;
+            S->lag = 0;
         }
         return;
     }
@@ -576,7 +582,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int ev
     if (blockIdx.x == 0) {
         const double rr = block_sum_partials(c.red, kSlotRR, S->rr_blocks);
         if (threadIdx.x == 0) {
-            if (even) S->lag = 0;
+            if (first) S->lag = 0;
             S->rr = rr;
             S->rr_last = rr;
             const int it = S->iter;
@@ -618,32 +624,56 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int ev
     block_partials<2>(rq_qq, c.red, kSlotRQ);
 }
 
-template <class V, class X, int kOdd>
+// x += alpha_0 P_0 + ... + alpha_{J-1} P_{J-1} over the whole vector: the
+// pending steps of a period cut before its last update.
+template <class V, class X, int J>
+__device__ __forceinline__ void cgm_flush_pending(const CgArgs<V, X>& c, const Scalars* S, int n) {
+    if constexpr (J > 0) {
+        constexpr int N = Vec16<X>::N;
+        const int g = gridDim.x, nv = n / N, stride = g * blockDim.x;
+        double al[J];
+#pragma unroll
+        for (int i = 0; i < J; ++i) al[i] = S->alpha_q[i];
+        for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride) {
+            double xe[N];
+            dload<N>(c.x, v, xe);
+#pragma unroll
+            for (int i = 0; i < J; ++i) {
+                X pp[N];
+                vload(c.pb[i], v, pp);
+#pragma unroll
+                for (int e = 0; e < N; ++e) xe[e] += al[i] * (double)pp[e];
+            }
+            dstore<N>(c.x, v, xe);
+        }
+        if (blockIdx.x == g - 1) {
+            const int i0 = nv * N + (int)threadIdx.x;
+            if (i0 < n) {
+                double xi = c.x[i0];
+#pragma unroll
+                for (int i = 0; i < J; ++i) xi += al[i] * (double)c.pb[i][i0];
+                c.x[i0] = xi;
+            }
+        }
+    }
+}
+
+// Update j = kJ of a period of kM iterations: c.p is P_j, c.p_alt is
+// P_{(j+1) mod kM}; the last update (kJ = kM - 1) also adds the period's
+// steps to x, reading P_0 .. P_{kM-2} (P_0 before p' replaces it).
+template <class V, class X, int kJ, int kM>
 __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int n) {
     constexpr int N = Vec16<X>::N;
+    constexpr bool kLast = kJ == kM - 1;
+    constexpr int kOld = kLast ? kM - 1 : 0;  // earlier directions read for x
     Scalars* S = c.S;
     const int g = gridDim.x;
     const int nv = n / N;
     const int stride = g * blockDim.x;
     const int v0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (S->done) {
-        // a pair cut after its even update: add the pending alpha_prev p_prev
-        if (kOdd && S->lag) {
-            const double ap = S->alpha_prev;
-            for (int v = v0; v < nv; v += stride) {
-                double xe[N];
-                X pp[N];
-                dload<N>(c.x, v, xe);
-                vload(c.p_alt, v, pp);
-#pragma unroll
-                for (int e = 0; e < N; ++e) xe[e] += ap * (double)pp[e];
-                dstore<N>(c.x, v, xe);
-            }
-            if (blockIdx.x == g - 1) {
-                const int i = nv * N + (int)threadIdx.x;
-                if (i < n) c.x[i] += ap * (double)c.p_alt[i];
-            }
-        }
+        // a period cut after its update kJ - 1: add the pending steps
+        if (kJ > 0 && S->lag) cgm_flush_pending<V, X, kJ>(c, S, n);
         return;
     }
     constexpr int kSlots[3] = {kSlotPQ, kSlotRQ, kSlotQQ};
@@ -654,13 +684,14 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
     // first vector in flight while the partial sums are re-added:
     // unconditional loads at a clamped index (the vectors hold n + 4 elements)
     double xv[N];
-    X pv[N], rv[N], qv[N], ppv[N];
+    X pv[N], rv[N], qv[N], ppv[kOld > 0 ? kOld : 1][N];
     const bool have = v0 < nv;
     {
         const int vs = have ? v0 : nv;
-        if (kOdd) {
+        if constexpr (kLast) {
             dload<N>(c.x, vs, xv);
-            vload(c.p_alt, vs, ppv);
+#pragma unroll
+            for (int i = 0; i < kOld; ++i) vload(c.pb[i], vs, ppv[i]);
         }
         vload(c.p, vs, pv);
         vload(c.r, vs, rv);
@@ -669,22 +700,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
     ps.finish(c.red, kSlots, gs, d3);
     const double pq = d3[0], rq = d3[1], qq = d3[2];
     if (!cg_pq_ok(pq)) {
-        if (kOdd) {  // keep the even iteration's step
-            const double ap = S->alpha_prev;
-            for (int v = v0; v < nv; v += stride) {
-                double xe[N];
-                X pp[N];
-                dload<N>(c.x, v, xe);
-                vload(c.p_alt, v, pp);
-#pragma unroll
-                for (int e = 0; e < N; ++e) xe[e] += ap * (double)pp[e];
-                dstore<N>(c.x, v, xe);
-            }
-            if (blockIdx.x == g - 1) {
-                const int i = nv * N + (int)threadIdx.x;
-                if (i < n) c.x[i] += ap * (double)c.p_alt[i];
-            }
-        }
+        cgm_flush_pending<V, X, kJ>(c, S, n);  // keep the period's earlier steps
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             S->breakdown = (pq == 0.0);
             S->nonfinite = !isfinite(pq);
@@ -698,33 +714,37 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
     double rr_next = rr - 2.0 * alpha * rq + alpha * alpha * qq;
     if (!(rr_next > 0.0)) rr_next = 0.0;  // rounding: restart the direction
     const double beta = rr_next / rr;
-    const double ap = kOdd ? S->alpha_prev : 0.0;
+    double al[kOld > 0 ? kOld : 1];
+#pragma unroll
+    for (int i = 0; i < kOld; ++i) al[i] = S->alpha_q[i];
     double acc[1] = {0.0};
     auto body = [&](int v, double (&xe)[N], const X (&pe)[N], const X (&re)[N], const X (&qe)[N],
-                    const X (&ppe)[N]) {
+                    const X (&ppe)[kOld > 0 ? kOld : 1][N]) {
         X rn[N], pn[N];
 #pragma unroll
         for (int e = 0; e < N; ++e) {
             const double p0 = (double)pe[e];
-            if (kOdd) {
-                xe[e] += ap * (double)ppe[e];
+            if constexpr (kLast) {
+#pragma unroll
+                for (int i = 0; i < kOld; ++i) xe[e] += al[i] * (double)ppe[i][e];
                 xe[e] += alpha * p0;
             }
             rn[e] = (X)((double)re[e] - alpha * (double)qe[e]);
             pn[e] = (X)((double)rn[e] + beta * p0);
             acc[0] += (double)rn[e] * (double)rn[e];
         }
-        if (kOdd) dstore<N>(c.x, v, xe);
+        if constexpr (kLast) dstore<N>(c.x, v, xe);
         vstore(c.r, v, rn);
         vstore(c.p_alt, v, pn);
     };
     if (have) body(v0, xv, pv, rv, qv, ppv);
     for (int v = v0 + stride; v < nv; v += stride) {
         double xe[N];
-        X pe[N], re[N], qe[N], ppe[N];
-        if (kOdd) {
+        X pe[N], re[N], qe[N], ppe[kOld > 0 ? kOld : 1][N];
+        if constexpr (kLast) {
             dload<N>(c.x, v, xe);
-            vload(c.p_alt, v, ppe);
+#pragma unroll
+            for (int i = 0; i < kOld; ++i) vload(c.pb[i], v, ppe[i]);
         }
         vload(c.p, v, pe);
         vload(c.r, v, re);
@@ -735,9 +755,10 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
         const int i = nv * N + (int)threadIdx.x;
         if (i < n) {
             const double p0 = (double)c.p[i];
-            if (kOdd) {
+            if constexpr (kLast) {
                 double xi = c.x[i];
-                xi += ap * (double)c.p_alt[i];
+#pragma unroll
+                for (int o = 0; o < kOld; ++o) xi += al[o] * (double)c.pb[o][i];
                 xi += alpha * p0;
                 c.x[i] = xi;
             }
@@ -753,10 +774,8 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
         S->beta = beta;
         S->pq = pq;
         S->rr_blocks = gridDim.x;
-        if (!kOdd) {
-            S->alpha_prev = alpha;
-            S->lag = 1;
-        }
+        if constexpr (!kLast) S->alpha_q[kJ] = alpha;
+        if constexpr (kJ == 0) S->lag = 1;
         ++S->iter;
     }
 }

@@ -264,6 +264,35 @@ def test_cg_lds_window_overflow_takes_gather_path():
     assert _rel(res[1][1], res[3][1]) <= 1e-5
 
 
+@pytest.mark.parametrize("max_iter,tol", [(7, 1e-30), (13, 1e-30), (33, 1e-30), (20000, 1e-9)])
+@pytest.mark.parametrize("check_every", [8, 32])
+def test_cg_mixed_x_period_is_bitwise_neutral(max_iter, tol, check_every):
+    """MIXED split CG adds alpha_k p_k to the FP64 x once per period of 2, 4
+    or 8 iterations, in iteration order: the same additions as every
+    iteration, so x, the iteration count and the history are bitwise equal
+    for every period, also when max_iter, convergence or a residual
+    replacement cuts a period.  (Host checks, and so replacements, fall
+    after whole chunks: check_every is a multiple of every period here.)"""
+    a, b, _ = gen.poisson3d(14, seed_mesh=2, seed_rhs=3)
+    out = {}
+    with _solver(a, b) as s:
+        for per in (2, 4, 8):
+            s.set_x(np.zeros(a.n))
+            rep = s.solve("cg", "tolerance", "mixed", tol=tol, max_iter=max_iter, cg_kernels=2, x_period=per,
+                          check_every=check_every)
+            out[per] = (rep, s.get_x(), s.history())
+    for per in (4, 8):
+        assert out[per][0]["iterations"] == out[2][0]["iterations"]
+        assert out[per][0]["replacements"] == out[2][0]["replacements"]
+        assert np.array_equal(out[per][1], out[2][1])
+        assert np.array_equal(out[per][2], out[2][2])
+    if tol > 1e-20:
+        assert out[2][0]["status"] == 0
+        assert np.linalg.norm(b - a.matvec(out[2][1])) / np.linalg.norm(b) <= tol
+    else:
+        assert out[2][0]["iterations"] == max_iter
+
+
 def test_cg_one_kernel_timing_hook():
     a, b, _ = gen.poisson3d(32, seed_mesh=1, seed_rhs=2)
     with _solver(a, b) as s:
