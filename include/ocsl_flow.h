@@ -95,12 +95,23 @@ typedef struct {
     int method_momentum, method_pressure, method_enthalpy;
     int mode;          /* ocsl_mode; REFERENCE is the drop-in */
     int precision;     /* ocsl_precision (TOLERANCE mode only) */
+    int support;       /* ocsl_flow_support: where the SIMPLE step's support loops run */
     ocsl_options solve_options;
 } ocsl_flow_params;
 
+/* Where the support loops of the SIMPLE step run (boundary updates, face mass
+ * fluxes, pressure and pressure-correction gradients, mass-flux correction,
+ * the field updates after each solve: cflow_solver.cpp:192-227, 286-476,
+ * cflow_solver_supply.cpp:25-214).  DEVICE (default): HIP kernels, the flow
+ * state stays resident in device memory between steps and the solutions are
+ * read where the solver leaves them.  HOST: a host restatement that uploads
+ * each equation's inputs, kept as an independent cross-check. */
+typedef enum { OCSL_SUPPORT_DEVICE = 0, OCSL_SUPPORT_HOST = 1 } ocsl_flow_support;
+
 /* Defaults: air (gamma 1.4, M 28.9651159e-3 kg/mol, Tref 298.15 K, Sutherland
  * 1.458e-6 / 110.4, Pr 0.713), transient, blended, inviscid, REFERENCE-mode
- * ILU(0)-BiCGSTAB for every equation, no boundary conditions. */
+ * ILU(0)-BiCGSTAB for every equation, no boundary conditions, support loops on
+ * the device. */
 void ocsl_flow_default_params(ocsl_flow_params* prm);
 
 /* Flow state (CFlow_solver::CFlow_cell / CFlow_face / CFlow_BC_face).  Any

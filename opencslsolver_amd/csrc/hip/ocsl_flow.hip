@@ -9,10 +9,13 @@
 // loop, then the cell loop: cflow_solver.cpp:268-283, 323-342, 429-446), which
 // keeps the assembly deterministic.
 //
-// Host part: the rest of CFlow_solver::iterate (boundary updates, face mass
-// fluxes, gradients, corrections, cflow_solver.cpp:229-476,
-// cflow_solver_supply.cpp) restated in C++ on host arrays, uploading what each
-// equation's kernel reads.
+// Support loops: the rest of CFlow_solver::iterate (boundary updates, face
+// mass fluxes, gradients, corrections, field updates: cflow_solver.cpp:229-476,
+// cflow_solver_supply.cpp) as device kernels over the resident state (the
+// default), or restated in C++ on host arrays that are uploaded for each
+// equation (OCSL_SUPPORT_HOST, a cross-check).  The gradient kernels gather
+// per cell in the same face order as the reference's scatter loops, so each
+// cell's sum is added in the reference's order.
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -84,9 +87,9 @@ struct MeshDev {
     int n, F, B;
     const double* vol;
     const int *fL, *fR;
-    const double *fS, *fe, *fdist, *fksi, *fgeo;
+    const double *fS, *fe, *fdist, *fksi, *fgeo, *falpha;
     const int *bL, *bmark;
-    const double *bS, *barea, *bksi, *bperp;
+    const double *bS, *barea, *bksi, *bperp, *be, *balpha, *bdist;
     const int *cf_ptr, *cf_code, *cf_pos;  // per cell: faces (f << 1 | side), neighbour slot position
     const int *cb_ptr, *cb_face;           // per cell: boundary faces
     const int *row_ptr, *dpos;             // caller layout (int32), diagonal position in the row
@@ -97,6 +100,7 @@ struct StateDev {
     double *f_mdot, *f_rho, *f_qy, *f_stress;
     double *b_T, *b_rho, *b_mdot, *b_h, *b_qy, *b_u;
     int* b_super;
+    double *gpc, *mach, *b_p, *b_p_out;
 };
 
 enum { kMomentum = 0, kPressure = 1, kEnthalpy = 2 };
@@ -351,6 +355,225 @@ __global__ void __launch_bounds__(kBlock) k_visc_stress(MeshDev m, StateDev s, G
     }
 }
 
+
+// ---------------------------------------------------------------- support loops
+// Face interpolation of calc_face_mdot / correct_face_mdot: the convective
+// interpolation (upwind or blended) by the sign of the mass flux.
+__device__ __forceinline__ double conv_interp_d(int blend, double geo, double mdot, double l, double r) {
+    return blend ? conv_interp<true>(geo, mdot, l, r) : conv_interp<false>(geo, mdot, l, r);
+}
+
+// set_boundary_conditions (cflow_solver.cpp:192-227), one thread per boundary face.
+__global__ void __launch_bounds__(kBlock) k_set_bcs(MeshDev m, StateDev s, FlowPhys ph) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.B) return;
+    const Gas& g = ph.gas;
+    const int L = m.bL[b], t = ph.bc[m.bmark[b]].type;
+    if (t == OCSL_BC_VELOCITY_INLET) {
+        s.b_p[b] = s.p[L];
+        s.b_rho[b] = g.density(s.p[L], s.b_T[b]);
+        s.b_mdot[b] = s.b_rho[b] * Gas::dot3(s.b_u + 3 * b, m.bS + 3 * b);
+        s.b_h[b] = g.enthalpy(s.b_T[b], s.b_u + 3 * b);
+    }
+    if (t == OCSL_BC_PRESSURE_OUTLET) {
+        s.b_T[b] = s.T[L];
+        for (int k = 0; k < 3; ++k) s.b_u[3 * b + k] = s.u[3 * L + k];
+        s.b_rho[b] = g.density(s.b_p[b], s.T[L]);
+        s.b_h[b] = g.enthalpy(s.T[L], s.u + 3 * L);
+        // both branches mark the face supersonic (cflow_solver.cpp:208-217)
+        s.b_super[b] = 1;
+        if ((s.b_mdot[b] / s.rho[L] / m.barea[b]) >= g.sound(s.T[L])) s.b_p[b] = s.p[L];
+        else s.b_p[b] = s.b_p_out[b];
+    }
+    if (t == OCSL_BC_WALL) s.b_p[b] = s.p[L] + Gas::dot3(s.gp + 3 * L, m.be + 3 * b) * m.bdist[b];
+}
+
+// calc_press_grad (cflow_solver_supply.cpp:132-175) gathered per cell: the
+// cell's faces in ascending order (+ as left cell, - as right), then its
+// boundary faces, then the division by the volume.
+__global__ void __launch_bounds__(kBlock) k_press_grad(MeshDev m, StateDev s, FlowPhys ph) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n) return;
+    double gv[3] = {0.0, 0.0, 0.0};
+    for (int e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
+        const int code = m.cf_code[e], f = code >> 1;
+        const int L = m.fL[f], R = m.fR[f];
+        double fp = (s.ap[L] + s.ap[R]);
+        fp = (s.ap[L] * s.p[L] + s.ap[R] * s.p[R]) / fp;
+        for (int k = 0; k < 3; ++k) {
+            if (code & 1) gv[k] = gv[k] - fp * m.fS[3 * f + k];
+            else gv[k] = gv[k] + fp * m.fS[3 * f + k];
+        }
+    }
+    for (int e = m.cb_ptr[i]; e < m.cb_ptr[i + 1]; ++e) {
+        const int b = m.cb_face[e], t = ph.bc[m.bmark[b]].type;
+        double pv;
+        if (t == OCSL_BC_WALL || t == OCSL_BC_SYMMETRY || t == OCSL_BC_EMPTY || t == OCSL_BC_VELOCITY_INLET)
+            pv = s.p[i];
+        else if (t == OCSL_BC_PRESSURE_OUTLET || t == OCSL_BC_SUPERSONIC_INLET)
+            pv = s.b_p[b];
+        else
+            continue;
+        for (int k = 0; k < 3; ++k) gv[k] = gv[k] + pv * m.bS[3 * b + k];
+    }
+    for (int k = 0; k < 3; ++k) s.gp[3 * i + k] = (1.0 / m.vol[i]) * gv[k];
+}
+
+// calc_press_corr_grad (cflow_solver_supply.cpp:177-214), gathered per cell.
+__global__ void __launch_bounds__(kBlock) k_press_corr_grad(MeshDev m, StateDev s, FlowPhys ph) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n) return;
+    double gv[3] = {0.0, 0.0, 0.0};
+    for (int e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
+        const int code = m.cf_code[e], f = code >> 1;
+        const int L = m.fL[f], R = m.fR[f];
+        double fp = m.fgeo[f] * s.pc[L];
+        fp += (1.0 - m.fgeo[f]) * s.pc[R];
+        for (int k = 0; k < 3; ++k) {
+            if (code & 1) gv[k] = gv[k] - fp * m.fS[3 * f + k];
+            else gv[k] = gv[k] + fp * m.fS[3 * f + k];
+        }
+    }
+    for (int e = m.cb_ptr[i]; e < m.cb_ptr[i + 1]; ++e) {
+        const int b = m.cb_face[e], t = ph.bc[m.bmark[b]].type;
+        if (t == OCSL_BC_WALL || t == OCSL_BC_SYMMETRY || t == OCSL_BC_EMPTY || t == OCSL_BC_VELOCITY_INLET ||
+            (s.b_super[b] && t == OCSL_BC_PRESSURE_OUTLET))
+            for (int k = 0; k < 3; ++k) gv[k] = gv[k] + s.pc[i] * m.bS[3 * b + k];
+    }
+    for (int k = 0; k < 3; ++k) s.gpc[3 * i + k] = (1.0 / m.vol[i]) * gv[k];
+}
+
+// calc_face_mdot, interior faces (cflow_solver_supply.cpp:25-46).
+__global__ void __launch_bounds__(kBlock) k_face_mdot(MeshDev m, StateDev s, int blend) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    const int L = m.fL[f], R = m.fR[f];
+    const double geo = m.fgeo[f];
+    const double qy = ((m.vol[L] / s.ap[L]) + (m.vol[R] / s.ap[R])) * Gas::dot3(m.falpha + 3 * f, m.fS + 3 * f);
+    s.f_qy[f] = qy;
+    double pgf[3], vf[3], dvec[3];
+    for (int k = 0; k < 3; ++k) {
+        pgf[k] = geo * s.gp[3 * L + k];
+        pgf[k] += (1.0 - geo) * s.gp[3 * R + k];
+        vf[k] = geo * s.u[3 * L + k];
+        vf[k] += (1.0 - geo) * s.u[3 * R + k];
+        dvec[k] = m.fdist[f] * m.fe[3 * f + k];
+    }
+    const double YF = qy * (s.p[R] - s.p[L] - Gas::dot3(pgf, dvec));
+    double mdot = Gas::dot3(vf, m.fS + 3 * f) - YF;
+    const double rf = conv_interp_d(blend, geo, mdot, s.rho[L], s.rho[R]);
+    s.f_rho[f] = rf;
+    mdot *= rf;
+    s.f_mdot[f] = mdot;
+}
+
+// calc_face_mdot, boundary faces (cflow_solver_supply.cpp:54-93).
+__global__ void __launch_bounds__(kBlock) k_bc_mdot(MeshDev m, StateDev s, FlowPhys ph) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.B) return;
+    const Gas& g = ph.gas;
+    const int L = m.bL[b], t = ph.bc[m.bmark[b]].type;
+    if (t == OCSL_BC_VELOCITY_INLET)
+        s.b_mdot[b] = g.density(s.p[L], s.b_T[b]) * Gas::dot3(s.b_u + 3 * b, m.bS + 3 * b);
+    if (t == OCSL_BC_SUPERSONIC_INLET)
+        s.b_qy[b] = (m.vol[L]) / (s.ap[L]) * Gas::dot3(m.balpha + 3 * b, m.bS + 3 * b);
+    if (t == OCSL_BC_PRESSURE_OUTLET) {
+        if (s.b_super[b]) {
+            s.b_mdot[b] = g.density(s.p[L], s.T[L]) * Gas::dot3(s.u + 3 * L, m.bS + 3 * b);
+            s.b_qy[b] = 1.0;
+        } else {
+            const double qy = (m.vol[L]) / (s.ap[L]) * Gas::dot3(m.balpha + 3 * b, m.bS + 3 * b);
+            s.b_qy[b] = qy;
+            double dvec[3];
+            for (int k = 0; k < 3; ++k) dvec[k] = m.bdist[b] * m.be[3 * b + k];
+            const double YF = qy * (s.b_p[b] - s.p[L] - 0.5 * Gas::dot3(s.gp + 3 * L, dvec));
+            double mdot = Gas::dot3(s.u + 3 * L, m.bS + 3 * b) - YF;
+            mdot *= s.b_rho[b];
+            s.b_mdot[b] = mdot;
+        }
+    }
+}
+
+// correct_face_mdot (cflow_solver_supply.cpp:96-130): threads [0, F) the
+// interior faces, [F, F + B) the boundary faces.
+__global__ void __launch_bounds__(kBlock) k_correct_mdot(MeshDev m, StateDev s, FlowPhys ph, int blend) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < m.F) {
+        const int f = q, L = m.fL[f], R = m.fR[f];
+        const double geo = m.fgeo[f], md = s.f_mdot[f];
+        const double Tf = conv_interp_d(blend, geo, md, s.T[L], s.T[R]);
+        const double pcf = conv_interp_d(blend, geo, md, s.pc[L], s.pc[R]);
+        double v = md;
+        v += md / s.f_rho[f] * ph.gas.dp_drho(Tf) * pcf;
+        v += s.f_qy[f] * s.f_rho[f] * (s.pc[L] - s.pc[R]);
+        s.f_mdot[f] = v;
+    } else if (q < m.F + m.B) {
+        const int b = q - m.F;
+        if (ph.bc[m.bmark[b]].type == OCSL_BC_PRESSURE_OUTLET && !s.b_super[b])
+            s.b_mdot[b] += s.b_qy[b] * s.b_rho[b] * (s.pc[m.bL[b]]);
+    }
+}
+
+// u_d = the momentum solve's solution (cflow_solver.cpp:290-299).
+__global__ void __launch_bounds__(kBlock) k_take_u(StateDev s, const double* __restrict__ x, int d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) s.u[3 * i + d] = x[i];
+}
+
+// p' = 0.5 x, p += p' (cflow_solver.cpp:350-357).
+__global__ void __launch_bounds__(kBlock) k_take_pc(StateDev s, const double* __restrict__ x, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double pc = kRelaxP * x[i];
+    s.pc[i] = pc;
+    s.p[i] += pc;
+}
+
+// Boundary pressure after the pressure update (cflow_solver.cpp:361-374).
+__global__ void __launch_bounds__(kBlock) k_bp_after_p(MeshDev m, StateDev s, FlowPhys ph) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.B) return;
+    const int t = ph.bc[m.bmark[b]].type;
+    if (t == OCSL_BC_VELOCITY_INLET) s.b_p[b] = s.p[m.bL[b]];
+    if (t == OCSL_BC_PRESSURE_OUTLET && s.b_super[b]) s.b_p[b] = s.p[m.bL[b]];
+}
+
+// Density from the new pressure and the velocity correction
+// (cflow_solver.cpp:390-396, 418-426): rho = rho(p, T); u -= 0.8 V grad p' / ap_vec.
+__global__ void __launch_bounds__(kBlock) k_rho_u(MeshDev m, StateDev s, FlowPhys ph) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n) return;
+    s.rho[i] = ph.gas.density(s.p[i], s.T[i]);
+    const double a = kRelaxU * (m.vol[i]);
+    for (int k = 0; k < 3; ++k) {
+        double den = s.ap_vec[3 * i + k];
+        if (den == 0.0) den = 2.2250738585072014e-308;  // DBL_MIN
+        s.u[3 * i + k] -= a * (s.gpc[3 * i + k] / den);
+    }
+}
+
+// Boundary density after the pressure update (cflow_solver.cpp:401-416).
+__global__ void __launch_bounds__(kBlock) k_brho(MeshDev m, StateDev s, FlowPhys ph) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.B) return;
+    const Gas& g = ph.gas;
+    const int L = m.bL[b], t = ph.bc[m.bmark[b]].type;
+    if (t == OCSL_BC_VELOCITY_INLET) s.b_rho[b] = g.density(s.p[L], s.b_T[b]);
+    if (t == OCSL_BC_PRESSURE_OUTLET)
+        s.b_rho[b] = s.b_super[b] ? g.density(s.p[L], s.T[L]) : g.density(s.b_p[b], s.T[L]);
+}
+
+// h = the enthalpy solve's solution, T = T(h, u), Mach (cflow_solver.cpp:462-472).
+__global__ void __launch_bounds__(kBlock) k_take_h(StateDev s, Gas g, const double* __restrict__ x, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double h = x[i];
+    s.h[i] = h;
+    const double T = g.temperature(h, s.u + 3 * i);
+    s.T[i] = T;
+    s.mach[i] = Gas::mag3(s.u + 3 * i) / g.sound(T);
+}
+
 }  // namespace ocsl
 
 using namespace ocsl;
@@ -376,14 +599,17 @@ struct ocsl_flow_s {
     double urf = 0.5;  // implicit_underrelaxation (cflow_solver.h:177)
     HostMesh mesh;
     HostFlow st;
+    bool dev = true;  // support loops on the device, state resident there
     // device mesh
-    DBuf<double> d_vol, d_fS, d_fe, d_fdist, d_fksi, d_fgeo, d_bS, d_barea, d_bksi, d_bperp;
+    DBuf<double> d_vol, d_fS, d_fe, d_fdist, d_fksi, d_fgeo, d_falpha, d_bS, d_barea, d_bksi, d_bperp, d_be, d_balpha,
+        d_bdist;
     DBuf<int> d_fL, d_fR, d_bL, d_bmark, d_cf_ptr, d_cf_code, d_cf_pos, d_cb_ptr, d_cb_face, d_row_ptr, d_dpos;
     // device state
     DBuf<double> d_u, d_u_old, d_rho, d_rho_old, d_p, d_p_old, d_pc, d_gp, d_T, d_h, d_h_old, d_ap, d_ap_vec, d_dt;
     DBuf<double> d_f_mdot, d_f_rho, d_f_qy, d_f_stress;
     DBuf<double> d_b_T, d_b_rho, d_b_mdot, d_b_h, d_b_qy, d_b_u;
     DBuf<int> d_b_super;
+    DBuf<double> d_gpc, d_mach, d_b_p, d_b_p_out;
     // assembled system
     DBuf<double> d_val, d_b, d_x0;
     int64_t nnz_in = 0;
@@ -391,14 +617,16 @@ struct ocsl_flow_s {
     bool initialized = false;
 
     MeshDev mesh_view() const {
-        return {mesh.n, mesh.F, mesh.B, d_vol.p, d_fL.p, d_fR.p, d_fS.p, d_fe.p, d_fdist.p, d_fksi.p, d_fgeo.p,
-                d_bL.p, d_bmark.p, d_bS.p, d_barea.p, d_bksi.p, d_bperp.p, d_cf_ptr.p, d_cf_code.p, d_cf_pos.p,
-                d_cb_ptr.p, d_cb_face.p, d_row_ptr.p, d_dpos.p};
+        return {mesh.n,     mesh.F,       mesh.B,      d_vol.p,    d_fL.p,      d_fR.p,       d_fS.p,
+                d_fe.p,     d_fdist.p,    d_fksi.p,    d_fgeo.p,   d_falpha.p,  d_bL.p,       d_bmark.p,
+                d_bS.p,     d_barea.p,    d_bksi.p,    d_bperp.p,  d_be.p,      d_balpha.p,   d_bdist.p,
+                d_cf_ptr.p, d_cf_code.p,  d_cf_pos.p,  d_cb_ptr.p, d_cb_face.p, d_row_ptr.p,  d_dpos.p};
     }
     StateDev state_view() const {
         return {d_u.p, d_u_old.p, d_rho.p, d_rho_old.p, d_p.p, d_p_old.p, d_pc.p, d_gp.p, d_T.p, d_h.p,
                 d_h_old.p, d_ap.p, d_ap_vec.p, d_dt.p, d_f_mdot.p, d_f_rho.p, d_f_qy.p, d_f_stress.p,
-                d_b_T.p, d_b_rho.p, d_b_mdot.p, d_b_h.p, d_b_qy.p, d_b_u.p, d_b_super.p};
+                d_b_T.p, d_b_rho.p, d_b_mdot.p, d_b_h.p, d_b_qy.p, d_b_u.p, d_b_super.p,
+                d_gpc.p, d_mach.p, d_b_p.p, d_b_p_out.p};
     }
     FlowPhys phys() const {
         FlowPhys ph{};
@@ -649,6 +877,77 @@ void upload_cells(ocsl_flow_s* fl, int eq) {
     }
 }
 
+// The whole state between host mirror and device (device-resident mode).
+void upload_all(ocsl_flow_s* fl) {
+    const HostFlow& s = fl->st;
+    hipStream_t st = fl->solver->stream;
+    up(fl->d_u, s.u, st);
+    up(fl->d_u_old, s.u_old, st);
+    up(fl->d_rho, s.rho, st);
+    up(fl->d_rho_old, s.rho_old, st);
+    up(fl->d_p, s.p, st);
+    up(fl->d_p_old, s.p_old, st);
+    up(fl->d_pc, s.pc, st);
+    up(fl->d_gp, s.gp, st);
+    up(fl->d_gpc, s.gpc, st);
+    up(fl->d_T, s.T, st);
+    up(fl->d_h, s.h, st);
+    up(fl->d_h_old, s.h_old, st);
+    up(fl->d_ap, s.ap, st);
+    up(fl->d_ap_vec, s.ap_vec, st);
+    up(fl->d_dt, s.dt, st);
+    up(fl->d_mach, s.mach, st);
+    up(fl->d_f_mdot, s.f_mdot, st);
+    up(fl->d_f_rho, s.f_rho, st);
+    up(fl->d_f_qy, s.f_qy, st);
+    up(fl->d_f_stress, s.f_stress, st);
+    up(fl->d_b_T, s.b_T, st);
+    up(fl->d_b_p, s.b_p, st);
+    up(fl->d_b_p_out, s.b_p_out, st);
+    up(fl->d_b_rho, s.b_rho, st);
+    up(fl->d_b_mdot, s.b_mdot, st);
+    up(fl->d_b_h, s.b_h, st);
+    up(fl->d_b_qy, s.b_qy, st);
+    up(fl->d_b_u, s.b_u, st);
+    up(fl->d_b_super, s.b_super, st);
+    OCSL_HIP(hipStreamSynchronize(st));
+}
+
+void download_all(ocsl_flow_s* fl) {
+    HostFlow& s = fl->st;
+    hipStream_t st = fl->solver->stream;
+    down(s.u, fl->d_u, st);
+    down(s.u_old, fl->d_u_old, st);
+    down(s.rho, fl->d_rho, st);
+    down(s.rho_old, fl->d_rho_old, st);
+    down(s.p, fl->d_p, st);
+    down(s.p_old, fl->d_p_old, st);
+    down(s.pc, fl->d_pc, st);
+    down(s.gp, fl->d_gp, st);
+    down(s.gpc, fl->d_gpc, st);
+    down(s.T, fl->d_T, st);
+    down(s.h, fl->d_h, st);
+    down(s.h_old, fl->d_h_old, st);
+    down(s.ap, fl->d_ap, st);
+    down(s.ap_vec, fl->d_ap_vec, st);
+    down(s.dt, fl->d_dt, st);
+    down(s.mach, fl->d_mach, st);
+    down(s.f_mdot, fl->d_f_mdot, st);
+    down(s.f_rho, fl->d_f_rho, st);
+    down(s.f_qy, fl->d_f_qy, st);
+    down(s.f_stress, fl->d_f_stress, st);
+    down(s.b_T, fl->d_b_T, st);
+    down(s.b_p, fl->d_b_p, st);
+    down(s.b_p_out, fl->d_b_p_out, st);
+    down(s.b_rho, fl->d_b_rho, st);
+    down(s.b_mdot, fl->d_b_mdot, st);
+    down(s.b_h, fl->d_b_h, st);
+    down(s.b_qy, fl->d_b_qy, st);
+    down(s.b_u, fl->d_b_u, st);
+    down(s.b_super, fl->d_b_super, st);
+    OCSL_HIP(hipStreamSynchronize(st));
+}
+
 template <int kEq>
 void launch_assemble(ocsl_flow_s* fl, int d) {
     const ocsl_flow_params& p = fl->prm;
@@ -678,11 +977,11 @@ void launch_assemble(ocsl_flow_s* fl, int d) {
 void assemble(ocsl_flow_s* fl, int eq) {
     OCSL_REQUIRE(eq >= OCSL_EQ_MOMENTUM_X && eq <= OCSL_EQ_ENTHALPY, OCSL_ERR_ARG, "bad equation");
     hipStream_t st = fl->solver->stream;
-    upload_cells(fl, eq);
+    if (!fl->dev) upload_cells(fl, eq);
     if (eq <= OCSL_EQ_MOMENTUM_Z) launch_assemble<kMomentum>(fl, eq);
     else if (eq == OCSL_EQ_PRESSURE) launch_assemble<kPressure>(fl, 0);
     else launch_assemble<kEnthalpy>(fl, 0);
-    if (eq <= OCSL_EQ_MOMENTUM_Z) {
+    if (eq <= OCSL_EQ_MOMENTUM_Z && !fl->dev) {
         down(fl->st.ap, fl->d_ap, st);
         down(fl->st.ap_vec, fl->d_ap_vec, st);
     }
@@ -700,14 +999,18 @@ void assemble(ocsl_flow_s* fl, int eq) {
 
 void viscous_stress(ocsl_flow_s* fl) {
     hipStream_t st = fl->solver->stream;
-    up(fl->d_u, fl->st.u, st);
-    up(fl->d_T, fl->st.T, st);
+    if (!fl->dev) {
+        up(fl->d_u, fl->st.u, st);
+        up(fl->d_T, fl->st.T, st);
+    }
     const int F = fl->mesh.F;
     if (F > 0)
         k_visc_stress<<<(F + kBlock - 1) / kBlock, kBlock, 0, st>>>(fl->mesh_view(), fl->state_view(), fl->gas);
     OCSL_HIP(hipGetLastError());
-    down(fl->st.f_stress, fl->d_f_stress, st);
-    OCSL_HIP(hipStreamSynchronize(st));
+    if (!fl->dev) {
+        down(fl->st.f_stress, fl->d_f_stress, st);
+        OCSL_HIP(hipStreamSynchronize(st));
+    }
 }
 
 // Solves the assembled system with the configured method; the solution lands
@@ -726,13 +1029,61 @@ void solve(ocsl_flow_s* fl, int method) {
         res = rep.residual;
     }
     fl->hist.push_back(res);
+    if (fl->dev) return;  // the solution stays in the handle's x for the device kernels
     fl->x_host.resize(fl->mesh.n);
     const ocsl_status st = ocsl_get_x(h, fl->x_host.data(), OCSL_HOST);
     if (st < 0) throw Error{st, std::string("flow solve: ") + ocsl_last_error()};
 }
 
+// CFlow_solver::iterate (cflow_solver.cpp:229-476) with the state resident in
+// device memory: the support loops as kernels on the solver's stream, the
+// solutions read from the handle's x.
+void iterate_dev(ocsl_flow_s* fl) {
+    const int n = fl->mesh.n, F = fl->mesh.F, B = fl->mesh.B;
+    hipStream_t st = fl->solver->stream;
+    const MeshDev m = fl->mesh_view();
+    const StateDev s = fl->state_view();
+    const FlowPhys ph = fl->phys();
+    const int blend = fl->prm.blended ? 1 : 0;
+    auto grid = [](int cnt) { return (cnt + kBlock - 1) / kBlock; };
+    const double* x = nullptr;
+    if (B) k_set_bcs<<<grid(B), kBlock, 0, st>>>(m, s, ph);
+    k_press_grad<<<grid(n), kBlock, 0, st>>>(m, s, ph);
+    k_fill<double><<<grid_for(n), kBlock, 0, st>>>(n, 0.0, s.ap);
+    OCSL_HIP(hipGetLastError());
+    if (fl->prm.viscous) viscous_stress(fl);
+    for (int d = 0; d < 3; ++d) {
+        assemble(fl, OCSL_EQ_MOMENTUM_X + d);
+        solve(fl, fl->prm.method_momentum);
+        x = fl->solver->x_own.p;
+        k_take_u<<<grid(n), kBlock, 0, st>>>(s, x, d, n);
+    }
+    if (F) k_face_mdot<<<grid(F), kBlock, 0, st>>>(m, s, blend);
+    if (B) k_bc_mdot<<<grid(B), kBlock, 0, st>>>(m, s, ph);
+    OCSL_HIP(hipGetLastError());
+    assemble(fl, OCSL_EQ_PRESSURE);
+    solve(fl, fl->prm.method_pressure);
+    x = fl->solver->x_own.p;
+    k_take_pc<<<grid(n), kBlock, 0, st>>>(s, x, n);
+    if (B) k_bp_after_p<<<grid(B), kBlock, 0, st>>>(m, s, ph);
+    k_press_corr_grad<<<grid(n), kBlock, 0, st>>>(m, s, ph);
+    if (F + B) k_correct_mdot<<<grid(F + B), kBlock, 0, st>>>(m, s, ph, blend);
+    k_rho_u<<<grid(n), kBlock, 0, st>>>(m, s, ph);
+    if (B) k_brho<<<grid(B), kBlock, 0, st>>>(m, s, ph);
+    OCSL_HIP(hipGetLastError());
+    assemble(fl, OCSL_EQ_ENTHALPY);
+    solve(fl, fl->prm.method_enthalpy);
+    x = fl->solver->x_own.p;
+    k_take_h<<<grid(n), kBlock, 0, st>>>(s, fl->gas, x, n);
+    OCSL_HIP(hipGetLastError());
+}
+
 // CFlow_solver::iterate (cflow_solver.cpp:229-476)
 void iterate(ocsl_flow_s* fl) {
+    if (fl->dev) {
+        iterate_dev(fl);
+        return;
+    }
     const HostMesh& m = fl->mesh;
     HostFlow& s = fl->st;
     const Gas& g = fl->gas;
@@ -788,10 +1139,20 @@ void iterate(ocsl_flow_s* fl) {
 // update_timestep (cflow_solver.cpp:497-519)
 void update_timestep(ocsl_flow_s* fl) {
     HostFlow& s = fl->st;
-    s.u_old = s.u;
-    s.rho_old = s.rho;
-    s.p_old = s.p;
-    s.h_old = s.h;
+    if (fl->dev) {
+        hipStream_t st = fl->solver->stream;
+        const size_t n = fl->mesh.n;
+        const auto d2d = hipMemcpyDeviceToDevice;
+        OCSL_HIP(hipMemcpyAsync(fl->d_u_old.p, fl->d_u.p, sizeof(double) * 3 * n, d2d, st));
+        OCSL_HIP(hipMemcpyAsync(fl->d_rho_old.p, fl->d_rho.p, sizeof(double) * n, d2d, st));
+        OCSL_HIP(hipMemcpyAsync(fl->d_p_old.p, fl->d_p.p, sizeof(double) * n, d2d, st));
+        OCSL_HIP(hipMemcpyAsync(fl->d_h_old.p, fl->d_h.p, sizeof(double) * n, d2d, st));
+    } else {
+        s.u_old = s.u;
+        s.rho_old = s.rho;
+        s.p_old = s.p;
+        s.h_old = s.h;
+    }
     const ocsl_flow_params& p = fl->prm;
     if (!p.transient && fl->urf < p.implicit_underrelax_max)
         fl->urf += (p.implicit_underrelax_max - p.implicit_underrelax) / p.ramp_iterations;
@@ -860,6 +1221,7 @@ void initialize(ocsl_flow_s* fl) {
     bc_mdot(fl);
     if (!p.transient) fl->urf = p.implicit_underrelax;
     else std::fill(s.dt.begin(), s.dt.end(), p.dt);
+    if (fl->dev) upload_all(fl);
     fl->initialized = true;
 }
 
@@ -890,6 +1252,11 @@ void alloc_state(ocsl_flow_s* fl) {
     for (auto* d : {&fl->d_b_T, &fl->d_b_rho, &fl->d_b_mdot, &fl->d_b_h, &fl->d_b_qy}) d->alloc(B);
     fl->d_b_u.alloc(3 * (size_t)B);
     fl->d_b_super.alloc(B);
+    fl->d_gpc.alloc(3 * (size_t)n);
+    fl->d_mach.alloc(n);
+    fl->d_b_p.alloc(B);
+    fl->d_b_p_out.alloc(B);
+    if (fl->dev) upload_all(fl);
 }
 
 template <class T>
@@ -940,7 +1307,10 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
                      OCSL_ERR_ARG, "bad mesh sizes");
         OCSL_REQUIRE(prm->n_bc >= 0 && prm->n_bc <= OCSL_FLOW_MAX_BC, OCSL_ERR_ARG, "bad boundary-condition count");
         OCSL_REQUIRE(prm->transient || prm->implicit_underrelax > 0.0, OCSL_ERR_ARG, "bad under-relaxation");
+        OCSL_REQUIRE(prm->support == OCSL_SUPPORT_DEVICE || prm->support == OCSL_SUPPORT_HOST, OCSL_ERR_ARG,
+                     "bad support location");
         fl->prm = *prm;
+        fl->dev = prm->support == OCSL_SUPPORT_DEVICE;
         const ocsl_status cs = ocsl_create(&fl->solver, mesh->n_cells, row_ptr, column_index);
         if (cs != OCSL_OK) throw Error{cs, std::string("flow solver handle: ") + ocsl_last_error()};
         HostMesh& m = fl->mesh;
@@ -1028,6 +1398,10 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
         up(fl->d_fdist, m.fdist, s);
         up(fl->d_fksi, m.fksi, s);
         up(fl->d_fgeo, m.fgeo, s);
+        up(fl->d_falpha, m.falpha, s);
+        up(fl->d_be, m.be, s);
+        up(fl->d_balpha, m.balpha, s);
+        up(fl->d_bdist, m.bdist, s);
         up(fl->d_bL, m.bL, s);
         up(fl->d_bmark, m.bmark, s);
         up(fl->d_bS, m.bS, s);
@@ -1089,6 +1463,7 @@ ocsl_status ocsl_flow_set_state(ocsl_flow fl, const ocsl_flow_state* st) {
     if (!fl || !st) return OCSL_ERR_ARG;
     return guarded_f([&] {
         HostFlow& s = fl->st;
+        if (fl->dev) download_all(fl);  // fields not given keep their device values
         copy_in(s.u, st->u);
         copy_in(s.u_old, st->u_old);
         copy_in(s.rho, st->rho);
@@ -1118,7 +1493,8 @@ ocsl_status ocsl_flow_set_state(ocsl_flow fl, const ocsl_flow_state* st) {
         copy_in(s.b_qy, st->b_qy);
         copy_in(s.b_u, st->b_u);
         copy_in(s.b_super, st->b_supersonic);
-        if (st->f_stress) up(fl->d_f_stress, s.f_stress, fl->solver->stream);
+        if (fl->dev) upload_all(fl);
+        else if (st->f_stress) up(fl->d_f_stress, s.f_stress, fl->solver->stream);
         OCSL_HIP(hipStreamSynchronize(fl->solver->stream));
         fl->initialized = true;
     });
@@ -1126,6 +1502,10 @@ ocsl_status ocsl_flow_set_state(ocsl_flow fl, const ocsl_flow_state* st) {
 
 ocsl_status ocsl_flow_get_state(ocsl_flow fl, ocsl_flow_state* st) {
     if (!fl || !st) return OCSL_ERR_ARG;
+    if (fl->dev) {
+        const ocsl_status rs = guarded_f([&] { download_all(fl); });
+        if (rs != OCSL_OK) return rs;
+    }
     const HostFlow& s = fl->st;
     copy_out(st->u, s.u);
     copy_out(st->u_old, s.u_old);
@@ -1199,7 +1579,13 @@ ocsl_status ocsl_flow_time_assemble(ocsl_flow fl, int equation, int reps, double
     return guarded_f([&] {
         OCSL_REQUIRE(equation >= OCSL_EQ_MOMENTUM_X && equation <= OCSL_EQ_ENTHALPY, OCSL_ERR_ARG, "bad equation");
         ocsl_solver* h = fl->solver;
-        upload_cells(fl, equation);
+        if (!fl->dev) upload_cells(fl, equation);
+        DBuf<double> ap_saved;
+        if (fl->dev) {  // the repeated momentum launches accumulate into ap
+            ap_saved.alloc(fl->mesh.n);
+            OCSL_HIP(hipMemcpyAsync(ap_saved.p, fl->d_ap.p, sizeof(double) * fl->mesh.n, hipMemcpyDeviceToDevice,
+                                    h->stream));
+        }
         auto launch = [&] {
             if (equation <= OCSL_EQ_MOMENTUM_Z) launch_assemble<kMomentum>(fl, equation);
             else if (equation == OCSL_EQ_PRESSURE) launch_assemble<kPressure>(fl, 0);
@@ -1215,7 +1601,11 @@ ocsl_status ocsl_flow_time_assemble(ocsl_flow fl, int equation, int reps, double
         *avg_ms = ms / reps;
         *bytes = assemble_bytes(fl, equation);
         // the repeated momentum launches accumulated into ap: restore it
-        up(fl->d_ap, fl->st.ap, h->stream);
+        if (fl->dev)
+            OCSL_HIP(hipMemcpyAsync(fl->d_ap.p, ap_saved.p, sizeof(double) * fl->mesh.n, hipMemcpyDeviceToDevice,
+                                    h->stream));
+        else
+            up(fl->d_ap, fl->st.ap, h->stream);
         OCSL_HIP(hipStreamSynchronize(h->stream));
     });
 }

@@ -1,7 +1,9 @@
 """Python side of include/ocsl_flow.h: the pressure-based flow solver whose
 coefficient assembly runs on the GPU (CFlow_solver's face/cell loops,
-cflow_solver_cell_face_loops.cpp) and whose linear systems go to the HIP
-solvers of the same library."""
+cflow_solver_cell_face_loops.cpp), whose linear systems go to the HIP
+solvers of the same library, and whose remaining SIMPLE-step loops run as
+device kernels over a device-resident state (or, with support=SUPPORT_HOST,
+as a host restatement kept as a cross-check)."""
 from __future__ import annotations
 
 import ctypes
@@ -20,6 +22,8 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 # ocsl_bc_type (enums.h:38 order)
 EMPTY, SYMMETRY, WALL, VELOCITY_INLET, SUPERSONIC_INLET, PRESSURE_OUTLET = range(6)
 MOMENTUM_X, MOMENTUM_Y, MOMENTUM_Z, PRESSURE, ENTHALPY = range(5)
+# ocsl_flow_support: where the SIMPLE step's support loops run
+SUPPORT_DEVICE, SUPPORT_HOST = range(2)
 MAX_BC = 16
 
 MESH_FLOAT = ("face_area_vector", "face_unit", "face_alpha", "face_area", "face_dist", "face_ksi",
@@ -52,7 +56,7 @@ class Params(ctypes.Structure):
                 ("t2", ctypes.c_double), ("n_bc", ctypes.c_int), ("bc", BC * MAX_BC),
                 ("method_momentum", ctypes.c_int), ("method_pressure", ctypes.c_int),
                 ("method_enthalpy", ctypes.c_int), ("mode", ctypes.c_int), ("precision", ctypes.c_int),
-                ("solve_options", _abi.Options)]
+                ("support", ctypes.c_int), ("solve_options", _abi.Options)]
 
 
 CELL_FIELDS = {"u": 3, "u_old": 3, "rho": 1, "rho_old": 1, "p": 1, "p_old": 1, "p_corr": 1, "grad_p": 3,

@@ -98,6 +98,62 @@ def test_flow_run_matches_reference(case):
     _check_history(hist, d["res_hist"])
 
 
+@pytest.mark.parametrize("case", CASES)
+def test_flow_support_device_matches_host(case):
+    """The SIMPLE step's support loops as device kernels over the resident
+    state (the default) against their host restatement, on the reference
+    configurations (walls, inlets, outlets, supersonic, steady): the same run
+    to rounding.  The device kernels contract multiply-adds and the host code
+    does not; after the fixture's steps the two differ by ~1e-11 relative in
+    h, so the bar is the reference-parity bar of the run test above, 1e-10."""
+    d = np.load(os.path.join(GOLDEN, f"flow_{case}.npz"))
+    cfg = FLOW_META[case]
+    out = {}
+    for support in (flow.SUPPORT_DEVICE, flow.SUPPORT_HOST):
+        prm = params_for(cfg)
+        prm.support = support
+        with flow.Flow(flow.mesh_from_npz(d), d["row_ptr"], d["col"], prm) as fl:
+            fl.initialize()
+            fl.run(cfg["steps"], cfg["inner"])
+            out[support] = (fl.get_state(), fl.solve_history())
+    dev, host = out[flow.SUPPORT_DEVICE][0], out[flow.SUPPORT_HOST][0]
+    for key in ("p", "T", "rho", "h", "u", "p_corr", "grad_p", "grad_pc", "mach", "f_mdot", "f_rho", "f_qy", "b_p",
+                "b_rho", "b_mdot", "b_h", "b_qy", "b_T"):
+        assert _close(dev[key], host[key], 1e-10), (case, key)
+    assert np.array_equal(dev["b_supersonic"], host["b_supersonic"])
+    assert len(out[flow.SUPPORT_DEVICE][1]) == len(out[flow.SUPPORT_HOST][1])
+
+
+@pytest.mark.parametrize("viscous,blended,transient", [(1, 1, 1), (0, 0, 0)])
+def test_flow_3d_device_run_matches_host(viscous, blended, transient):
+    """Two SIMPLE steps (two inner iterations each) on a permuted 8^3 hex mesh
+    with a velocity inlet, a pressure outlet, no-slip and slip walls and a
+    symmetry plane: device-resident support loops against the host
+    restatement, TOLERANCE-mode FP64 solves."""
+    rng = np.random.default_rng(5)
+    n = 8
+    perm = rng.permutation(n ** 3)
+    mesh, rp, col = gen_hex(n, perm=perm)
+    bcs = [dict(type=flow.VELOCITY_INLET, velocity=(3.0, 0.5, -0.2), temperature=300.0),
+           dict(type=flow.PRESSURE_OUTLET, pressure=1e5),
+           dict(type=flow.WALL, no_slip=1), dict(type=flow.WALL, no_slip=0),
+           dict(type=flow.SYMMETRY), dict(type=flow.SYMMETRY)]
+    out = {}
+    for support in (flow.SUPPORT_DEVICE, flow.SUPPORT_HOST):
+        prm = flow.default_params(viscous=viscous, blended=blended, transient=transient, dt=1e-4,
+                                  implicit_underrelax=0.6, p0=1e5, t0=300.0, u0=(2.0, 0.0, 0.0), bcs=bcs,
+                                  mode=1, method_pressure=0, support=support,
+                                  solve_options=dict(tol=1e-12, max_iter=2000))
+        with flow.Flow(mesh, rp, col, prm) as fl:
+            fl.initialize()
+            fl.run(2, 2)
+            out[support] = fl.get_state()
+    dev, host = out[flow.SUPPORT_DEVICE], out[flow.SUPPORT_HOST]
+    for key in ("p", "T", "rho", "h", "u", "f_mdot", "b_mdot", "b_p", "b_rho"):
+        assert np.all(np.isfinite(dev[key])), key
+        assert _close(dev[key], host[key], 1e-9), key
+
+
 def _sod(tag, steps):
     d = np.load(os.path.join(GOLDEN, f"sod_{tag}_{steps}steps.npz"))
     mesh = np.load(os.path.join(GOLDEN, "flow_sod1000_mesh.npz"))
