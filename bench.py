@@ -16,7 +16,9 @@ already resident in HBM.  The JSON line also carries the side workloads:
 * adjoint -- forward CG solve plus the adjoint (transpose CG solve and the
   sampled outer product dL/dA on the CSR pattern) on the 128^3 system;
 * assembly -- the GPU coefficient assembly of each equation on a 128^3 hex
-  mesh, beside the oracle's host restatement of the reference loops.
+  mesh, beside the oracle's host restatement of the reference loops;
+* flow -- SIMPLE iterations of the flow solver on a 64^3 hex mesh, support
+  loops on the device (state resident) and on the host.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--sections all]
 """
@@ -389,14 +391,58 @@ def assembly(args, out):
                        "gpu_total_us": round(sum(v["avg_us"] for v in res.values()), 2), "cpu_baseline": cpu}
 
 
+def flow_step(args, out):
+    """One SIMPLE iteration of the flow solver (three momentum solves with
+    ILU(0)-BiCGSTAB, the pressure correction with CG, enthalpy; REFERENCE-mode
+    solves, the drop-in semantics) on a permuted n^3 hex mesh with an inlet,
+    an outlet, walls and symmetry planes, viscous, blended, transient: wall
+    time per iteration with the support loops as device kernels over the
+    resident state, and with their host restatement (per-equation uploads)."""
+    import torch
+
+    from opencslsolver_amd import flow, gen
+
+    n = args.flow_n
+    _log(f"bench: SIMPLE iterations on a {n}^3 hex mesh")
+    rng = np.random.default_rng(31)
+    perm = rng.permutation(n ** 3)
+    mesh, rp, col = gen.hex_mesh(n, perm=perm)
+    bcs = [dict(type=flow.VELOCITY_INLET, velocity=(3.0, 0.5, -0.2), temperature=300.0),
+           dict(type=flow.PRESSURE_OUTLET, pressure=1e5), dict(type=flow.WALL, no_slip=1),
+           dict(type=flow.WALL, no_slip=0), dict(type=flow.SYMMETRY), dict(type=flow.SYMMETRY)]
+    res = {}
+    for name, support in (("device", flow.SUPPORT_DEVICE), ("host_support", flow.SUPPORT_HOST)):
+        prm = flow.default_params(viscous=1, blended=1, transient=1, dt=1e-4, p0=1e5, t0=300.0,
+                                  u0=(2.0, 0.0, 0.0), bcs=bcs, method_pressure=0, support=support)
+        with flow.Flow(mesh, rp, col, prm) as fl:
+            fl.initialize()
+            fl.run(1, 1)  # warm: layouts, level schedules, graphs
+            torch.cuda.synchronize()
+            iters = max(1, args.steps)
+            t0 = time.perf_counter()
+            fl.run(iters, 1)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / iters
+            st = fl.get_state()
+        res[name] = {"ms_per_iteration": round(ms, 3), "p_mean": float(np.mean(st["p"])),
+                     "T_mean": float(np.mean(st["T"]))}
+    out["flow_step"] = {"workload": f"SIMPLE iteration, {n}^3 permuted hex mesh (N={n ** 3}), inlet/outlet/walls/"
+                                    "symmetry, viscous, blended, transient dt 1e-4, REFERENCE-mode solves "
+                                    "(BiCGSTAB momentum and enthalpy, CG pressure)",
+                        **res,
+                        "speedup_device_vs_host_support": round(res["host_support"]["ms_per_iteration"]
+                                                                / res["device"]["ms_per_iteration"], 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--sections", default="adjoint,multi,batch,assembly",
-                    help="side workloads to add to the headline: adjoint, multi, batch, assembly (or none)")
+    ap.add_argument("--sections", default="adjoint,multi,batch,assembly,flow",
+                    help="side workloads to add to the headline: adjoint, multi, batch, assembly, flow (or none)")
+    ap.add_argument("--flow-n", type=int, default=64, help="hex mesh size of the flow-step section")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reuse-tuning", action="store_true",
                     help="apply the cached tuned launch when present instead of tuning again")
@@ -433,6 +479,8 @@ def main():
         batch(args, out)
     if "assembly" in args.sections:
         assembly(args, out)
+    if "flow" in args.sections:
+        flow_step(args, out)
     print(json.dumps(out), flush=True)
 
 
