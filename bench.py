@@ -446,6 +446,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reuse-tuning", action="store_true",
                     help="apply the cached tuned launch when present instead of tuning again")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="run only the side sections (for profiling them); the JSON line then has no value")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the measured kernels, --steps times each (for rocprofv3)")
     args = ap.parse_args()
@@ -472,7 +474,8 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic (seeded generator, regenerated on the box)",
     }
-    headline(args, out)
+    if not args.skip_headline:
+        headline(args, out)
     if "multi" in args.sections:
         multi_rhs(args, out)
     if "batch" in args.sections:
