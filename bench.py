@@ -327,6 +327,10 @@ def profile_kernels(args):
         for prec in ("mixed", "fp64"):
             ms, by = s.time_cg_spmv(prec, reps=args.steps)
             res[f"cg_spmv_{prec}"] = {"avg_us": ms * 1e3, "bytes": by, "GBps": by / (ms * 1e-3) / 1e9}
+            for rpl in (1, 2, 4):  # every layout, so that whichever a later tuning picks has a traffic entry
+                ms1, by1 = s.time_cg_spmv(prec, reps=args.steps, rows_per_lane=rpl)
+                res[f"cg_spmv_{prec}_r{rpl}"] = {"avg_us": ms1 * 1e3, "bytes": by1,
+                                                 "GBps": by1 / (ms1 * 1e-3) / 1e9}
             ms2, by2 = s.time_cg_iteration(prec, pairs=args.steps)
             for par, name in ((0, "even"), (1, "odd")):
                 res[f"cg_iter_{prec}_{name}"] = {"avg_us": ms2[par] * 1e3, "bytes": by2[par],

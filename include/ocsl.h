@@ -192,10 +192,12 @@ ocsl_status ocsl_time_cg_iteration(ocsl_handle h, int precision, int pairs, doub
  * of the TOLERANCE-mode CG solve of the handle's system for every launch
  * candidate -- iteration form {1, 2, 3} x rows per lane {1, 2, 4} x threads
  * per workgroup {256, 512, 1024} x SpMV workgroups {256 .. 2048} x (split
- * form) vector-update workgroups -- timed with HIP events, and keeps the
- * fastest in the handle for later solves of that precision whose options
- * leave those fields at 0.  best (optional) receives it; table (optional)
- * every candidate's time per iteration.  x is preserved. */
+ * form) vector-update workgroups -- timed with HIP events; the six fastest
+ * run again over four times the iterations and the fastest of those is kept
+ * in the handle for later solves of that precision whose options leave those
+ * fields at 0.  best (optional) receives it and best_us_per_iter its refined
+ * time; table (optional) every candidate's time in the sweep.  x is
+ * preserved. */
 typedef struct {
     int rows_per_lane, block_size, grid_blocks, vec_blocks, cg_kernels;
     double us_per_iteration;

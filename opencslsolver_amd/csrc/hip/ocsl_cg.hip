@@ -11,6 +11,7 @@
 //                  FP64 every iteration and the FP64 true residual b - A x
 //                  replacing the FP32 recurrence residual whenever the latter
 //                  has dropped by `replace_ratio` (reliable updates).
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 #include <cstdio>
@@ -651,10 +652,11 @@ extern "C" ocsl_status ocsl_time_cg_iteration(ocsl_handle h, int precision, int 
     }
 }
 
-// Autotuner of the tolerance-CG launch: every candidate (rows per lane x
-// threads per workgroup x SpMV workgroups x vector workgroups) runs
-// `iterations` iterations of the real solve on the handle's system, timed
-// with HIP events; the fastest per iteration is kept in the handle and used by
+// Autotuner of the tolerance-CG launch: every candidate (iteration form x
+// rows per lane x threads per workgroup x SpMV workgroups x vector workgroups)
+// runs `iterations` iterations of the real solve on the handle's system,
+// timed with HIP events; the six fastest run again for four times as many
+// iterations, and the fastest of those is kept in the handle and used by
 // later solves whose options leave these fields at 0.  The caller's x is
 // restored afterwards.
 extern "C" ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations, ocsl_options* best,
@@ -677,11 +679,18 @@ extern "C" ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations
         const int blocks[] = {256, 512, 1024};
         const int grids[] = {256, 512, 1024, 2048};
         const int vgrids[] = {0, 128, 256, 1024, 2048};
-        ocsl_options bo{};
-        double best_us = 1e300;
         int count = 0;
         const int pi = precision == OCSL_MIXED ? 1 : 0;
         h->tuned[pi] = ocsl_solver_tuned{};
+        auto time_one = [&](const ocsl_options& o) {
+            OCSL_HIP(hipMemsetAsync(h->x_own.p, 0, sizeof(double) * n, h->stream));
+            ocsl_report r{};
+            cg_tolerance(h, precision == OCSL_MIXED, o, &r);  // warm: layout, graphs
+            OCSL_HIP(hipMemsetAsync(h->x_own.p, 0, sizeof(double) * n, h->stream));
+            cg_tolerance(h, precision == OCSL_MIXED, o, &r);
+            return r.iterations > 0 ? r.solve_ms * 1e3 / r.iterations : 1e300;
+        };
+        std::vector<std::pair<double, ocsl_options>> cand;
         for (int kn : kernels)
             for (int rpl : rpls)
                 for (int blk : blocks)
@@ -699,20 +708,26 @@ extern "C" ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations
                             o.block_size = blk;
                             o.grid_blocks = gr;
                             o.vec_blocks = vg;
-                            OCSL_HIP(hipMemsetAsync(h->x_own.p, 0, sizeof(double) * n, h->stream));
-                            ocsl_report r{};
-                            cg_tolerance(h, precision == OCSL_MIXED, o, &r);  // warm: layout, graphs
-                            OCSL_HIP(hipMemsetAsync(h->x_own.p, 0, sizeof(double) * n, h->stream));
-                            cg_tolerance(h, precision == OCSL_MIXED, o, &r);
-                            const double us = r.iterations > 0 ? r.solve_ms * 1e3 / r.iterations : 1e300;
+                            const double us = time_one(o);
                             if (table && count < table_cap)
                                 table[count] = ocsl_tune_entry{rpl, blk, gr, vg, kn, us};
                             ++count;
-                            if (us < best_us) {
-                                best_us = us;
-                                bo = o;
-                            }
+                            cand.emplace_back(us, o);
                         }
+        // the few fastest again over four times the iterations: a short
+        // sweep's timing noise (a few percent) exceeds the gaps between them
+        std::sort(cand.begin(), cand.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        ocsl_options bo = cand.front().second;
+        double best_us = 1e300;
+        for (size_t i = 0; i < std::min<size_t>(6, cand.size()); ++i) {
+            ocsl_options o = cand[i].second;
+            o.max_iter = 4 * iterations;
+            const double us = time_one(o);
+            if (us < best_us) {
+                best_us = us;
+                bo = cand[i].second;
+            }
+        }
         h->tuned[pi] =
             ocsl_solver_tuned{bo.rows_per_lane, bo.block_size, bo.grid_blocks, bo.vec_blocks, bo.cg_kernels};
         drop_graphs(h);

@@ -313,7 +313,11 @@ def test_cg_autotuner_keeps_x_and_tuned_solves_converge():
         assert best["cg_kernels"] in (1, 2, 3)
         assert {t["cg_kernels"] for t in table} == {1, 2, 3}
         assert len(table) >= 36 and all(t["us_per_iteration"] > 0 for t in table)
-        assert us == min(t["us_per_iteration"] for t in table)
+        # the pick is the fastest of the sweep's leaders re-timed over more
+        # iterations: one of the table's configurations
+        assert us > 0
+        keys = ("cg_kernels", "rows_per_lane", "block_size", "grid_blocks", "vec_blocks")
+        assert any(all(t[k] == best[k] for k in keys) for t in table)
         s.set_x(np.zeros(a.n))
         rep = s.solve("cg", "tolerance", "mixed", tol=1e-8)
         assert rep["status"] == 0, rep
