@@ -172,57 +172,64 @@ __device__ __forceinline__ double block_sum_partials(RedScratch red, int slot, i
 
 // Several slots at once: one pass, one pair of barriers.
 
-// The same sums in two phases.  PartialSums::issue starts this thread's loads
-// of the first two rounds of partials (all slots, no wait); a kernel calls it
-// before its other global loads, because vector loads complete in issue
-// order: waiting for the partials then waits for nothing issued after them.
-// finish adds any further rounds and completes the block reduction.  Every
-// block runs the same code, so all get the same bits.
+// The same sums in two phases, read by the block's first wave only, with
+// 16-byte loads.  issue() starts the wave's loads of the first 512 partials
+// of every slot (raw loads at clamped indices, nothing computed from them, so
+// that nothing waits for them yet); a kernel calls it before its other global
+// loads, because
+// vector loads complete in issue order: waiting for the partials then waits
+// for nothing issued after them.  finish() adds any further partials, reduces
+// across the wave and hands the totals to the whole block through LDS with a
+// single barrier -- so each instantiation's finish() may run once per kernel.
+// Every block runs the same code, so all get the same bits.
 template <int K>
 struct PartialSums {
-    double v[2][K];
+    static constexpr int kRounds = 4;  // double2 loads per lane and slot issued up front
+    double2 v[K][kRounds];
     __device__ __forceinline__ void issue(RedScratch red, const int (&slot)[K], const int (&G)[K]) {
-        const int bd = blockDim.x, i = threadIdx.x;
-        // unconditional loads (clamped index, result dropped), so that the
-        // compiler can count them
+        if (threadIdx.x >= kWave) return;
+        const int lane = threadIdx.x;
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
+        for (int k = 0; k < K; ++k) {
+            const double2* src = reinterpret_cast<const double2*>(red.partial + slot[k] * kMaxRedBlocks);
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const double t = red.partial[slot[k] * kMaxRedBlocks + min(i + u * bd, kMaxRedBlocks - 1)];
-                v[u][k] = i + u * bd < G[k] ? t : 0.0;
-            }
+            for (int r = 0; r < kRounds; ++r) v[k][r] = src[min(lane + kWave * r, kMaxRedBlocks / 2 - 1)];
+        }
     }
     __device__ __forceinline__ void finish(RedScratch red, const int (&slot)[K], const int (&G)[K],
                                            double (&out)[K]) {
-        const int bd = blockDim.x;
-        double a[K];
-        int gmax = 0;
+        __shared__ double sh[K];
+        if (threadIdx.x < kWave) {
+            const int lane = threadIdx.x;
+            double a[K];
+            int gmax = 0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            a[k] = v[0][k] + v[1][k];
-            gmax = max(gmax, G[k]);
-        }
-        for (int i = threadIdx.x + 2 * bd; i < gmax; i += bd)
+            for (int k = 0; k < K; ++k) {
+                a[k] = 0.0;
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (i < G[k]) a[k] += red.partial[slot[k] * kMaxRedBlocks + i];
-        __shared__ double sh[K][16];
-        const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-        const int nw = blockDim.x / kWave;
+                for (int r = 0; r < kRounds; ++r) {
+                    const int i2 = lane + kWave * r;  // partials past G[k] (stale or clamped) add 0
+                    a[k] += (2 * i2 < G[k] ? v[k][r].x : 0.0) + (2 * i2 + 1 < G[k] ? v[k][r].y : 0.0);
+                }
+                gmax = max(gmax, G[k]);
+            }
+            for (int i2 = lane + kWave * kRounds; 2 * i2 < gmax; i2 += kWave)
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const double s = wave_sum(a[k]);
-            if (lane == 0) sh[k][wid] = s;
+                for (int k = 0; k < K; ++k)
+                    if (2 * i2 < G[k]) {
+                        const double2 t = reinterpret_cast<const double2*>(red.partial + slot[k] * kMaxRedBlocks)[i2];
+                        a[k] += t.x;
+                        if (2 * i2 + 1 < G[k]) a[k] += t.y;
+                    }
+#pragma unroll
+            for (int k = 0; k < K; ++k) a[k] = wave_sum(a[k]);
+            if (lane == 0)
+#pragma unroll
+                for (int k = 0; k < K; ++k) sh[k] = a[k];
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            double t = 0.0;
-            for (int w = 0; w < nw; ++w) t += sh[k][w];
-            out[k] = t;
-        }
-        __syncthreads();
+        for (int k = 0; k < K; ++k) out[k] = sh[k];
     }
 };
 
