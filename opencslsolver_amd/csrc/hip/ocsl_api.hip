@@ -110,12 +110,12 @@ void ensure_sellp_values(ocsl_solver* h, SellP& s, bool need64, bool need32) {
     if (!s.val64.p || !s.diag64.p) {
         drop_graphs(h);
         s.val64.alloc(s.total);
-        s.diag64.alloc(s.n);
+        s.diag64.alloc(s.padded());
     }
     if (need32 && (!s.val32.p || !s.diag32.p)) {
         drop_graphs(h);
         s.val32.alloc(s.total);
-        s.diag32.alloc(s.n);
+        s.diag32.alloc(s.padded());
     }
     const int g = grid_for(s.total);
     if (want64) {

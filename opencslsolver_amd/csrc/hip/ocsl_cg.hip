@@ -37,7 +37,7 @@ static void launch_spmv(ocsl_solver* h, const Sell& s, double a, const double* x
 // ||b - A x|| / ||b|| over SELL-P: the residual kernel writing into scratch.
 static double true_relres_p(ocsl_solver* h, const SellP& s, const double* b, const double* x) {
     const int n = s.n;
-    double* r = work_buf(h, h->work.d[11], n);
+    double* r = work_buf(h, h->work.d[11], s.padded());
     const int g = std::max(1, std::min(kMaxRedBlocks, (s.n_slices + 15) / 16 * 1));
     with_rpl(s.rpl, [&](auto RC) {
         k_cg_residual<false, decltype(RC)::value><<<g, 1024, 0, h->stream>>>(s.view<double>(), b, x, r, nullptr,
@@ -285,10 +285,12 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     Work& w = h->work;
     // b, x in FP64 (d5, d6); r, p, q in the Krylov precision, and for the
     // one-kernel form their second buffers (MIXED f3..f5, FP64 d10, d12, d13).
-    // The vectors hold n + 4 elements: the update kernels load at clamped
-    // indices past the last full 16-byte vector.
-    const size_t nb = (size_t)n + 4;
-    double* b = work_buf(h, w.d[5], n);
+    // The vectors are allocated to the layout's whole slices (the SpMV-type
+    // kernels load row groups without a tail guard) plus 4 elements (the
+    // update kernels load at clamped indices past the last full 16-byte
+    // vector).
+    const size_t nb = s.padded() + 4;
+    double* b = work_buf(h, w.d[5], nb);
     double* x = work_buf(h, w.d[6], nb);
     work_buf(h, h->hist, (size_t)o.max_iter + 2);
     CgArgs<double, double> c64{};
@@ -510,16 +512,17 @@ extern "C" ocsl_status ocsl_time_cg_spmv(ocsl_handle h, int precision, int reord
         DBuf<double> p64, q64;
         DBuf<float> p32, q32;
         double ms = 0.0;
+        const size_t nb = s.padded() + 4;  // whole slices, as the SpMV loads them
         if (mixed) {
-            p32.alloc(n);
-            q32.alloc(n);
+            p32.alloc(nb);
+            q32.alloc(nb);
             k_fill<float><<<grid_for(n), kBlock, 0, h->stream>>>(n, 1.0f, p32.p);
             with_rpl(s.rpl, [&](auto RC) {
                 ms = time_spmv<decltype(RC)::value, float, float>(h, s, p32.p, q32.p, reps, o);
             });
         } else {
-            p64.alloc(n);
-            q64.alloc(n);
+            p64.alloc(nb);
+            q64.alloc(nb);
             k_fill<double><<<grid_for(n), kBlock, 0, h->stream>>>(n, 1.0, p64.p);
             with_rpl(s.rpl, [&](auto RC) {
                 ms = time_spmv<decltype(RC)::value, double, double>(h, s, p64.p, q64.p, reps, o);
@@ -550,13 +553,14 @@ static void time_iteration_pairs(ocsl_solver* h, SellP& s, const ocsl_options& o
     const int* perm = s.identity ? nullptr : s.perm.p;
     const CgLaunch L = cg_launch(s.n_slices, n, o);
     Work& w = h->work;
+    const size_t nb = s.padded() + 4;
     DBuf<double> b, x;
-    b.alloc(n);
-    x.alloc(n);
+    b.alloc(nb);
+    x.alloc(nb);
     X* buf[6];
     for (int i = 0; i < 6; ++i) {
-        if constexpr (sizeof(X) == 4) buf[i] = work_buf(h, w.f[i], n);
-        else buf[i] = work_buf(h, w.d[7 + i], n);
+        if constexpr (sizeof(X) == 4) buf[i] = work_buf(h, w.f[i], nb);
+        else buf[i] = work_buf(h, w.d[7 + i], nb);
     }
     const CgfArgs<V, X> f{s.view<V>(), {buf[0], buf[3]}, {buf[2], buf[4]}, {buf[1], buf[5]}, x.p, h->S.p, nullptr,
                           h->red()};

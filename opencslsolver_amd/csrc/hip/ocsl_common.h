@@ -323,6 +323,22 @@ __device__ __forceinline__ void load_rows_t(const T* __restrict__ a, int row0, i
         for (int j = 0; j < R; ++j) out[j] = row0 + j < n ? a[row0 + j] : T(0);
     }
 }
+// The same without the tail guard, for vectors allocated to the layout's
+// padded row count (whole slices): one vector load, nothing to branch on, so
+// the compiler keeps the load in flight past the code that follows.  Values
+// of padding rows are unspecified and must not reach a result.
+template <int R, class T>
+__device__ __forceinline__ void load_rows_p(const T* __restrict__ a, int row0, double (&out)[R]) {
+    const Pack<T, R> v = ldp<T, R>(a + row0);
+#pragma unroll
+    for (int j = 0; j < R; ++j) out[j] = (double)v.v[j];
+}
+template <int R, class T>
+__device__ __forceinline__ void load_rows_tp(const T* __restrict__ a, int row0, T (&out)[R]) {
+    const Pack<T, R> v = ldp<T, R>(a + row0);
+#pragma unroll
+    for (int j = 0; j < R; ++j) out[j] = v.v[j];
+}
 template <int R, class T>
 __device__ __forceinline__ void store_rows(T* __restrict__ a, int row0, int n, const T (&v)[R]) {
     if (row0 + R <= n) {
@@ -341,6 +357,15 @@ __device__ __forceinline__ void store_rows(T* __restrict__ a, int row0, int n, c
 // l owning rows s*64R + l*R + j.  Pair t of row j of lane l is at element
 // base + 128 R t + 2 R l + 2 j, so a lane fetches 2R values (and R packed
 // column words) per access; R = 1 is the one-row-per-lane layout.  kDiff:
+// difference form, diag = row sum s_i: sum_off a_ij (x_j - x_i) + s_i x_i;
+// otherwise sum_off a_ij x_j + a_ii x_i.  Arithmetic in FP64 whatever the
+// storage; per row the terms are added in the same order for every R.
+// gather(c) returns the vector's element c as a double (a plain load, or a
+// value formed from several vectors).
+// R rows of A x over SELL-P with R rows per lane: slices of 64 R rows, lane
+// l owning rows s*64R + l*R + j.  Pair t of row j of lane l is at element
+// base + 128 R t + 2 R l + 2 j, so a lane fetches 2R values (and R packed
+// column words) per access; R = 1 is the one-row-per-lane layout.  kDiff:
 // difference form, diag = row sum s_i: s_i x_i + sum_off a_ij (x_j - x_i);
 // otherwise a_ii x_i + sum_off a_ij x_j.  Arithmetic in FP64 whatever the
 // storage; per row the terms are added in the same order for every R.
@@ -352,7 +377,7 @@ __device__ __forceinline__ void sellp_rows_g(const SellPView<V>& A, int s, int l
     const int base = A.slice_ptr[s];
     const int np = (A.slice_ptr[s + 1] - base) / (128 * R);  // pairs per row
     double dg[R];
-    load_rows<R>(A.diag, row0, A.n, dg);
+    load_rows_p<R>(A.diag, row0, dg);  // diagonals are allocated to whole slices
 #pragma unroll
     for (int j = 0; j < R; ++j) acc[j] = dg[j] * xi[j];
     for (int t0 = 0; t0 < np; t0 += kPairs) {

@@ -52,6 +52,9 @@ struct SellP {
     DBuf<double> val64, diag64;
     DBuf<float> val32, diag32;              // diag32: row sums (difference form)
     bool identity = true;
+    // rows of whole slices: the allocation size of every vector the SELL-P
+    // kernels read by row groups without a tail guard
+    size_t padded() const { return (size_t)n_slices * 64 * rpl; }
     template <class V>
     SellPView<V> view() const;
 };

@@ -303,9 +303,9 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_residual(SellPView<double> A
     for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
         const int row0 = s * kWave * R + lane * R;
         double xi[R], ax[R], bv[R];
-        load_rows<R>(x, row0, A.n, xi);
+        load_rows_p<R>(x, row0, xi);
         sellp_rows<double, double, false, R>(A, s, lane, row0, x, xi, ax);
-        load_rows<R>(b, row0, A.n, bv);
+        load_rows_p<R>(b, row0, bv);
         double rv[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) {
@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_spmv(CgArgs<V, X> c) {
     for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
         const int row0 = s * kWave * R + lane * R;
         double pi[R], qd[R];
-        load_rows<R>(c.p, row0, c.A.n, pi);
+        load_rows_p<R>(c.p, row0, pi);
         sellp_rows<V, X, sizeof(V) == 4, R>(c.A, s, lane, row0, c.p, pi, qd);
         X qv[R];
 #pragma unroll
@@ -608,9 +608,9 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int fi
     for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
         const int row0 = s * kWave * R + lane * R;
         double pi[R], qd[R], rv[R];
-        load_rows<R>(c.p, row0, c.A.n, pi);
+        load_rows_p<R>(c.p, row0, pi);
         sellp_rows<V, X, sizeof(V) == 4, R>(c.A, s, lane, row0, c.p, pi, qd);
-        load_rows<R>(c.r, row0, c.A.n, rv);
+        load_rows_p<R>(c.r, row0, rv);
         X qv[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) {
@@ -837,9 +837,9 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgf_first(CgfArgs<V, X> c) {
     for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
         const int row0 = s * kWave * R + lane * R;
         double pi[R], qd[R], rv[R];
-        load_rows<R>(p, row0, c.A.n, pi);
+        load_rows_p<R>(p, row0, pi);
         sellp_rows<V, X, sizeof(V) == 4, R>(c.A, s, lane, row0, p, pi, qd);
-        load_rows<R>(c.r[0], row0, c.A.n, rv);
+        load_rows_p<R>(c.r[0], row0, rv);
         X qv[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) {
@@ -936,9 +936,9 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgf_iter(CgfArgs<V, X> c, int k
     X rv[R], qv[R], pv[R];
     if (sw < s1) {
         const int row0 = sw * kWave * R + lane * R;
-        load_rows_t<R>(rin, row0, n, rv);
-        load_rows_t<R>(qin, row0, n, qv);
-        load_rows_t<R>(pin, row0, n, pv);
+        load_rows_tp<R>(rin, row0, rv);
+        load_rows_tp<R>(qin, row0, qv);
+        load_rows_tp<R>(pin, row0, pv);
     }
     ps.finish(c.red, kIn, gs, d4);
     const double pq = d4[0], rq = d4[1], qq = d4[2], rr = d4[3];
@@ -948,8 +948,8 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgf_iter(CgfArgs<V, X> c, int k
             for (int s = sw; s < s1; s += sstep) {
                 const int row0 = s * kWave * R + lane * R;
                 double xv[R], pp[R];
-                load_rows<R>(c.x, row0, n, xv);
-                load_rows<R>(pout, row0, n, pp);
+                load_rows_p<R>(c.x, row0, xv);
+                load_rows_p<R>(pout, row0, pp);
 #pragma unroll
                 for (int j = 0; j < R; ++j) xv[j] += ap * pp[j];
                 store_rows<R>(c.x, row0, n, xv);
@@ -969,9 +969,9 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgf_iter(CgfArgs<V, X> c, int k
     for (int s = sw; s < s1; s += sstep) {
         const int row0 = s * kWave * R + lane * R;
         if (s != sw) {
-            load_rows_t<R>(rin, row0, n, rv);
-            load_rows_t<R>(qin, row0, n, qv);
-            load_rows_t<R>(pin, row0, n, pv);
+            load_rows_tp<R>(rin, row0, rv);
+            load_rows_tp<R>(qin, row0, qv);
+            load_rows_tp<R>(pin, row0, pv);
         }
         X rn[R], pn[R];
         double pnd[R];
@@ -983,8 +983,8 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgf_iter(CgfArgs<V, X> c, int k
         }
         if constexpr (kOdd) {
             double xv[R], pp[R];
-            load_rows<R>(c.x, row0, n, xv);
-            load_rows<R>(pout, row0, n, pp);
+            load_rows_p<R>(c.x, row0, xv);
+            load_rows_p<R>(pout, row0, pp);
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 xv[j] += ap * pp[j];
@@ -1075,9 +1075,9 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgl_iter(CgfArgs<float, float> 
         const int s = sw + t * sstep;
         if (s < s1) {
             const int row0 = s * kWave * R + lane * R;
-            load_rows_t<R>(rin, row0, n, rv[t]);
-            load_rows_t<R>(qin, row0, n, qv[t]);
-            load_rows_t<R>(pin, row0, n, pv[t]);
+            load_rows_tp<R>(rin, row0, rv[t]);
+            load_rows_tp<R>(qin, row0, qv[t]);
+            load_rows_tp<R>(pin, row0, pv[t]);
         }
     }
     if (threadIdx.x == 0) {
@@ -1132,8 +1132,8 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgl_iter(CgfArgs<float, float> 
                 if (s < s1) {
                     const int row0 = s * kWave * R + lane * R;
                     double xe[R], pe[R];
-                    load_rows<R>(c.x, row0, n, xe);
-                    load_rows<R>(pout, row0, n, pe);
+                    load_rows_p<R>(c.x, row0, xe);
+                    load_rows_p<R>(pout, row0, pe);
 #pragma unroll
                     for (int j = 0; j < R; ++j) xe[j] += ap * pe[j];
                     store_rows<R>(c.x, row0, n, xe);
@@ -1164,8 +1164,8 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgl_iter(CgfArgs<float, float> 
             }
             if constexpr (kOdd) {
                 double xe[R], pe[R];
-                load_rows<R>(c.x, row0, n, xe);
-                load_rows<R>(pout, row0, n, pe);
+                load_rows_p<R>(c.x, row0, xe);
+                load_rows_p<R>(pout, row0, pe);
 #pragma unroll
                 for (int j = 0; j < R; ++j) {
                     xe[j] += ap * pe[j];
