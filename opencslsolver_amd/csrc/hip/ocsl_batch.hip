@@ -70,6 +70,51 @@ __device__ __forceinline__ double sys_sum(const RedScratchBig& red, int slot, in
     return t;
 }
 
+// K slots of system k's partials (blocks [b0, b1)) in two phases, as the
+// single-system solver does it: issue() starts the first wave's loads of up
+// to 512 partials per slot (raw loads at clamped indices, before the kernel's
+// vector loads, which complete after them); finish() adds the rest, reduces
+// across the wave and hands the totals to the block through LDS with one
+// barrier (once per kernel).  Identical in all of the system's blocks.
+template <int K>
+struct SysSums {
+    static constexpr int kRounds = 8;
+    double v[K][kRounds];
+    __device__ __forceinline__ void issue(const RedScratchBig& red, const int (&slot)[K], int b0, int b1) {
+        if (threadIdx.x >= kWave) return;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int r = 0; r < kRounds; ++r)
+                v[k][r] = red.partial[(size_t)slot[k] * red.stride + min(b0 + (int)threadIdx.x + kWave * r, b1 - 1)];
+    }
+    __device__ __forceinline__ void finish(const RedScratchBig& red, const int (&slot)[K], int b0, int b1,
+                                           double (&out)[K]) {
+        __shared__ double sh[K];
+        if (threadIdx.x < kWave) {
+            const int lane = threadIdx.x;
+            double a[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                a[k] = 0.0;
+#pragma unroll
+                for (int r = 0; r < kRounds; ++r) a[k] += b0 + lane + kWave * r < b1 ? v[k][r] : 0.0;
+            }
+            for (int i = b0 + lane + kWave * kRounds; i < b1; i += kWave)
+#pragma unroll
+                for (int k = 0; k < K; ++k) a[k] += red.partial[(size_t)slot[k] * red.stride + i];
+#pragma unroll
+            for (int k = 0; k < K; ++k) a[k] = wave_sum(a[k]);
+            if (lane == 0)
+#pragma unroll
+                for (int k = 0; k < K; ++k) sh[k] = a[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] = sh[k];
+    }
+};
+
 template <int NR>
 __device__ __forceinline__ void bt_partials(const double (&v)[NR], const RedScratchBig& red, int slot0, int gb) {
     __shared__ double sh[NR][16];
@@ -197,8 +242,11 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
     SysScalars* S = c.sys + k;
     if (S->done) return;
     const int b0 = c.map.bptr[k], b1 = c.map.bptr[k + 1];
+    constexpr int kSl[3] = {kSlotPQ, kSlotRQ, kSlotQQ};
+    SysSums<3> ps;
+    ps.issue(c.red, kSl, b0, b1);
     // the block's rows are whole 64-row slices: 16-byte vectors, the first
-    // one loaded before the partial sums are re-added
+    // one in flight while the partial sums are re-added
     const int v0 = c.map.bs0[gb] * (kWave / N), v1 = c.map.bs1[gb] * (kWave / N);
     const int va = v0 + (int)threadIdx.x;
     double xa[N];
@@ -209,9 +257,9 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
         vload(c.r, va, ra);
         vload(c.q, va, qa);
     }
-    const double pq = sys_sum(c.red, kSlotPQ, b0, b1);
-    const double rq = sys_sum(c.red, kSlotRQ, b0, b1);
-    const double qq = sys_sum(c.red, kSlotQQ, b0, b1);
+    double d3[3];
+    ps.finish(c.red, kSl, b0, b1, d3);
+    const double pq = d3[0], rq = d3[1], qq = d3[2];
     if (!(pq != 0.0 && isfinite(pq))) {
         if (gb == b0 && threadIdx.x == 0) {
             S->breakdown = (pq == 0.0);
