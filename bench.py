@@ -272,6 +272,8 @@ def batch(args, out):
     with Batch([(a.row_ptr, a.col) for a, _ in systems]) as bt:
         bt.set_values([a.val for a, _ in systems])
         bt.set_rhs([b for _, b in systems])
+        # launch shape (rows per lane x workgroup size) from the autotuner
+        shape, us_it, table = bt.tune("mixed", iterations=32)
         bt.solve("mixed", tol=1e-8)  # warm (graph capture)
         t0 = time.perf_counter()
         xs, reps, lat = bt.solve("mixed", tol=1e-8)
@@ -296,6 +298,8 @@ def batch(args, out):
         "iterations": [r["iterations"] for r in reps],
         "max_fp64_relres": max(rel),
         "sequential_sum_ms": round(seq, 3),
+        "launch": dict(shape, us_per_iteration=round(us_it, 2)),
+        "tuning": [[t["rows_per_lane"], t["block_size"], round(t["us_per_iteration"], 2)] for t in table],
     }
 
 

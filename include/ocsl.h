@@ -232,6 +232,24 @@ ocsl_status ocsl_batch_set_rhs(ocsl_batch bt, const double* const* b, ocsl_where
  * of the solve until its convergence was observed (at chunk granularity). */
 ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* opt, double* const* x,
                              ocsl_where where, ocsl_report* reps, double* latency_ms);
+/* Launch shape of the batch: rows per lane of the concatenated SELL-P layout
+ * (1, 2 or 4: slices of 64, 128 or 256 rows, one per wave) and threads per
+ * SpMV workgroup (256, 512 or 1024); values and right-hand side already set
+ * follow into the new layout.  Default 1 and 1024. */
+ocsl_status ocsl_batch_set_launch(ocsl_batch bt, int rows_per_lane, int block_size);
+ocsl_status ocsl_batch_get_launch(ocsl_batch bt, int* rows_per_lane, int* block_size);
+/* Average device time of one CG iteration of the whole batch in the current
+ * launch shape (every system restarted from x = 0 and kept running), over
+ * `iterations` iterations replayed as one hipGraph. */
+ocsl_status ocsl_batch_time_iterations(ocsl_batch bt, int precision, int iterations, double* us_per_iter);
+/* Autotuner of the batched launch: times every rows-per-lane x workgroup-size
+ * shape with ocsl_batch_time_iterations and keeps the fastest.  table
+ * (optional, table_cap entries) receives each measurement with grid_blocks =
+ * workgroups per launch, vec_blocks = threads per update workgroup and
+ * cg_kernels = 2 (an iteration is two kernels). */
+ocsl_status ocsl_batch_tune(ocsl_batch bt, int precision, int iterations, int* best_rows_per_lane,
+                            int* best_block_size, double* best_us_per_iter, ocsl_tune_entry* table, int table_cap,
+                            int* table_len);
 
 /* ---- differentiation (no counterpart in the reference, which has no
  * derivative path; for gradient-based callers of CMatrix_solver::solve,

@@ -121,6 +121,11 @@ SIGNATURES = {
     "ocsl_batch_solve": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.POINTER(Options),
                                         ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                         ctypes.POINTER(Report), _f64p]),
+    "ocsl_batch_set_launch": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int]),
+    "ocsl_batch_get_launch": (ctypes.c_int, [c_handle, _i32p, _i32p]),
+    "ocsl_batch_time_iterations": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, _f64p]),
+    "ocsl_batch_tune": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p,
+                                       ctypes.POINTER(TuneEntry), ctypes.c_int, _i32p]),
     "ocsl_last_error": (ctypes.c_char_p, []),
 }
 

@@ -1,8 +1,9 @@
 // Batched solves of many independent CSR systems in the same launches.
 //
 // The systems' SELL-P structures (reverse Cuthill-McKee order, 16-bit column
-// offsets) are concatenated, each padded to whole 64-row slices so no slice
-// and no workgroup straddles two systems.  One block map -- workgroup ->
+// offsets, R = 1, 2 or 4 rows per lane) are concatenated, each padded to
+// whole slices of 64 R rows so no slice and no workgroup straddles two
+// systems.  One block map -- workgroup ->
 // (system, slice range) -- serves every kernel, so the partial sums of system
 // k always sit in the contiguous block range bptr[k]..bptr[k+1] and a
 // consuming block re-adds only its own system's partials.  Every system has
@@ -10,8 +11,11 @@
 // merely share the launches.  An iteration is two kernels (the merged-
 // reduction CG of the single-system mixed solver); blocks of converged
 // systems return at once, and chunks of iterations replay as one hipGraph.
-// FP64 residual replacement (MIXED) is done per system when its FP32
-// recurrence has dropped by replace_ratio or claims convergence.
+// MIXED: when a system's FP32 recurrence has dropped by replace_ratio since
+// its last replacement, the system's next iteration is an FP64 residual
+// replacement (r = b - A x) inside the same launches, so the replacement
+// comes one iteration after its trigger whatever the chunk length; a claim
+// of convergence is confirmed on the true FP64 residual between chunks.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -29,9 +33,13 @@ constexpr int kBtUpdateBlock = 256;
 struct SysScalars {
     double rr, bb, r_ref, tol2, alpha, beta, replace_ratio;
     int iter, max_iter, done, halt, need_update, breakdown, nonfinite, replacements;
-    int verified;  // convergence confirmed on the true FP64 residual (MIXED)
-    int pad;
+    int verified;     // convergence confirmed on the true FP64 residual (MIXED)
+    int replace_now;  // the next SpMV kernel replaces r instead (set by the update kernel)
+    int replaced;     // this iteration's SpMV kernel replaced r (set by the SpMV kernel)
 };
+
+// partials of the true r.r of an in-iteration replacement
+constexpr int kSlotTR = 5;
 
 struct BatchMap {
     const int* bsys;    // block -> system
@@ -52,6 +60,9 @@ struct BtArgs {
     double* x;
     SysScalars* sys;
     RedScratchBig red;
+    int slice_rows;        // 64 R
+    SellPView<double> A64; // FP64 values (the residual replacement)
+    const double* b;
 };
 
 // Sum of system k's partials of one slot; identical in all its blocks.
@@ -134,15 +145,17 @@ __device__ __forceinline__ void bt_partials(const double (&v)[NR], const RedScra
         }
 }
 
-// A system takes the residual pass when its recurrence asks for a
-// replacement, or claims convergence not yet confirmed on the true residual.
-__device__ __forceinline__ bool bt_wants_residual(const SysScalars& S) {
-    return S.need_update || (S.done && !S.verified && !S.nonfinite && !S.breakdown);
+// A system takes the residual pass between chunks when it claims convergence
+// not yet confirmed on the true residual.
+__device__ __host__ __forceinline__ bool bt_wants_residual(const SysScalars& S) {
+    return S.done && !S.verified && !S.nonfinite && !S.breakdown;
 }
 
 // r = b - A x (FP64) for the systems being (re)started; p = r on the first
-// call; partials of r.r and b.b.
-template <class X>
+// call; partials of r.r and b.b.  One wave per slice of 64 R rows; every row
+// of the concatenation exists (systems are padded to whole slices, padding
+// rows are zero rows with zero right-hand side).
+template <class X, int R>
 __global__ void __launch_bounds__(1024) k_bt_residual(SellPView<double> A, BatchMap map, const double* __restrict__ b,
                                                     const double* __restrict__ x, X* r, X* p, SysScalars* sys,
                                                     RedScratchBig red, int first) {
@@ -153,16 +166,21 @@ __global__ void __launch_bounds__(1024) k_bt_residual(SellPView<double> A, Batch
     double acc[2] = {0.0, 0.0};
     for (int s = map.bs0[gb] + (int)(threadIdx.x / kWave); s < map.bs1[gb];
          s += (int)(blockDim.x / kWave)) {
-        const int row = s * kWave + lane;
-        const double ax = sellp_row<double, double, false>(A, s, lane, row, x, row < A.n ? x[row] : 0.0);
-        if (row < A.n) {
-            const double bv = b[row];
-            const double rv = bv - ax;
-            acc[0] += rv * rv;
-            acc[1] += bv * bv;
-            r[row] = (X)rv;
-            if (first) p[row] = (X)rv;
+        const int row0 = s * kWave * R + lane * R;
+        double xi[R], ax[R], bv[R];
+        load_rows_p<R>(x, row0, xi);
+        load_rows_p<R>(b, row0, bv);
+        sellp_rows<double, double, false, R>(A, s, lane, row0, x, xi, ax);
+        X rv[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const double rj = bv[j] - ax[j];
+            acc[0] += rj * rj;
+            acc[1] += bv[j] * bv[j];
+            rv[j] = (X)rj;
         }
+        store_rows<R>(r, row0, A.n, rv);
+        if (first) store_rows<R>(p, row0, A.n, rv);
     }
     bt_partials<2>(acc, red, kSlotRR, gb);
 }
@@ -188,17 +206,44 @@ __global__ void k_bt_fin_residual(BatchMap map, SysScalars* sys, RedScratchBig r
         if (!isfinite(rr)) { S.nonfinite = 1; S.done = 1; S.verified = 1; }
         S.halt = S.done;
         S.need_update = 0;
+        S.replace_now = 0;
+        S.replaced = 0;
     }
 }
 
-// K1: per-system rr (first block of the system), q = A p, partials p.q, r.q, q.q
-template <class V, class X>
+// K1: per-system rr (first block of the system), q = A p, partials p.q, r.q,
+// q.q; or, for a system whose replacement is due, r = b - A x in FP64 with
+// partials of r.r (the iteration's update kernel then adopts it).
+template <class V, class X, int R>
 __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
     const int gb = c.map.block();
     const int k = c.map.bsys[gb];
     SysScalars* S = c.sys + k;
     if (S->done) {
         if (gb == c.map.bptr[k] && threadIdx.x == 0) S->halt = 1;
+        return;
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    if (S->replace_now) {
+        double acc[1] = {0.0};
+        for (int s = c.map.bs0[gb] + (int)(threadIdx.x / kWave); s < c.map.bs1[gb];
+             s += (int)(blockDim.x / kWave)) {
+            const int row0 = s * kWave * R + lane * R;
+            double xi[R], ax[R], bv[R];
+            load_rows_p<R>(c.x, row0, xi);
+            load_rows_p<R>(c.b, row0, bv);
+            sellp_rows<double, double, false, R>(c.A64, s, lane, row0, c.x, xi, ax);
+            X rv[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const double rj = bv[j] - ax[j];
+                acc[0] += rj * rj;
+                rv[j] = (X)rj;
+            }
+            store_rows<R>(c.r, row0, c.A.n, rv);
+        }
+        bt_partials<1>(acc, c.red, kSlotTR, gb);
+        if (gb == c.map.bptr[k] && threadIdx.x == 0) S->replaced = 1;
         return;
     }
     if (gb == c.map.bptr[k]) {
@@ -208,22 +253,26 @@ __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
             const double scale = S->bb > 0.0 ? S->bb : 1.0;
             if (!(rr > S->tol2 * scale) || S->iter >= S->max_iter) S->done = 1;
             if (!isfinite(rr)) { S->nonfinite = 1; S->done = 1; }
-            if (sqrt(rr) < S->replace_ratio * S->r_ref) S->need_update = 1;
+            S->need_update = sqrt(rr) < S->replace_ratio * S->r_ref;
+            S->replaced = 0;
         }
     }
-    const int lane = threadIdx.x & (kWave - 1);
     double acc[3] = {0.0, 0.0, 0.0};
     for (int s = c.map.bs0[gb] + (int)(threadIdx.x / kWave); s < c.map.bs1[gb]; s += (int)(blockDim.x / kWave)) {
-        const int row = s * kWave + lane;
-        const X pi = row < c.A.n ? c.p[row] : (X)0;
-        const X qv = (X)sellp_row<V, X, sizeof(V) == 4>(c.A, s, lane, row, c.p, (double)pi);
-        if (row < c.A.n) {
-            const double rv = (double)c.r[row];
-            c.q[row] = qv;
-            acc[0] += (double)pi * (double)qv;
-            acc[1] += rv * (double)qv;
-            acc[2] += (double)qv * (double)qv;
+        const int row0 = s * kWave * R + lane * R;
+        double pi[R], rv[R], qd[R];
+        load_rows_p<R>(c.p, row0, pi);
+        load_rows_p<R>(c.r, row0, rv);
+        sellp_rows<V, X, sizeof(V) == 4, R>(c.A, s, lane, row0, c.p, pi, qd);
+        X qv[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            qv[j] = (X)qd[j];
+            acc[0] += pi[j] * (double)qv[j];
+            acc[1] += rv[j] * (double)qv[j];
+            acc[2] += (double)qv[j] * (double)qv[j];
         }
+        store_rows<R>(c.q, row0, c.A.n, qv);
     }
     // p.q, then r.q and q.q in slots of their own: the r.r slot is being read
     // by the first block of each system during this kernel
@@ -242,12 +291,29 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
     SysScalars* S = c.sys + k;
     if (S->done) return;
     const int b0 = c.map.bptr[k], b1 = c.map.bptr[k + 1];
+    if (S->replaced) {
+        // the replaced residual's r.r partials become the next rr; its norm
+        // the reference of the next replacement test
+        if (threadIdx.x == 0)
+            c.red.partial[(size_t)kSlotRR * c.red.stride + gb] = c.red.partial[(size_t)kSlotTR * c.red.stride + gb];
+        if (gb == b0) {
+            const double tr = sys_sum(c.red, kSlotTR, b0, b1);
+            if (threadIdx.x == 0) {
+                S->r_ref = sqrt(tr);
+                S->replacements++;
+                S->replace_now = 0;
+                S->need_update = 0;
+                if (!isfinite(tr)) { S->nonfinite = 1; S->done = 1; }
+            }
+        }
+        return;
+    }
     constexpr int kSl[3] = {kSlotPQ, kSlotRQ, kSlotQQ};
     SysSums<3> ps;
     ps.issue(c.red, kSl, b0, b1);
-    // the block's rows are whole 64-row slices: 16-byte vectors, the first
-    // one in flight while the partial sums are re-added
-    const int v0 = c.map.bs0[gb] * (kWave / N), v1 = c.map.bs1[gb] * (kWave / N);
+    // the block's rows are whole slices: 16-byte vectors, the first one in
+    // flight while the partial sums are re-added
+    const int v0 = c.map.bs0[gb] * (c.slice_rows / N), v1 = c.map.bs1[gb] * (c.slice_rows / N);
     const int va = v0 + (int)threadIdx.x;
     double xa[N];
     X pa[N], ra[N], qa[N];
@@ -303,6 +369,7 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
         S->alpha = alpha;
         S->beta = beta;
         ++S->iter;
+        S->replace_now = S->need_update;
     }
 }
 
@@ -318,8 +385,13 @@ struct ocsl_batch_s {
     std::vector<int64_t> n_sys;          // rows per system
     std::vector<int64_t> nnz_in;         // caller entries per system
     std::vector<int> row_off;            // padded row offset per system (m + 1)
-    int n_tot = 0, ns_tot = 0, nblocks = 0, block = 1024;
+    int n_tot = 0, ns_tot = 0, nblocks = 0;
+    int rpl = 1;                         // rows per lane: slices of 64 rpl rows
+    int block = 1024;                    // threads per SpMV/residual workgroup (one slice per wave)
     std::vector<int> bptr_host;
+    std::vector<HostCSR> csr;            // the systems' structures (real entries)
+    std::vector<std::vector<int>> order; // their reverse Cuthill-McKee orders
+    std::vector<int> ns_sys;             // slices per system in the current layout
     // concatenated SELL-P (global numbering)
     DBuf<int> slice_ptr, src, diag_src, bsys, bs0, bs1, bptr, perm;  // perm: padded row -> (flat input index)
     DBuf<int> active;                    // compacted list of the blocks of unfinished systems
@@ -345,16 +417,23 @@ struct ocsl_batch_s {
         if (stream) (void)hipStreamDestroy(stream);
     }
     RedScratchBig red() const { return {red_partial.p, (size_t)nblocks}; }
+    // threads of the update kernel: one 16-byte vector each over a block's rows
+    int upd_block(bool mixed) const {
+        const int vecs = (block / kWave) * kWave * rpl / (mixed ? 4 : 2);
+        return std::min(kBtUpdateBlock, std::max(kWave, vecs));
+    }
     std::vector<int64_t> flat_off() const {
         std::vector<int64_t> o(m + 1, 0);
         for (int k = 0; k < m; ++k) o[k + 1] = o[k] + n_sys[k];
         return o;
     }
     SellPView<double> view64() const {
-        return {n_tot, ns_tot, slice_ptr.p, compact ? nullptr : col.p, compact ? colw.p : nullptr, val64.p, diag64.p};
+        return {n_tot, ns_tot, slice_ptr.p, compact ? nullptr : col.p, compact ? colw.p : nullptr, val64.p, diag64.p,
+                nullptr, nullptr};
     }
     SellPView<float> view32() const {
-        return {n_tot, ns_tot, slice_ptr.p, compact ? nullptr : col.p, compact ? colw.p : nullptr, val32.p, diag32.p};
+        return {n_tot, ns_tot, slice_ptr.p, compact ? nullptr : col.p, compact ? colw.p : nullptr, val32.p, diag32.p,
+                nullptr, nullptr};
     }
     BatchMap map(bool compact = false) const { return {bsys.p, bs0.p, bs1.p, bptr.p, compact ? active.p : nullptr}; }
 };
@@ -390,20 +469,299 @@ __global__ void k_bt_scatter(int n, const int* __restrict__ perm, const double* 
     }
 }
 
-// Per-slice-width conversion of the concatenated value arrays; the FP32
-// diagonal is the FP64 row sum (difference form).
-__global__ void __launch_bounds__(kBlock) k_bt_rowsum(SellPView<double> A, float* __restrict__ diag32) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
-    const int nwaves = gridDim.x * kWavesPerBlock;
-    for (int s = wave; s < A.n_slices; s += nwaves) {
-        const int row = s * kWave + lane;
-        const int base = A.slice_ptr[s];
-        const int w = (A.slice_ptr[s + 1] - base) >> 6;
-        double sum = A.diag[row];
-        for (int k = 0; k < w; ++k) sum += A.val[base + (k / 2) * 2 * kWave + 2 * lane + (k & 1)];
-        diag32[row] = (float)sum;
+void bt_drop_graphs(ocsl_batch bt) {
+    for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
+    bt->graphs.clear();
+}
+
+// Concatenated SELL-P of the batch with rpl rows per lane.  Row r of system
+// k's reverse Cuthill-McKee order is row row_off[k] + r; in slice sl lane l
+// owns rows sl*64R + l*R + j, and off-diagonal kk of such a row sits at
+// base + (kk/2)*128R + 2R l + 2j + (kk&1) -- the single-system SELL-P
+// layout, so the same row routines serve.  Padding rows of a system are
+// zero rows whose columns are themselves.
+void bt_build_layout(ocsl_batch bt, int rpl) {
+    OCSL_REQUIRE(rpl == 1 || rpl == 2 || rpl == 4, OCSL_ERR_ARG, "rows_per_lane must be 1, 2 or 4");
+    const int m = bt->m, H = kWave * rpl;
+    std::vector<int> sp_all{0}, colg, srcg, dsrc, perm, row_off{0}, ns_sys(m);
+    bool compact = true;
+    int64_t flat_base = 0;
+    for (int k = 0; k < m; ++k) {
+        const HostCSR& a = bt->csr[k];
+        const std::vector<int>& order = bt->order[k];
+        const int n = a.n, ns = (n + H - 1) / H;
+        OCSL_REQUIRE((int64_t)row_off.back() + (int64_t)ns * H < ((int64_t)1 << 30), OCSL_ERR_ARG, "batch too large");
+        std::vector<int> iperm(n);
+        for (int i = 0; i < n; ++i) iperm[order[i]] = i;
+        const int r0 = row_off.back();
+        const int64_t vb = bt->val_off[k];
+        for (int sl = 0; sl < ns; ++sl) {
+            int w = 0;
+            for (int q = 0; q < H && sl * H + q < n; ++q) {
+                const int o = order[sl * H + q];
+                w = std::max(w, a.rp[o + 1] - a.rp[o] - 1);
+            }
+            w += w & 1;
+            const int base = sp_all.back();
+            OCSL_REQUIRE((int64_t)base + (int64_t)w * H < ((int64_t)1 << 31), OCSL_ERR_ARG, "batch too large");
+            sp_all.push_back(base + w * H);
+            colg.resize(base + (size_t)w * H);
+            srcg.resize(base + (size_t)w * H);
+            for (int l = 0; l < kWave; ++l)
+                for (int j = 0; j < rpl; ++j) {
+                    const int r = sl * H + l * rpl + j;  // local row (new order)
+                    auto slot = [&](int kk) {
+                        return (int64_t)base + (int64_t)(kk / 2) * 2 * H + 2 * rpl * l + 2 * j + (kk & 1);
+                    };
+                    int kk = 0;
+                    if (r < n) {
+                        const int o = order[r];
+                        for (int p = a.rp[o]; p < a.rp[o + 1]; ++p) {
+                            if (p == a.diag[o]) continue;
+                            const int64_t q = slot(kk++);
+                            colg[q] = r0 + iperm[a.col[p]];
+                            srcg[q] = (int)(vb + a.src[p]);
+                            const int d = colg[q] - (r0 + r);
+                            if (d < -32768 || d > 32767) compact = false;
+                        }
+                    }
+                    for (; kk < w; ++kk) {
+                        const int64_t q = slot(kk);
+                        colg[q] = r0 + r;
+                        srcg[q] = -1;
+                    }
+                }
+        }
+        for (int r = 0; r < ns * H; ++r) {
+            perm.push_back(r < n ? (int)(flat_base + order[r]) : -1);
+            dsrc.push_back(r < n ? (int)(vb + a.src[a.diag[order[r]]]) : -1);
+        }
+        ns_sys[k] = ns;
+        row_off.push_back(r0 + ns * H);
+        flat_base += n;
     }
+    bt_drop_graphs(bt);
+    hipStream_t s = bt->stream;
+    OCSL_HIP(hipStreamSynchronize(s));  // earlier uploads may still read host buffers of an earlier build
+    bt->rpl = rpl;
+    bt->row_off = row_off;
+    bt->ns_sys = ns_sys;
+    bt->n_tot = row_off.back();
+    bt->ns_tot = (int)(sp_all.size() - 1);
+    bt->total = sp_all.back();
+    bt->compact = compact;
+    bt->slice_ptr.upload(sp_all.data(), sp_all.size(), s);
+    if (compact) {
+        // word of pair t of row j of lane l: base / 2 + t H + l R + j
+        std::vector<unsigned> colw(bt->total / 2);
+        for (int sl = 0; sl < bt->ns_tot; ++sl) {
+            const int np = (sp_all[sl + 1] - sp_all[sl]) / (2 * H);
+            for (int t = 0; t < np; ++t)
+                for (int l = 0; l < kWave; ++l)
+                    for (int j = 0; j < rpl; ++j) {
+                        const int64_t q = sp_all[sl] + (int64_t)t * 2 * H + 2 * rpl * l + 2 * j;
+                        const int row = sl * H + l * rpl + j;
+                        const unsigned lo = (unsigned)(uint16_t)(int16_t)(colg[q] - row);
+                        const unsigned hi = (unsigned)(uint16_t)(int16_t)(colg[q + 1] - row);
+                        colw[sp_all[sl] / 2 + (int64_t)t * H + l * rpl + j] = lo | (hi << 16);
+                    }
+        }
+        bt->colw.upload(colw.data(), colw.size(), s);
+        OCSL_HIP(hipStreamSynchronize(s));
+    } else {
+        bt->col.upload(colg.data(), colg.size(), s);
+    }
+    bt->src.upload(srcg.data(), srcg.size(), s);
+    bt->diag_src.upload(dsrc.data(), dsrc.size(), s);
+    bt->perm.upload(perm.data(), perm.size(), s);
+    OCSL_HIP(hipStreamSynchronize(s));
+}
+
+// Workgroup -> (system, slice range) map for SpMV workgroups of `block`
+// threads, one slice per wave; the update kernel covers the same rows.
+void bt_build_map(ocsl_batch bt, int block) {
+    OCSL_REQUIRE(block == 256 || block == 512 || block == 1024, OCSL_ERR_ARG, "block_size must be 256, 512 or 1024");
+    const int spb = block / kWave;
+    std::vector<int> bsys, bs0, bs1, bptr{0};
+    int sb = 0;
+    for (int k = 0; k < bt->m; ++k) {
+        const int ns = bt->ns_sys[k];
+        for (int s0 = 0; s0 < ns; s0 += spb) {
+            bsys.push_back(k);
+            bs0.push_back(sb + s0);
+            bs1.push_back(sb + std::min(ns, s0 + spb));
+        }
+        bptr.push_back((int)bsys.size());
+        sb += ns;
+    }
+    bt_drop_graphs(bt);
+    hipStream_t s = bt->stream;
+    OCSL_HIP(hipStreamSynchronize(s));
+    bt->block = block;
+    bt->nblocks = (int)bsys.size();
+    bt->bptr_host = bptr;
+    bt->bsys.upload(bsys.data(), bsys.size(), s);
+    bt->bs0.upload(bs0.data(), bs0.size(), s);
+    bt->bs1.upload(bs1.data(), bs1.size(), s);
+    bt->bptr.upload(bptr.data(), bptr.size(), s);
+    bt->red_partial.alloc((size_t)bt->nblocks * 8);
+    bt->active.alloc(bt->nblocks);
+    OCSL_HIP(hipStreamSynchronize(s));
+}
+
+// Matrix values of the current layout from the caller's values (val_in):
+// FP64 entries and diagonal, FP32 entries and the FP64 row sums as the FP32
+// diagonal (difference form).
+void bt_gather_values(ocsl_batch bt) {
+    hipStream_t s = bt->stream;
+    const void* before[4] = {bt->val64.p, bt->diag64.p, bt->val32.p, bt->diag32.p};
+    bt->val64.alloc(bt->total);
+    bt->diag64.alloc(bt->n_tot);
+    bt->val32.alloc(bt->total);
+    bt->diag32.alloc(bt->n_tot);
+    const void* after[4] = {bt->val64.p, bt->diag64.p, bt->val32.p, bt->diag32.p};
+    if (!std::equal(before, before + 4, after)) bt_drop_graphs(bt);  // graphs hold the arrays' addresses
+    k_gather_vals<double><<<grid_for(bt->total), kBlock, 0, s>>>(bt->total, bt->src.p, bt->val_in.p, bt->val64.p);
+    k_gather_vals<double><<<grid_for(bt->n_tot), kBlock, 0, s>>>(bt->n_tot, bt->diag_src.p, bt->val_in.p,
+                                                                bt->diag64.p);
+    k_gather_vals<float><<<grid_for(bt->total), kBlock, 0, s>>>(bt->total, bt->src.p, bt->val_in.p, bt->val32.p);
+    with_rpl(bt->rpl, [&](auto RC) {
+        k_sellp_rowsum<decltype(RC)::value>
+            <<<grid_for((int64_t)bt->ns_tot * kWave), kBlock, 0, s>>>(bt->view64(), bt->diag32.p);
+    });
+    OCSL_HIP(hipGetLastError());
+}
+
+// Layout with rpl rows per lane and workgroups of `block` threads; values
+// and right-hand side, when set, follow into the new layout.
+void bt_set_launch(ocsl_batch bt, int rpl, int block) {
+    if (rpl != bt->rpl) {
+        bt_build_layout(bt, rpl);
+        if (bt->have_vals) bt_gather_values(bt);
+        if (bt->have_b) {
+            bt->b_glob.alloc(bt->n_tot);
+            k_bt_gather<<<grid_for(bt->n_tot), kBlock, 0, bt->stream>>>(bt->n_tot, bt->perm.p, bt->b_in.p,
+                                                                       bt->b_glob.p);
+            OCSL_HIP(hipGetLastError());
+        }
+        bt_build_map(bt, block);
+    } else if (block != bt->block) {
+        bt_build_map(bt, block);
+    }
+    OCSL_HIP(hipStreamSynchronize(bt->stream));
+}
+
+// Krylov vectors of the precision (zeroed: padding rows stay zero) and fresh
+// per-system scalars for a solve to tol within max_iter iterations.
+void bt_prepare(ocsl_batch bt, bool mixed, double tol, int max_iter, double replace_ratio) {
+    hipStream_t s = bt->stream;
+    const int N = bt->n_tot, m = bt->m;
+    const bool remap = mixed ? (!bt->f[0].p || bt->f[0].n < (size_t)N) : (!bt->d[0].p || bt->d[0].n < (size_t)N);
+    if (remap) bt_drop_graphs(bt);
+    for (int i = 0; i < 3; ++i) {
+        if (mixed) {
+            bt->f[i].alloc(N);
+            OCSL_HIP(hipMemsetAsync(bt->f[i].p, 0, sizeof(float) * N, s));
+        } else {
+            bt->d[i].alloc(N);
+            OCSL_HIP(hipMemsetAsync(bt->d[i].p, 0, sizeof(double) * N, s));
+        }
+    }
+    std::vector<SysScalars> init(m);
+    for (auto& S : init) {
+        S = SysScalars{};
+        S.tol2 = tol * tol;
+        S.max_iter = max_iter;
+        S.replace_ratio = mixed ? replace_ratio : 0.0;
+    }
+    OCSL_HIP(hipMemcpyAsync(bt->sys.p, init.data(), sizeof(SysScalars) * m, hipMemcpyHostToDevice, s));
+    OCSL_HIP(hipStreamSynchronize(s));
+}
+
+// The batch's kernel launches in one precision over the current layout.
+struct BtLaunch {
+    ocsl_batch bt;
+    bool mixed;
+    int UB;
+    BtArgs<float, float> c32;
+    BtArgs<double, double> c64;
+    BtLaunch(ocsl_batch b, bool mx)
+        : bt(b), mixed(mx), UB(b->upd_block(mx)),
+          c32{b->view32(), b->map(), b->f[0].p, b->f[1].p, b->f[2].p, b->x_glob.p, b->sys.p, b->red(), kWave * b->rpl,
+              b->view64(), b->b_glob.p},
+          c64{b->view64(), b->map(), b->d[0].p, b->d[1].p, b->d[2].p, b->x_glob.p, b->sys.p, b->red(),
+              kWave * b->rpl, b->view64(), b->b_glob.p} {}
+    void set_map(const BatchMap& mp) { c32.map = c64.map = mp; }
+    // r = b - A x (and p = r when first) for the systems of the map's blocks
+    void residual(int first, const BatchMap& mp, int grid) {
+        hipStream_t s = bt->stream;
+        with_rpl(bt->rpl, [&](auto RC) {
+            constexpr int R = decltype(RC)::value;
+            if (mixed)
+                k_bt_residual<float, R><<<grid, bt->block, 0, s>>>(bt->view64(), mp, bt->b_glob.p, bt->x_glob.p,
+                                                                   c32.r, c32.p, bt->sys.p, bt->red(), first);
+            else
+                k_bt_residual<double, R><<<grid, bt->block, 0, s>>>(bt->view64(), mp, bt->b_glob.p, bt->x_glob.p,
+                                                                    c64.r, c64.p, bt->sys.p, bt->red(), first);
+        });
+        k_bt_fin_residual<<<bt->m, 256, 0, s>>>(bt->map(), bt->sys.p, bt->red(), first, bt->m);
+        OCSL_HIP(hipGetLastError());
+    }
+    // one CG iteration of every unfinished system of the map's blocks
+    void iter(int grid) {
+        hipStream_t s = bt->stream;
+        with_rpl(bt->rpl, [&](auto RC) {
+            constexpr int R = decltype(RC)::value;
+            if (mixed) {
+                k_bt_spmv<float, float, R><<<grid, bt->block, 0, s>>>(c32);
+                k_bt_update<float, float><<<grid, UB, 0, s>>>(c32);
+            } else {
+                k_bt_spmv<double, double, R><<<grid, bt->block, 0, s>>>(c64);
+                k_bt_update<double, double><<<grid, UB, 0, s>>>(c64);
+            }
+        });
+    }
+};
+
+// Average time of one iteration of the whole batch: every system restarted
+// from x = 0 with tolerance 0 (none finishes), `iters` iterations captured
+// as one graph, the second of two replays timed with events.
+double bt_time_iterations(ocsl_batch bt, bool mixed, int iters) {
+    hipStream_t s = bt->stream;
+    bt->x_glob.alloc(bt->n_tot);
+    hipGraphExec_t ex = nullptr;
+    hipEvent_t e1 = nullptr;
+    float ms = 0.0f;
+    try {
+        OCSL_HIP(hipEventCreate(&e1));
+        for (int rep = 0; rep < 2; ++rep) {
+            OCSL_HIP(hipMemsetAsync(bt->x_glob.p, 0, sizeof(double) * bt->n_tot, s));
+            bt_prepare(bt, mixed, 0.0, 1 << 30, 0.0);
+            BtLaunch L(bt, mixed);
+            L.residual(1, bt->map(), bt->nblocks);
+            if (!ex) {
+                hipGraph_t gr;
+                OCSL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                for (int i = 0; i < iters; ++i) L.iter(bt->nblocks);
+                OCSL_HIP(hipStreamEndCapture(s, &gr));
+                const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                OCSL_HIP(e);
+            }
+            OCSL_HIP(hipEventRecord(bt->ev0, s));
+            OCSL_HIP(hipGraphLaunch(ex, s));
+            OCSL_HIP(hipEventRecord(e1, s));
+            OCSL_HIP(hipStreamSynchronize(s));
+            OCSL_HIP(hipEventElapsedTime(&ms, bt->ev0, e1));
+        }
+    } catch (...) {
+        if (ex) (void)hipGraphExecDestroy(ex);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipGraphExecDestroy(ex);
+    (void)hipEventDestroy(e1);
+    return 1e3 * (double)ms / iters;
 }
 
 }  // namespace
@@ -426,114 +784,24 @@ ocsl_status ocsl_batch_create(ocsl_batch* out, int m, const int64_t* n_rows, con
         OCSL_HIP(hipEventCreate(&bt->ev0));
         bt->m = m;
         bt->n_sys.assign(n_rows, n_rows + m);
-        std::vector<int> sp_all{0}, colg, srcg, dsrc, perm, bsys, bs0, bs1, bptr{0};
-        std::vector<int> row_off{0};
-        int64_t val_base = 0;
+        bt->csr.resize(m);
+        bt->order.resize(m);
         bt->val_off.assign(m + 1, 0);
-        bool compact = true;
-        const int spb = bt->block / kWave;  // slices per block: one per wave
-        int slice_base = 0;
-        int64_t flat_base = 0;  // offset of system k in the flat (concatenated) vectors
+        int64_t flat = 0;
         for (int k = 0; k < m; ++k) {
-            HostCSR a;
+            HostCSR& a = bt->csr[k];
             analyse_structure(n_rows[k], row_ptr[k], column_index[k], a);
-            const int n = a.n, ns = (n + kWave - 1) / kWave;
-            std::vector<int> order = rcm_order(a);
-            std::vector<int> iperm(n);
-            for (int i = 0; i < n; ++i) iperm[order[i]] = i;
-            // slice widths
-            const int r0 = row_off.back();
-            for (int sl = 0; sl < ns; ++sl) {
-                int w = 0;
-                for (int l = 0; l < kWave && sl * kWave + l < n; ++l) {
-                    const int o = order[sl * kWave + l];
-                    w = std::max(w, a.rp[o + 1] - a.rp[o] - 1);
-                }
-                w += w & 1;
-                const int base = sp_all.back();
-                sp_all.push_back(base + w * kWave);
-                colg.resize(base + (size_t)w * kWave);
-                srcg.resize(base + (size_t)w * kWave);
-                for (int l = 0; l < kWave; ++l) {
-                    const int r = sl * kWave + l;  // local new row
-                    int kk = 0;
-                    if (r < n) {
-                        const int o = order[r];
-                        for (int p = a.rp[o]; p < a.rp[o + 1]; ++p) {
-                            if (p == a.diag[o]) continue;
-                            const int64_t q = base + (int64_t)(kk / 2) * 2 * kWave + 2 * l + (kk & 1);
-                            colg[q] = r0 + iperm[a.col[p]];
-                            srcg[q] = (int)(val_base + a.src[p]);
-                            const int d = colg[q] - (r0 + r);
-                            if (d < -32768 || d > 32767) compact = false;
-                            ++kk;
-                        }
-                    }
-                    for (; kk < w; ++kk) {
-                        const int64_t q = base + (int64_t)(kk / 2) * 2 * kWave + 2 * l + (kk & 1);
-                        colg[q] = r0 + r;  // padding rows of a system are its own (zero) rows
-                        srcg[q] = -1;
-                    }
-                }
-            }
-            for (int r = 0; r < ns * kWave; ++r) {
-                perm.push_back(r < n ? (int)(flat_base + order[r]) : -1);
-                dsrc.push_back(r < n ? (int)(val_base + a.src[a.diag[order[r]]]) : -1);
-            }
-            // blocks of this system
-            for (int s0 = 0; s0 < ns; s0 += spb) {
-                bsys.push_back(k);
-                bs0.push_back(slice_base + s0);
-                bs1.push_back(slice_base + std::min(ns, s0 + spb));
-            }
-            bptr.push_back((int)bsys.size());
-            slice_base += ns;
-            row_off.push_back(r0 + ns * kWave);
-            flat_base += n;
-            val_base += a.nnz_in;
+            bt->order[k] = rcm_order(a);
             bt->nnz_in.push_back(a.nnz_in);
-            bt->val_off[k + 1] = val_base;
+            bt->val_off[k + 1] = bt->val_off[k] + a.nnz_in;
+            flat += a.n;
         }
-        OCSL_REQUIRE(val_base < ((int64_t)1 << 31) && row_off.back() < (1 << 30), OCSL_ERR_ARG, "batch too large");
-        bt->n_flat = flat_base;
-        bt->row_off = row_off;
-        bt->n_tot = row_off.back();
-        bt->ns_tot = slice_base;
-        bt->nblocks = (int)bsys.size();
-        bt->bptr_host = bptr;
-        bt->total = sp_all.back();
-        bt->compact = compact;
-        hipStream_t s = bt->stream;
-        bt->slice_ptr.upload(sp_all.data(), sp_all.size(), s);
-        if (compact) {
-            std::vector<unsigned> colw(bt->total / 2);
-            for (int sl = 0; sl < bt->ns_tot; ++sl) {
-                const int np = (sp_all[sl + 1] - sp_all[sl]) / (2 * kWave);
-                for (int t = 0; t < np; ++t)
-                    for (int l = 0; l < kWave; ++l) {
-                        const int64_t q = sp_all[sl] + (int64_t)t * 2 * kWave + 2 * l;
-                        const int row = sl * kWave + l;
-                        const unsigned lo = (unsigned)(uint16_t)(int16_t)(colg[q] - row);
-                        const unsigned hi = (unsigned)(uint16_t)(int16_t)(colg[q + 1] - row);
-                        colw[sp_all[sl] / 2 + (int64_t)t * kWave + l] = lo | (hi << 16);
-                    }
-            }
-            bt->colw.upload(colw.data(), colw.size(), s);
-        } else {
-            bt->col.upload(colg.data(), colg.size(), s);
-        }
-        bt->src.upload(srcg.data(), srcg.size(), s);
-        bt->diag_src.upload(dsrc.data(), dsrc.size(), s);
-        bt->perm.upload(perm.data(), perm.size(), s);
-        bt->bsys.upload(bsys.data(), bsys.size(), s);
-        bt->bs0.upload(bs0.data(), bs0.size(), s);
-        bt->bs1.upload(bs1.data(), bs1.size(), s);
-        bt->bptr.upload(bptr.data(), bptr.size(), s);
-        bt->red_partial.alloc((size_t)bt->nblocks * 8);
-        bt->active.alloc(bt->nblocks);
+        OCSL_REQUIRE(bt->val_off[m] < ((int64_t)1 << 31) && flat < ((int64_t)1 << 30), OCSL_ERR_ARG, "batch too large");
+        bt->n_flat = flat;
+        bt->rpl = 0;  // forces the first build
+        bt_set_launch(bt, 1, 1024);
         bt->sys.alloc(m);
         OCSL_HIP(hipHostMalloc((void**)&bt->sys_host, sizeof(SysScalars) * m, hipHostMallocDefault));
-        OCSL_HIP(hipStreamSynchronize(s));
     });
     if (st != OCSL_OK) {
         delete bt;
@@ -569,16 +837,7 @@ ocsl_status ocsl_batch_set_values(ocsl_batch bt, const double* const* values, oc
             OCSL_HIP(hipMemcpyAsync(bt->val_in.p + bt->val_off[k], values[k], sizeof(double) * bt->nnz_in[k],
                                     where == OCSL_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
         }
-        bt->val64.alloc(bt->total);
-        bt->diag64.alloc(bt->n_tot);
-        bt->val32.alloc(bt->total);
-        bt->diag32.alloc(bt->n_tot);
-        k_gather_vals<double><<<grid_for(bt->total), kBlock, 0, s>>>(bt->total, bt->src.p, bt->val_in.p, bt->val64.p);
-        k_gather_vals<double><<<grid_for(bt->n_tot), kBlock, 0, s>>>(bt->n_tot, bt->diag_src.p, bt->val_in.p,
-                                                                    bt->diag64.p);
-        k_gather_vals<float><<<grid_for(bt->total), kBlock, 0, s>>>(bt->total, bt->src.p, bt->val_in.p, bt->val32.p);
-        k_bt_rowsum<<<grid_for((int64_t)bt->ns_tot * kWave), kBlock, 0, s>>>(bt->view64(), bt->diag32.p);
-        OCSL_HIP(hipGetLastError());
+        bt_gather_values(bt);
         OCSL_HIP(hipStreamSynchronize(s));
         bt->have_vals = true;
     });
@@ -601,10 +860,71 @@ static void bt_flat_in(ocsl_batch bt, const double* const* v, ocsl_where where, 
 ocsl_status ocsl_batch_set_rhs(ocsl_batch bt, const double* const* b, ocsl_where where) {
     if (!bt || !b) return OCSL_ERR_ARG;
     return guarded_b([&] {
+        const double* before = bt->b_glob.p;
         bt->b_glob.alloc(bt->n_tot);
+        if (before != bt->b_glob.p) bt_drop_graphs(bt);  // graphs hold b's address
         bt_flat_in(bt, b, where, bt->b_in, bt->b_glob.p);
         OCSL_HIP(hipStreamSynchronize(bt->stream));
         bt->have_b = true;
+    });
+}
+
+ocsl_status ocsl_batch_set_launch(ocsl_batch bt, int rows_per_lane, int block_size) {
+    if (!bt) return OCSL_ERR_ARG;
+    return guarded_b([&] { bt_set_launch(bt, rows_per_lane, block_size); });
+}
+
+ocsl_status ocsl_batch_get_launch(ocsl_batch bt, int* rows_per_lane, int* block_size) {
+    if (!bt || !rows_per_lane || !block_size) return OCSL_ERR_ARG;
+    *rows_per_lane = bt->rpl;
+    *block_size = bt->block;
+    return OCSL_OK;
+}
+
+ocsl_status ocsl_batch_time_iterations(ocsl_batch bt, int precision, int iterations, double* us_per_iter) {
+    if (!bt || !us_per_iter || iterations <= 0) return OCSL_ERR_ARG;
+    return guarded_b([&] {
+        OCSL_REQUIRE(precision == OCSL_FP64 || precision == OCSL_MIXED, OCSL_ERR_ARG, "bad precision");
+        OCSL_REQUIRE(bt->have_vals && bt->have_b, OCSL_ERR_STATE, "values or rhs not set");
+        *us_per_iter = bt_time_iterations(bt, precision == OCSL_MIXED, iterations);
+    });
+}
+
+ocsl_status ocsl_batch_tune(ocsl_batch bt, int precision, int iterations, int* best_rpl, int* best_block,
+                            double* best_us, ocsl_tune_entry* table, int table_cap, int* table_len) {
+    if (!bt || iterations <= 0 || (table && table_cap < 0)) return OCSL_ERR_ARG;
+    if (table_len) *table_len = 0;
+    return guarded_b([&] {
+        OCSL_REQUIRE(precision == OCSL_FP64 || precision == OCSL_MIXED, OCSL_ERR_ARG, "bad precision");
+        OCSL_REQUIRE(bt->have_vals && bt->have_b, OCSL_ERR_STATE, "values or rhs not set");
+        const bool mixed = precision == OCSL_MIXED;
+        int br = bt->rpl, bb = bt->block, len = 0;
+        double bu = 0.0;
+        for (int R : {1, 2, 4})
+            for (int B : {256, 512, 1024}) {
+                bt_set_launch(bt, R, B);
+                const double us = bt_time_iterations(bt, mixed, iterations);
+                if (table && len < table_cap) {
+                    ocsl_tune_entry& e = table[len];
+                    e.rows_per_lane = R;
+                    e.block_size = B;
+                    e.grid_blocks = bt->nblocks;
+                    e.vec_blocks = bt->upd_block(mixed);
+                    e.cg_kernels = 2;
+                    e.us_per_iteration = us;
+                }
+                ++len;
+                if (bu == 0.0 || us < bu) {
+                    bu = us;
+                    br = R;
+                    bb = B;
+                }
+            }
+        bt_set_launch(bt, br, bb);
+        if (best_rpl) *best_rpl = br;
+        if (best_block) *best_block = bb;
+        if (best_us) *best_us = bu;
+        if (table_len) *table_len = table ? std::min(len, table_cap) : len;
     });
 }
 
@@ -621,61 +941,20 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
         OCSL_REQUIRE(o.max_iter > 0 && o.check_every > 0 && o.tol >= 0.0, OCSL_ERR_ARG, "bad options");
         const bool mixed = precision == OCSL_MIXED;
         hipStream_t s = bt->stream;
-        const int m = bt->m, N = bt->n_tot, G = bt->nblocks, B = bt->block;
+        const int m = bt->m, N = bt->n_tot, G = bt->nblocks;
         DBuf<double> flat;
         bt->x_glob.alloc(N);
         bt_flat_in(bt, x, where, flat, bt->x_glob.p);
-        // vectors in the Krylov precision
-        const bool remap = mixed ? (!bt->f[0].p || bt->f[0].n < (size_t)N) : (!bt->d[0].p || bt->d[0].n < (size_t)N);
-        if (remap) {
-            for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
-            bt->graphs.clear();
-        }
-        for (int i = 0; i < 3; ++i) {
-            if (mixed) bt->f[i].alloc(N);
-            else bt->d[i].alloc(N);
-        }
-        std::vector<SysScalars> init(m);
-        for (auto& S : init) {
-            S = SysScalars{};
-            S.tol2 = o.tol * o.tol;
-            S.max_iter = o.max_iter;
-            S.replace_ratio = mixed ? o.replace_ratio : 0.0;
-        }
-        OCSL_HIP(hipMemcpyAsync(bt->sys.p, init.data(), sizeof(SysScalars) * m, hipMemcpyHostToDevice, s));
-        // padded rows must stay zero in every vector
-        for (int i = 0; i < 3; ++i) {
-            if (mixed) OCSL_HIP(hipMemsetAsync(bt->f[i].p, 0, sizeof(float) * N, s));
-            else OCSL_HIP(hipMemsetAsync(bt->d[i].p, 0, sizeof(double) * N, s));
-        }
+        bt_prepare(bt, mixed, o.tol, o.max_iter, o.replace_ratio);
         // launched blocks: all, or (compacted) the blocks of unfinished systems
         int G_cur = G;
         bool comp = false;
-        BtArgs<float, float> c32{bt->view32(), bt->map(), bt->f[0].p, bt->f[1].p, bt->f[2].p, bt->x_glob.p,
-                                 bt->sys.p, bt->red()};
-        BtArgs<double, double> c64{bt->view64(), bt->map(), bt->d[0].p, bt->d[1].p, bt->d[2].p, bt->x_glob.p,
-                                   bt->sys.p, bt->red()};
+        BtLaunch L(bt, mixed);
         auto residual = [&](int first, bool all) {
-            const BatchMap mp = bt->map(comp && !all);
-            const int gl = (comp && !all) ? G_cur : G;
-            if (mixed)
-                k_bt_residual<float><<<gl, B, 0, s>>>(bt->view64(), mp, bt->b_glob.p, bt->x_glob.p, c32.r, c32.p,
-                                                      bt->sys.p, bt->red(), first);
-            else
-                k_bt_residual<double><<<gl, B, 0, s>>>(bt->view64(), mp, bt->b_glob.p, bt->x_glob.p, c64.r, c64.p,
-                                                       bt->sys.p, bt->red(), first);
-            k_bt_fin_residual<<<m, 256, 0, s>>>(bt->map(), bt->sys.p, bt->red(), first, m);
-            OCSL_HIP(hipGetLastError());
+            const bool c = comp && !all;
+            L.residual(first, bt->map(c), c ? G_cur : G);
         };
-        auto iter = [&] {
-            if (mixed) {
-                k_bt_spmv<float, float><<<G_cur, B, 0, s>>>(c32);
-                k_bt_update<float, float><<<G_cur, kBtUpdateBlock, 0, s>>>(c32);
-            } else {
-                k_bt_spmv<double, double><<<G_cur, B, 0, s>>>(c64);
-                k_bt_update<double, double><<<G_cur, kBtUpdateBlock, 0, s>>>(c64);
-            }
-        };
+        auto iter = [&] { L.iter(G_cur); };
         auto read_sys = [&] {
             OCSL_HIP(hipMemcpyAsync(bt->sys_host, bt->sys.p, sizeof(SysScalars) * m, hipMemcpyDeviceToHost, s));
             OCSL_HIP(hipStreamSynchronize(s));
@@ -705,22 +984,21 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
             OCSL_HIP(hipStreamSynchronize(s));
             G_cur = (int)act.size();
             comp = true;
-            c32.map = c64.map = bt->map(true);
+            L.set_map(bt->map(true));
         };
         OCSL_HIP(hipEventRecord(bt->ev0, s));
         residual(1, true);
         read_sys();
         stamp();
         const int chunk = o.check_every;
-        int launched = 0;
-        if (bt->graphs.size() > 64) {
-            for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
-            bt->graphs.clear();
-        }
+        int64_t launched = 0;
+        if (bt->graphs.size() > 64) bt_drop_graphs(bt);
         while (true) {
             bool all = true;
             for (int k = 0; k < m; ++k) all = all && finished(bt->sys_host[k]);
-            if (all || launched >= o.max_iter) break;
+            // an iteration is a CG step or (MIXED) a residual replacement,
+            // never two replacements in a row: 2 max_iter launches bound it
+            if (all || launched >= 2 * (int64_t)o.max_iter) break;
             compact();
             // graphs hold the launch size and map; the active list is read
             // through its (fixed) device address when the graph runs
@@ -746,10 +1024,7 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
             read_sys();
             if (mixed) {
                 bool any = false;
-                for (int k = 0; k < m; ++k) {
-                    const SysScalars& S = bt->sys_host[k];
-                    any = any || S.need_update || (S.done && !S.verified && !S.nonfinite && !S.breakdown);
-                }
+                for (int k = 0; k < m; ++k) any = any || bt_wants_residual(bt->sys_host[k]);
                 if (any) {
                     residual(0, false);
                     read_sys();
@@ -770,7 +1045,8 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
         // true residuals: one more residual pass for every system
         std::vector<SysScalars> before(bt->sys_host, bt->sys_host + m);
         for (int k = 0; k < m; ++k) {
-            bt->sys_host[k].need_update = 1;
+            bt->sys_host[k].done = 1;
+            bt->sys_host[k].verified = 0;
             bt->sys_host[k].nonfinite = 0;
             bt->sys_host[k].breakdown = 0;
         }

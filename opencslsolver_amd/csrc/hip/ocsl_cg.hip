@@ -8,9 +8,12 @@
 //                  form (three kernels, two for MIXED).  MIXED runs the
 //                  iterations in FP32 (matrix in difference form, r, p, q)
 //                  with FP64 dot accumulation, the solution accumulated in
-//                  FP64 every iteration and the FP64 true residual b - A x
-//                  replacing the FP32 recurrence residual whenever the latter
-//                  has dropped by `replace_ratio` (reliable updates).
+//                  FP64 (once per period of x_period iterations) and the FP64
+//                  true residual b - A x replacing the FP32 recurrence
+//                  residual whenever the latter has dropped by
+//                  `replace_ratio` (reliable updates); the chunk stops at the
+//                  trigger, so the replacement comes before the next
+//                  iteration whatever the chunk length.
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -36,7 +39,6 @@ static void launch_spmv(ocsl_solver* h, const Sell& s, double a, const double* x
 
 // ||b - A x|| / ||b|| over SELL-P: the residual kernel writing into scratch.
 static double true_relres_p(ocsl_solver* h, const SellP& s, const double* b, const double* x) {
-    const int n = s.n;
     double* r = work_buf(h, h->work.d[11], s.padded());
     const int g = std::max(1, std::min(kMaxRedBlocks, (s.n_slices + 15) / 16 * 1));
     with_rpl(s.rpl, [&](auto RC) {

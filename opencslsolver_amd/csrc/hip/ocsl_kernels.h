@@ -597,7 +597,15 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int fi
             if (c.hist && it < S->hist_cap) c.hist[it] = sqrt(rr / scale);
             if (!(rr > S->tol2 * scale) || it >= S->max_iter) S->done = 1;
             if (!isfinite(rr)) { S->nonfinite = 1; S->done = 1; }
-            if (sqrt(rr) < S->replace_ratio * S->r_ref) S->need_update = 1;
+            // the FP64 residual replacement follows its trigger at once: the
+            // solve stops here (as on a convergence claim) and the host
+            // replaces r before the next iteration.  Iterations run on past
+            // the trigger can let the FP32 recurrence drift beyond recovery
+            // on systems with condition near 1 / eps32.
+            if (sqrt(rr) < S->replace_ratio * S->r_ref) {
+                S->need_update = 1;
+                S->done = 1;
+            }
         }
     }
     const int lane = threadIdx.x & (kWave - 1);
@@ -874,13 +882,16 @@ __device__ __forceinline__ bool cgf_stop(Scalars* S, double* hist, double pq, do
     const bool conv = !(rr > S->tol2 * scale);
     const bool fin = isfinite(rr);
     const bool pq_ok = cg_pq_ok(pq);
-    const bool stop = conv || !fin || !pq_ok || k >= kstop;
+    // a due FP64 residual replacement stops the solve too: the host replaces
+    // r before the next iteration
+    const bool replace = sqrt(rr) < S->replace_ratio * S->r_ref;
+    const bool stop = conv || !fin || !pq_ok || replace || k >= kstop;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const int it = S->iter;
         if (hist && it < S->hist_cap) hist[it] = sqrt(rr / scale);
         S->rr = rr;
         S->rr_last = rr;
-        if (sqrt(rr) < S->replace_ratio * S->r_ref) S->need_update = 1;
+        if (replace) S->need_update = 1;
         if (stop) {
             S->fstop[odd] = 1;
             S->done = 1;

@@ -362,3 +362,33 @@ class Batch:
                                         reps, lat.ctypes.data_as(_f64p))
         _check(st, allow_warn=True)
         return xs, [r.as_dict() for r in reps], lat
+
+    # -- launch shape ------------------------------------------------------
+    def set_launch(self, rows_per_lane=1, block_size=1024):
+        """Rows per lane (1, 2, 4) of the concatenated layout and threads per
+        SpMV workgroup (256, 512, 1024); values and rhs follow."""
+        _check(self._lib.ocsl_batch_set_launch(self._h, rows_per_lane, block_size))
+
+    def launch(self):
+        r, b = ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.ocsl_batch_get_launch(self._h, ctypes.byref(r), ctypes.byref(b)))
+        return {"rows_per_lane": r.value, "block_size": b.value}
+
+    def time_iterations(self, precision="mixed", iterations=32):
+        """Device us per CG iteration of the whole batch in the current shape."""
+        us = ctypes.c_double()
+        _check(self._lib.ocsl_batch_time_iterations(self._h, PRECISIONS[precision], iterations, ctypes.byref(us)))
+        return us.value
+
+    def tune(self, precision="mixed", iterations=32):
+        """Times every rows-per-lane x workgroup-size shape, keeps the fastest.
+        Returns (best shape dict, best us/iteration, table)."""
+        r, b, us, ln = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        cap = 32
+        table = (_abi.TuneEntry * cap)()
+        _check(self._lib.ocsl_batch_tune(self._h, PRECISIONS[precision], iterations, ctypes.byref(r),
+                                         ctypes.byref(b), ctypes.byref(us), table, cap, ctypes.byref(ln)))
+        keys = ("rows_per_lane", "block_size", "grid_blocks", "vec_blocks")
+        rows = [dict({k: getattr(table[i], k) for k in keys}, us_per_iteration=table[i].us_per_iteration)
+                for i in range(ln.value)]
+        return {"rows_per_lane": r.value, "block_size": b.value}, us.value, rows

@@ -325,3 +325,43 @@ def test_cg_autotuner_keeps_x_and_tuned_solves_converge():
         s.set_tuned_cg("mixed")  # cleared: defaults again
         s.set_x(np.zeros(a.n))
         assert s.solve("cg", "tolerance", "mixed", tol=1e-8)["status"] == 0
+
+
+@pytest.mark.parametrize("kernels", [1, 2])
+@pytest.mark.parametrize("check_every", [1, 8, 32])
+def test_cg_mixed_replacement_right_after_trigger(kernels, check_every):
+    # Pressure systems with a near-null constant mode (smallest eigenvalue
+    # ~1e-6, condition ~1e7) and right-hand sides that excite it: the FP32
+    # recurrence drifts fast, and iterations run on past the replacement
+    # trigger (to the end of a chunk) let it diverge for good on the 27-row
+    # system.  The chunk stops at the trigger, so every chunk length and both
+    # kernel forms converge with the same number of iterations.
+    its = []
+    for n in (3, 5, 7):
+        a, _, _ = gen.poisson3d(n, seed_mesh=100, seed_rhs=101)
+        b = gen.uniform(70, a.n)
+        with Solver(a.row_ptr, a.col) as s:
+            s.set_values(a.val)
+            s.set_rhs(b)
+            s.set_x(np.zeros(a.n))
+            rep = s.solve("cg", "tolerance", "mixed", tol=1e-8, check_every=check_every, max_iter=3000,
+                          cg_kernels=kernels)
+            x = s.get_x()
+        assert rep["status"] == 0, (n, rep)
+        assert np.linalg.norm(b - a.matvec(x)) / np.linalg.norm(b) <= 1.05e-8
+        assert rep["replacements"] >= 1
+        its.append(rep["iterations"])
+    # the same runs with the shortest chunk (same kernel form: the forms add
+    # their dot products in different orders, which moves the count a little
+    # on systems this ill-conditioned)
+    ref = []
+    for n in (3, 5, 7):
+        a, _, _ = gen.poisson3d(n, seed_mesh=100, seed_rhs=101)
+        b = gen.uniform(70, a.n)
+        with Solver(a.row_ptr, a.col) as s:
+            s.set_values(a.val)
+            s.set_rhs(b)
+            s.set_x(np.zeros(a.n))
+            ref.append(s.solve("cg", "tolerance", "mixed", tol=1e-8, check_every=1, max_iter=3000,
+                               cg_kernels=kernels)["iterations"])
+    assert all(abs(u - v) <= 2 for u, v in zip(its, ref)), (its, ref)
