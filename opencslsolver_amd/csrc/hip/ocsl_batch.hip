@@ -38,8 +38,9 @@ struct SysScalars {
     int replaced;     // this iteration's SpMV kernel replaced r (set by the SpMV kernel)
 };
 
-// partials of the true r.r of an in-iteration replacement
-constexpr int kSlotTR = 5;
+// partials of the true r.r of an in-iteration replacement (the batch's
+// partial scratch holds 8 slots per block)
+constexpr int kBtSlotTR = 5;
 
 struct BatchMap {
     const int* bsys;    // block -> system
@@ -236,13 +237,12 @@ __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
             X rv[R];
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                const double rj = bv[j] - ax[j];
-                acc[0] += rj * rj;
-                rv[j] = (X)rj;
+                rv[j] = (X)(bv[j] - ax[j]);
+                acc[0] += (double)rv[j] * (double)rv[j];  // r.r of the stored r
             }
             store_rows<R>(c.r, row0, c.A.n, rv);
         }
-        bt_partials<1>(acc, c.red, kSlotTR, gb);
+        bt_partials<1>(acc, c.red, kBtSlotTR, gb);
         if (gb == c.map.bptr[k] && threadIdx.x == 0) S->replaced = 1;
         return;
     }
@@ -295,9 +295,9 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
         // the replaced residual's r.r partials become the next rr; its norm
         // the reference of the next replacement test
         if (threadIdx.x == 0)
-            c.red.partial[(size_t)kSlotRR * c.red.stride + gb] = c.red.partial[(size_t)kSlotTR * c.red.stride + gb];
+            c.red.partial[(size_t)kSlotRR * c.red.stride + gb] = c.red.partial[(size_t)kBtSlotTR * c.red.stride + gb];
         if (gb == b0) {
-            const double tr = sys_sum(c.red, kSlotTR, b0, b1);
+            const double tr = sys_sum(c.red, kBtSlotTR, b0, b1);
             if (threadIdx.x == 0) {
                 S->r_ref = sqrt(tr);
                 S->replacements++;
