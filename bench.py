@@ -399,6 +399,54 @@ def assembly(args, out):
                        "gpu_total_us": round(sum(v["avg_us"] for v in res.values()), 2), "cpu_baseline": cpu}
 
 
+def structure(args, out):
+    """Matrix structure of the n^3 hex mesh (CFVM::define_matrix_structure) built
+    on the device: from host arrays (PCIe included) and from device arrays."""
+    import ctypes
+
+    import torch
+
+    from opencslsolver_amd import gen
+    from opencslsolver_amd._native import hip_lib
+    from opencslsolver_amd.solver import build_structure
+
+    n = args.n
+    _log(f"bench: structure of the {n}^3 mesh")
+    mesh, rp_g, col_g = gen.hex_mesh(n)
+    fl, fr, bl = mesh["face_left"], mesh["face_right"], mesh["bc_left"]
+    build_structure(n ** 3, fl, fr, bl)  # warm (module load)
+    t0 = time.perf_counter()
+    rp, col, fs, ds = build_structure(n ** 3, fl, fr, bl, slots=True)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    ok = bool(np.array_equal(rp, rp_g) and np.array_equal(col, col_g))
+    dev = torch.device("cuda:0")
+    d_fl, d_fr, d_bl = (torch.from_numpy(a).to(dev) for a in (fl, fr, bl))
+    d_rp = torch.empty(n ** 3 + 1, dtype=torch.int64, device=dev)
+    d_col = torch.empty(col.shape[0], dtype=torch.int64, device=dev)
+    d_fs = torch.empty(2 * fl.shape[0], dtype=torch.int64, device=dev)
+    d_ds = torch.empty(n ** 3, dtype=torch.int64, device=dev)
+    lib = hip_lib()
+    nnz = ctypes.c_int64()
+    vp = ctypes.c_void_p
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = lib.ocsl_build_structure(n ** 3, fl.shape[0], vp(d_fl.data_ptr()), vp(d_fr.data_ptr()), bl.shape[0],
+                                      vp(d_bl.data_ptr()), 1, vp(d_rp.data_ptr()), vp(d_col.data_ptr()),
+                                      col.shape[0], ctypes.byref(nnz), vp(d_fs.data_ptr()), vp(d_ds.data_ptr()))
+        assert st == 0
+    torch.cuda.synchronize()
+    dev_ms = (time.perf_counter() - t0) * 1e3 / reps
+    out["structure"] = {
+        "workload": f"CSR structure + face and diagonal slot maps of the {n}^3 hex mesh: {n ** 3} cells, "
+                    f"{fl.shape[0]} interior faces, {bl.shape[0]} boundary faces, {col.shape[0]} entries",
+        "device_arrays_ms": round(dev_ms, 3),
+        "host_arrays_ms": round(host_ms, 3),
+        "equals_generator": ok,
+    }
+
+
 def flow_step(args, out):
     """One SIMPLE iteration of the flow solver (three momentum solves with
     ILU(0)-BiCGSTAB, the pressure correction with CG, enthalpy; REFERENCE-mode
@@ -448,8 +496,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--sections", default="adjoint,multi,batch,assembly,flow",
-                    help="side workloads to add to the headline: adjoint, multi, batch, assembly, flow (or none)")
+    ap.add_argument("--sections", default="adjoint,multi,batch,assembly,flow,structure",
+                    help="side workloads to add to the headline: adjoint, multi, batch, assembly, flow, "
+                         "structure (or none)")
     ap.add_argument("--flow-n", type=int, default=64, help="hex mesh size of the flow-step section")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reuse-tuning", action="store_true",
@@ -492,6 +541,8 @@ def main():
         assembly(args, out)
     if "flow" in args.sections:
         flow_step(args, out)
+    if "structure" in args.sections:
+        structure(args, out)
     print(json.dumps(out), flush=True)
 
 

@@ -101,6 +101,25 @@ void ocsl_default_options(ocsl_options* opt);
 /* Number of HIP devices (0 when none); never fails. */
 int ocsl_device_count(void);
 
+/* Builds, on the device, the CSR structure CFVM::define_matrix_structure
+ * makes (cfvm.cpp:74-98, each element inserted through CCSR::insert_element,
+ * cmatrix_solver.cpp:57-100): row i holds, ascending and each once, i, the
+ * cells across its faces and, when it owns a boundary face, the sentinel
+ * column (uint64_t)-1 (the -1 of its neighbour list, cgeometry.cpp:602-612).
+ * Input: n_cells, the n_faces interior faces (face_left[f], face_right[f]:
+ * the two cells) and the cells of the n_bc boundary faces.  Output: row_ptr
+ * (n_cells + 1), column_index (column_cap entries), *nnz; optionally
+ * face_slot (2 n_faces: the slot of column face_right[f] in row face_left[f],
+ * then of face_left[f] in row face_right[f]) and diag_slot (n_cells: the slot
+ * of each diagonal).  With column_index NULL only *nnz and (if given)
+ * row_ptr are produced: the size query.  All arrays are host or device
+ * memory per `where`.  Cell indices out of range, or a face with one cell on
+ * both sides, return OCSL_ERR_STRUCTURE before anything is filled. */
+ocsl_status ocsl_build_structure(int64_t n_cells, int64_t n_faces, const int32_t* face_left,
+                                 const int32_t* face_right, int64_t n_bc, const int32_t* bc_left,
+                                 ocsl_where where, uint64_t* row_ptr, uint64_t* column_index,
+                                 int64_t column_cap, int64_t* nnz, int64_t* face_slot, int64_t* diag_slot);
+
 /* Creates a handle for an n x n CSR structure (the reference's CCSR layout).
  * Replaces CFVM::define_matrix_structure's CMatrix_solver construction
  * (cfvm.cpp:74-98: CMatrix_solver(n, n, nnz), M = matrix,

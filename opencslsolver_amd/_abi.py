@@ -126,6 +126,10 @@ SIGNATURES = {
     "ocsl_batch_time_iterations": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, _f64p]),
     "ocsl_batch_tune": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p,
                                        ctypes.POINTER(TuneEntry), ctypes.c_int, _i32p]),
+    "ocsl_build_structure": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_int64, _i64p, ctypes.c_void_p,
+                                            ctypes.c_void_p]),
     "ocsl_last_error": (ctypes.c_char_p, []),
 }
 

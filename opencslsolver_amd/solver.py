@@ -301,6 +301,35 @@ class Solver:
         return ms.value, by.value
 
 
+def build_structure(n_cells, face_left, face_right, bc_left, slots=False):
+    """The CSR structure of CFVM::define_matrix_structure (cfvm.cpp:74-98),
+    built on the device from the mesh's face lists: per row, ascending and
+    each once, the cell, the cells across its faces and, when it owns a
+    boundary face, the sentinel column 2^64 - 1.  Returns (row_ptr, col) as
+    uint64 arrays, and with slots=True also (face_slot[F, 2], diag_slot[n]):
+    the slots of each face's two off-diagonal entries (row face_left, then
+    row face_right) and of each diagonal."""
+    lib = hip_lib()
+    fl = np.ascontiguousarray(face_left, dtype=np.int32)
+    fr = np.ascontiguousarray(face_right, dtype=np.int32)
+    bl = np.ascontiguousarray(bc_left, dtype=np.int32)
+    if fl.shape != fr.shape:
+        raise ValueError("face_left and face_right differ in length")
+    nnz = ctypes.c_int64()
+    vp = ctypes.c_void_p
+    args = (int(n_cells), fl.shape[0], fl.ctypes.data_as(vp), fr.ctypes.data_as(vp), bl.shape[0],
+            bl.ctypes.data_as(vp), _abi.HOST)
+    row_ptr = np.empty(int(n_cells) + 1, np.uint64)
+    _check(lib.ocsl_build_structure(*args, row_ptr.ctypes.data_as(vp), None, 0, ctypes.byref(nnz), None, None))
+    col = np.empty(nnz.value, np.uint64)
+    fs = np.empty((fl.shape[0], 2), np.int64) if slots else None
+    ds = np.empty(int(n_cells), np.int64) if slots else None
+    _check(lib.ocsl_build_structure(*args, row_ptr.ctypes.data_as(vp), col.ctypes.data_as(vp), col.shape[0],
+                                    ctypes.byref(nnz), fs.ctypes.data_as(vp) if slots else None,
+                                    ds.ctypes.data_as(vp) if slots else None))
+    return (row_ptr, col, fs, ds) if slots else (row_ptr, col)
+
+
 class Batch:
     """Many independent CSR systems solved by CG in the same launches."""
 

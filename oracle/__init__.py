@@ -253,3 +253,48 @@ def ref_gemv(A, x, alpha=1.0, beta=0.0, z=None):
     out = np.empty(A.n)
     ref().ref_gemv(*_csr(A), _p(A.val), _d(alpha), _p(x), _d(beta), _p(z), _p(out))
     return out
+
+
+SENTINEL_COL = (1 << 64) - 1
+
+
+def define_matrix_structure(n_cells, face_left, face_right, bc_left):
+    """CSR structure of CFVM::define_matrix_structure (cfvm.cpp:74-98).
+
+    Every cell i inserts (i, i), then (i, j) for each entry j of its neighbour
+    list as Cgeometry builds it (cgeometry.cpp:602-612: per face the cell
+    across it, -1 for a boundary face; the -1 converts to the size_t column
+    2^64 - 1).  Insertion follows CCSR::insert_element (cmatrix_solver.cpp:
+    57-100): a column already in the row is only overwritten; otherwise it
+    goes before the first larger column of the row, or at the row's end.  An
+    insertion into row i changes no other row's columns (it only shifts later
+    row pointers), so rows are built one at a time here.  Pure-Python loops:
+    small meshes only.  Returns (row_ptr, col) as uint64 arrays."""
+    n = int(n_cells)
+    neigh = [[] for _ in range(n)]
+    for l, r in zip(np.asarray(face_left).tolist(), np.asarray(face_right).tolist()):
+        neigh[l].append(r)
+        neigh[r].append(l)
+    for c in np.asarray(bc_left).tolist():
+        neigh[c].append(-1)
+    row_ptr = [0]
+    cols = []
+    for i in range(n):
+        row = []
+
+        def insert(j):
+            j &= SENTINEL_COL
+            if j in row:
+                return
+            for k, c in enumerate(row):
+                if j < c:
+                    row.insert(k, j)
+                    return
+            row.append(j)
+
+        insert(i)
+        for j in neigh[i]:
+            insert(j)
+        cols.extend(row)
+        row_ptr.append(len(cols))
+    return np.array(row_ptr, np.uint64), np.array(cols, np.uint64)

@@ -14,6 +14,7 @@ How each oracle function is pinned:
 | or_ilu_bicgstab_reference| reference golden results (seeded systems and Sod momentum/enthalpy systems) |
 | or_cg_tol / or_bicgstab_tol | numpy dense solve; residual bound; n-step exactness |
 | or_transpose / or_sampled_outer | dense transpose / outer product |
+| define_matrix_structure  | the reference's own Sod structure (sod_*_2steps.npz, from its mesh in flow_sod1000_mesh.npz); the vectorised hex-mesh generator (gen.hex_mesh) with and without a cell permutation |
 """
 import glob
 import os
@@ -283,3 +284,32 @@ def test_oracle_assembly_matches_reference_states(case):
         if eq < 3:
             assert rel(ap, d["state_ap"][k]) <= 1e-12, (case, k)
             assert rel(ap_vec[:, eq], d["state_ap_vec"][k][:, eq]) <= 1e-12, (case, k)
+
+
+def test_structure_matches_reference_sod_structure():
+    """The 1000-cell shock tube: 999 interior faces, 4002 boundary faces (4
+    per cell plus the two ends): one sentinel per row, as the reference's
+    CCSR stores a repeated column once."""
+    m = np.load(os.path.join(GOLDEN, "flow_sod1000_mesh.npz"))
+    rp, col = oracle.define_matrix_structure(1000, m["mesh_face_left"], m["mesh_face_right"], m["mesh_bc_left"])
+    for tag in ("cg", "bicgstab", "split"):
+        d = np.load(os.path.join(GOLDEN, f"sod_{tag}_2steps.npz"))
+        assert np.array_equal(rp, d["row_ptr"]) and np.array_equal(col, d["col"]), tag
+    assert int(np.sum(col == oracle.SENTINEL_COL)) == 1000
+
+
+@pytest.mark.parametrize("n,permuted", [(3, False), (5, True), (6, True)])
+def test_structure_matches_hex_mesh_generator(n, permuted):
+    perm = np.random.default_rng(n).permutation(n ** 3) if permuted else None
+    mesh, rp_g, col_g = gen.hex_mesh(n, perm=perm)
+    rp, col = oracle.define_matrix_structure(n ** 3, mesh["face_left"], mesh["face_right"], mesh["bc_left"])
+    assert np.array_equal(rp, rp_g) and np.array_equal(col, col_g)
+
+
+def test_structure_repeats_and_isolated_cells():
+    """A face listed twice and a cell with three boundary faces store their
+    columns once; a cell without faces holds its diagonal only."""
+    rp, col = oracle.define_matrix_structure(4, [0, 1, 1], [1, 0, 2], [2, 2, 2, 0])
+    S = oracle.SENTINEL_COL
+    assert rp.tolist() == [0, 3, 6, 9, 10]
+    assert col.tolist() == [0, 1, S, 0, 1, 2, 1, 2, S, 3]
