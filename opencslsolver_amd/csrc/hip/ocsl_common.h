@@ -357,15 +357,6 @@ __device__ __forceinline__ void store_rows(T* __restrict__ a, int row0, int n, c
 // l owning rows s*64R + l*R + j.  Pair t of row j of lane l is at element
 // base + 128 R t + 2 R l + 2 j, so a lane fetches 2R values (and R packed
 // column words) per access; R = 1 is the one-row-per-lane layout.  kDiff:
-// difference form, diag = row sum s_i: sum_off a_ij (x_j - x_i) + s_i x_i;
-// otherwise sum_off a_ij x_j + a_ii x_i.  Arithmetic in FP64 whatever the
-// storage; per row the terms are added in the same order for every R.
-// gather(c) returns the vector's element c as a double (a plain load, or a
-// value formed from several vectors).
-// R rows of A x over SELL-P with R rows per lane: slices of 64 R rows, lane
-// l owning rows s*64R + l*R + j.  Pair t of row j of lane l is at element
-// base + 128 R t + 2 R l + 2 j, so a lane fetches 2R values (and R packed
-// column words) per access; R = 1 is the one-row-per-lane layout.  kDiff:
 // difference form, diag = row sum s_i: s_i x_i + sum_off a_ij (x_j - x_i);
 // otherwise a_ii x_i + sum_off a_ij x_j.  Arithmetic in FP64 whatever the
 // storage; per row the terms are added in the same order for every R.
