@@ -166,6 +166,7 @@ def headline(args, out):
         cpu = cpu_baseline(a, b, r64["iterations"])
 
     kname, kname64 = dom_m["kernel"], dom_64["kernel"]
+    triad = stream_triad()
     out.update({
         "value": round(msm, 3),
         "ms_per_step": round(msm, 3),
@@ -191,7 +192,9 @@ def headline(args, out):
                      "fp64_kernel": {"kernel": kname64, "achieved": round(ach_64, 1),
                                      "avg_launch_us": round(dom_64["ms"] * 1e3, 2),
                                      "bytes_per_launch": dom_64["bytes"],
-                                     "traffic": measured_traffic(kname64)}},
+                                     "traffic": measured_traffic(kname64)},
+                     "stream_triad": triad,
+                     "frac_of_triad": round(ach_m / triad["hbm_GBps"], 4)},
         "cpu_baseline": cpu,
     })
 
@@ -397,6 +400,25 @@ def assembly(args, out):
                                    f"boundary faces={B}), viscous, blended, transient, permuted cells",
                        "kernel": "k_assemble", "gpu": res,
                        "gpu_total_us": round(sum(v["avg_us"] for v in res.values()), 2), "cpu_baseline": cpu}
+
+
+def stream_triad():
+    """Measured STREAM-triad bandwidth (a = b + 3 c, FP64, 16-byte accesses):
+    three 512 MiB arrays, far beyond the 256 MB last-level cache (the HBM
+    rate), and three 32 MiB ones, which the cache holds across launches."""
+    import ctypes
+
+    from opencslsolver_amd._native import hip_lib
+
+    lib = hip_lib()
+    res = {}
+    for key, n, reps in (("hbm_GBps", 64 << 20, 20), ("cache_resident_GBps", 4 << 20, 200)):
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        st = lib.ocsl_time_stream_triad(n, reps, ctypes.byref(ms), ctypes.byref(by))
+        if st != 0:
+            raise RuntimeError(f"ocsl_time_stream_triad failed ({st})")
+        res[key] = round(by.value / (ms.value * 1e-3) / 1e9, 1)
+    return res
 
 
 def structure(args, out):

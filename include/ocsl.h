@@ -292,6 +292,13 @@ ocsl_status ocsl_sampled_outer(ocsl_handle h, double alpha, const double* u, con
  * (values written, columns read, row offsets, u and v once). */
 ocsl_status ocsl_time_sampled_outer(ocsl_handle h, int reps, double* avg_ms, double* bytes_per_launch);
 
+/* Measurement hook (no counterpart in the reference): STREAM triad
+ * a = b + 3 c over three device arrays of n doubles (n even), 16-byte
+ * accesses; avg_ms over `reps` launches after a warm-up, bytes_per_launch =
+ * 24 n.  The bandwidth a plain streaming kernel reaches on the device, for
+ * comparison with the published HBM peak. */
+ocsl_status ocsl_time_stream_triad(int64_t n, int reps, double* avg_ms, double* bytes_per_launch);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ocsl_last_error(void);
 

@@ -130,6 +130,7 @@ SIGNATURES = {
                                             ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_int64, _i64p, ctypes.c_void_p,
                                             ctypes.c_void_p]),
+    "ocsl_time_stream_triad": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _f64p, _f64p]),
     "ocsl_last_error": (ctypes.c_char_p, []),
 }
 

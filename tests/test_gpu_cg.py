@@ -365,3 +365,17 @@ def test_cg_mixed_replacement_right_after_trigger(kernels, check_every):
             ref.append(s.solve("cg", "tolerance", "mixed", tol=1e-8, check_every=1, max_iter=3000,
                                cg_kernels=kernels)["iterations"])
     assert all(abs(u - v) <= 2 for u, v in zip(its, ref)), (its, ref)
+
+
+def test_stream_triad_timing_hook():
+    import ctypes
+
+    from opencslsolver_amd._native import hip_lib
+
+    lib = hip_lib()
+    ms, by = ctypes.c_double(), ctypes.c_double()
+    assert lib.ocsl_time_stream_triad(1 << 22, 10, ctypes.byref(ms), ctypes.byref(by)) == 0
+    assert by.value == 24.0 * (1 << 22) and ms.value > 0.0
+    gbps = by.value / (ms.value * 1e-3) / 1e9
+    assert 100.0 < gbps < 20000.0, gbps
+    assert lib.ocsl_time_stream_triad(3, 10, ctypes.byref(ms), ctypes.byref(by)) != 0  # odd length
