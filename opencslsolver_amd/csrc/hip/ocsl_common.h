@@ -361,8 +361,11 @@ __device__ __forceinline__ void store_rows(T* __restrict__ a, int row0, int n, c
 // otherwise a_ii x_i + sum_off a_ij x_j.  Arithmetic in FP64 whatever the
 // storage; per row the terms are added in the same order for every R.
 // gather(c) returns the vector's element c as a double (a plain load, or a
-// value formed from several vectors).
-template <class V, bool kDiff, int R, class G, int kPairs = (R == 1 ? 4 : 2)>
+// value formed from several vectors).  kPairs pairs are loaded per round: 4
+// at R = 1, 2 at R = 2, and 1 at R = 4, where 2 would take the MIXED SpMV
+// from 78 to 96 registers (6 to 5 waves per SIMD) and measured 21.3 against
+// 20.4 us at 128^3.
+template <class V, bool kDiff, int R, class G, int kPairs = (R == 1 ? 4 : (R == 4 ? 1 : 2))>
 __device__ __forceinline__ void sellp_rows_g(const SellPView<V>& A, int s, int lane, int row0, const G& gather,
                                              const double (&xi)[R], double (&acc)[R]) {
     const int base = A.slice_ptr[s];
