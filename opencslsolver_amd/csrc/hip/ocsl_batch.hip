@@ -26,8 +26,8 @@
 
 namespace ocsl {
 
-// Threads per workgroup of the update kernel: one 16-byte FP32 vector per
-// thread over a block's 16 slices of 64 rows.
+// Most threads per workgroup of the update kernel, which covers the rows of
+// one SpMV workgroup with one 16-byte vector per thread and round.
 constexpr int kBtUpdateBlock = 256;
 
 struct SysScalars {
