@@ -53,6 +53,7 @@ void ilu_decompose_device(ocsl_solver* h, int sweeps, bool want32) {
     double* prev = h->ilu_old.p;
     if (sweeps % 2 == 1) std::swap(cur, prev);  // the last sweep lands in ilu_m
     OCSL_HIP(hipMemcpyAsync(cur, h->csr_val.p, sizeof(double) * nnz, hipMemcpyDeviceToDevice, h->stream));
+    ++h->ilu_epoch;  // the level-ordered copies of the factor are stale
     const int g = grid_for(nnz);
     for (int s = 0; s < sweeps; ++s) {
         std::swap(cur, prev);
@@ -74,54 +75,70 @@ struct TriCSR {
     const int* diag;
 };
 
-// lower: out_i = in_i - sum_{col < i} M x_col (unit diagonal)
-// upper: out_i = (in_i - sum_{col > i} M x_col) / M_ii
-// The sum is formed first and subtracted once, as the reference does.
+// One row of a triangular solve, at position k of the direction's level
+// order: lower: out_i = in_i - sum_{col < i} M x_col (unit diagonal); upper:
+// out_i = (in_i - sum_{col > i} M x_col) / M_ii.  The sum is formed first and
+// subtracted once, as the reference does, over the row's entries in their
+// order.  The row index, its entry range and its diagonal are read by
+// position from level-ordered copies, so a row costs three rounds of
+// dependent loads (position data, entries, gathered values) rather than five
+// (row, row pointers and diagonal slot, entries, values, gathered values).
+struct LevelTri {
+    const int* rows;
+    const int* eptr;
+    const int* ecol;
+};
 template <class VM, class X, int NR>
-__device__ __forceinline__ void tri_row(int i, TriCSR t, const VM* __restrict__ m, const X* in, X* out,
-                                        bool lower) {
+__device__ __forceinline__ void tri_row_level(int k, LevelTri t, const VM* __restrict__ ev,
+                                              const VM* __restrict__ dv, const X* in, X* out, bool lower) {
+    const int i = t.rows[k];
+    const int p0 = t.eptr[k], p1 = t.eptr[k + 1];
+    const double dd = lower ? 1.0 : (double)dv[k];
     double sum[NR];
 #pragma unroll
     for (int c = 0; c < NR; ++c) sum[c] = 0.0;
-    const int d = t.diag[i];
-    const int p0 = lower ? t.rp[i] : d + 1;
-    const int p1 = lower ? d : t.rp[i + 1];
     for (int p = p0; p < p1; ++p) {
-        const double mv = (double)m[p];
-        const int j = t.col[p];
+        const double mv = (double)ev[p];
+        const int j = t.ecol[p];
 #pragma unroll
         for (int c = 0; c < NR; ++c) sum[c] += mv * (double)out[(size_t)j * NR + c];
     }
-    const double dv = lower ? 1.0 : (double)m[d];
 #pragma unroll
     for (int c = 0; c < NR; ++c) {
         const double v = (double)in[(size_t)i * NR + c] - sum[c];
-        out[(size_t)i * NR + c] = (X)(lower ? v : v / dv);
+        out[(size_t)i * NR + c] = (X)(lower ? v : v / dd);
     }
 }
 
 template <class VM, class X, int NR>
-__global__ void k_tri_level(const int* __restrict__ rows, int begin, int count, TriCSR t,
-                            const VM* __restrict__ m, const X* in, X* out, int lower,
-                            const Scalars* S) {
+__global__ void k_tri_level(LevelTri t, int begin, int count, const VM* __restrict__ ev, const VM* __restrict__ dv,
+                            const X* in, X* out, int lower, const Scalars* S) {
     if (S && S->halt) return;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x)
-        tri_row<VM, X, NR>(rows[begin + k], t, m, in, out, lower != 0);
+        tri_row_level<VM, X, NR>(begin + k, t, ev, dv, in, out, lower != 0);
 }
 
 // All levels in one workgroup; a barrier separates consecutive levels.
 constexpr int kTriBlock = 1024;
 template <class VM, class X, int NR>
-__global__ void __launch_bounds__(kTriBlock) k_tri_block(const int* __restrict__ rows,
-                                                         const int* __restrict__ lptr, int n_levels,
-                                                         TriCSR t, const VM* __restrict__ m, const X* in,
-                                                         X* out, int lower, const Scalars* S) {
+__global__ void __launch_bounds__(kTriBlock) k_tri_block(LevelTri t, const int* __restrict__ lptr, int n_levels,
+                                                         const VM* __restrict__ ev, const VM* __restrict__ dv,
+                                                         const X* in, X* out, int lower, const Scalars* S) {
     if (S && S->halt) return;
     for (int l = 0; l < n_levels; ++l) {
         const int b = lptr[l], e = lptr[l + 1];
-        for (int k = b + threadIdx.x; k < e; k += kTriBlock) tri_row<VM, X, NR>(rows[k], t, m, in, out, lower != 0);
+        for (int k = b + threadIdx.x; k < e; k += kTriBlock) tri_row_level<VM, X, NR>(k, t, ev, dv, in, out, lower != 0);
         __syncthreads();
     }
+}
+
+// Factor values into level order: ev[e] = m[esrc[e]], dv[k] = m[dslot[k]].
+template <class VM>
+__global__ void k_level_gather(int ne, const int* __restrict__ esrc, int n, const int* __restrict__ dslot,
+                               const VM* __restrict__ m, VM* __restrict__ ev, VM* __restrict__ dv) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += stride) ev[e] = m[esrc[e]];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) dv[k] = m[dslot[k]];
 }
 
 // One damped Jacobi sweep: out = w * (in - off(M) old) [/ M_ii] + (1 - w) old
@@ -169,19 +186,54 @@ bool use_block_tri(const Levels& L, int n) {
     return n <= 65536 || (int64_t)L.n_levels * 256 > n;
 }
 
+// The factor m's values in L's level order, gathered once per factor
+// (address) and decomposition.
+template <class VM>
+void level_values(ocsl_solver* h, Levels& L, const VM* m, const VM** ev, const VM** dv) {
+    DBuf<VM>* e;
+    DBuf<VM>* d;
+    const void** from;
+    int64_t* epoch;
+    if constexpr (sizeof(VM) == 8) {
+        e = &L.eval64;
+        d = &L.dval64;
+        from = &L.from64;
+        epoch = &L.epoch64;
+    } else {
+        e = &L.eval32;
+        d = &L.dval32;
+        from = &L.from32;
+        epoch = &L.epoch32;
+    }
+    const int n = h->host.n;
+    const int ne = (int)L.esrc.n;
+    if (*from != (const void*)m || *epoch != h->ilu_epoch || !e->p) {
+        e->alloc(std::max(ne, 1));
+        d->alloc(n);
+        k_level_gather<VM><<<grid_for(std::max(ne, n)), kBlock, 0, h->stream>>>(ne, L.esrc.p, n, L.dslot.p, m, e->p,
+                                                                                 d->p);
+        OCSL_HIP(hipGetLastError());
+        *from = (const void*)m;
+        *epoch = h->ilu_epoch;
+    }
+    *ev = e->p;
+    *dv = d->p;
+}
+
 // Exact solve (lower or upper) of NR interleaved vectors.
 template <class VM, class X, int NR>
 void tri_exact(ocsl_solver* h, const VM* m, const X* in, X* out, bool lower, const Scalars* S) {
-    const Levels& L = lower ? h->lev_fwd : h->lev_bwd;
+    Levels& L = lower ? h->lev_fwd : h->lev_bwd;
     const int n = h->host.n;
+    const VM *ev, *dv;
+    level_values<VM>(h, L, m, &ev, &dv);
+    const LevelTri t{L.rows.p, L.eptr.p, L.ecol.p};
     if (use_block_tri(L, n)) {
-        k_tri_block<VM, X, NR><<<1, kTriBlock, 0, h->stream>>>(L.rows.p, L.ptr.p, L.n_levels, tri_csr(h), m, in,
-                                                               out, lower, S);
+        k_tri_block<VM, X, NR><<<1, kTriBlock, 0, h->stream>>>(t, L.ptr.p, L.n_levels, ev, dv, in, out, lower, S);
     } else {
         for (int l = 0; l < L.n_levels; ++l) {
             const int b = L.ptr_host[l], cnt = L.ptr_host[l + 1] - b;
-            k_tri_level<VM, X, NR><<<grid_for(cnt), kBlock, 0, h->stream>>>(L.rows.p, b, cnt, tri_csr(h), m,
-                                                                           in, out, lower, S);
+            k_tri_level<VM, X, NR><<<grid_for(cnt), kBlock, 0, h->stream>>>(t, b, cnt, ev, dv, in, out, lower, S);
         }
     }
 }

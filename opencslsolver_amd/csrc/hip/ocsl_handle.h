@@ -75,6 +75,18 @@ struct Levels {
     int max_width = 0;
     std::vector<int> ptr_host;  // n_levels + 1
     DBuf<int> rows, ptr;
+    // The direction's entries in level order: position k (row rows[k]) owns
+    // entries eptr[k] .. eptr[k+1] (the row's strictly lower or strictly
+    // upper part, in the row's own order), with their columns ecol and their
+    // slots esrc in the factor; dslot[k] is the slot of the row's diagonal.
+    DBuf<int> eptr, ecol, esrc, dslot;
+    // Factor values gathered into that order, and the factor (address and
+    // decomposition epoch) they were gathered from.
+    DBuf<double> eval64, dval64;
+    DBuf<float> eval32, dval32;
+    const void* from64 = nullptr;
+    const void* from32 = nullptr;
+    int64_t epoch64 = -1, epoch32 = -1;
 };
 
 // Fine-grained ILU(0) sweep tables (cmatrix_solver.cpp:519-551): for each
@@ -127,6 +139,7 @@ struct ocsl_solver {
     ocsl::IluTables ilu;
     bool ilu_built = false;
     ocsl::DBuf<double> ilu_m, ilu_old;
+    int64_t ilu_epoch = 0;  // bumped by every decomposition (ilu_m, ilu_m32 rewritten)
     ocsl::DBuf<float> ilu_m32;
 
     ocsl::Work work;

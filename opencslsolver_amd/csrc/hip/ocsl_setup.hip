@@ -321,6 +321,27 @@ void build_levels(const HostCSR& a, bool lower, Levels& L, hipStream_t st) {
     L.ptr_host = ptr;
     L.rows.upload(rows.data(), rows.size(), st);
     L.ptr.upload(ptr.data(), ptr.size(), st);
+    // the entries in level order (one position per row)
+    std::vector<int> eptr(n + 1, 0), ecol, esrc, dslot(n);
+    ecol.reserve(a.col.size() / 2 + 1);
+    esrc.reserve(a.col.size() / 2 + 1);
+    for (int k = 0; k < n; ++k) {
+        const int i = rows[k];
+        const int p0 = lower ? a.rp[i] : a.diag[i] + 1;
+        const int p1 = lower ? a.diag[i] : a.rp[i + 1];
+        for (int q = p0; q < p1; ++q) {
+            ecol.push_back(a.col[q]);
+            esrc.push_back(q);
+        }
+        eptr[k + 1] = (int)ecol.size();
+        dslot[k] = a.diag[i];
+    }
+    L.eptr.upload(eptr.data(), eptr.size(), st);
+    L.ecol.upload(ecol.data(), ecol.size(), st);
+    L.esrc.upload(esrc.data(), esrc.size(), st);
+    L.dslot.upload(dslot.data(), dslot.size(), st);
+    L.from64 = L.from32 = nullptr;
+    L.epoch64 = L.epoch32 = -1;
     OCSL_HIP(hipStreamSynchronize(st));
 }
 
