@@ -464,7 +464,7 @@ ocsl_status ocsl_gemv(ocsl_handle h, double a, const double* x, double b, const 
         Sell& s = sell_for(h, false);
         ensure_sell_values(h, s, true, false);
         k_sell_axpby<double><<<grid_for((int64_t)s.n_slices * kWave), kBlock, 0, h->stream>>>(
-            s.view<double>(), a, dx, b, dz, dout);
+            s.view<double>(), a, nullptr, dx, b, dz, dout);
         OCSL_HIP(hipGetLastError());
         if (where == OCSL_HOST)
             OCSL_HIP(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));

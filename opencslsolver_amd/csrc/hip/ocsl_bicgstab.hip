@@ -14,6 +14,7 @@
 //                          storage, FP64 arithmetic, FP64 solution and
 //                          residual replacement).
 #include <cfloat>
+#include <cstddef>
 #include <cmath>
 
 #include "ocsl_handle.h"
@@ -270,6 +271,27 @@ static void apply_precond(ocsl_solver* h, const VM* m, const X* in, X* tmp, X* t
 }
 
 // ------------------------------------------------------------------ reference mode
+// Device-resident scalars of the REFERENCE BiCGSTAB: the dot products land
+// in S->aux (0: r0.r, 1: r0.v, 2: r.r, 3: b.b, 4: dum.dum2, 5: dum.dum), a
+// one-thread kernel forms alpha, the residual and omega with the reference's
+// formulas, and the vector updates read their coefficients from S.  The
+// reference's early exit (residual below 1e-20, before x = h) sets S->halt,
+// after which the updates and the triangular solves do nothing.
+__global__ void k_bref_scalars(Scalars* S, double* hist, int step, int it) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || S->halt) return;
+    if (step == 0) {
+        S->alpha = S->aux[0] / (S->aux[1] + DBL_MIN);
+    } else if (step == 1) {
+        const double res = S->aux[2] / S->aux[3] + DBL_MIN;
+        S->rr = res;
+        hist[it] = res;
+        S->iter = it + 1;
+        if (res < 1.0e-20) S->halt = 1;
+    } else {
+        S->omega = S->aux[4] / (S->aux[5] + 1.0e-40);
+    }
+}
+
 void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
     const int n = h->host.n;
     ensure_csr_values(h, false);
@@ -282,23 +304,36 @@ void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
     double *hh = w.d[6].p, *t = w.d[7].p, *y = w.d[8].p, *z = w.d[9].p, *dum2 = w.d[10].p;
     double* x = h->x_own.p;
     const double* b = h->b_dev;
-    ocsl_options o;
-    ocsl_default_options(&o);
     const int gv = grid_for(n);
     auto spmv = [&](double a, const double* xin, double bz, const double* zin, double* out) {
         k_sell_axpby<double><<<grid_for((int64_t)s.n_slices * kWave), kBlock, 0, h->stream>>>(
-            s.view<double>(), a, xin, bz, zin, out);
+            s.view<double>(), a, nullptr, xin, bz, zin, out);
     };
-    auto axpby = [&](double a, const double* xa, double bb, const double* ya, double* out) {
-        k_axpby<<<gv, kBlock, 0, h->stream>>>(n, a, xa, bb, ya, out);
+    Scalars* S = h->S.p;
+    // device addresses of fields of S (S is never dereferenced on the host)
+    auto field = [S](size_t off) { return reinterpret_cast<double*>(reinterpret_cast<char*>(S) + off); };
+    double* aux = field(offsetof(Scalars, aux));
+    double* hist_d = work_buf(h, h->hist, 4);
+    const int gd = red_blocks(kBlock * 4, n);
+    // dot products into S->aux[slot], in the reduction device_dots uses
+    auto dot1 = [&](const double* a1, const double* b1, int slot) {
+        DotArgs<1> d{{a1}, {b1}, {0}};
+        k_dots<1><<<gd, kBlock, 0, h->stream>>>(n, d, h->red());
+        k_finalize<1, FinStore<1>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinStore<1>{aux + slot});
     };
-    auto dot2 = [&](const double* a1, const double* b1, const double* a2, const double* b2, double* out) {
-        const double* aa[2] = {a1, a2};
-        const double* bb[2] = {b1, b2};
-        const int kind[2] = {0, 0};
-        device_dots(h, n, 2, aa, bb, kind, out);
+    auto dot2 = [&](const double* a1, const double* b1, const double* a2, const double* b2, int slot) {
+        DotArgs<2> d{{a1, a2}, {b1, b2}, {0, 0}};
+        k_dots<2><<<gd, kBlock, 0, h->stream>>>(n, d, h->red());
+        k_finalize<2, FinStore<2>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinStore<2>{aux + slot});
     };
-    std::vector<double> hist;
+    auto upd = [&](double ca, const double* pa, const double* xa, double cb, const double* pb, const double* ya,
+                   double* out) {
+        k_axpby_s<<<gv, kBlock, 0, h->stream>>>(n, ca, pa, xa, cb, pb, ya, out, S);
+    };
+    Scalars init{};
+    init.alpha = 1.0;
+    init.omega = 1.0;
+    OCSL_HIP(hipMemcpyAsync(S, &init, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     OCSL_HIP(hipEventRecord(h->ev0, h->stream));
     ilu_decompose_device(h, 3, false);  // M.copy_same_structure + ILU_decompose
     const double* m = h->ilu_m.p;       // allocated by the call above on first use
@@ -306,37 +341,43 @@ void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
     OCSL_HIP(hipMemsetAsync(v, 0, sizeof(double) * n, h->stream));
     spmv(-1.0, x, 1.0, b, r);
     OCSL_HIP(hipMemcpyAsync(r0, r, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
-    double alpha = 1.0, omega = 1.0, res = 0.0;
-    int it = 0;
-    for (; it < 3; ++it) {
-        const double rhon = device_dot(h, r0, r, n);
-        axpby(1.0, p, -1.0 * omega, v, dum);
-        axpby(1.0, r, omega, dum, p);
-        tri_exact<double, double, 1>(h, m, p, dum, true, nullptr);
-        tri_exact<double, double, 1>(h, m, dum, y, false, nullptr);
+    const double* alpha = field(offsetof(Scalars, alpha));
+    const double* omega = field(offsetof(Scalars, omega));
+    for (int it = 0; it < 3; ++it) {
+        dot1(r0, r, 0);                                                // rhon
+        upd(1.0, nullptr, p, -1.0, omega, v, dum);                     // p - omega v
+        upd(1.0, nullptr, r, 1.0, omega, dum, p);                      // p = r + omega (p - omega v)
+        tri_exact<double, double, 1>(h, m, p, dum, true, S);
+        tri_exact<double, double, 1>(h, m, dum, y, false, S);
         spmv(1.0, y, 0.0, nullptr, v);
-        const double dum1 = device_dot(h, r0, v, n);
-        alpha = rhon / (dum1 + DBL_MIN);
-        axpby(1.0, x, alpha, y, hh);
-        double d[2];
-        dot2(r, r, b, b, d);
-        res = d[0] / d[1] + DBL_MIN;
-        hist.push_back(res);
-        if (res < 1.0e-20) { ++it; break; }
-        axpby(-1.0 * alpha, v, 1.0, r, sv);
-        tri_exact<double, double, 1>(h, m, sv, dum, true, nullptr);
-        tri_exact<double, double, 1>(h, m, dum, z, false, nullptr);
+        dot1(r0, v, 1);
+        k_bref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 0, it);    // alpha
+        upd(1.0, nullptr, x, 1.0, alpha, y, hh);                       // h = x + alpha y
+        dot2(r, r, b, b, 2);
+        k_bref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 1, it);    // residual, early exit
+        upd(-1.0, alpha, v, 1.0, nullptr, r, sv);                      // s = r - alpha v
+        tri_exact<double, double, 1>(h, m, sv, dum, true, S);
+        tri_exact<double, double, 1>(h, m, dum, z, false, S);
         spmv(1.0, z, 0.0, nullptr, t);
-        tri_exact<double, double, 1>(h, m, t, dum, true, nullptr);
-        tri_exact<double, double, 1>(h, m, sv, dum2, true, nullptr);
-        dot2(dum, dum2, dum, dum, d);
-        omega = d[0] / (d[1] + 1.0e-40);
-        axpby(1.0, hh, omega, z, x);
-        axpby(-1.0 * omega, t, 1.0, sv, r);
+        tri_exact<double, double, 1>(h, m, t, dum, true, S);
+        tri_exact<double, double, 1>(h, m, sv, dum2, true, S);
+        dot2(dum, dum2, dum, dum, 4);
+        k_bref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 2, it);    // omega
+        upd(1.0, nullptr, hh, 1.0, omega, z, x);                       // x = h + omega z
+        upd(-1.0, omega, t, 1.0, nullptr, sv, r);                      // r = s - omega t
     }
     OCSL_HIP(hipGetLastError());
     OCSL_HIP(hipEventRecord(h->ev1, h->stream));
+    // the one read-back of the solve: iterations, residual, history
+    OCSL_HIP(hipMemcpyAsync(h->S_host, S, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
     sync(h);
+    const int it = h->S_host->iter;
+    const double res = h->S_host->rr;
+    std::vector<double> hist(it);
+    if (it) {
+        OCSL_HIP(hipMemcpyAsync(hist.data(), hist_d, sizeof(double) * it, hipMemcpyDeviceToHost, h->stream));
+        sync(h);
+    }
     float ms = 0.f;
     OCSL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->hist_host = hist;

@@ -16,6 +16,7 @@
 //                  iteration whatever the chunk length.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <type_traits>
 #include <cstdio>
 
@@ -24,16 +25,10 @@
 
 namespace ocsl {
 
-static void launch_axpby(ocsl_solver* h, double a, const double* x, double b, const double* y,
-                         double* out, int n) {
-    k_axpby<<<grid_for(n), kBlock, 0, h->stream>>>(n, a, x, b, y, out);
-    OCSL_HIP(hipGetLastError());
-}
-
 static void launch_spmv(ocsl_solver* h, const Sell& s, double a, const double* x, double b,
-                        const double* z, double* out) {
+                        const double* z, double* out, const double* a_dev = nullptr) {
     k_sell_axpby<double><<<grid_for((int64_t)s.n_slices * kWave), kBlock, 0, h->stream>>>(
-        s.view<double>(), a, x, b, z, out);
+        s.view<double>(), a, a_dev, x, b, z, out);
     OCSL_HIP(hipGetLastError());
 }
 
@@ -66,6 +61,24 @@ static double true_relres(ocsl_solver* h, const Sell& s, const double* b, const 
     return bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
 }
 
+// Device-resident scalars of the REFERENCE CG: r.r and p.q land in S->aux[0],
+// S->aux[1], r'.r' in S->aux[2]; a one-thread kernel forms a, the residual
+// and beta with the reference's formulas; the reference's early exit (after
+// x += a p, on r'.r' < 1e-6) sets S->halt, after which the updates do nothing.
+__global__ void k_cgref_scalars(Scalars* S, double* hist, int step, int it) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || S->halt) return;
+    if (step == 0) {
+        S->alpha = S->aux[0] / (S->aux[1] + 1.0e-40);
+    } else {
+        const double res = S->aux[2];
+        S->rr = res;
+        hist[it] = res;
+        S->iter = it + 1;
+        if (res < 1.0e-6) S->halt = 1;
+        else S->beta = res / (S->aux[0] + 1.0e-40);
+    }
+}
+
 void cg_reference(ocsl_solver* h, ocsl_report* rep) {
     const int n = h->host.n;
     Sell& s = sell_for(h, false);
@@ -75,33 +88,49 @@ void cg_reference(ocsl_solver* h, ocsl_report* rep) {
     double *r = w.d[0].p, *rn = w.d[1].p, *p = w.d[2].p, *pn = w.d[3].p, *q = w.d[4].p;
     double* x = h->x_own.p;
     const double* b = h->b_dev;
-    std::vector<double> hist;
+    Scalars* S = h->S.p;
+    double* hist_d = work_buf(h, h->hist, 4);
+    // device addresses of fields of S (S is never dereferenced on the host)
+    auto field = [S](size_t off) { return reinterpret_cast<double*>(reinterpret_cast<char*>(S) + off); };
+    double* aux = field(offsetof(Scalars, aux));
+    const double* alpha = field(offsetof(Scalars, alpha));
+    const double* beta = field(offsetof(Scalars, beta));
+    const int gd = red_blocks(kBlock * 4, n);
+    const int gv = grid_for(n);
+    Scalars init{};
+    OCSL_HIP(hipMemcpyAsync(S, &init, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     OCSL_HIP(hipEventRecord(h->ev0, h->stream));
     // r = b - A x ; p = r
     launch_spmv(h, s, -1.0, x, 1.0, b, r);
     OCSL_HIP(hipMemcpyAsync(p, r, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
-    double res = 0.0;
-    int it = 0;
-    for (; it < 4; ++it) {
+    for (int it = 0; it < 4; ++it) {
         launch_spmv(h, s, 1.0, p, 0.0, nullptr, q);
-        const double* da[2] = {r, p};
-        const double* db[2] = {r, q};
-        const int kind[2] = {0, 0};
-        double d[2];
-        device_dots(h, n, 2, da, db, kind, d);
-        const double a = d[0] / (d[1] + 1.0e-40);
-        launch_axpby(h, 1.0, x, a, p, x, n);
-        launch_spmv(h, s, -1.0 * a, p, 1.0, r, rn);
-        res = device_dot(h, rn, rn, n);
-        hist.push_back(res);
-        if (res < 1.0e-6) { ++it; break; }
-        const double beta = res / (d[0] + 1.0e-40);
-        launch_axpby(h, 1.0, rn, beta, p, pn, n);
+        DotArgs<2> d2{{r, p}, {r, q}, {0, 0}};
+        k_dots<2><<<gd, kBlock, 0, h->stream>>>(n, d2, h->red());
+        k_finalize<2, FinStore<2>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinStore<2>{aux});
+        k_cgref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 0, it);            // a
+        k_axpby_s<<<gv, kBlock, 0, h->stream>>>(n, 1.0, nullptr, x, 1.0, alpha, p, x, S);  // x += a p
+        launch_spmv(h, s, -1.0, p, 1.0, r, rn, alpha);                          // r' = -a (A p) + r
+        DotArgs<1> d1{{rn}, {rn}, {0}};
+        k_dots<1><<<gd, kBlock, 0, h->stream>>>(n, d1, h->red());
+        k_finalize<1, FinStore<1>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinStore<1>{aux + 2});
+        k_cgref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 1, it);            // residual, early exit, beta
+        k_axpby_s<<<gv, kBlock, 0, h->stream>>>(n, 1.0, nullptr, rn, 1.0, beta, p, pn, S);  // p' = r' + beta p
         std::swap(r, rn);
         std::swap(p, pn);
     }
+    OCSL_HIP(hipGetLastError());
     OCSL_HIP(hipEventRecord(h->ev1, h->stream));
+    // the one read-back of the solve: iterations, residual, history
+    OCSL_HIP(hipMemcpyAsync(h->S_host, S, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
     sync(h);
+    const int it = h->S_host->iter;
+    const double res = h->S_host->rr;
+    std::vector<double> hist(it);
+    if (it) {
+        OCSL_HIP(hipMemcpyAsync(hist.data(), hist_d, sizeof(double) * it, hipMemcpyDeviceToHost, h->stream));
+        sync(h);
+    }
     float ms = 0.f;
     OCSL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->hist_host = hist;

@@ -58,6 +58,18 @@ __global__ void k_fill(int n, T v, T* __restrict__ out) {
         out[i] = a * x[i] + b * y[i];
 }
 
+// out = (ca * a) x + (cb * b) y with a = *pa and b = *pb where given (device
+// scalars; else 1); nothing once S->halt is set.
+[[maybe_unused]] static __global__ void k_axpby_s(int n, double ca, const double* pa, const double* __restrict__ x,
+                                                  double cb, const double* pb, const double* __restrict__ y,
+                                                  double* out, const Scalars* S) {
+    if (S->halt) return;
+    const double a = pa ? ca * *pa : ca;
+    const double b = pb ? cb * *pb : cb;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = a * x[i] + b * y[i];
+}
+
 // Up to 3 dot products / sums in one pass; results land in out[0..NR-1].
 // kind: 0 = a.b, 1 = sum(a), 2 = sum|a|.
 template <int NR>
@@ -96,10 +108,12 @@ struct FinStore {
 // out = a*(A x) + b*z over a SELL matrix whose row order is the vector order
 // (cmatrix_solver.cpp:338-453).  z may be null.
 template <class V>
-__global__ void __launch_bounds__(kBlock) k_sell_axpby(SellView<V> A, double a,
+__global__ void __launch_bounds__(kBlock) k_sell_axpby(SellView<V> A, double a, const double* a_dev,
                                                        const double* __restrict__ x, double b,
                                                        const double* __restrict__ z,
                                                        double* __restrict__ out) {
+    // a_dev (optional): a device scalar the coefficient a multiplies
+    if (a_dev) a *= *a_dev;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
     const int nwaves = gridDim.x * kWavesPerBlock;
