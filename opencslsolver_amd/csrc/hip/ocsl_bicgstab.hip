@@ -119,6 +119,19 @@ __global__ void k_tri_level(LevelTri t, int begin, int count, const VM* __restri
         tri_row_level<VM, X, NR>(begin + k, t, ev, dv, in, out, lower != 0);
 }
 
+// Two independent solves with the same factor and direction in one pass
+// over the levels (each vector's rows computed exactly as alone).
+template <class VM>
+__global__ void k_tri_level_pair(LevelTri t, int begin, int count, const VM* __restrict__ ev,
+                                 const VM* __restrict__ dv, const double* in0, double* out0, const double* in1,
+                                 double* out1, int lower, const Scalars* S) {
+    if (S && S->halt) return;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+        tri_row_level<VM, double, 1>(begin + k, t, ev, dv, in0, out0, lower != 0);
+        tri_row_level<VM, double, 1>(begin + k, t, ev, dv, in1, out1, lower != 0);
+    }
+}
+
 // All levels in one workgroup; a barrier separates consecutive levels.
 constexpr int kTriBlock = 1024;
 template <class VM, class X, int NR>
@@ -236,6 +249,28 @@ void tri_exact(ocsl_solver* h, const VM* m, const X* in, X* out, bool lower, con
             const int b = L.ptr_host[l], cnt = L.ptr_host[l + 1] - b;
             k_tri_level<VM, X, NR><<<grid_for(cnt), kBlock, 0, h->stream>>>(t, b, cnt, ev, dv, in, out, lower, S);
         }
+    }
+}
+
+// Two exact solves (same factor, same direction) of independent vectors:
+// out0 = T^-1 in0 and out1 = T^-1 in1, one launch per level for both.
+template <class VM>
+void tri_exact_pair(ocsl_solver* h, const VM* m, const double* in0, double* out0, const double* in1, double* out1,
+                    bool lower, const Scalars* S) {
+    Levels& L = lower ? h->lev_fwd : h->lev_bwd;
+    const int n = h->host.n;
+    if (use_block_tri(L, n)) {
+        tri_exact<VM, double, 1>(h, m, in0, out0, lower, S);
+        tri_exact<VM, double, 1>(h, m, in1, out1, lower, S);
+        return;
+    }
+    const VM *ev, *dv;
+    level_values<VM>(h, L, m, &ev, &dv);
+    const LevelTri t{L.rows.p, L.eptr.p, L.ecol.p};
+    for (int l = 0; l < L.n_levels; ++l) {
+        const int b = L.ptr_host[l], cnt = L.ptr_host[l + 1] - b;
+        k_tri_level_pair<VM><<<grid_for(cnt), kBlock, 0, h->stream>>>(t, b, cnt, ev, dv, in0, out0, in1, out1, lower,
+                                                                      S);
     }
 }
 
@@ -359,8 +394,7 @@ void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
         tri_exact<double, double, 1>(h, m, sv, dum, true, S);
         tri_exact<double, double, 1>(h, m, dum, z, false, S);
         spmv(1.0, z, 0.0, nullptr, t);
-        tri_exact<double, double, 1>(h, m, t, dum, true, S);
-        tri_exact<double, double, 1>(h, m, sv, dum2, true, S);
+        tri_exact_pair<double>(h, m, t, dum, sv, dum2, true, S);       // L^-1 t, L^-1 s
         dot2(dum, dum2, dum, dum, 4);
         k_bref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 2, it);    // omega
         upd(1.0, nullptr, hh, 1.0, omega, z, x);                       // x = h + omega z
