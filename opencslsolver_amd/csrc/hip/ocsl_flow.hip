@@ -93,6 +93,10 @@ struct MeshDev {
     const int *cf_ptr, *cf_code, *cf_pos;  // per cell: faces (f << 1 | side), neighbour slot position
     const int *cb_ptr, *cb_face;           // per cell: boundary faces
     const int *row_ptr, *dpos;             // caller layout (int32), diagonal position in the row
+    // per cell-face entry, in the order of cf_code: the cell across the face and
+    // the face's static weights, so a cell's faces read contiguous memory
+    const int* cf_nb;
+    const double *cf_geo, *cf_ksi;
 };
 
 struct StateDev {
@@ -165,37 +169,48 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshDev m, StateDev s, Flow
     for (int k = 0; k < kMaxRow; ++k) a[k] = 0.0;
     double rhs = 0.0;
     const int dp = m.dpos[i];
-    // interior faces (cflow_solver_cell_face_loops.cpp:38-58, 186-218, 334-346)
+    // this cell's own values, the same for each of its faces
+    double own_q = 0.0, own_T = 0.0;
+    if constexpr (kEq == kMomentum) own_q = s.u[3 * i + d];
+    else if constexpr (kEq == kPressure) own_q = s.pc[i];
+    else own_q = s.h[i];
+    if constexpr (kEq != kMomentum || kVisc) own_T = s.T[i];
+    // interior faces (cflow_solver_cell_face_loops.cpp:38-58, 186-218, 334-346);
+    // (l, r) are the face's left and right cell values whichever side this is
     for (int e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
         const int code = m.cf_code[e];
         const int f = code >> 1, side = code & 1, op = m.cf_pos[e];
-        const int L = m.fL[f], R = m.fR[f];
-        const double geo = m.fgeo[f];
+        const int j = m.cf_nb[e];
+        const double geo = m.cf_geo[e];
         const double mdot = s.f_mdot[f];
         if constexpr (kEq == kMomentum) {
-            conv_row<kBlend>(a, rhs, side, dp, op, mdot, s.u[3 * L + d], s.u[3 * R + d], geo);
+            const double qj = s.u[3 * j + d];
+            conv_row<kBlend>(a, rhs, side, dp, op, mdot, side ? qj : own_q, side ? own_q : qj, geo);
             if constexpr (kVisc) {
-                const double Tf = face_avg(geo, s.T[L], s.T[R]);
+                const double Tj = s.T[j];
+                const double Tf = face_avg(geo, side ? Tj : own_T, side ? own_T : Tj);
                 double D = gas.viscosity(Tf);
-                D = D * m.fksi[f];
+                D = D * m.cf_ksi[e];
                 diff_row(a, dp, op, D);
                 const double st = s.f_stress[3 * f + d];
                 rhs += side == 0 ? st : -st;
             }
         } else if constexpr (kEq == kPressure) {
             rhs += side == 0 ? -mdot : mdot;
-            const double Tf = conv_interp<kBlend>(geo, mdot, s.T[L], s.T[R]);
+            const double Tj = s.T[j], qj = s.pc[j];
+            const double Tf = conv_interp<kBlend>(geo, mdot, side ? Tj : own_T, side ? own_T : Tj);
             double dum = gas.dp_drho(Tf);
             dum = dum / s.f_rho[f];
             dum *= mdot;
-            conv_row<kBlend>(a, rhs, side, dp, op, dum, s.pc[L], s.pc[R], geo);
+            conv_row<kBlend>(a, rhs, side, dp, op, dum, side ? qj : own_q, side ? own_q : qj, geo);
             const double avgD = s.f_qy[f] * s.f_rho[f];
             diff_row(a, dp, op, avgD);
         } else {
-            conv_row<kBlend>(a, rhs, side, dp, op, mdot, s.h[L], s.h[R], geo);
-            const double Tf = face_avg(geo, s.T[L], s.T[R]);
+            const double Tj = s.T[j], qj = s.h[j];
+            conv_row<kBlend>(a, rhs, side, dp, op, mdot, side ? qj : own_q, side ? own_q : qj, geo);
+            const double Tf = face_avg(geo, side ? Tj : own_T, side ? own_T : Tj);
             double avgD = gas.viscosity(Tf);
-            avgD = avgD / gas.pr * m.fksi[f];
+            avgD = avgD / gas.pr * m.cf_ksi[e];
             diff_row(a, dp, op, avgD);
         }
     }
@@ -395,11 +410,14 @@ __global__ void __launch_bounds__(kBlock) k_press_grad(MeshDev m, StateDev s, Fl
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.n) return;
     double gv[3] = {0.0, 0.0, 0.0};
+    const double ap_i = s.ap[i], p_i = s.p[i];
     for (int e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
-        const int code = m.cf_code[e], f = code >> 1;
-        const int L = m.fL[f], R = m.fR[f];
-        double fp = (s.ap[L] + s.ap[R]);
-        fp = (s.ap[L] * s.p[L] + s.ap[R] * s.p[R]) / fp;
+        const int code = m.cf_code[e], f = code >> 1, j = m.cf_nb[e];
+        const double ap_j = s.ap[j], p_j = s.p[j];
+        const double apL = code & 1 ? ap_j : ap_i, apR = code & 1 ? ap_i : ap_j;
+        const double pL = code & 1 ? p_j : p_i, pR = code & 1 ? p_i : p_j;
+        double fp = (apL + apR);
+        fp = (apL * pL + apR * pR) / fp;
         for (int k = 0; k < 3; ++k) {
             if (code & 1) gv[k] = gv[k] - fp * m.fS[3 * f + k];
             else gv[k] = gv[k] + fp * m.fS[3 * f + k];
@@ -424,11 +442,12 @@ __global__ void __launch_bounds__(kBlock) k_press_corr_grad(MeshDev m, StateDev 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.n) return;
     double gv[3] = {0.0, 0.0, 0.0};
+    const double pc_i = s.pc[i];
     for (int e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
         const int code = m.cf_code[e], f = code >> 1;
-        const int L = m.fL[f], R = m.fR[f];
-        double fp = m.fgeo[f] * s.pc[L];
-        fp += (1.0 - m.fgeo[f]) * s.pc[R];
+        const double geo = m.cf_geo[e], pc_j = s.pc[m.cf_nb[e]];
+        double fp = geo * (code & 1 ? pc_j : pc_i);
+        fp += (1.0 - geo) * (code & 1 ? pc_i : pc_j);
         for (int k = 0; k < 3; ++k) {
             if (code & 1) gv[k] = gv[k] - fp * m.fS[3 * f + k];
             else gv[k] = gv[k] + fp * m.fS[3 * f + k];
@@ -604,6 +623,8 @@ struct ocsl_flow_s {
     DBuf<double> d_vol, d_fS, d_fe, d_fdist, d_fksi, d_fgeo, d_falpha, d_bS, d_barea, d_bksi, d_bperp, d_be, d_balpha,
         d_bdist;
     DBuf<int> d_fL, d_fR, d_bL, d_bmark, d_cf_ptr, d_cf_code, d_cf_pos, d_cb_ptr, d_cb_face, d_row_ptr, d_dpos;
+    DBuf<int> d_cf_nb;
+    DBuf<double> d_cf_geo, d_cf_ksi;
     // device state
     DBuf<double> d_u, d_u_old, d_rho, d_rho_old, d_p, d_p_old, d_pc, d_gp, d_T, d_h, d_h_old, d_ap, d_ap_vec, d_dt;
     DBuf<double> d_f_mdot, d_f_rho, d_f_qy, d_f_stress;
@@ -620,7 +641,8 @@ struct ocsl_flow_s {
         return {mesh.n,     mesh.F,       mesh.B,      d_vol.p,    d_fL.p,      d_fR.p,       d_fS.p,
                 d_fe.p,     d_fdist.p,    d_fksi.p,    d_fgeo.p,   d_falpha.p,  d_bL.p,       d_bmark.p,
                 d_bS.p,     d_barea.p,    d_bksi.p,    d_bperp.p,  d_be.p,      d_balpha.p,   d_bdist.p,
-                d_cf_ptr.p, d_cf_code.p,  d_cf_pos.p,  d_cb_ptr.p, d_cb_face.p, d_row_ptr.p,  d_dpos.p};
+                d_cf_ptr.p, d_cf_code.p,  d_cf_pos.p,  d_cb_ptr.p, d_cb_face.p, d_row_ptr.p,  d_dpos.p,
+                d_cf_nb.p,  d_cf_geo.p,   d_cf_ksi.p};
     }
     StateDev state_view() const {
         return {d_u.p, d_u_old.p, d_rho.p, d_rho_old.p, d_p.p, d_p_old.p, d_pc.p, d_gp.p, d_T.p, d_h.p,
@@ -1375,7 +1397,8 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
             cf[m.fR[f]].push_back((int)((f << 1) | 1));
         }
         for (size_t b = 0; b < B; ++b) cb[m.bL[b]].push_back((int)b);
-        std::vector<int> cf_ptr(n + 1, 0), cf_code, cf_pos, cb_ptr(n + 1, 0), cb_face;
+        std::vector<int> cf_ptr(n + 1, 0), cf_code, cf_pos, cf_nb, cb_ptr(n + 1, 0), cb_face;
+        std::vector<double> cf_geo, cf_ksi;
         for (size_t i = 0; i < n; ++i) {
             for (int code : cf[i]) {
                 const int f = code >> 1;
@@ -1384,6 +1407,9 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
                 OCSL_REQUIRE(pos >= 0, OCSL_ERR_STRUCTURE, "face neighbour missing from the matrix row");
                 cf_code.push_back(code);
                 cf_pos.push_back(pos);
+                cf_nb.push_back(other);
+                cf_geo.push_back(m.fgeo[f]);
+                cf_ksi.push_back(m.fksi[f]);
             }
             cf_ptr[i + 1] = (int)cf_code.size();
             for (int b : cb[i]) cb_face.push_back(b);
@@ -1411,6 +1437,9 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
         up(fl->d_cf_ptr, cf_ptr, s);
         up(fl->d_cf_code, cf_code, s);
         up(fl->d_cf_pos, cf_pos, s);
+        up(fl->d_cf_nb, cf_nb, s);
+        up(fl->d_cf_geo, cf_geo, s);
+        up(fl->d_cf_ksi, cf_ksi, s);
         up(fl->d_cb_ptr, cb_ptr, s);
         up(fl->d_cb_face, cb_face, s);
         up(fl->d_row_ptr, rp32, s);

@@ -120,3 +120,15 @@ def test_errors():
     st = lib.ocsl_build_structure(3, 2, fl.ctypes.data_as(vp), fr.ctypes.data_as(vp), 0, None, 0, None,
                                   col.ctypes.data_as(vp), 2, ctypes.byref(nnz), None, None)
     assert st != 0 and nnz.value == 7  # 7 entries do not fit in 2
+
+
+def test_meshes_without_interior_faces():
+    """Cells with no interior face: each row holds its diagonal, and the
+    sentinel when the cell owns boundary faces."""
+    rp, col = build_structure(3, [], [], [])
+    assert rp.tolist() == [0, 1, 2, 3] and col.tolist() == [0, 1, 2]
+    rp, col, fs, ds = build_structure(1, [], [], [0, 0, 0, 0, 0, 0], slots=True)
+    assert rp.tolist() == [0, 2] and col.tolist() == [0, oracle.SENTINEL_COL]
+    assert fs.shape == (0, 2) and ds.tolist() == [0]
+    rp_o, col_o = oracle.define_matrix_structure(1, [], [], [0, 0, 0, 0, 0, 0])
+    assert np.array_equal(rp, rp_o) and np.array_equal(col, col_o)
