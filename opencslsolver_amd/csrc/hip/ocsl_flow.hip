@@ -16,9 +16,11 @@
 // equation (OCSL_SUPPORT_HOST, a cross-check).  The gradient kernels gather
 // per cell in the same face order as the reference's scatter loops, so each
 // cell's sum is added in the reference's order.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <numeric>
 #include <vector>
 
 #include "ocsl_flow.h"
@@ -618,6 +620,12 @@ struct ocsl_flow_s {
     double urf = 0.5;  // implicit_underrelaxation (cflow_solver.h:177)
     HostMesh mesh;
     HostFlow st;
+    // The device keeps interior faces sorted by their left cell, so a cell's
+    // faces as owner are contiguous: device face k is the caller's face
+    // fperm[k].  The host mirror keeps the caller's order.
+    std::vector<int> fperm;
+    DBuf<int> d_fperm;
+    DBuf<double> d_fstage;  // caller-order staging of per-face state
     bool dev = true;  // support loops on the device, state resident there
     // device mesh
     DBuf<double> d_vol, d_fS, d_fe, d_fdist, d_fksi, d_fgeo, d_falpha, d_bS, d_barea, d_bksi, d_bperp, d_be, d_balpha,
@@ -685,6 +693,53 @@ void up(DBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
 template <class T>
 void down(std::vector<T>& h, const DBuf<T>& d, hipStream_t s) {
     if (!h.empty()) OCSL_HIP(hipMemcpyAsync(h.data(), d.p, sizeof(T) * h.size(), hipMemcpyDeviceToHost, s));
+}
+
+// Face data between the caller's face order and the handle's: device face k
+// is the caller's face fperm[k]; c components per face.
+template <bool kToHandle>
+__global__ void k_face_perm(int F, int c, const int* __restrict__ fperm, const double* __restrict__ in,
+                            double* __restrict__ out) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < F; k += gridDim.x * blockDim.x) {
+        const size_t f = (size_t)fperm[k];
+        for (int q = 0; q < c; ++q) {
+            if (kToHandle) out[(size_t)k * c + q] = in[f * c + q];
+            else out[f * c + q] = in[(size_t)k * c + q];
+        }
+    }
+}
+
+// Set-up upload of a per-face mesh array, permuted on the host.
+template <class T>
+void up_mesh_f(const ocsl_flow_s* fl, DBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
+    const size_t F = fl->fperm.size(), c = F ? h.size() / F : 0;
+    std::vector<T> t(h.size());
+    for (size_t k = 0; k < F; ++k)
+        for (size_t q = 0; q < c; ++q) t[k * c + q] = h[(size_t)fl->fperm[k] * c + q];
+    d.upload(t.data(), t.size(), s);
+    OCSL_HIP(hipStreamSynchronize(s));  // t is freed on return
+}
+
+// Per-face state (1 or 3 components per face) from the host mirror, in the
+// caller's order, to the device, in the handle's order: staged, then gathered.
+void up_f(ocsl_flow_s* fl, DBuf<double>& d, const std::vector<double>& h, hipStream_t s) {
+    const int F = (int)fl->fperm.size();
+    d.alloc(h.size());
+    if (h.empty() || F == 0) return;
+    fl->d_fstage.upload(h.data(), h.size(), s);
+    k_face_perm<true><<<grid_for(F), kBlock, 0, s>>>(F, (int)(h.size() / F), fl->d_fperm.p, fl->d_fstage.p, d.p);
+    OCSL_HIP(hipGetLastError());
+}
+
+// The reverse: scattered into the staging buffer, then copied out (ordered
+// on the stream; the caller synchronises before reading h).
+void down_f(ocsl_flow_s* fl, std::vector<double>& h, const DBuf<double>& d, hipStream_t s) {
+    const int F = (int)fl->fperm.size();
+    if (h.empty() || F == 0) return;
+    fl->d_fstage.alloc(h.size());
+    k_face_perm<false><<<grid_for(F), kBlock, 0, s>>>(F, (int)(h.size() / F), fl->d_fperm.p, d.p, fl->d_fstage.p);
+    OCSL_HIP(hipGetLastError());
+    OCSL_HIP(hipMemcpyAsync(h.data(), fl->d_fstage.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
 }
 
 inline const double* v3(const std::vector<double>& a, int i) { return a.data() + 3 * i; }
@@ -868,7 +923,7 @@ void upload_cells(ocsl_flow_s* fl, int eq) {
     up(fl->d_T, s.T, st);
     up(fl->d_rho, s.rho, st);
     up(fl->d_dt, s.dt, st);
-    up(fl->d_f_mdot, s.f_mdot, st);
+    up_f(fl, fl->d_f_mdot, s.f_mdot, st);
     up(fl->d_b_mdot, s.b_mdot, st);
     if (eq <= OCSL_EQ_MOMENTUM_Z) {
         up(fl->d_u, s.u, st);
@@ -877,13 +932,13 @@ void upload_cells(ocsl_flow_s* fl, int eq) {
         up(fl->d_gp, s.gp, st);
         up(fl->d_ap, s.ap, st);
         up(fl->d_ap_vec, s.ap_vec, st);
-        up(fl->d_f_stress, s.f_stress, st);
+        up_f(fl, fl->d_f_stress, s.f_stress, st);
         up(fl->d_b_u, s.b_u, st);
     } else if (eq == OCSL_EQ_PRESSURE) {
         up(fl->d_rho_old, s.rho_old, st);
         up(fl->d_pc, s.pc, st);
-        up(fl->d_f_rho, s.f_rho, st);
-        up(fl->d_f_qy, s.f_qy, st);
+        up_f(fl, fl->d_f_rho, s.f_rho, st);
+        up_f(fl, fl->d_f_qy, s.f_qy, st);
         up(fl->d_b_T, s.b_T, st);
         up(fl->d_b_rho, s.b_rho, st);
         up(fl->d_b_qy, s.b_qy, st);
@@ -919,10 +974,10 @@ void upload_all(ocsl_flow_s* fl) {
     up(fl->d_ap_vec, s.ap_vec, st);
     up(fl->d_dt, s.dt, st);
     up(fl->d_mach, s.mach, st);
-    up(fl->d_f_mdot, s.f_mdot, st);
-    up(fl->d_f_rho, s.f_rho, st);
-    up(fl->d_f_qy, s.f_qy, st);
-    up(fl->d_f_stress, s.f_stress, st);
+    up_f(fl, fl->d_f_mdot, s.f_mdot, st);
+    up_f(fl, fl->d_f_rho, s.f_rho, st);
+    up_f(fl, fl->d_f_qy, s.f_qy, st);
+    up_f(fl, fl->d_f_stress, s.f_stress, st);
     up(fl->d_b_T, s.b_T, st);
     up(fl->d_b_p, s.b_p, st);
     up(fl->d_b_p_out, s.b_p_out, st);
@@ -954,10 +1009,10 @@ void download_all(ocsl_flow_s* fl) {
     down(s.ap_vec, fl->d_ap_vec, st);
     down(s.dt, fl->d_dt, st);
     down(s.mach, fl->d_mach, st);
-    down(s.f_mdot, fl->d_f_mdot, st);
-    down(s.f_rho, fl->d_f_rho, st);
-    down(s.f_qy, fl->d_f_qy, st);
-    down(s.f_stress, fl->d_f_stress, st);
+    down_f(fl, s.f_mdot, fl->d_f_mdot, st);
+    down_f(fl, s.f_rho, fl->d_f_rho, st);
+    down_f(fl, s.f_qy, fl->d_f_qy, st);
+    down_f(fl, s.f_stress, fl->d_f_stress, st);
     down(s.b_T, fl->d_b_T, st);
     down(s.b_p, fl->d_b_p, st);
     down(s.b_p_out, fl->d_b_p_out, st);
@@ -1030,7 +1085,7 @@ void viscous_stress(ocsl_flow_s* fl) {
         k_visc_stress<<<(F + kBlock - 1) / kBlock, kBlock, 0, st>>>(fl->mesh_view(), fl->state_view(), fl->gas);
     OCSL_HIP(hipGetLastError());
     if (!fl->dev) {
-        down(fl->st.f_stress, fl->d_f_stress, st);
+        down_f(fl, fl->st.f_stress, fl->d_f_stress, st);
         OCSL_HIP(hipStreamSynchronize(st));
     }
 }
@@ -1263,7 +1318,7 @@ void alloc_state(ocsl_flow_s* fl) {
     s.b_u.assign(3 * (size_t)B, 0.0);
     s.b_super.assign(B, 0);
     hipStream_t st = fl->solver->stream;
-    up(fl->d_f_stress, s.f_stress, st);
+    up_f(fl, fl->d_f_stress, s.f_stress, st);
     up(fl->d_ap, s.ap, st);
     up(fl->d_ap_vec, s.ap_vec, st);
     for (auto* d : {&fl->d_u, &fl->d_u_old, &fl->d_gp}) d->alloc(3 * (size_t)n);
@@ -1391,10 +1446,19 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
             dpos[i] = find((int)i, (int)i);
             OCSL_REQUIRE(dpos[i] >= 0, OCSL_ERR_STRUCTURE, "row without diagonal");
         }
+        fl->fperm.resize(F);
+        std::iota(fl->fperm.begin(), fl->fperm.end(), 0);
+        std::stable_sort(fl->fperm.begin(), fl->fperm.end(), [&](int x, int y) { return m.fL[x] < m.fL[y]; });
+        up(fl->d_fperm, fl->fperm, fl->solver->stream);
+        OCSL_HIP(hipStreamSynchronize(fl->solver->stream));
+        std::vector<int> finv(F);
+        for (size_t k = 0; k < F; ++k) finv[fl->fperm[k]] = (int)k;
+        // each cell's faces in the caller's ascending face order (the order
+        // the reference's face loop adds them in), coded with device indices
         std::vector<std::vector<int>> cf(n), cb(n);
         for (size_t f = 0; f < F; ++f) {
-            cf[m.fL[f]].push_back((int)(f << 1));
-            cf[m.fR[f]].push_back((int)((f << 1) | 1));
+            cf[m.fL[f]].push_back((int)f << 1);
+            cf[m.fR[f]].push_back(((int)f << 1) | 1);
         }
         for (size_t b = 0; b < B; ++b) cb[m.bL[b]].push_back((int)b);
         std::vector<int> cf_ptr(n + 1, 0), cf_code, cf_pos, cf_nb, cb_ptr(n + 1, 0), cb_face;
@@ -1405,7 +1469,7 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
                 const int other = (code & 1) ? m.fL[f] : m.fR[f];
                 const int pos = find((int)i, other);
                 OCSL_REQUIRE(pos >= 0, OCSL_ERR_STRUCTURE, "face neighbour missing from the matrix row");
-                cf_code.push_back(code);
+                cf_code.push_back((finv[f] << 1) | (code & 1));
                 cf_pos.push_back(pos);
                 cf_nb.push_back(other);
                 cf_geo.push_back(m.fgeo[f]);
@@ -1417,14 +1481,14 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
         }
         hipStream_t s = fl->solver->stream;
         up(fl->d_vol, m.vol, s);
-        up(fl->d_fL, m.fL, s);
-        up(fl->d_fR, m.fR, s);
-        up(fl->d_fS, m.fS, s);
-        up(fl->d_fe, m.fe, s);
-        up(fl->d_fdist, m.fdist, s);
-        up(fl->d_fksi, m.fksi, s);
-        up(fl->d_fgeo, m.fgeo, s);
-        up(fl->d_falpha, m.falpha, s);
+        up_mesh_f(fl, fl->d_fL, m.fL, s);
+        up_mesh_f(fl, fl->d_fR, m.fR, s);
+        up_mesh_f(fl, fl->d_fS, m.fS, s);
+        up_mesh_f(fl, fl->d_fe, m.fe, s);
+        up_mesh_f(fl, fl->d_fdist, m.fdist, s);
+        up_mesh_f(fl, fl->d_fksi, m.fksi, s);
+        up_mesh_f(fl, fl->d_fgeo, m.fgeo, s);
+        up_mesh_f(fl, fl->d_falpha, m.falpha, s);
         up(fl->d_be, m.be, s);
         up(fl->d_balpha, m.balpha, s);
         up(fl->d_bdist, m.bdist, s);
@@ -1523,7 +1587,7 @@ ocsl_status ocsl_flow_set_state(ocsl_flow fl, const ocsl_flow_state* st) {
         copy_in(s.b_u, st->b_u);
         copy_in(s.b_super, st->b_supersonic);
         if (fl->dev) upload_all(fl);
-        else if (st->f_stress) up(fl->d_f_stress, s.f_stress, fl->solver->stream);
+        else if (st->f_stress) up_f(fl, fl->d_f_stress, s.f_stress, fl->solver->stream);
         OCSL_HIP(hipStreamSynchronize(fl->solver->stream));
         fl->initialized = true;
     });
