@@ -377,14 +377,24 @@ def assembly(args, out):
               b_mdot=rng.normal(size=B) * 1e-3, b_h=np.full(B, 3e5), b_qy=rng.random(B) * 1e-3,
               b_u=rng.normal(size=(B, 3)), b_supersonic=np.zeros(B, np.int32))
     names = ["momentum_x", "momentum_y", "momentum_z", "pressure", "enthalpy"]
-    res = {}
-    with flow.Flow(mesh, rp, col, prm) as fl:
-        fl.set_state(**st)
-        for eq, name in enumerate(names):
-            ms, by = fl.time_assemble(eq, reps=10)
-            res[name] = {"avg_us": round(ms * 1e3, 2), "bytes_per_launch": by,
-                         "achieved_GBps": round(by / (ms * 1e-3) / 1e9, 1),
-                         "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    def timed(m, r, c):
+        res = {}
+        with flow.Flow(m, r, c, prm) as fl:
+            fl.set_state(**st)
+            for eq, name in enumerate(names):
+                ms, by = fl.time_assemble(eq, reps=10)
+                res[name] = {"avg_us": round(ms * 1e3, 2), "bytes_per_launch": by,
+                             "achieved_GBps": round(by / (ms * 1e-3) / 1e9, 1),
+                             "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        return res
+
+    res = timed(mesh, rp, col)
+    # the same loops with the generator's lexicographic cell numbering, where
+    # a cell's neighbours sit near it in memory
+    mesh_n, rp_n, col_n = gen.hex_mesh(n)
+    res_nat = timed(mesh_n, rp_n, col_n)
+    del mesh_n, rp_n, col_n
     cpu = None
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -399,7 +409,10 @@ def assembly(args, out):
     out["assembly"] = {"workload": f"face/boundary/cell coefficient loops, {n}^3 hex mesh (N={N}, faces={F}, "
                                    f"boundary faces={B}), viscous, blended, transient, permuted cells",
                        "kernel": "k_assemble", "gpu": res,
-                       "gpu_total_us": round(sum(v["avg_us"] for v in res.values()), 2), "cpu_baseline": cpu}
+                       "gpu_total_us": round(sum(v["avg_us"] for v in res.values()), 2),
+                       "gpu_natural_order": {k: {"avg_us": v["avg_us"], "frac": v["frac"]} for k, v in res_nat.items()},
+                       "gpu_natural_order_total_us": round(sum(v["avg_us"] for v in res_nat.values()), 2),
+                       "cpu_baseline": cpu}
 
 
 def stream_triad():

@@ -92,11 +92,16 @@ struct MeshDev {
     const double *fS, *fe, *fdist, *fksi, *fgeo, *falpha;
     const int *bL, *bmark;
     const double *bS, *barea, *bksi, *bperp, *be, *balpha, *bdist;
-    const int *cf_ptr, *cf_code, *cf_pos;  // per cell: faces (f << 1 | side), neighbour slot position
+    // Per cell, its interior faces in ascending caller face order, sliced 64
+    // cells (one wave) to a slice: entry k of cell i is at cf_sp[i / 64] +
+    // 64 k + i % 64, so each step of the face loop is one contiguous load per
+    // wave; code = f << 1 | side, or -1 after the cell's last face; pos = the
+    // neighbour's slot position in the row.
+    const int *cf_sp, *cf_code, *cf_pos;
     const int *cb_ptr, *cb_face;           // per cell: boundary faces
     const int *row_ptr, *dpos;             // caller layout (int32), diagonal position in the row
-    // per cell-face entry, in the order of cf_code: the cell across the face and
-    // the face's static weights, so a cell's faces read contiguous memory
+    // per cell-face entry, beside cf_code: the cell across the face and the
+    // face's static weights
     const int* cf_nb;
     const double *cf_geo, *cf_ksi;
 };
@@ -179,8 +184,9 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshDev m, StateDev s, Flow
     if constexpr (kEq != kMomentum || kVisc) own_T = s.T[i];
     // interior faces (cflow_solver_cell_face_loops.cpp:38-58, 186-218, 334-346);
     // (l, r) are the face's left and right cell values whichever side this is
-    for (int e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
+    for (int e = m.cf_sp[i >> 6] + (i & 63), e1 = m.cf_sp[(i >> 6) + 1]; e < e1; e += 64) {
         const int code = m.cf_code[e];
+        if (code < 0) break;
         const int f = code >> 1, side = code & 1, op = m.cf_pos[e];
         const int j = m.cf_nb[e];
         const double geo = m.cf_geo[e];
@@ -413,8 +419,10 @@ __global__ void __launch_bounds__(kBlock) k_press_grad(MeshDev m, StateDev s, Fl
     if (i >= m.n) return;
     double gv[3] = {0.0, 0.0, 0.0};
     const double ap_i = s.ap[i], p_i = s.p[i];
-    for (int e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
-        const int code = m.cf_code[e], f = code >> 1, j = m.cf_nb[e];
+    for (int e = m.cf_sp[i >> 6] + (i & 63), e1 = m.cf_sp[(i >> 6) + 1]; e < e1; e += 64) {
+        const int code = m.cf_code[e];
+        if (code < 0) break;
+        const int f = code >> 1, j = m.cf_nb[e];
         const double ap_j = s.ap[j], p_j = s.p[j];
         const double apL = code & 1 ? ap_j : ap_i, apR = code & 1 ? ap_i : ap_j;
         const double pL = code & 1 ? p_j : p_i, pR = code & 1 ? p_i : p_j;
@@ -445,8 +453,10 @@ __global__ void __launch_bounds__(kBlock) k_press_corr_grad(MeshDev m, StateDev 
     if (i >= m.n) return;
     double gv[3] = {0.0, 0.0, 0.0};
     const double pc_i = s.pc[i];
-    for (int e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
-        const int code = m.cf_code[e], f = code >> 1;
+    for (int e = m.cf_sp[i >> 6] + (i & 63), e1 = m.cf_sp[(i >> 6) + 1]; e < e1; e += 64) {
+        const int code = m.cf_code[e];
+        if (code < 0) break;
+        const int f = code >> 1;
         const double geo = m.cf_geo[e], pc_j = s.pc[m.cf_nb[e]];
         double fp = geo * (code & 1 ? pc_j : pc_i);
         fp += (1.0 - geo) * (code & 1 ? pc_i : pc_j);
@@ -630,7 +640,7 @@ struct ocsl_flow_s {
     // device mesh
     DBuf<double> d_vol, d_fS, d_fe, d_fdist, d_fksi, d_fgeo, d_falpha, d_bS, d_barea, d_bksi, d_bperp, d_be, d_balpha,
         d_bdist;
-    DBuf<int> d_fL, d_fR, d_bL, d_bmark, d_cf_ptr, d_cf_code, d_cf_pos, d_cb_ptr, d_cb_face, d_row_ptr, d_dpos;
+    DBuf<int> d_fL, d_fR, d_bL, d_bmark, d_cf_sp, d_cf_code, d_cf_pos, d_cb_ptr, d_cb_face, d_row_ptr, d_dpos;
     DBuf<int> d_cf_nb;
     DBuf<double> d_cf_geo, d_cf_ksi;
     // device state
@@ -649,7 +659,7 @@ struct ocsl_flow_s {
         return {mesh.n,     mesh.F,       mesh.B,      d_vol.p,    d_fL.p,      d_fR.p,       d_fS.p,
                 d_fe.p,     d_fdist.p,    d_fksi.p,    d_fgeo.p,   d_falpha.p,  d_bL.p,       d_bmark.p,
                 d_bS.p,     d_barea.p,    d_bksi.p,    d_bperp.p,  d_be.p,      d_balpha.p,   d_bdist.p,
-                d_cf_ptr.p, d_cf_code.p,  d_cf_pos.p,  d_cb_ptr.p, d_cb_face.p, d_row_ptr.p,  d_dpos.p,
+                d_cf_sp.p,  d_cf_code.p,  d_cf_pos.p,  d_cb_ptr.p, d_cb_face.p, d_row_ptr.p,  d_dpos.p,
                 d_cf_nb.p,  d_cf_geo.p,   d_cf_ksi.p};
     }
     StateDev state_view() const {
@@ -1479,6 +1489,34 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
             for (int b : cb[i]) cb_face.push_back(b);
             cb_ptr[i + 1] = (int)cb_face.size();
         }
+        // the sliced form of the cell-face lists (MeshDev::cf_sp)
+        const size_t nsl = (n + 63) / 64;
+        std::vector<int> cf_sp(nsl + 1, 0);
+        for (size_t sl = 0; sl < nsl; ++sl) {
+            int w = 0;
+            for (size_t i = sl * 64; i < std::min(n, sl * 64 + 64); ++i) w = std::max(w, cf_ptr[i + 1] - cf_ptr[i]);
+            OCSL_REQUIRE((int64_t)cf_sp[sl] + 64 * (int64_t)w < (1ll << 31), OCSL_ERR_ARG, "mesh too large");
+            cf_sp[sl + 1] = cf_sp[sl] + 64 * w;
+        }
+        {
+            const size_t tot = (size_t)cf_sp[nsl];
+            std::vector<int> sc(tot, -1), sp(tot, 0), sn(tot, 0);
+            std::vector<double> sg(tot, 0.0), sk(tot, 0.0);
+            for (size_t i = 0; i < n; ++i)
+                for (int k = 0; k < cf_ptr[i + 1] - cf_ptr[i]; ++k) {
+                    const size_t e = (size_t)cf_sp[i / 64] + 64 * (size_t)k + i % 64, q = (size_t)cf_ptr[i] + k;
+                    sc[e] = cf_code[q];
+                    sp[e] = cf_pos[q];
+                    sn[e] = cf_nb[q];
+                    sg[e] = cf_geo[q];
+                    sk[e] = cf_ksi[q];
+                }
+            cf_code.swap(sc);
+            cf_pos.swap(sp);
+            cf_nb.swap(sn);
+            cf_geo.swap(sg);
+            cf_ksi.swap(sk);
+        }
         hipStream_t s = fl->solver->stream;
         up(fl->d_vol, m.vol, s);
         up_mesh_f(fl, fl->d_fL, m.fL, s);
@@ -1498,7 +1536,7 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
         up(fl->d_barea, m.barea, s);
         up(fl->d_bksi, m.bksi, s);
         up(fl->d_bperp, m.bperp, s);
-        up(fl->d_cf_ptr, cf_ptr, s);
+        up(fl->d_cf_sp, cf_sp, s);
         up(fl->d_cf_code, cf_code, s);
         up(fl->d_cf_pos, cf_pos, s);
         up(fl->d_cf_nb, cf_nb, s);
