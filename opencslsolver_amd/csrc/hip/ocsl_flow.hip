@@ -116,11 +116,12 @@ struct StateDev {
 
 enum { kMomentum = 0, kPressure = 1, kEnthalpy = 2 };
 
-template <class T>
-__device__ __forceinline__ void add_at(T (&a)[kMaxRow], int pos, T v) {
-#pragma unroll
-    for (int k = 0; k < kMaxRow; ++k)
-        if (k == pos) a[k] += v;
+// a: the cell's row, staged in LDS; pos: the slot position in the row.  The
+// reference adds each term, already rounded, to the slot: the empty asm keeps
+// the compiler from fusing the product that formed v into this add.
+__device__ __forceinline__ void add_at(double* a, int pos, double v) {
+    asm volatile("" : "+v"(v));
+    a[pos] += v;
 }
 
 __device__ __forceinline__ double face_avg(double geo, double l, double r) {
@@ -139,7 +140,7 @@ __device__ __forceinline__ double conv_interp(double geo, double mdot, double l,
 // side 0 = left (owner), 1 = right; dp / op: positions of the diagonal and of
 // the other cell's column in this row.
 template <bool kBlend>
-__device__ __forceinline__ void conv_row(double (&a)[kMaxRow], double& rhs, int side, int dp, int op,
+__device__ __forceinline__ void conv_row(double* a, double& rhs, int side, int dp, int op,
                                          double mdot, double l, double r, double geo) {
     if (mdot >= 0.0) {
         if (side == 0) add_at(a, dp, mdot);
@@ -159,21 +160,15 @@ __device__ __forceinline__ void conv_row(double (&a)[kMaxRow], double& rhs, int 
 }
 
 // Diffusive_flux (solvers_common.h:42-53): +D on the diagonal, -D off it.
-__device__ __forceinline__ void diff_row(double (&a)[kMaxRow], int dp, int op, double D) {
+__device__ __forceinline__ void diff_row(double* a, int dp, int op, double D) {
     add_at(a, dp, D);
     add_at(a, op, -D);
 }
 
 template <int kEq, bool kVisc, bool kBlend, bool kTransient>
-__global__ void __launch_bounds__(kBlock) k_assemble(MeshDev m, StateDev s, FlowPhys ph, int d,
-                                                     double* __restrict__ val, double* __restrict__ b,
-                                                     double* __restrict__ x0) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m.n) return;
+__device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& s, const FlowPhys& ph, int d,
+                                              int i, double* a, double* __restrict__ b, double* __restrict__ x0) {
     const Gas& gas = ph.gas;
-    double a[kMaxRow];
-#pragma unroll
-    for (int k = 0; k < kMaxRow; ++k) a[k] = 0.0;
     double rhs = 0.0;
     const int dp = m.dpos[i];
     // this cell's own values, the same for each of its faces
@@ -294,10 +289,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshDev m, StateDev s, Flow
     }
     // cell terms (cflow_solver_cell_face_loops.cpp:143-175, 299-322, 423-446)
     const double V = m.vol[i];
-    double diag = 0.0;
-#pragma unroll
-    for (int k = 0; k < kMaxRow; ++k)
-        if (k == dp) diag = a[k];
+    double diag = a[dp];
     if constexpr (kEq == kMomentum) {
         double dum = 0.0, dum2 = 0.0;
         if constexpr (kTransient) {
@@ -342,11 +334,25 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshDev m, StateDev s, Flow
         }
         x0[i] = s.h[i];
     }
-    const int p0 = m.row_ptr[i], len = m.row_ptr[i + 1] - p0;
-#pragma unroll
-    for (int k = 0; k < kMaxRow; ++k)
-        if (k < len) val[p0 + k] = (k == dp) ? diag : a[k];
+    a[dp] = diag;
     b[i] = rhs;
+}
+
+// One thread per cell.  The block's rows are accumulated in LDS, laid out
+// as they lie in the caller's value array (sentinel slots stay 0), and
+// written out together.
+template <int kEq, bool kVisc, bool kBlend, bool kTransient>
+__global__ void __launch_bounds__(kBlock) k_assemble(MeshDev m, StateDev s, FlowPhys ph, int d,
+                                                     double* __restrict__ val, double* __restrict__ b,
+                                                     double* __restrict__ x0) {
+    extern __shared__ double rows[];
+    const int i0 = blockIdx.x * blockDim.x, i = i0 + threadIdx.x;
+    const int base = m.row_ptr[i0], span = m.row_ptr[min(i0 + (int)blockDim.x, m.n)] - base;
+    for (int k = threadIdx.x; k < span; k += blockDim.x) rows[k] = 0.0;
+    __syncthreads();
+    if (i < m.n) assemble_cell<kEq, kVisc, kBlend, kTransient>(m, s, ph, d, i, rows + (m.row_ptr[i] - base), b, x0);
+    __syncthreads();
+    for (int k = threadIdx.x; k < span; k += blockDim.x) val[base + k] = rows[k];
 }
 
 // calc_flux_explicit_visc_part (cflow_solver_cell_face_loops.cpp:25-35):
@@ -634,6 +640,7 @@ struct ocsl_flow_s {
     // faces as owner are contiguous: device face k is the caller's face
     // fperm[k].  The host mirror keeps the caller's order.
     std::vector<int> fperm;
+    size_t asm_lds = 0;  // LDS bytes of one assembly block: its longest run of rows
     DBuf<int> d_fperm;
     DBuf<double> d_fstage;  // caller-order staging of per-face state
     bool dev = true;  // support loops on the device, state resident there
@@ -1044,7 +1051,8 @@ void launch_assemble(ocsl_flow_s* fl, int d) {
     const StateDev s = fl->state_view();
     const FlowPhys ph = fl->phys();
     double *val = fl->d_val.p, *b = fl->d_b.p, *x0 = fl->d_x0.p;
-#define OCSL_ASM(V, BL, TR) k_assemble<kEq, V, BL, TR><<<grid, kBlock, 0, st>>>(m, s, ph, d, val, b, x0)
+#define OCSL_ASM(V, BL, TR) \
+    k_assemble<kEq, V, BL, TR><<<grid, kBlock, fl->asm_lds, st>>>(m, s, ph, d, val, b, x0)
     const int key = (p.viscous ? 4 : 0) + (p.blended ? 2 : 0) + (p.transient ? 1 : 0);
     switch (key) {
         case 0: OCSL_ASM(false, false, false); break;
@@ -1456,6 +1464,9 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
             dpos[i] = find((int)i, (int)i);
             OCSL_REQUIRE(dpos[i] >= 0, OCSL_ERR_STRUCTURE, "row without diagonal");
         }
+        int span = 1;  // at most kBlock * kMaxRow entries: 32 KB
+        for (size_t i0 = 0; i0 < n; i0 += kBlock) span = std::max(span, rp32[std::min(n, i0 + kBlock)] - rp32[i0]);
+        fl->asm_lds = sizeof(double) * (size_t)span;
         fl->fperm.resize(F);
         std::iota(fl->fperm.begin(), fl->fperm.end(), 0);
         std::stable_sort(fl->fperm.begin(), fl->fperm.end(), [&](int x, int y) { return m.fL[x] < m.fL[y]; });
