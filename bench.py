@@ -18,7 +18,9 @@ already resident in HBM.  The JSON line also carries the side workloads:
 * assembly -- the GPU coefficient assembly of each equation on a 128^3 hex
   mesh, beside the oracle's host restatement of the reference loops;
 * flow -- SIMPLE iterations of the flow solver on a 64^3 hex mesh, support
-  loops on the device (state resident) and on the host.
+  loops on the device (state resident) and on the host;
+* structure -- the CSR pattern of the 128^3 mesh built on the device;
+* mesh -- SU2 reading, connectivity and geometry of a 64^3 box on the host.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--sections all]
 """
@@ -434,6 +436,46 @@ def stream_triad():
     return res
 
 
+def mesh_read(args, out):
+    """SU2 mesh set-up on the host (libocsl_mesh.so): parsing, cell/face
+    connectivity, geometry and the finite-volume face factors of a 64^3
+    hexahedral box in random node and cell numbering (seed 31), beside the
+    reference's own Read_SU2_mesh + CFVM::create_FVM_from_geometry (the
+    reference build in oracle/_ref, when present) on a 32^3 sample."""
+    import tempfile
+
+    from opencslsolver_amd import mesh
+
+    res = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for n in (32, 64):
+            _log(f"bench: SU2 mesh {n}^3")
+            path = os.path.join(tmp, f"box{n}.su2")
+            with open(path, "w") as f:
+                f.write(mesh.box_su2(n, "hex", h=1.0 / n, seed=31))
+            t0 = time.perf_counter()
+            m = mesh.read_su2(path)
+            res[n] = (time.perf_counter() - t0, m["cell_volume"].shape[0], m["face_left"].shape[0],
+                      m["bc_left"].shape[0])
+        cpu = None
+        import oracle
+
+        if not args.no_cpu_baseline and oracle.ref_available():
+            t0 = time.perf_counter()
+            oracle.ref_su2_arrays(os.path.join(tmp, "box32.su2"))
+            ref_s = time.perf_counter() - t0
+            cpu = {"value": round(ref_s * 1e3, 1), "unit": "ms for the 32^3 file", "cores": 1, "kind": "reference",
+                   "sample": "the reference's Read_SU2_mesh, create_FVM_from_geometry and define_matrix_structure "
+                             "(built from its sources) on the 32^3 file",
+                   "speedup_32": round(ref_s / res[32][0], 1)}
+    t64 = res[64]
+    out["mesh"] = {"workload": f"SU2 hexahedral box, 64^3 cells ({t64[1]} cells, {t64[2]} interior faces, "
+                               f"{t64[3]} boundary faces), random numbering (seed 31): read, connect, geometry, "
+                               "face factors",
+                   "ms_64": round(t64[0] * 1e3, 1), "cells_per_s": round(t64[1] / t64[0]),
+                   "ms_32": round(res[32][0] * 1e3, 1), "cpu_baseline": cpu}
+
+
 def structure(args, out):
     """Matrix structure of the n^3 hex mesh (CFVM::define_matrix_structure) built
     on the device: from host arrays (PCIe included) and from device arrays."""
@@ -531,7 +573,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--sections", default="adjoint,multi,batch,assembly,flow,structure",
+    ap.add_argument("--sections", default="adjoint,multi,batch,assembly,flow,structure,mesh",
                     help="side workloads to add to the headline: adjoint, multi, batch, assembly, flow, "
                          "structure (or none)")
     ap.add_argument("--flow-n", type=int, default=64, help="hex mesh size of the flow-step section")
@@ -578,6 +620,8 @@ def main():
         flow_step(args, out)
     if "structure" in args.sections:
         structure(args, out)
+    if "mesh" in args.sections:
+        mesh_read(args, out)
     print(json.dumps(out), flush=True)
 
 

@@ -12,6 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 GEN_LIB_PATH = os.path.join(_HERE, "libocsl_gen.so")
 HIP_LIB_PATH = os.path.join(_HERE, "libocsl_hip.so")
+MESH_LIB_PATH = os.path.join(_HERE, "libocsl_mesh.so")
 
 
 class NativeLibraryMissing(RuntimeError):
@@ -38,4 +39,22 @@ def hip_lib() -> ctypes.CDLL:
 
     lib = _load(HIP_LIB_PATH)
     _abi.declare(lib)
+    return lib
+
+
+@functools.lru_cache(maxsize=None)
+def mesh_lib() -> ctypes.CDLL:
+    lib = _load(MESH_LIB_PATH)
+    c = ctypes
+    p = c.c_void_p
+    lib.ocsl_mesh_read_su2.argtypes = [c.c_char_p, c.POINTER(p)]
+    lib.ocsl_mesh_parse_su2.argtypes = [c.c_char_p, c.c_int64, c.POINTER(p)]
+    lib.ocsl_mesh_destroy.argtypes = [p]
+    lib.ocsl_mesh_destroy.restype = None
+    lib.ocsl_mesh_last_error.restype = c.c_char_p
+    lib.ocsl_mesh_sizes.argtypes = [p] + [c.POINTER(c.c_int64)] * 4 + [c.POINTER(c.c_int32)]
+    lib.ocsl_mesh_marker_tag.argtypes = [p, c.c_int32, c.c_char_p, c.c_int64]
+    lib.ocsl_mesh_cells.argtypes = [p, p, p]
+    lib.ocsl_mesh_faces.argtypes = [p] + [p] * 9
+    lib.ocsl_mesh_bc_faces.argtypes = [p] + [p] * 9
     return lib

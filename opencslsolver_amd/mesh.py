@@ -1,0 +1,191 @@
+"""SU2 meshes: the reader (libocsl_mesh.so, include/ocsl_mesh.h) and a box
+generator that writes SU2 text.
+
+read_su2 / parse_su2 return the finite-volume arrays that ``flow.Flow`` takes
+(the keys of ``gen.hex_mesh``), in the reference's order: interior faces as
+Cgeometry::calc_cell_face_connectivity discovers them (cgeometry.cpp:620-660),
+boundary faces marker by marker in file order; values as
+CFVM::create_FVM_from_geometry computes them (cfvm.cpp:100-170).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._native import mesh_lib
+
+
+class MeshError(RuntimeError):
+    pass
+
+
+def _arr(shape, dt):
+    return np.empty(shape, dt)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _collect(lib, h) -> dict:
+    c = ctypes
+    nn, nc, nf, nb = (c.c_int64() for _ in range(4))
+    nm = c.c_int32()
+    lib.ocsl_mesh_sizes(h, c.byref(nn), c.byref(nc), c.byref(nf), c.byref(nb), c.byref(nm))
+    N, F, B = nc.value, nf.value, nb.value
+    tags = []
+    for i in range(nm.value):
+        buf = c.create_string_buffer(4096)
+        lib.ocsl_mesh_marker_tag(h, i, buf, 4096)
+        tags.append(buf.value.decode())
+    vol, ctr = _arr(N, np.float64), _arr((N, 3), np.float64)
+    lib.ocsl_mesh_cells(h, _ptr(vol), _ptr(ctr))
+    f = {k: _arr(F, np.int32) for k in ("face_left", "face_right")}
+    for k in ("face_area_vector", "face_unit", "face_alpha"):
+        f[k] = _arr((F, 3), np.float64)
+    for k in ("face_dist", "face_ksi", "face_geo_aver", "face_area"):
+        f[k] = _arr(F, np.float64)
+    lib.ocsl_mesh_faces(h, *(_ptr(f[k]) for k in ("face_left", "face_right", "face_area_vector", "face_unit",
+                                                   "face_dist", "face_alpha", "face_ksi", "face_geo_aver",
+                                                   "face_area")))
+    b = {k: _arr(B, np.int32) for k in ("bc_left", "bc_marker")}
+    for k in ("bc_area_vector", "bc_unit", "bc_alpha"):
+        b[k] = _arr((B, 3), np.float64)
+    for k in ("bc_dist", "bc_ksi", "bc_perp", "bc_area"):
+        b[k] = _arr(B, np.float64)
+    lib.ocsl_mesh_bc_faces(h, *(_ptr(b[k]) for k in ("bc_left", "bc_marker", "bc_area_vector", "bc_unit",
+                                                      "bc_dist", "bc_alpha", "bc_ksi", "bc_perp", "bc_area")))
+    out = {"cell_volume": vol, "cell_center": ctr, **f, **b, "n_nodes": nn.value, "tags": tags}
+    return out
+
+
+def _finish(lib, st, h) -> dict:
+    if st != 0:
+        raise MeshError(f"SU2 mesh: {lib.ocsl_mesh_last_error().decode()} (status {st})")
+    try:
+        return _collect(lib, h)
+    finally:
+        lib.ocsl_mesh_destroy(h)
+
+
+def read_su2(path: str) -> dict:
+    """Reads an SU2 file (Read_SU2_mesh, cgeometry_supply.h:201-433) into the
+    finite-volume arrays, plus cell_center, n_nodes and the marker tags."""
+    lib = mesh_lib()
+    h = ctypes.c_void_p()
+    st = lib.ocsl_mesh_read_su2(str(path).encode(), ctypes.byref(h))
+    return _finish(lib, st, h)
+
+
+def parse_su2(text: str | bytes) -> dict:
+    """The same from SU2 text in memory."""
+    lib = mesh_lib()
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    h = ctypes.c_void_p()
+    st = lib.ocsl_mesh_parse_su2(data, len(data), ctypes.byref(h))
+    return _finish(lib, st, h)
+
+
+# ------------------------------------------------------------------ writer
+SU2_TET, SU2_HEX, SU2_PRISM, SU2_TRI, SU2_QUAD = 10, 12, 13, 5, 9
+
+
+def write_su2_text(nodes, elements, markers) -> str:
+    """SU2 text: nodes [n, 3]; elements a list of (type code, node list);
+    markers a list of (tag, [(type code, node list), ...])."""
+    out = ["NDIME= 3", f"NELEM= {len(elements)}"]
+    out += [f"{t} " + " ".join(str(int(v)) for v in ns) + f" {i}" for i, (t, ns) in enumerate(elements)]
+    out.append(f"NPOIN= {len(nodes)}")
+    out += [f"{x!r} {y!r} {z!r} {i}" for i, (x, y, z) in enumerate(np.asarray(nodes, np.float64).tolist())]
+    out.append(f"NMARK= {len(markers)}")
+    for tag, elems in markers:
+        out += [f"MARKER_TAG= {tag}", f"MARKER_ELEMS= {len(elems)}"]
+        out += [f"{t} " + " ".join(str(int(v)) for v in ns) for t, ns in elems]
+    return "\n".join(out) + "\n"
+
+
+# Kuhn subdivision of a cube into six tetrahedra along the diagonal 000-111:
+# one per order of the three axes; corners as (x, y, z) bits.
+_KUHN = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+
+
+def box_su2(n, kind: str = "hex", h: float = 1.0, seed: int | None = None) -> str:
+    """An n^3 box of cubes of side h as SU2 text, each cube one hexahedron,
+    two prisms (split along its 100-010 diagonal) or six tetrahedra (Kuhn
+    subdivision); positively oriented elements; six markers xmin, xmax, ymin,
+    ymax, zmin, zmax.  With a seed, nodes and elements are numbered in a
+    random order (the connectivity order depends on the numbering)."""
+    n = int(n)
+    m = n + 1
+    rng = np.random.default_rng(seed) if seed is not None else None
+    nid = np.arange(m ** 3) if rng is None else rng.permutation(m ** 3)
+
+    def node(i, j, k):
+        return int(nid[i + m * (j + m * k)])
+
+    def corner(i, j, k, b):
+        return node(i + (b & 1), j + ((b >> 1) & 1), k + ((b >> 2) & 1))
+
+    coords = np.empty((m ** 3, 3))
+    g = np.arange(m) * h
+    kk, jj, ii = np.meshgrid(g, g, g, indexing="ij")
+    coords[nid] = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1)
+
+    def vol6(ns):
+        p = coords[list(ns)]
+        return float(np.dot(p[1] - p[0], np.cross(p[2] - p[0], p[3] - p[0])))
+
+    elems = []
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                c = [corner(i, j, k, b) for b in range(8)]
+                if kind == "hex":
+                    elems.append((SU2_HEX, [c[0], c[1], c[3], c[2], c[4], c[5], c[7], c[6]]))
+                elif kind == "prism":
+                    elems.append((SU2_PRISM, [c[0], c[1], c[2], c[4], c[5], c[6]]))
+                    elems.append((SU2_PRISM, [c[1], c[3], c[2], c[5], c[7], c[6]]))
+                elif kind == "tet":
+                    for order in _KUHN:
+                        b, path = 0, [c[0]]
+                        for ax in order:
+                            b |= 1 << ax
+                            path.append(c[b])
+                        if vol6(path) < 0:
+                            path[2], path[3] = path[3], path[2]
+                        elems.append((SU2_TET, path))
+                else:
+                    raise ValueError("kind must be hex, prism or tet")
+    markers = []
+    for axis, name in enumerate("xyz"):
+        for side, tag in ((0, name + "min"), (1, name + "max")):
+            faces = []
+            a, b = [d for d in range(3) if d != axis]
+            for u in range(n):
+                for v in range(n):
+                    def pt(du, dv):
+                        idx = [0, 0, 0]
+                        idx[axis] = side * n
+                        idx[a] = u + du
+                        idx[b] = v + dv
+                        return node(*idx)
+                    q = [pt(0, 0), pt(1, 0), pt(1, 1), pt(0, 1)]
+                    split = kind == "tet" or (kind == "prism" and axis == 2)
+                    if not split:
+                        faces.append((SU2_QUAD, q))
+                    elif kind == "prism":
+                        # the prisms' 100-010 diagonal on the z sides
+                        p00, p10, p11, p01 = q
+                        faces.append((SU2_TRI, [p00, p10, p01]))
+                        faces.append((SU2_TRI, [p10, p11, p01]))
+                    else:
+                        # Kuhn faces split along the diagonal from the low corner
+                        p00, p10, p11, p01 = q
+                        faces.append((SU2_TRI, [p00, p10, p11]))
+                        faces.append((SU2_TRI, [p00, p11, p01]))
+            markers.append((tag, faces))
+    if rng is not None:
+        order = rng.permutation(len(elems))
+        elems = [elems[q] for q in order]
+    return write_su2_text(coords, elems, markers)

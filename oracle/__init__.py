@@ -219,6 +219,38 @@ def assemble(eq, mesh, row_ptr, col, params, state, urf=0.5, fast=False):
 
 # --- reference build (_ref) --------------------------------------------------
 
+def ref_su2_arrays(path: str) -> dict:
+    """The reference's mesh set-up on one SU2 file -- Read_SU2_mesh
+    (cgeometry_supply.h:201-433), CFVM::create_FVM_from_geometry
+    (cfvm.cpp:100-170), CFVM::define_matrix_structure (cfvm.cpp:74-98) --
+    as arrays (ref_su2_driver.cpp).  Its progress lines go to stderr."""
+    import sys
+
+    r = ref()
+    sz = (ctypes.c_int64 * 5)()
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        r.ref_su2_read(str(path).encode(), sz)
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+    N, F, B, nnz, _ = list(sz)
+    spec = [("cell_volume", N, np.float64), ("cell_center", (N, 3), np.float64),
+            ("face_left", F, np.int32), ("face_right", F, np.int32),
+            ("face_area_vector", (F, 3), np.float64), ("face_unit", (F, 3), np.float64),
+            ("face_dist", F, np.float64), ("face_alpha", (F, 3), np.float64), ("face_ksi", F, np.float64),
+            ("face_geo_aver", F, np.float64), ("face_area", F, np.float64),
+            ("bc_left", B, np.int32), ("bc_marker", B, np.int32), ("bc_area_vector", (B, 3), np.float64),
+            ("bc_unit", (B, 3), np.float64), ("bc_dist", B, np.float64), ("bc_alpha", (B, 3), np.float64),
+            ("bc_ksi", B, np.float64), ("bc_perp", B, np.float64), ("bc_area", B, np.float64),
+            ("row_ptr", N + 1, np.uint64), ("col", nnz, np.uint64)]
+    out = {k: np.empty(shape, dt) for k, shape, dt in spec}
+    r.ref_su2_copy(*(out[k].ctypes.data_as(ctypes.c_void_p) for k, _, _ in spec))
+    return out
+
+
 def ref_cg_solve(A, b, x0):
     x = np.array(x0, dtype=np.float64)
     res = ref().ref_cg_solve(*_csr(A), _p(A.val), _p(b), _p(x))

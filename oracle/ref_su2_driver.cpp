@@ -1,0 +1,84 @@
+// ref_su2_driver.cpp -- C entry points into the reference's mesh set-up,
+// compiled with the reference's own sources where they lie (oracle/Makefile).
+// TEST INFRASTRUCTURE: pins libocsl_mesh.so; never shipped, never on the
+// product path.
+//
+// ref_su2_read runs Read_SU2_mesh (cgeometry_supply.h:201-433),
+// CFVM::create_FVM_from_geometry (cfvm.cpp:100-170) and
+// CFVM::define_matrix_structure (cfvm.cpp:74-98) on one file and keeps the
+// result; ref_su2_copy copies its arrays out in the reference's order.
+#include <cstdint>
+#include <cstring>
+#include <memory>
+
+#include "cfvm.h"
+#include "cgeometry_supply.h"
+#include "cmatrix_solver.h"
+#include "croot.h"
+
+namespace {
+std::shared_ptr<CFVM> g_fvm;
+std::shared_ptr<CMatrix_solver> g_ms;
+}  // namespace
+
+extern "C" {
+
+// sizes: n_cells, n_faces, n_bc, nnz, n_markers
+int ref_su2_read(const char* path, int64_t* sizes) {
+    root.geometry = std::make_shared<Cgeometry>(Read_SU2_mesh(path));
+    g_fvm = std::make_shared<CFVM>();
+    root.FVM = g_fvm;
+    g_fvm->create_FVM_from_geometry();
+    g_fvm->define_matrix_structure(g_ms);
+    sizes[0] = (int64_t)g_fvm->cells.size();
+    sizes[1] = (int64_t)g_fvm->faces.size();
+    sizes[2] = (int64_t)g_fvm->BC_faces.size();
+    sizes[3] = (int64_t)g_ms->matrix.column_index.size();
+    sizes[4] = (int64_t)root.geometry->n_boundary;
+    return 0;
+}
+
+static void put3(double* d, size_t i, Svector v) {
+    for (int k = 0; k < 3; ++k) d[3 * i + k] = v[k];
+}
+
+void ref_su2_copy(double* vol, double* center, int32_t* fl, int32_t* fr, double* fS, double* fe, double* fdist,
+                  double* falpha, double* fksi, double* fgeo, double* farea, int32_t* bl, int32_t* bm, double* bS,
+                  double* be, double* bdist, double* balpha, double* bksi, double* bperp, double* barea,
+                  uint64_t* row_ptr, uint64_t* col) {
+    const CFVM& F = *g_fvm;
+    for (size_t i = 0; i < F.cells.size(); ++i) {
+        vol[i] = F.cells[i].volume;
+        put3(center, i, root.geometry->cells[i].center);
+    }
+    for (size_t i = 0; i < F.faces.size(); ++i) {
+        const CFVM::CFVM_face& f = F.faces[i];
+        fl[i] = f.left_cell_index;
+        fr[i] = f.right_cell_index;
+        put3(fS, i, f.area_vector);
+        put3(fe, i, f.cell_center_unit_vector);
+        fdist[i] = f.cell_center_distance;
+        put3(falpha, i, f.alpha);
+        fksi[i] = f.diffusion_ksi_A;
+        fgeo[i] = f.geo_aver;
+        farea[i] = f.area;
+    }
+    for (size_t i = 0; i < F.BC_faces.size(); ++i) {
+        const CFVM::CFVM_BC_face& f = F.BC_faces[i];
+        bl[i] = f.left_cell_index;
+        bm[i] = root.geometry->faces[(size_t)f.global_face_index].boundary_marker;
+        put3(bS, i, f.area_vector);
+        put3(be, i, f.cell_center_unit_vector);
+        bdist[i] = f.cell_center_distance;
+        put3(balpha, i, f.alpha);
+        bksi[i] = f.diffusion_ksi_A;
+        bperp[i] = f.perp_dist;
+        barea[i] = f.area;
+    }
+    const auto& rp = g_ms->matrix.row_ptr;
+    const auto& ci = g_ms->matrix.column_index;
+    for (size_t i = 0; i < rp.size(); ++i) row_ptr[i] = rp[i];
+    for (size_t i = 0; i < ci.size(); ++i) col[i] = ci[i];
+}
+
+}  // extern "C"

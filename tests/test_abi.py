@@ -17,7 +17,7 @@ def _declared(header):
 
 
 @pytest.mark.parametrize("header,lib", [("ocsl.h", "libocsl_hip.so"), ("ocsl_flow.h", "libocsl_hip.so"),
-                                        ("ocsl_gen.h", "libocsl_gen.so")])
+                                        ("ocsl_gen.h", "libocsl_gen.so"), ("ocsl_mesh.h", "libocsl_mesh.so")])
 def test_exports(header, lib):
     path = os.path.join(PKG, lib)
     assert os.path.exists(path), f"{lib} not built"

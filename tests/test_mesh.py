@@ -1,0 +1,136 @@
+"""SU2 reader and finite-volume geometry (libocsl_mesh.so) against the
+reference's own outputs (fixtures from the reference build, and the build
+itself when present), the reference's geometry unit test values
+(test_geo_calculations, cgeometry_supply.h:81-196) and the divergence
+theorem."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from opencslsolver_amd import mesh
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+INT_KEYS = ("face_left", "face_right", "bc_left", "bc_marker")
+FLOAT_KEYS = ("cell_volume", "cell_center", "face_area_vector", "face_unit", "face_dist", "face_alpha",
+              "face_ksi", "face_geo_aver", "face_area", "bc_area_vector", "bc_unit", "bc_dist", "bc_alpha",
+              "bc_ksi", "bc_perp", "bc_area")
+
+
+def _same(mine, ref):
+    """Integer arrays (face order, owners, markers) exactly; values to 1e-13
+    of each array's largest magnitude (the reference build uses -ffast-math,
+    and components that are zero on paper carry ~1e-16 rounding)."""
+    for k in INT_KEYS:
+        assert np.array_equal(mine[k], ref[k]), k
+    for k in FLOAT_KEYS:
+        scale = np.abs(ref[k]).max() if ref[k].size else 1.0
+        assert np.allclose(mine[k], ref[k], rtol=1e-13, atol=1e-13 * scale), k
+
+
+@pytest.mark.parametrize("name", ["hex3_s5", "prism2_s7", "tet2_s9"])
+def test_golden_reference_arrays(name):
+    d = np.load(os.path.join(GOLDEN, f"su2_{name}.npz"))
+    m = mesh.parse_su2(d["su2_text"].tobytes())
+    _same(m, d)
+    assert m["tags"] == ["xmin", "xmax", "ymin", "ymax", "zmin", "zmax"]
+    # the matrix pattern CFVM::define_matrix_structure builds from this mesh
+    rp, col = oracle.define_matrix_structure(m["cell_volume"].shape[0], m["face_left"], m["face_right"],
+                                             m["bc_left"])
+    assert np.array_equal(rp, d["row_ptr"]) and np.array_equal(col, d["col"])
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("kind,n,seed", [("hex", 5, 1), ("hex", 4, None), ("prism", 4, 2), ("tet", 3, 3)])
+def test_against_reference_build(kind, n, seed, tmp_path):
+    path = str(tmp_path / "box.su2")
+    with open(path, "w") as f:
+        f.write(mesh.box_su2(n, kind, h=0.1, seed=seed))
+    _same(mesh.read_su2(path), oracle.ref_su2_arrays(path))
+
+
+def _one_cell(code, nodes, coords, faces):
+    return mesh.write_su2_text(coords, [(code, nodes)], [("all", faces)])
+
+
+CUBE = [(0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (1.0, 1.0, 0.0),
+        (0.0, 0.0, 1.0), (1.0, 0.0, 1.0), (0.0, 1.0, 1.0), (1.0, 1.0, 1.0)]
+
+
+def _bc_center(m, q):
+    return m["cell_center"][m["bc_left"][q]] + m["bc_dist"][q] * m["bc_unit"][q]
+
+
+def test_reference_unit_test_values():
+    """test_geo_calculations' cells and faces: tetra 0-1-2-4, prism 0-1-2-4-5-6
+    and hexahedron 0..7 of the unit cube; triangle 0-1-2 and quadrilateral
+    0-1-3-2 (1e-15, as there)."""
+    tet = mesh.parse_su2(_one_cell(10, [0, 1, 2, 4], CUBE, [(5, [0, 1, 2]), (5, [0, 1, 4]), (5, [1, 2, 4]),
+                                                             (5, [0, 2, 4])]))
+    assert abs(tet["cell_volume"][0] - 1.0 / 6.0) < 1e-15
+    assert np.all(np.abs(tet["cell_center"][0] - 0.25) < 1e-15)
+    assert abs(tet["bc_area"][0] - 0.5) < 1e-15
+    assert np.all(np.abs(_bc_center(tet, 0) - [1.0 / 3.0, 1.0 / 3.0, 0.0]) < 1e-15)
+    prism = mesh.parse_su2(_one_cell(13, [0, 1, 2, 4, 5, 6], CUBE,
+                                     [(5, [0, 1, 2]), (5, [4, 5, 6]), (9, [0, 1, 5, 4]), (9, [1, 2, 6, 5]),
+                                      (9, [2, 0, 4, 6])]))
+    assert abs(prism["cell_volume"][0] - 0.5) < 1e-15
+    assert np.all(np.abs(prism["cell_center"][0] - [1.0 / 3.0, 1.0 / 3.0, 0.5]) < 1e-15)
+    hexa = mesh.parse_su2(_one_cell(12, [0, 1, 3, 2, 4, 5, 7, 6], CUBE,
+                                    [(9, [0, 1, 3, 2]), (9, [4, 5, 7, 6]), (9, [0, 1, 5, 4]), (9, [2, 3, 7, 6]),
+                                     (9, [0, 2, 6, 4]), (9, [1, 3, 7, 5])]))
+    assert abs(hexa["cell_volume"][0] - 1.0) < 1e-15
+    assert np.all(np.abs(hexa["cell_center"][0] - 0.5) < 1e-15)
+    assert abs(hexa["bc_area"][0] - 1.0) < 1e-15
+    assert np.all(np.abs(_bc_center(hexa, 0) - [0.5, 0.5, 0.0]) < 1e-15)
+    for m in (tet, prism, hexa):
+        assert m["face_left"].shape == (0,) and np.all(m["bc_left"] == 0)
+
+
+@pytest.mark.parametrize("kind,seed", [("hex", 2), ("prism", 3), ("tet", 4)])
+def test_closed_cells_and_orientation(kind, seed):
+    """Every cell's outward area vectors sum to zero (divergence theorem), the
+    volumes fill the box, interior area vectors point from the left cell to
+    the right one, and each boundary face's vector points out of the box."""
+    n, h = 3, 0.5
+    m = mesh.parse_su2(mesh.box_su2(n, kind, h=h, seed=seed))
+    N = m["cell_volume"].shape[0]
+    tot = np.zeros((N, 3))
+    np.add.at(tot, m["face_left"], m["face_area_vector"])
+    np.add.at(tot, m["face_right"], -m["face_area_vector"])
+    np.add.at(tot, m["bc_left"], m["bc_area_vector"])
+    assert np.abs(tot).max() < 1e-15
+    assert np.all(m["cell_volume"] > 0) and abs(m["cell_volume"].sum() - (n * h) ** 3) < 1e-13
+    assert np.all(np.einsum("ij,ij->i", m["face_area_vector"], m["face_unit"]) > 0)
+    c = m["cell_center"][m["bc_left"]] + m["bc_dist"][:, None] * m["bc_unit"]
+    assert np.all(np.einsum("ij,ij->i", m["bc_area_vector"], c - 0.5 * n * h) > 0)
+    assert np.allclose(m["face_ksi"], m["face_area"] ** 2 / np.einsum("ij,ij->i", m["face_area_vector"],
+                                                                        m["face_dist"][:, None] * m["face_unit"]))
+
+
+def test_file_and_memory_agree(tmp_path):
+    text = mesh.box_su2(3, "prism", h=0.2, seed=11)
+    p = tmp_path / "m.su2"
+    p.write_text(text)
+    a, b = mesh.read_su2(str(p)), mesh.parse_su2(text)
+    for k in INT_KEYS + FLOAT_KEYS:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("text,needle", [
+    ("NDIME= 3\nNELEM= 1\n14 0 1 2 3 4 0\n", "unsupported element type 14"),
+    ("NDIME= 3\nNELEM= 1\n10 0 1 2 3 0\n", "NPOIN"),
+    ("NDIME= 3\nNELEM= 1\n10 0 1 2 9 0\nNPOIN= 4\n0 0 0 0\n1 0 0 1\n0 1 0 2\n0 0 1 3\nNMARK= 0\n", "out of range"),
+    ("NDIME= 3\nNELEM= 1\n10 0 1 2 3 0\nNPOIN= 5\n0 0 0 0\n1 0 0 1\n0 1 0 2\n0 0 1 3\n1 1 1 4\n"
+     "NMARK= 1\nMARKER_TAG= w\nMARKER_ELEMS= 1\n5 0 1 4\n", "share no cell"),
+    ("NDIME= 3\nNELEM= 1\n10 0 1 x 3 0\n", "integer"),
+])
+def test_malformed_input_is_rejected(text, needle):
+    with pytest.raises(mesh.MeshError, match=needle):
+        mesh.parse_su2(text)
+
+
+def test_missing_file():
+    with pytest.raises(mesh.MeshError, match="cannot open"):
+        mesh.read_su2("/nonexistent/mesh.su2")
