@@ -67,6 +67,26 @@ int ocsl_mesh_faces(ocsl_mesh m, int32_t* left, int32_t* right, double* area_vec
 int ocsl_mesh_bc_faces(ocsl_mesh m, int32_t* left, int32_t* marker, double* area_vector, double* unit,
                        double* dist, double* alpha, double* ksi, double* perp, double* area);
 
+/* Cell fields for ocsl_mesh_write_vtk, one value (vectors: three) per cell;
+ * a NULL array is left out of the file. */
+typedef struct ocsl_vtk_fields {
+    const double* density;
+    const double* pressure;
+    const double* temperature;
+    const double* viscosity;
+    const double* mach;
+    const double* velocity;      /* [n][3] */
+    const double* pressure_grad; /* [n][3] */
+} ocsl_vtk_fields;
+
+/* Legacy-VTK unstructured grid of the mesh: nodes, cells (VTK node order and
+ * types 10 / 13 / 12), then cell data Volume[m3], Center and the given fields
+ * with the reference's names and order.  Numbers are written %.{precision}g:
+ * 6 reproduces the reference's stream output, 17 round-trips every double.
+ * Reference: Cpost_processing::write_VTK_file / write_VTK_data
+ * (cpost_processing.cpp:26-127, 260-333). */
+int ocsl_mesh_write_vtk(ocsl_mesh m, const char* path, const ocsl_vtk_fields* fields, int precision);
+
 #ifdef __cplusplus
 }
 #endif

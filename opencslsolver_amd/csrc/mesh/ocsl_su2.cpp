@@ -577,4 +577,66 @@ int ocsl_mesh_bc_faces(ocsl_mesh m, int32_t* left, int32_t* marker, double* area
     return 0;
 }
 
+// Cpost_processing::write_VTK_file (cpost_processing.cpp:260-333) and
+// write_VTK_data (:26-127): VTK node order per cell type (prism 0 2 1 3 5 4,
+// hexahedron back to the file's order), then the cell data.
+int ocsl_mesh_write_vtk(ocsl_mesh m, const char* path, const ocsl_vtk_fields* fields, int precision) {
+    if (!m || !path || precision < 1 || precision > 17) {
+        g_err = "bad argument";
+        return -1;
+    }
+    FILE* f = std::fopen(path, "w");
+    if (!f) {
+        g_err = std::string("cannot open ") + path + " for writing";
+        return -2;
+    }
+    const int p = precision;
+    const size_t n = m->cells.size();
+    std::fprintf(f, "# vtk DataFile Version 1.0\nSolver Data File\nASCII\nDATASET UNSTRUCTURED_GRID\nPOINTS %zu double\n",
+                 m->nodes.size());
+    for (const V3& x : m->nodes) std::fprintf(f, "%.*g %.*g %.*g\n", p, x.v[0], p, x.v[1], p, x.v[2]);
+    size_t list = 0;
+    for (const Cell& c : m->cells) list += 1 + (size_t)kNodes[c.type];
+    std::fprintf(f, "CELLS %zu %zu\n", n, list);
+    static const int kVtkOrder[3][8] = {{0, 1, 2, 3}, {0, 2, 1, 3, 5, 4}, {0, 1, 3, 2, 4, 5, 7, 6}};
+    for (const Cell& c : m->cells) {
+        std::fprintf(f, "%d", kNodes[c.type]);
+        for (int k = 0; k < kNodes[c.type]; ++k) std::fprintf(f, " %d", c.node[kVtkOrder[c.type][k]]);
+        std::fputc('\n', f);
+    }
+    static const int kVtkType[3] = {10, 13, 12};
+    std::fprintf(f, "CELL_TYPES %zu\n", n);
+    for (const Cell& c : m->cells) std::fprintf(f, "%d\n", kVtkType[c.type]);
+    std::fprintf(f, "CELL_DATA %zu\n", n);
+    auto scalars = [&](const char* name, const double* v) {
+        if (!v) return;
+        std::fprintf(f, "SCALARS %s double 1\nLOOKUP_TABLE default\n", name);
+        for (size_t c = 0; c < n; ++c) std::fprintf(f, "%.*g\n", p, v[c]);
+    };
+    auto vectors = [&](const char* name, const double* v) {
+        if (!v) return;
+        std::fprintf(f, "VECTORS %s double\n", name);
+        for (size_t c = 0; c < n; ++c) std::fprintf(f, "%.*g %.*g %.*g\n", p, v[3 * c], p, v[3 * c + 1], p, v[3 * c + 2]);
+    };
+    scalars("Volume[m3]", m->vol.data());
+    std::vector<double> ctr(3 * n);
+    for (size_t c = 0; c < n; ++c) put3(ctr.data(), c, m->center[c]);
+    vectors("Center", ctr.data());
+    if (fields) {
+        scalars("Density[Kg/m^3]", fields->density);
+        scalars("Pressure[Pa]", fields->pressure);
+        scalars("Temperature[K]", fields->temperature);
+        scalars("viscosity", fields->viscosity);
+        scalars("MACH", fields->mach);
+        vectors("Velocity[m/s]", fields->velocity);
+        vectors("Pres_grad", fields->pressure_grad);
+    }
+    const bool ok = !std::ferror(f);
+    if (std::fclose(f) != 0 || !ok) {
+        g_err = std::string("write to ") + path + " failed";
+        return -2;
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -87,6 +87,60 @@ def parse_su2(text: str | bytes) -> dict:
     return _finish(lib, st, h)
 
 
+# ------------------------------------------------------------------ VTK output
+VTK_FIELDS = ("density", "pressure", "temperature", "viscosity", "mach", "velocity", "pressure_grad")
+
+
+class _VtkFields(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in VTK_FIELDS]
+
+
+def vtk_fields(state: dict, sutherland) -> dict:
+    """The reference's output fields from a flow state (``Flow.get_state()``):
+    density, pressure, temperature, Sutherland viscosity
+    s0 T^1.5 / (T + s1) (Cfluid_properties::get_viscosity,
+    cfluid_properties.cpp:123-128), Mach, velocity, pressure gradient."""
+    T = np.asarray(state["T"], np.float64)
+    mu = sutherland[0] * np.power(T, 1.5)
+    mu = mu / (T + sutherland[1])
+    return {"density": state["rho"], "pressure": state["p"], "temperature": T, "viscosity": mu,
+            "mach": state["mach"], "velocity": state["u"], "pressure_grad": state["grad_p"]}
+
+
+def write_vtk(su2, path: str, fields: dict | None = None, precision: int = 6) -> None:
+    """Writes the mesh of `su2` (SU2 text, or a path to an SU2 file) with cell
+    fields as a legacy-VTK file, as Cpost_processing::write_VTK_file does
+    (cpost_processing.cpp:26-127, 260-333): fields by the keys of VTK_FIELDS
+    (velocity and pressure_grad [n, 3]); precision 6 gives the reference's
+    text, 17 round-trips every value."""
+    lib = mesh_lib()
+    h = ctypes.c_void_p()
+    if isinstance(su2, (bytes, str)) and ("\n" in (su2 if isinstance(su2, str) else su2.decode(errors="ignore"))):
+        data = su2.encode() if isinstance(su2, str) else bytes(su2)
+        st = lib.ocsl_mesh_parse_su2(data, len(data), ctypes.byref(h))
+    else:
+        st = lib.ocsl_mesh_read_su2(str(su2).encode(), ctypes.byref(h))
+    if st != 0:
+        raise MeshError(f"SU2 mesh: {lib.ocsl_mesh_last_error().decode()} (status {st})")
+    try:
+        n = ctypes.c_int64()
+        lib.ocsl_mesh_sizes(h, None, ctypes.byref(n), None, None, None)
+        keep, f = [], _VtkFields()
+        for k, v in (fields or {}).items():
+            if k not in VTK_FIELDS:
+                raise ValueError(f"unknown VTK field {k!r}")
+            a = np.ascontiguousarray(v, dtype=np.float64)
+            if a.size != n.value * (3 if k in ("velocity", "pressure_grad") else 1):
+                raise ValueError(f"field {k!r} has {a.size} values for {n.value} cells")
+            keep.append(a)
+            setattr(f, k, a.ctypes.data)
+        st = lib.ocsl_mesh_write_vtk(h, str(path).encode(), ctypes.byref(f) if fields else None, int(precision))
+        if st != 0:
+            raise MeshError(f"VTK: {lib.ocsl_mesh_last_error().decode()} (status {st})")
+    finally:
+        lib.ocsl_mesh_destroy(h)
+
+
 # ------------------------------------------------------------------ writer
 SU2_TET, SU2_HEX, SU2_PRISM, SU2_TRI, SU2_QUAD = 10, 12, 13, 5, 9
 

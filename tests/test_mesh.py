@@ -134,3 +134,85 @@ def test_malformed_input_is_rejected(text, needle):
 def test_missing_file():
     with pytest.raises(mesh.MeshError, match="cannot open"):
         mesh.read_su2("/nonexistent/mesh.su2")
+
+
+def _read_vtk(path):
+    """A minimal legacy-VTK reader: points, cells, types, and the cell data
+    sections in file order."""
+    tok = open(path).read().split("\n")
+    i = tok.index(next(t for t in tok if t.startswith("POINTS")))
+    n_pts = int(tok[i].split()[1])
+    pts = np.array([[float(x) for x in tok[i + 1 + k].split()] for k in range(n_pts)])
+    i += 1 + n_pts
+    n_cells = int(tok[i].split()[1])
+    cells = [[int(x) for x in tok[i + 1 + k].split()][1:] for k in range(n_cells)]
+    i += 1 + n_cells
+    types = [int(tok[i + 1 + k]) for k in range(n_cells)]
+    i += 1 + n_cells
+    assert tok[i] == f"CELL_DATA {n_cells}"
+    i += 1
+    data = {}
+    while i < len(tok) and tok[i]:
+        head = tok[i].split()
+        if head[0] == "SCALARS":
+            data[head[1]] = np.array([float(tok[i + 2 + k]) for k in range(n_cells)])
+            i += 2 + n_cells
+        else:
+            data[head[1]] = np.array([[float(x) for x in tok[i + 1 + k].split()] for k in range(n_cells)])
+            i += 1 + n_cells
+    return pts, cells, types, data
+
+
+# the reference's cell-data sections and their order (cpost_processing.cpp:26-127)
+VTK_NAMES = ["Volume[m3]", "Center", "Density[Kg/m^3]", "Pressure[Pa]", "Temperature[K]", "viscosity", "MACH",
+             "Velocity[m/s]", "Pres_grad"]
+
+
+@pytest.mark.parametrize("kind", ["hex", "prism", "tet"])
+def test_vtk_round_trip_and_vtk_node_order(kind, tmp_path):
+    """precision 17 returns every value; the cells follow VTK's node order
+    (hexahedron: base 0-3 with its right-hand normal towards 4-7; wedge: base
+    0-1-2 with its normal away from 3-4-5; tetrahedron positively oriented)."""
+    text = mesh.box_su2(2, kind, h=0.3, seed=17)
+    m = mesh.parse_su2(text)
+    n = m["cell_volume"].shape[0]
+    rng = np.random.default_rng(5)
+    fields = {k: rng.normal(size=(n, 3) if k in ("velocity", "pressure_grad") else n) for k in mesh.VTK_FIELDS}
+    path = str(tmp_path / "out.vtk")
+    mesh.write_vtk(text, path, fields, precision=17)
+    pts, cells, types, data = _read_vtk(path)
+    assert list(data) == VTK_NAMES
+    assert np.array_equal(data["Volume[m3]"], m["cell_volume"]) and np.array_equal(data["Center"], m["cell_center"])
+    for k, name in zip(mesh.VTK_FIELDS, VTK_NAMES[2:]):
+        assert np.array_equal(data[name], fields[k]), k
+    code = {"hex": 12, "prism": 13, "tet": 10}[kind]
+    assert set(types) == {code}
+    for c in cells:
+        p = pts[c]
+        if kind == "tet":
+            assert np.dot(p[1] - p[0], np.cross(p[2] - p[0], p[3] - p[0])) > 0
+        elif kind == "prism":
+            nb = np.cross(p[1] - p[0], p[2] - p[0])
+            assert np.dot(nb, p[3:].mean(0) - p[:3].mean(0)) < 0
+        else:
+            nb = np.cross(p[1] - p[0], p[3] - p[0])
+            assert np.dot(nb, p[4:].mean(0) - p[:4].mean(0)) > 0
+    # volume by cell matches the mesh's own
+    assert abs(data["Volume[m3]"].sum() - (2 * 0.3) ** 3) < 1e-13
+
+
+def test_vtk_reference_precision_and_errors(tmp_path):
+    """precision 6 is the reference's default stream output (%g, 6 significant
+    digits); unknown or mis-sized fields and bad paths are rejected."""
+    text = mesh.box_su2(1, "hex", h=1.0 / 3.0)
+    path = str(tmp_path / "r.vtk")
+    mesh.write_vtk(text, path, {"pressure": [101325.123456789]}, precision=6)
+    body = open(path).read()
+    assert "\n0.333333 0 0\n" in body and "\n101325\n" in body and "\n0.037037\n" in body
+    assert body.startswith("# vtk DataFile Version 1.0\nSolver Data File\nASCII\nDATASET UNSTRUCTURED_GRID\n")
+    with pytest.raises(ValueError):
+        mesh.write_vtk(text, path, {"pressure": [1.0, 2.0]})
+    with pytest.raises(ValueError):
+        mesh.write_vtk(text, path, {"speed": [1.0]})
+    with pytest.raises(mesh.MeshError):
+        mesh.write_vtk(text, "/nonexistent/dir/x.vtk")
