@@ -14,6 +14,8 @@
 #include "cfvm.h"
 #include "cgeometry_supply.h"
 #include "cmatrix_solver.h"
+#include "cflow_solver.h"
+#include "cfluid_properties.h"
 #include "croot.h"
 
 namespace {
@@ -79,6 +81,42 @@ void ref_su2_copy(double* vol, double* center, int32_t* fl, int32_t* fr, double*
     const auto& ci = g_ms->matrix.column_index;
     for (size_t i = 0; i < rp.size(); ++i) row_ptr[i] = rp[i];
     for (size_t i = 0; i < ci.size(); ++i) col[i] = ci[i];
+}
+
+// After a reference run (ref_sod_run): its mesh as nodes and SU2-ordered
+// cells (the inverse of the reader's hexahedron reordering,
+// cgeometry_supply.h:266-276), and the cell fields
+// Cpost_processing::write_VTK_data writes (cpost_processing.cpp:26-127),
+// viscosity from Cfluid_properties::get_viscosity.
+void ref_last_run_sizes(int64_t* n_nodes, int64_t* n_cells) {
+    *n_nodes = (int64_t)root.geometry->nodes.size();
+    *n_cells = (int64_t)root.geometry->cells.size();
+}
+
+void ref_last_run_dump(double* nodes, int32_t* cell_type, int32_t* cell_nodes, double* rho, double* p, double* T,
+                       double* mu, double* mach, double* u, double* gp) {
+    const Cgeometry& g = *root.geometry;
+    for (size_t i = 0; i < g.nodes.size(); ++i) put3(nodes, i, g.nodes[i]);
+    const int hex_su2[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    for (size_t c = 0; c < g.cells.size(); ++c) {
+        const auto& cell = g.cells[c];
+        const int code = cell.ecell_type == enums::Ecell_type::tetra ? 10
+                         : cell.ecell_type == enums::Ecell_type::wedge ? 13 : 12;
+        cell_type[c] = code;
+        for (int k = 0; k < 8; ++k) cell_nodes[8 * c + k] = -1;
+        for (size_t k = 0; k < cell.i_nodes.size(); ++k)
+            cell_nodes[8 * c + k] = cell.i_nodes[code == 12 ? hex_su2[k] : k];
+    }
+    const auto& cells = root.Flow_solver->cells;
+    for (size_t c = 0; c < cells.size(); ++c) {
+        rho[c] = cells[c].Density;
+        p[c] = cells[c].Pressure;
+        T[c] = cells[c].Temperature;
+        mu[c] = root.Fluid_properties->get_viscosity(cells[c].Temperature);
+        mach[c] = cells[c].Mach;
+        put3(u, c, cells[c].Velocity);
+        put3(gp, c, cells[c].Pressure_grad);
+    }
 }
 
 }  // extern "C"

@@ -216,3 +216,15 @@ def test_vtk_reference_precision_and_errors(tmp_path):
         mesh.write_vtk(text, path, {"speed": [1.0]})
     with pytest.raises(mesh.MeshError):
         mesh.write_vtk(text, "/nonexistent/dir/x.vtk")
+
+
+def test_vtk_text_equals_the_reference_file(tmp_path):
+    """The reference's own solution.vtk after a 40-cell Sod run (fixture from
+    the reference build) against ocsl_mesh_write_vtk on the same mesh and
+    cell fields at precision 6: the same text, byte for byte."""
+    d = np.load(os.path.join(GOLDEN, "vtk_sod40.npz"))
+    elems = [(int(t), [int(v) for v in ns if v >= 0]) for t, ns in zip(d["cell_type"], d["cell_nodes"])]
+    text = mesh.write_su2_text(d["nodes"], elems, [])
+    path = str(tmp_path / "mine.vtk")
+    mesh.write_vtk(text, path, {k: d[k] for k in mesh.VTK_FIELDS}, precision=6)
+    assert open(path, "rb").read() == d["vtk_text"].tobytes()
