@@ -125,6 +125,8 @@ struct Text {
         }
         return false;
     }
+    // the token copied into a scratch buffer ended there, not at its size
+    bool at_end(size_t n) const { return p + n >= end || std::strchr(" \t\n\r\v\f", p[n]); }
     void skip_ws() {
         while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r' || *p == '\v' || *p == '\f')) ++p;
     }
@@ -136,7 +138,7 @@ struct Text {
         buf[n] = 0;
         char* q = nullptr;
         const long long v = std::strtoll(buf, &q, 10);
-        if (n == 0 || q != buf + n) fail(-3, std::string("expected an integer (") + what + ")");
+        if (n == 0 || q != buf + n || !at_end(n)) fail(-3, std::string("expected an integer (") + what + ")");
         p += n;
         return v;
     }
@@ -148,7 +150,7 @@ struct Text {
         buf[n] = 0;
         char* q = nullptr;
         const double v = std::strtod(buf, &q);
-        if (n == 0 || q != buf + n) fail(-3, std::string("expected a number (") + what + ")");
+        if (n == 0 || q != buf + n || !at_end(n)) fail(-3, std::string("expected a number (") + what + ")");
         p += n;
         return v;
     }

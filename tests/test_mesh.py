@@ -125,6 +125,7 @@ def test_file_and_memory_agree(tmp_path):
     ("NDIME= 3\nNELEM= 1\n10 0 1 2 3 0\nNPOIN= 5\n0 0 0 0\n1 0 0 1\n0 1 0 2\n0 0 1 3\n1 1 1 4\n"
      "NMARK= 1\nMARKER_TAG= w\nMARKER_ELEMS= 1\n5 0 1 4\n", "share no cell"),
     ("NDIME= 3\nNELEM= 1\n10 0 1 x 3 0\n", "integer"),
+    ("NDIME= 3\nNELEM= 1\n10 0 1 " + "2" * 80 + " 3 0\n", "integer"),
 ])
 def test_malformed_input_is_rejected(text, needle):
     with pytest.raises(mesh.MeshError, match=needle):
