@@ -50,13 +50,20 @@ def lib(fast: bool = False) -> ctypes.CDLL:
     return _lib("liboracle_fast.so" if fast else "liboracle.so")
 
 
-def ref_available() -> bool:
-    return os.path.exists(os.path.join(_HERE, "_ref", "libocsl_ref.so"))
+def _ref_path(variant: str = "") -> str:
+    """The reference build: "" its own flags (make -C oracle ref), "ieee"
+    without contraction or fast-math (make -C oracle ref-ieee)."""
+    return os.path.join(_HERE, "_ref", variant, "libocsl_ref.so") if variant else \
+        os.path.join(_HERE, "_ref", "libocsl_ref.so")
+
+
+def ref_available(variant: str = "") -> bool:
+    return os.path.exists(_ref_path(variant))
 
 
 @functools.lru_cache(maxsize=None)
-def ref() -> ctypes.CDLL:
-    path = os.path.join(_HERE, "_ref", "libocsl_ref.so")
+def ref(variant: str = "") -> ctypes.CDLL:
+    path = _ref_path(variant)
     if not os.path.exists(path):
         raise RuntimeError(f"{path} missing: run make -C oracle ref (needs the reference sources)")
     r = ctypes.CDLL(path)
@@ -251,15 +258,15 @@ def ref_su2_arrays(path: str) -> dict:
     return out
 
 
-def ref_cg_solve(A, b, x0):
+def ref_cg_solve(A, b, x0, variant=""):
     x = np.array(x0, dtype=np.float64)
-    res = ref().ref_cg_solve(*_csr(A), _p(A.val), _p(b), _p(x))
+    res = ref(variant).ref_cg_solve(*_csr(A), _p(A.val), _p(b), _p(x))
     return res, x
 
 
-def ref_ilu_bicgstab_solve(A, b, x0):
+def ref_ilu_bicgstab_solve(A, b, x0, variant=""):
     x = np.array(x0, dtype=np.float64)
-    res = ref().ref_ilu_bicgstab_solve(*_csr(A), _p(A.val), _p(b), _p(x))
+    res = ref(variant).ref_ilu_bicgstab_solve(*_csr(A), _p(A.val), _p(b), _p(x))
     return res, x
 
 

@@ -143,3 +143,32 @@ def test_batch_mixed_replacement_independent_of_check_every(check_every):
         assert np.linalg.norm(b - a.matvec(xs[k])) / np.linalg.norm(b) <= 1.05e-8
         assert reps[k]["replacements"] >= 1
         assert abs(reps[k]["iterations"] - ref[k]["iterations"]) <= 3, (k, reps[k], ref[k])
+
+
+def test_batch_full_plan_64_systems():
+    """The full uneven batch: 64 pressure systems of n^3 cells, n log-uniform
+    in [22, 80] from seed 7 (about 7.5 M cells), MIXED CG to 1e-8 in one
+    batched solve.  Every system's FP64 relative residual is recomputed by a
+    host SpMV; the smallest, the median and the largest systems are compared
+    with the oracle's FP64 solutions."""
+    import parity
+
+    oracle.set_threads(8)
+    ns, sm, sr = gen.batch_plan(7, 64, 22, 80)
+    assert len(ns) == 64 and min(ns) >= 22 and max(ns) <= 80
+    sysl = [gen.poisson3d(int(ns[k]), seed_mesh=int(sm[k]), seed_rhs=int(sr[k]))[:2] for k in range(64)]
+    with Batch([(a.row_ptr, a.col) for a, _ in sysl]) as bt:
+        bt.set_values([a.val for a, _ in sysl])
+        bt.set_rhs([b for _, b in sysl])
+        xs, reps, lat = bt.solve("mixed", tol=1e-8)
+    worst = 0.0
+    for k, (a, b) in enumerate(sysl):
+        assert reps[k]["status"] == 0, (k, reps[k])
+        worst = max(worst, float(np.linalg.norm(b - a.matvec(xs[k])) / np.linalg.norm(b)))
+    parity.check("batch64 worst host relres", worst, 1.05e-8)
+    order = np.argsort([a.n for a, _ in sysl])
+    for k in (order[0], order[32], order[-1]):
+        a, b = sysl[k]
+        _, x_o, _, _ = oracle.cg_tol(a, b, tol=1e-10)
+        err = float(np.linalg.norm(xs[k] - x_o) / np.linalg.norm(x_o))
+        parity.check(f"batch64 system {k} (n={a.n}) error vs oracle fp64", err, 1e-5)

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
+import parity
 from opencslsolver_amd import gen
 from opencslsolver_amd.gen import CSR
 from opencslsolver_amd.solver import Solver
@@ -36,6 +37,10 @@ def _one_thread():
 
 @pytest.mark.parametrize("path", SOLVER_FIXTURES, ids=lambda p: os.path.basename(p))
 def test_reference_mode_matches_reference(path):
+    """REFERENCE ILU(0)-BiCGSTAB against the reference's own results: factors,
+    returned residual, solution, and the per-iteration residuals against the
+    oracle's restatement."""
+    name = os.path.basename(path)[:-4]
     d = np.load(path)
     a = CSR(d["row_ptr"], d["col"], d["val"])
     with _solver(a, d["b"], d["x0"]) as s:
@@ -43,12 +48,12 @@ def test_reference_mode_matches_reference(path):
         res = s.ILU_BICGSTAB_solve()
         x = s.get_x()
         hist = s.history()
-    assert _rel(m, d["ilu_m"]) <= 1e-12
-    assert res == pytest.approx(float(d["bicg_res"]), rel=1e-8, abs=1e-28)
-    assert _rel(x, d["bicg_x"]) <= 1e-9
+    parity.check(f"{name} ilu_m", parity.rel(m, d["ilu_m"]), 1e-12)
+    parity.check(f"{name} bicg residual", parity.res_rel(res, d["bicg_res"]), parity.bar("solver", name, "bicg_res"))
+    parity.check(f"{name} bicg x", parity.rel(x, d["bicg_x"]), parity.bar("solver", name, "bicg_x"))
     _, _, h_or = oracle.ilu_bicgstab_reference(a, d["b"], d["x0"])
-    assert len(hist) == len(h_or)
-    assert np.allclose(hist, h_or, rtol=1e-8, atol=1e-28)
+    parity.check(f"{name} bicg history vs oracle", parity.hist_rel(hist, h_or, floor=parity.RES_FLOOR),
+                 parity.FLOOR_BAR)
 
 
 @pytest.mark.parametrize("tag", ["bicgstab", "split"])
@@ -58,6 +63,7 @@ def test_reference_mode_on_sod_systems(tag):
     d = np.load(os.path.join(GOLDEN, f"sod_{tag}_2steps.npz"))
     rp, col = d["row_ptr"], d["col"]
     checked = 0
+    worst_res = worst_x = 0.0
     for k in range(d["dump_val"].shape[0]):
         if d["dump_method"][k] != 1:
             continue
@@ -65,10 +71,12 @@ def test_reference_mode_on_sod_systems(tag):
         with _solver(a, d["dump_b"][k], d["dump_x0"][k]) as s:
             res = s.ILU_BICGSTAB_solve()
             x = s.get_x()
-        assert res == pytest.approx(d["res_hist"][k], rel=1e-8, abs=1e-28), k
-        assert _rel(x, d["dump_x"][k]) <= 1e-9, k
+        worst_res = max(worst_res, parity.res_rel(res, d["res_hist"][k]))
+        worst_x = max(worst_x, parity.rel(x, d["dump_x"][k]))
         checked += 1
     assert checked >= 4
+    parity.check(f"sod {tag} systems residual", worst_res, parity.bar("sod_systems", tag, "res"))
+    parity.check(f"sod {tag} systems x", worst_x, parity.bar("sod_systems", tag, "x"))
 
 
 @pytest.mark.parametrize("sweeps", [1, 2, 5])
@@ -132,3 +140,35 @@ def test_three_rhs_equal_single_solves(precision):
         rel = np.linalg.norm(B[k] - a.matvec(X[k])) / np.linalg.norm(B[k])
         assert rel <= 1e-10
         assert _rel(X[k], singles[k]) <= (1e-12 if precision == "fp64" else 1e-9)
+
+
+def test_convdiff_96cubed_three_rhs_against_oracle():
+    """The 96^3 Taylor-Green convection-diffusion system (cells permuted by
+    seed 3) with three right-hand sides (seeds 4-6) sharing A: ILU(0)-
+    BiCGSTAB to 1e-10 in one multi-RHS solve, FP64 and MIXED (FP32 inner
+    iterations, FP64 refinement).  Every solution's relative residual is
+    recomputed by a host SpMV; each is compared with the oracle's FP64
+    solution of the same right-hand side, and each MIXED solution with the
+    FP64 one (errors printed per right-hand side)."""
+    oracle.set_threads(8)
+    a, _ = gen.convdiff3d(96, seed_perm=3)
+    assert (a.n, a.nnz) == (884736, 6137856)
+    B = np.stack([gen.uniform(sd, a.n) for sd in (4, 5, 6)])
+    ref = [oracle.bicgstab_tol(a, B[k], tol=1e-10) for k in range(3)]
+    X = {}
+    with Solver(a.row_ptr, a.col) as s:
+        s.set_values(a.val)
+        for prec in ("fp64", "mixed"):
+            X[prec], reps = s.solve_multi(B, precision=prec, tol=1e-10)
+            for k in range(3):
+                assert reps[k]["status"] == 0, (prec, k, reps[k])
+                relres = np.linalg.norm(B[k] - a.matvec(X[prec][k])) / np.linalg.norm(B[k])
+                parity.check(f"convdiff96 {prec} rhs{k} host relres", relres, 1e-10)
+    for k in range(3):
+        it_o, x_o, _, _ = ref[k]
+        parity.check(f"convdiff96 fp64 rhs{k} vs oracle fp64", parity.rel(X["fp64"][k], x_o), 1e-10)
+        err = np.linalg.norm(X["mixed"][k] - X["fp64"][k]) / np.linalg.norm(X["fp64"][k])
+        # both solutions have relative residual <= 1e-10; the system is
+        # diagonally dominant (rho V / dt), so their distance is of that order
+        parity.check(f"convdiff96 mixed rhs{k} error vs fp64", float(err), 1e-9)
+        parity.check(f"convdiff96 mixed rhs{k} vs oracle fp64", parity.rel(X["mixed"][k], x_o), 1e-9)

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
+import parity
 from opencslsolver_amd import gen
 from opencslsolver_amd.gen import CSR
 from opencslsolver_amd.solver import OcslError, Solver
@@ -31,23 +32,24 @@ def _solver(a, b, x0=None):
 
 @pytest.mark.parametrize("path", SOLVER_FIXTURES, ids=lambda p: os.path.basename(p))
 def test_cg_reference_mode_matches_reference(path):
+    name = os.path.basename(path)[:-4]
     d = np.load(path)
     a = CSR(d["row_ptr"], d["col"], d["val"])
     with _solver(a, d["b"], d["x0"]) as s:
         res = s.CG_solve()
         x = s.get_x()
         hist = s.history()
-    assert res == pytest.approx(float(d["cg_res"]), rel=1e-10)
-    assert _rel(x, d["cg_x"]) <= 1e-10
+    parity.check(f"{name} cg residual", parity.res_rel(res, d["cg_res"], floor=0.0), parity.bar("solver", name, "cg_res"))
+    parity.check(f"{name} cg x", parity.rel(x, d["cg_x"]), parity.bar("solver", name, "cg_x"))
     _, _, h_or = oracle.cg_reference(a, d["b"], d["x0"])
-    assert len(hist) == len(h_or)
-    assert np.allclose(hist, h_or, rtol=1e-10, atol=0)
+    parity.check(f"{name} cg history vs oracle", parity.hist_rel(hist, h_or, floor=0.0), parity.FLOOR_BAR)
 
 
 def test_cg_reference_mode_on_sod_systems():
     d = np.load(os.path.join(GOLDEN, "sod_split_2steps.npz"))
     rp, col = d["row_ptr"], d["col"]
     checked = 0
+    worst_res = worst_x = 0.0
     for k in range(d["dump_val"].shape[0]):
         if d["dump_method"][k] != 0:
             continue
@@ -55,10 +57,12 @@ def test_cg_reference_mode_on_sod_systems():
         with _solver(a, d["dump_b"][k], d["dump_x0"][k]) as s:
             res = s.CG_solve()
             x = s.get_x()
-        assert res == pytest.approx(d["res_hist"][k], rel=1e-10, abs=1e-28)
-        assert _rel(x, d["dump_x"][k]) <= 1e-10
+        worst_res = max(worst_res, parity.res_rel(res, d["res_hist"][k]))
+        worst_x = max(worst_x, parity.rel(x, d["dump_x"][k]))
         checked += 1
     assert checked >= 2
+    parity.check("sod split cg systems residual", worst_res, parity.FLOOR_BAR)
+    parity.check("sod split cg systems x", worst_x, parity.FLOOR_BAR)
 
 
 def test_blas_helpers_match_reference():
