@@ -481,8 +481,8 @@ __global__ void __launch_bounds__(kBlock) k_bi_spmv_v(BiArgs<V, X, NR> c) {
     const int lane = threadIdx.x & (kWave - 1);
     const int g = gridDim.x;
     const int lb = xcd_block(blockIdx.x, g);
-    const int spb = (c.A.n_slices + g - 1) / g;
-    const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
+    const SliceRange sr = block_slices(lb, g, c.A.n_slices);
+    const int s0 = sr.s0, s1 = sr.s1;
     double acc[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) acc[k] = 0.0;
@@ -567,8 +567,8 @@ __global__ void __launch_bounds__(kBlock) k_bi_spmv_t(BiArgs<V, X, NR> c) {
     const int lane = threadIdx.x & (kWave - 1);
     const int g = gridDim.x;
     const int lb = xcd_block(blockIdx.x, g);
-    const int spb = (c.A.n_slices + g - 1) / g;
-    const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
+    const SliceRange sr = block_slices(lb, g, c.A.n_slices);
+    const int s0 = sr.s0, s1 = sr.s1;
     double acc[2 * NR];
 #pragma unroll
     for (int k = 0; k < 2 * NR; ++k) acc[k] = 0.0;
@@ -682,8 +682,8 @@ __global__ void __launch_bounds__(kBlock) k_bi_residual(SellView<double> A, cons
     const int lane = threadIdx.x & (kWave - 1);
     const int g = gridDim.x;
     const int lb = xcd_block(blockIdx.x, g);
-    const int spb = (A.n_slices + g - 1) / g;
-    const int s0 = lb * spb, s1 = min(A.n_slices, s0 + spb);
+    const SliceRange sr = block_slices(lb, g, A.n_slices);
+    const int s0 = sr.s0, s1 = sr.s1;
     double acc[3 * NR];
 #pragma unroll
     for (int k = 0; k < 3 * NR; ++k) acc[k] = 0.0;
@@ -826,7 +826,7 @@ void bicgstab_run(ocsl_solver* h, bool mixed, const ocsl_options& o, const doubl
         int chunk = std::max(1, std::min({o.check_every, left, std::max(1, done_iters)}));
         const bool graph = o.use_graph && chunk == o.check_every;
         if (graph) {
-            auto key = std::make_pair(kind, chunk);
+            const GraphKey key = bicgstab_graph_key(kind, chunk, o.tri_weight);
             auto it = h->graphs.find(key);
             if (it == h->graphs.end()) {
                 hipGraph_t gr;

@@ -35,9 +35,10 @@ static void launch_spmv(ocsl_solver* h, const Sell& s, double a, const double* x
 // ||b - A x|| / ||b|| over SELL-P: the residual kernel writing into scratch.
 static double true_relres_p(ocsl_solver* h, const SellP& s, const double* b, const double* x) {
     double* r = work_buf(h, h->work.d[11], s.padded());
-    const int g = std::max(1, std::min(kMaxRedBlocks, (s.n_slices + 15) / 16 * 1));
+    const CgLaunch L = residual_launch(s.n_slices);
+    const int g = L.grid;
     with_rpl(s.rpl, [&](auto RC) {
-        k_cg_residual<false, decltype(RC)::value><<<g, 1024, 0, h->stream>>>(s.view<double>(), b, x, r, nullptr,
+        k_cg_residual<false, decltype(RC)::value><<<g, L.block, 0, h->stream>>>(s.view<double>(), b, x, r, nullptr,
                                                                             nullptr, nullptr, h->red(), 0);
     });
     k_finalize<2, FinStore<2>><<<1, kBlock, 0, h->stream>>>(h->red(), g, FinStore<2>{h->dscal.p}, kSlotRR);
@@ -161,32 +162,6 @@ int x_period(const ocsl_options& o) { return o.x_period == 2 || o.x_period == 4 
 
 namespace {
 
-struct CgLaunch {
-    int grid, block;  // SpMV and residual kernels
-    int vgrid;        // vector-update kernels
-};
-
-int round8(int g) { return g >= kXcds ? (g / kXcds) * kXcds : g; }
-
-// Default launch: 512 blocks of 1024 threads (2 blocks, 8 waves per SIMD, on
-// each of the 256 CUs).  Every block of a consuming kernel re-adds one
-// partial per producing block, so fewer, larger blocks keep that cheap.  Small
-// systems get one wave per slice.  All kernels of a solve share the launch:
-// each re-adds partial sums the others stored.
-CgLaunch cg_launch(int n_slices, int n, const ocsl_options& o) {
-    int block = o.block_size > 0 ? o.block_size : 1024;
-    block = std::max(kWave, std::min(kCgMaxBlock, block / kWave * kWave));
-    const int waves = block / kWave;
-    int cap = o.grid_blocks > 0 ? o.grid_blocks : 512;
-    cap = std::min(cap, kMaxRedBlocks);
-    const int grid = round8(std::max(1, std::min(cap, (n_slices + waves - 1) / waves)));
-    int vcap = o.vec_blocks > 0 ? o.vec_blocks : grid;
-    vcap = std::min(vcap, kMaxRedBlocks);
-    // one 16-byte vector per thread at most
-    const int vneed = (int)std::max<int64_t>(1, ((int64_t)n / 2 + block - 1) / block);
-    return {grid, block, round8(std::max(1, std::min(vcap, vneed)))};
-}
-
 // Update j of a period of M iterations.
 template <class V, class X, int M, int J = 0>
 void launch_cgm_update(ocsl_solver* h, const CgArgs<V, X>& ck, int n, CgLaunch L, int j) {
@@ -216,13 +191,6 @@ void launch_iter(ocsl_solver* h, const CgArgs<V, X>& c, int n, CgLaunch L, int k
         k_cg_update_r<V, X><<<L.vgrid, L.block, 0, h->stream>>>(c, n);
         k_cg_update_xp<V, X><<<L.vgrid, L.block, 0, h->stream>>>(c, n);
     }
-}
-
-// Slices per wave of the one-kernel launch over a SELL-P layout.
-int slices_per_wave(int n_slices, CgLaunch L) {
-    const int spb = (n_slices + L.grid - 1) / L.grid;
-    const int waves = L.block / kWave;
-    return (spb + waves - 1) / waves;
 }
 
 // One-kernel iteration k of a chunk (parity = k & 1).  kLds (FP32): the
@@ -259,10 +227,7 @@ void run_chunk(ocsl_solver* h, int chunk, bool use_graph, int kind, int R, CgLau
         OCSL_HIP(hipGetLastError());
         return;
     }
-    // kind < 64, grid and vgrid <= 2048, block / 64 <= 16
-    const int64_t k1 = (int64_t)1 << 48 | (int64_t)kind << 40 | (int64_t)L.grid << 24 | (int64_t)L.vgrid << 8 |
-                       (L.block / kWave);
-    auto key = std::make_pair(k1, chunk * 8 + R);
+    const GraphKey key = cg_graph_key(kind, L, chunk, R);
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
         hipGraph_t gr;
@@ -388,20 +353,13 @@ void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o_in, ocsl_rep
     const int kind = (mixed ? 1 : 0) + (s.identity ? 0 : 2) + (fused ? 4 : 0) + (lds ? 8 : 0);
     // chunks hold whole periods: MIXED split M iterations, one-kernel pairs
     const int per = fused ? 2 : (mixed ? M : 1);
-    const bool pairs = per > 1;
-    const int every = (o.check_every + per - 1) / per * per;
     while (!h->S_host->done) {
+        const int left = o.max_iter - h->S_host->iter;
+        if (left < 1) break;
+        const CgChunk cp = cg_chunk(o.check_every, per, left, o.use_graph != 0);
+        const bool graph = cp.graph;
+        const int chunk = cp.chunk, kstop = cp.kstop;
         const int iter0 = h->S_host->iter;
-        const int left = o.max_iter - iter0;
-        const bool graph = o.use_graph && left > every;
-        int chunk = graph ? every : std::max(1, std::min(every, left));
-        if (pairs) {
-            // the kernel after the last allowed iteration records its residual
-            // and stops; iterations past max_iter stop on their own
-            if (!graph && chunk == left) ++chunk;
-            chunk = (chunk + per - 1) / per * per;
-        }
-        const int kstop = std::min(chunk, left);
         with_rpl(s.rpl, [&](auto RC) {
             constexpr int R = decltype(RC)::value;
             if (fused) {

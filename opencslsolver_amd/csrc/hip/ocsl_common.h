@@ -10,21 +10,12 @@
 #include <vector>
 
 #include "ocsl.h"
+#include "ocsl_launch.h"
 
 namespace ocsl {
 
-constexpr int kWave = 64;         // CDNA wavefront width
-constexpr int kBlock = 256;       // threads per workgroup (4 waves)
-constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kXcds = 8;          // MI355X: 8 XCDs, workgroups dealt round-robin
-constexpr int kMaxRedBlocks = 2048;
-constexpr int kMaxRed = 16;       // reductions finalised by one kernel
 
 // ---------------------------------------------------------------- errors
-struct Error {
-    ocsl_status status;
-    std::string what;
-};
 
 void set_last_error(const std::string& msg);
 
@@ -109,10 +100,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Logical block id so that each XCD processes one contiguous range of blocks:
 // hardware block b runs on XCD b % 8.
-__device__ __forceinline__ int xcd_block(int b, int g) {
-    if (g % kXcds) return b;
-    return (b % kXcds) * (g / kXcds) + b / kXcds;
-}
+__device__ __forceinline__ int xcd_block(int b, int g) { return xcd_map(b, g); }
 
 // Deterministic grid-wide reductions in two launches.  The producing kernel
 // stores one partial sum per block and reduction (block_partials, plain

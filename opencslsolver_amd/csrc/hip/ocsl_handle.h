@@ -158,8 +158,8 @@ struct ocsl_solver {
     uint64_t adj_epoch = ~0ull;
     ocsl::DBuf<double> grad_own;
 
-    // hipGraph cache of iteration chunks, keyed by (kind, chunk)
-    std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
+    // hipGraph cache of iteration chunks (cg_graph_key, bicgstab_graph_key)
+    std::map<ocsl::GraphKey, hipGraphExec_t> graphs;
 
     ocsl::RedScratch red() const { return {red_partial.p}; }
 };

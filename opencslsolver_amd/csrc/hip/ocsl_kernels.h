@@ -144,7 +144,6 @@ __global__ void __launch_bounds__(kBlock) k_sell_axpby(SellView<V> A, double a, 
 // and K2; K1 turns it into `halt`, which stops K3 -- K3 cannot test `done`
 // because its own block 0 writes it.  Only block 0 writes shared scalars, and
 // never one that other blocks of the same kernel read.
-constexpr int kCgMaxBlock = 1024;  // CG kernels run with 64..1024 threads per block
 constexpr int kSlotPQ = 0;  // p.q of K1
 constexpr int kSlotRR = 1;  // r.r of K2 and of the residual kernel
 constexpr int kSlotBB = 2;  // b.b of the residual kernel
@@ -311,8 +310,8 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_residual(SellPView<double> A
     const int lane = threadIdx.x & (kWave - 1);
     const int g = gridDim.x;
     const int lb = xcd_block(blockIdx.x, g);
-    const int spb = (A.n_slices + g - 1) / g;
-    const int s0 = lb * spb, s1 = min(A.n_slices, s0 + spb);
+    const SliceRange sr = block_slices(lb, g, A.n_slices);
+    const int s0 = sr.s0, s1 = sr.s1;
     double acc[2] = {0.0, 0.0};
     for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
         const int row0 = s * kWave * R + lane * R;
@@ -386,8 +385,8 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_spmv(CgArgs<V, X> c) {
     }
     const int lane = threadIdx.x & (kWave - 1);
     const int lb = xcd_block(blockIdx.x, g);
-    const int spb = (c.A.n_slices + g - 1) / g;
-    const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
+    const SliceRange sr = block_slices(lb, g, c.A.n_slices);
+    const int s0 = sr.s0, s1 = sr.s1;
     double acc[1] = {0.0};
     for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
         const int row0 = s * kWave * R + lane * R;
@@ -624,8 +623,8 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int fi
     }
     const int lane = threadIdx.x & (kWave - 1);
     const int lb = xcd_block(blockIdx.x, g);
-    const int spb = (c.A.n_slices + g - 1) / g;
-    const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
+    const SliceRange sr = block_slices(lb, g, c.A.n_slices);
+    const int s0 = sr.s0, s1 = sr.s1;
     double acc[3] = {0.0, 0.0, 0.0};
     for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
         const int row0 = s * kWave * R + lane * R;
@@ -852,8 +851,8 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgf_first(CgfArgs<V, X> c) {
     const int lane = threadIdx.x & (kWave - 1);
     const int g = gridDim.x;
     const int lb = xcd_block(blockIdx.x, g);
-    const int spb = (c.A.n_slices + g - 1) / g;
-    const int s0 = lb * spb, s1 = min(c.A.n_slices, s0 + spb);
+    const SliceRange sr = block_slices(lb, g, c.A.n_slices);
+    const int s0 = sr.s0, s1 = sr.s1;
     const X* __restrict__ p = c.p[0];
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
@@ -941,8 +940,8 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgf_iter(CgfArgs<V, X> c, int k
     const int lane = threadIdx.x & (kWave - 1);
     const int g = gridDim.x;
     const int lb = xcd_block(blockIdx.x, g);
-    const int spb = (c.A.n_slices + g - 1) / g;
-    const int s0 = min(c.A.n_slices, lb * spb), s1 = min(c.A.n_slices, s0 + spb);
+    const SliceRange sr = block_slices(lb, g, c.A.n_slices);
+    const int s0 = sr.s0, s1 = sr.s1;
     const int n = c.A.n;
     const X* __restrict__ rin = c.r[kOdd];
     const X* __restrict__ qin = c.q[kOdd];
@@ -1071,8 +1070,8 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgl_iter(CgfArgs<float, float> 
     const int lane = threadIdx.x & (kWave - 1);
     const int g = gridDim.x;
     const int lb = xcd_block(blockIdx.x, g);
-    const int spb = (c.A.n_slices + g - 1) / g;
-    const int s0 = min(c.A.n_slices, lb * spb), s1 = min(c.A.n_slices, s0 + spb);
+    const SliceRange sr = block_slices(lb, g, c.A.n_slices);
+    const int s0 = sr.s0, s1 = sr.s1;
     const int n = c.A.n;
     const X* __restrict__ rin = c.r[kOdd];
     const X* __restrict__ qin = c.q[kOdd];

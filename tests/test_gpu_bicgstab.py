@@ -172,3 +172,28 @@ def test_convdiff_96cubed_three_rhs_against_oracle():
         # diagonally dominant (rho V / dt), so their distance is of that order
         parity.check(f"convdiff96 mixed rhs{k} error vs fp64", float(err), 1e-9)
         parity.check(f"convdiff96 mixed rhs{k} vs oracle fp64", parity.rel(X["mixed"][k], x_o), 1e-9)
+
+
+def test_graph_cache_follows_the_jacobi_weight():
+    """TOLERANCE BiCGSTAB with damped-Jacobi triangular sweeps replays its
+    iteration chunks as graphs whose sweeps take the weight as an argument: a
+    handle that solved with one weight and then solves with another must give
+    what a fresh handle gives with the second weight (and the oracle's
+    iteration count)."""
+    a, _ = gen.convdiff3d(16, seed_perm=3, dt=1.0)
+    b = gen.uniform(4, a.n)
+    opts = dict(tol=1e-10, tri_mode=1, tri_sweeps=3, check_every=2)
+    with _solver(a, b) as s:
+        s.solve("ilu_bicgstab", "tolerance", "fp64", tri_weight=0.9, **opts)
+        s.set_x(np.zeros(a.n))
+        rep = s.solve("ilu_bicgstab", "tolerance", "fp64", tri_weight=0.7, **opts)
+        x_reused = s.get_x()
+    assert rep["iterations"] > 4, rep  # chunks of check_every iterations replayed as graphs
+    with _solver(a, b) as s:
+        rep_f = s.solve("ilu_bicgstab", "tolerance", "fp64", tri_weight=0.7, **opts)
+        x_fresh = s.get_x()
+    assert rep["iterations"] == rep_f["iterations"]
+    assert np.array_equal(x_reused, x_fresh)
+    it_o, x_o, _, _ = oracle.bicgstab_tol(a, b, tol=1e-10, ilu_sweeps=3, tri_sweeps=3, tri_w=0.7)
+    assert abs(rep["iterations"] - it_o) <= 1
+    assert parity.rel(x_reused, x_o) <= 1e-7
