@@ -186,6 +186,38 @@ ocsl_status ocsl_gemv(ocsl_handle h, double a, const double* x, double b, const 
  * values M in the reference's slot order after `sweeps` sweeps. */
 ocsl_status ocsl_ILU_decompose(ocsl_handle h, int sweeps, double* m_out, ocsl_where where);
 
+/* CMatrix_solver::ILU_decompose(A, M, uptr) as called on an M that need not
+ * hold A's values (cmatrix_solver.cpp:519-551: the sweeps start from M's
+ * current values): m0 is that M^0 in the handle's slot order, A the handle's
+ * current values.  Sentinel slots of m_out are 0. */
+ocsl_status ocsl_ILU_decompose_from(ocsl_handle h, int sweeps, const double* m0, double* m_out,
+                                    ocsl_where where);
+
+/* CMatrix_solver::ILU_forsub (cmatrix_solver.cpp:555-571): y = L^-1 b with
+ * the unit-lower part of the factor m (the handle's slot order);
+ * CMatrix_solver::ILU_backsub (cmatrix_solver.cpp:595-610): x = U^-1 y with
+ * its upper part.  Exact level-scheduled solves in the reference's order of
+ * terms per row; the diagonal positions are the handle's own (what
+ * extract_diag_indexes gives, cmatrix_solver.cpp:455-469). */
+ocsl_status ocsl_ILU_forsub(ocsl_handle h, const double* m, const double* b, double* y, ocsl_where where);
+ocsl_status ocsl_ILU_backsub(ocsl_handle h, const double* m, const double* y, double* x, ocsl_where where);
+
+/* CMatrix_solver::norm, the static sum and ABSsum (cmatrix_solver.cpp:233-263),
+ * product (:265-272) and both axpy forms (:275-335: out = a x + b y, or a x
+ * when y is NULL), on n-vectors, without a handle: the reference calls sum
+ * and ABSsum as static members (CMatrix_solver::ABSsum(b_vector),
+ * cflow_solver.cpp:346).  They run on a stream the library keeps for the
+ * current device (created at the first call, held until the process ends;
+ * calls are serialised per device).  Device operands must be complete when
+ * the call is made.  Reductions add in a fixed order; each product of axpy is
+ * rounded before the sum, as the reference's loops compute it. */
+ocsl_status ocsl_vec_norm(int64_t n, const double* a, const double* b, ocsl_where where, double* out);
+ocsl_status ocsl_vec_sum(int64_t n, const double* a, ocsl_where where, double* out);
+ocsl_status ocsl_vec_ABSsum(int64_t n, const double* a, ocsl_where where, double* out);
+ocsl_status ocsl_vec_product(int64_t n, const double* x, const double* y, double* out, ocsl_where where);
+ocsl_status ocsl_vec_axpy(int64_t n, double a, const double* x, double b, const double* y, double* out,
+                          ocsl_where where);
+
 /* Measurement hook (no counterpart in the reference): launches the tolerance-
  * mode CG SpMV kernel (q = A p with its fused dot products) `reps` times on
  * the handle's stream between two HIP events and returns its average duration

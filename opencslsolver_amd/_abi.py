@@ -99,6 +99,17 @@ SIGNATURES = {
     "ocsl_gemv": (ctypes.c_int, [c_handle, ctypes.c_double, ctypes.c_void_p, ctypes.c_double,
                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "ocsl_ILU_decompose": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
+    "ocsl_ILU_decompose_from": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_int]),
+    "ocsl_ILU_forsub": (ctypes.c_int, [c_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "ocsl_ILU_backsub": (ctypes.c_int, [c_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "ocsl_vec_norm": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _f64p]),
+    "ocsl_vec_sum": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, _f64p]),
+    "ocsl_vec_ABSsum": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, _f64p]),
+    "ocsl_vec_product": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_int]),
+    "ocsl_vec_axpy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_double, ctypes.c_void_p, ctypes.c_double,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "ocsl_time_cg_spmv": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _f64p, _f64p]),
     "ocsl_time_cg_iteration": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, _f64p, _f64p]),

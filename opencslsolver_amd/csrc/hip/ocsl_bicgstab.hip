@@ -40,7 +40,7 @@ __global__ void k_to_float(int n, const double* __restrict__ in, float* __restri
         out[i] = (float)in[i];
 }
 
-void ilu_decompose_device(ocsl_solver* h, int sweeps, bool want32) {
+void ilu_decompose_device(ocsl_solver* h, int sweeps, bool want32, const double* m0) {
     OCSL_REQUIRE(sweeps >= 0, OCSL_ERR_ARG, "ILU sweeps must be >= 0");
     if (!h->ilu_built) {
         build_ilu_tables(h->host, h->ilu, h->stream);
@@ -49,11 +49,12 @@ void ilu_decompose_device(ocsl_solver* h, int sweeps, bool want32) {
     const int nnz = (int)h->host.col.size();
     h->ilu_m.alloc(nnz);
     h->ilu_old.alloc(nnz);
-    // M^0 = A (copy_same_structure, cmatrix_solver.cpp:702)
+    // M^0 = A (copy_same_structure, cmatrix_solver.cpp:702), or the
+    // caller's M (ILU_decompose starts from whatever M holds, :520)
     double* cur = h->ilu_m.p;
     double* prev = h->ilu_old.p;
     if (sweeps % 2 == 1) std::swap(cur, prev);  // the last sweep lands in ilu_m
-    OCSL_HIP(hipMemcpyAsync(cur, h->csr_val.p, sizeof(double) * nnz, hipMemcpyDeviceToDevice, h->stream));
+    OCSL_HIP(hipMemcpyAsync(cur, m0 ? m0 : h->csr_val.p, sizeof(double) * nnz, hipMemcpyDeviceToDevice, h->stream));
     ++h->ilu_epoch;  // the level-ordered copies of the factor are stale
     const int g = grid_for(nnz);
     for (int s = 0; s < sweeps; ++s) {
@@ -944,3 +945,97 @@ void bicgstab_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_
 }
 
 }  // namespace ocsl
+
+// ------------------------------------------------------------------ factor-level entry points
+namespace ocsl {
+namespace {
+
+template <class F>
+ocsl_status tri_guard(F&& f) {
+    try {
+        f();
+        return OCSL_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what);
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return OCSL_ERR_NOMEM;
+    } catch (...) {
+        set_last_error("unexpected exception");
+        return OCSL_ERR_ARG;
+    }
+}
+
+// The caller's factor (its slot order, sentinels included) in the handle's
+// real-entry order.
+void gather_factor(ocsl_solver* h, const double* m, int where, DBuf<double>& stage, DBuf<double>& real) {
+    const double* md = m;
+    if (where == OCSL_HOST) {
+        stage.upload(m, (size_t)h->host.nnz_in, h->stream);
+        md = stage.p;
+    }
+    const int64_t nr = (int64_t)h->host.col.size();
+    real.alloc((size_t)nr);
+    k_gather_vals<double><<<grid_for(nr), kBlock, 0, h->stream>>>(nr, h->src.p, md, real.p);
+    OCSL_HIP(hipGetLastError());
+}
+
+// y = L^-1 b (lower) or x = U^-1 y (upper) with the caller's factor m.
+ocsl_status tri_api(ocsl_handle h, const double* m, const double* in, double* out, int where, bool lower) {
+    if (!h || !m || !in || !out || (where != OCSL_HOST && where != OCSL_DEVICE)) return OCSL_ERR_ARG;
+    return tri_guard([&] {
+        const int n = h->host.n;
+        DBuf<double> mstage, mreal, ib, ob;
+        gather_factor(h, m, where, mstage, mreal);
+        ensure_levels(h);
+        ++h->ilu_epoch;  // the level-ordered copies follow this factor
+        const double* din = in;
+        double* dout = out;
+        if (where == OCSL_HOST) {
+            ib.upload(in, (size_t)n, h->stream);
+            ob.alloc((size_t)n);
+            din = ib.p;
+            dout = ob.p;
+        }
+        tri_exact<double, double, 1>(h, mreal.p, din, dout, lower, nullptr);
+        OCSL_HIP(hipGetLastError());
+        if (where == OCSL_HOST)
+            OCSL_HIP(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+        sync(h);
+    });
+}
+
+}  // namespace
+}  // namespace ocsl
+
+using namespace ocsl;
+
+extern "C" ocsl_status ocsl_ILU_forsub(ocsl_handle h, const double* m, const double* b, double* y, ocsl_where where) {
+    return tri_api(h, m, b, y, where, true);
+}
+
+extern "C" ocsl_status ocsl_ILU_backsub(ocsl_handle h, const double* m, const double* y, double* x,
+                                       ocsl_where where) {
+    return tri_api(h, m, y, x, where, false);
+}
+
+extern "C" ocsl_status ocsl_ILU_decompose_from(ocsl_handle h, int sweeps, const double* m0, double* m_out,
+                                              ocsl_where where) {
+    if (!h || !m0 || !m_out || sweeps < 0 || (where != OCSL_HOST && where != OCSL_DEVICE)) return OCSL_ERR_ARG;
+    return tri_guard([&] {
+        ensure_csr_values(h, false);
+        DBuf<double> mstage, mreal, full;
+        gather_factor(h, m0, where, mstage, mreal);
+        ilu_decompose_device(h, sweeps, false, mreal.p);
+        const int64_t nnz_in = h->host.nnz_in;
+        full.alloc((size_t)nnz_in);
+        k_fill<double><<<grid_for(nnz_in), kBlock, 0, h->stream>>>((int)nnz_in, 0.0, full.p);
+        k_permute_out<double><<<grid_for((int64_t)h->host.col.size()), kBlock, 0, h->stream>>>(
+            (int)h->host.col.size(), h->src.p, h->ilu_m.p, full.p);
+        OCSL_HIP(hipGetLastError());
+        OCSL_HIP(hipMemcpyAsync(m_out, full.p, sizeof(double) * nnz_in,
+                                where == OCSL_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, h->stream));
+        sync(h);
+    });
+}

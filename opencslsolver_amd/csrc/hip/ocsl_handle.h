@@ -220,7 +220,9 @@ void cg_reference(ocsl_solver* h, ocsl_report* rep);
 void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report* rep);
 void bicgstab_reference(ocsl_solver* h, ocsl_report* rep);
 void bicgstab_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report* rep);
-void ilu_decompose_device(ocsl_solver* h, int sweeps, bool want32);
+// The reference's ILU sweeps from M^0 = m0 (device, real-entry order), or
+// from A when m0 is null; the result lands in h->ilu_m.
+void ilu_decompose_device(ocsl_solver* h, int sweeps, bool want32, const double* m0 = nullptr);
 void bicgstab_multi(ocsl_solver* h, bool mixed, const ocsl_options& o, int k, const double* B_dev,
                     double* X_dev, ocsl_report* reps);
 
