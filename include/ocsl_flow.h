@@ -2,9 +2,11 @@
  * coefficient loops of CFlow_solver (cflow_solver_cell_face_loops.cpp with the
  * stencil helpers of solvers_common.h) run as HIP kernels that write the
  * system's CSR values, right-hand side and initial guess on the device, in the
- * caller's slot layout, where the solver handle reads them.  The SIMPLE-like
- * step (CFlow_solver::iterate, cflow_solver.cpp:229-476) is host C++ around
- * those kernels and the handle's solvers.
+ * caller's slot layout, where the solver handle reads them.  The rest of the
+ * SIMPLE-like step (CFlow_solver::iterate, cflow_solver.cpp:229-476) runs as
+ * device kernels over a state kept in device memory (OCSL_SUPPORT_DEVICE, the
+ * default), or as a host restatement around the assembly kernels
+ * (OCSL_SUPPORT_HOST).
  *
  * Arrays of 3-vectors are x, y, z interleaved.  Enumerations follow the
  * reference's (enums.h).  All calls return ocsl_status as in ocsl.h. */
@@ -97,7 +99,22 @@ typedef struct {
     int precision;     /* ocsl_precision (TOLERANCE mode only) */
     int support;       /* ocsl_flow_support: where the SIMPLE step's support loops run */
     ocsl_options solve_options;
+    int cell_order;    /* ocsl_flow_cell_order: the order the device keeps cells in */
 } ocsl_flow_params;
+
+/* The order in which the device stores cells (and interior faces, by their
+ * left cell) for the kernels that gather over a cell's neighbours.  The
+ * caller's numbering, rows, right-hand side and solution order are unchanged
+ * whatever the choice, and so are all results, bit for bit: each cell's sums
+ * run over its faces in the caller's face order either way.  AUTO (default):
+ * a reverse Cuthill-McKee order when the caller's numbering puts face
+ * neighbours more than twice as far apart on average, else the caller's.
+ * CALLER / LOCALITY force one. */
+typedef enum { OCSL_CELL_ORDER_AUTO = 0, OCSL_CELL_ORDER_CALLER = 1, OCSL_CELL_ORDER_LOCALITY = 2 } ocsl_flow_cell_order;
+
+/* The device cell order chosen at create: order[k] = the caller's cell stored
+ * k-th (n_cells entries). */
+ocsl_status ocsl_flow_cell_order_get(ocsl_flow f, int32_t* order);
 
 /* Where the support loops of the SIMPLE step run (boundary updates, face mass
  * fluxes, pressure and pressure-correction gradients, mass-flux correction,

@@ -104,6 +104,10 @@ struct MeshDev {
     // face's static weights
     const int* cf_nb;
     const double *cf_geo, *cf_ksi;
+    // Cells stored in an order of their own (cperm != nullptr): device cell
+    // k is the caller's cell cperm[k], whose row starts at rstart[k] in the
+    // caller's value array and holds rlen[k] slots.
+    const int *cperm, *rstart, *rlen;
 };
 
 struct StateDev {
@@ -116,10 +120,19 @@ struct StateDev {
 
 enum { kMomentum = 0, kPressure = 1, kEnthalpy = 2 };
 
-// a: the cell's row, staged in LDS; pos: the slot position in the row.  The
-// reference adds each term, already rounded, to the slot: the empty asm keeps
-// the compiler from fusing the product that formed v into this add.
-__device__ __forceinline__ void add_at(double* a, int pos, double v) {
+// A cell's row staged in LDS: slot q at p[q * stride] (stride 1: the row
+// laid out as in the caller's value array; the block size: one column of
+// slots per thread, free of bank conflicts).
+struct RowAcc {
+    double* p;
+    int stride;
+    __device__ __forceinline__ double& operator[](int q) const { return p[q * stride]; }
+};
+
+// a: the cell's row; pos: the slot position in the row.  The reference adds
+// each term, already rounded, to the slot: the empty asm keeps the compiler
+// from fusing the product that formed v into this add.
+__device__ __forceinline__ void add_at(RowAcc a, int pos, double v) {
     asm volatile("" : "+v"(v));
     a[pos] += v;
 }
@@ -140,7 +153,7 @@ __device__ __forceinline__ double conv_interp(double geo, double mdot, double l,
 // side 0 = left (owner), 1 = right; dp / op: positions of the diagonal and of
 // the other cell's column in this row.
 template <bool kBlend>
-__device__ __forceinline__ void conv_row(double* a, double& rhs, int side, int dp, int op,
+__device__ __forceinline__ void conv_row(RowAcc a, double& rhs, int side, int dp, int op,
                                          double mdot, double l, double r, double geo) {
     if (mdot >= 0.0) {
         if (side == 0) add_at(a, dp, mdot);
@@ -160,14 +173,15 @@ __device__ __forceinline__ void conv_row(double* a, double& rhs, int side, int d
 }
 
 // Diffusive_flux (solvers_common.h:42-53): +D on the diagonal, -D off it.
-__device__ __forceinline__ void diff_row(double* a, int dp, int op, double D) {
+__device__ __forceinline__ void diff_row(RowAcc a, int dp, int op, double D) {
     add_at(a, dp, D);
     add_at(a, op, -D);
 }
 
 template <int kEq, bool kVisc, bool kBlend, bool kTransient>
 __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& s, const FlowPhys& ph, int d,
-                                              int i, double* a, double* __restrict__ b, double* __restrict__ x0) {
+                                              int i, int oi, RowAcc a, double* __restrict__ b,
+                                              double* __restrict__ x0) {
     const Gas& gas = ph.gas;
     double rhs = 0.0;
     const int dp = m.dpos[i];
@@ -307,7 +321,7 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
         }
         s.ap[i] += diag / 3.0;
         s.ap_vec[3 * i + d] = diag;
-        x0[i] = s.u[3 * i + d];
+        x0[oi] = s.u[3 * i + d];
     } else if constexpr (kEq == kPressure) {
         if constexpr (kTransient) {
             const double dum = V / s.dt[i];
@@ -318,7 +332,7 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
         } else {
             diag = diag / ph.urf;
         }
-        x0[i] = s.pc[i];
+        x0[oi] = s.pc[i];
     } else {
         if constexpr (kTransient) {
             const double dum = V / s.dt[i];
@@ -332,27 +346,42 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
             diag = dd / ph.urf;
             rhs += (1.0 - ph.urf) / ph.urf * dd * s.h_old[i];
         }
-        x0[i] = s.h[i];
+        x0[oi] = s.h[i];
     }
     a[dp] = diag;
-    b[i] = rhs;
+    b[oi] = rhs;
 }
 
-// One thread per cell.  The block's rows are accumulated in LDS, laid out
-// as they lie in the caller's value array (sentinel slots stay 0), and
-// written out together.
-template <int kEq, bool kVisc, bool kBlend, bool kTransient>
+// One thread per cell; each XCD takes a contiguous range of cells.  Rows are
+// accumulated in LDS.  Cells in the caller's order (kPerm false): the block's
+// rows lie together in the caller's value array, so they are staged as laid
+// out there (sentinel slots stay 0) and written out as one coalesced run.
+// Cells in an order of their own (kPerm): each thread's row sits in a column
+// of LDS slots and is written to the caller's row of its cell; b and x0 go
+// to the caller's index of the cell.
+template <int kEq, bool kVisc, bool kBlend, bool kTransient, bool kPerm>
 __global__ void __launch_bounds__(kBlock) k_assemble(MeshDev m, StateDev s, FlowPhys ph, int d,
                                                      double* __restrict__ val, double* __restrict__ b,
                                                      double* __restrict__ x0) {
     extern __shared__ double rows[];
-    const int i0 = blockIdx.x * blockDim.x, i = i0 + threadIdx.x;
-    const int base = m.row_ptr[i0], span = m.row_ptr[min(i0 + (int)blockDim.x, m.n)] - base;
-    for (int k = threadIdx.x; k < span; k += blockDim.x) rows[k] = 0.0;
-    __syncthreads();
-    if (i < m.n) assemble_cell<kEq, kVisc, kBlend, kTransient>(m, s, ph, d, i, rows + (m.row_ptr[i] - base), b, x0);
-    __syncthreads();
-    for (int k = threadIdx.x; k < span; k += blockDim.x) val[base + k] = rows[k];
+    const int i0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x, i = i0 + threadIdx.x;
+    if constexpr (!kPerm) {
+        const int base = m.row_ptr[i0], span = m.row_ptr[min(i0 + (int)blockDim.x, m.n)] - base;
+        for (int k = threadIdx.x; k < span; k += blockDim.x) rows[k] = 0.0;
+        __syncthreads();
+        if (i < m.n)
+            assemble_cell<kEq, kVisc, kBlend, kTransient>(m, s, ph, d, i, i, RowAcc{rows + (m.row_ptr[i] - base), 1},
+                                                          b, x0);
+        __syncthreads();
+        for (int k = threadIdx.x; k < span; k += blockDim.x) val[base + k] = rows[k];
+    } else {
+        if (i >= m.n) return;
+        const int t = threadIdx.x, nt = blockDim.x;
+        const int len = m.rlen[i], rs = m.rstart[i];
+        for (int q = 0; q < len; ++q) rows[q * nt + t] = 0.0;
+        assemble_cell<kEq, kVisc, kBlend, kTransient>(m, s, ph, d, i, m.cperm[i], RowAcc{rows + t, nt}, b, x0);
+        for (int q = 0; q < len; ++q) val[rs + q] = rows[q * nt + t];
+    }
 }
 
 // calc_flux_explicit_visc_part (cflow_solver_cell_face_loops.cpp:25-35):
@@ -552,16 +581,20 @@ __global__ void __launch_bounds__(kBlock) k_correct_mdot(MeshDev m, StateDev s, 
 }
 
 // u_d = the momentum solve's solution (cflow_solver.cpp:290-299).
-__global__ void __launch_bounds__(kBlock) k_take_u(StateDev s, const double* __restrict__ x, int d, int n) {
+// x is in the caller's cell order; cperm (null: the same order) maps each
+// device cell to it.
+__global__ void __launch_bounds__(kBlock) k_take_u(StateDev s, const double* __restrict__ x, const int* cperm, int d,
+                                                   int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) s.u[3 * i + d] = x[i];
+    if (i < n) s.u[3 * i + d] = x[cperm ? cperm[i] : i];
 }
 
 // p' = 0.5 x, p += p' (cflow_solver.cpp:350-357).
-__global__ void __launch_bounds__(kBlock) k_take_pc(StateDev s, const double* __restrict__ x, int n) {
+__global__ void __launch_bounds__(kBlock) k_take_pc(StateDev s, const double* __restrict__ x, const int* cperm,
+                                                    int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double pc = kRelaxP * x[i];
+    const double pc = kRelaxP * x[cperm ? cperm[i] : i];
     s.pc[i] = pc;
     s.p[i] += pc;
 }
@@ -601,10 +634,11 @@ __global__ void __launch_bounds__(kBlock) k_brho(MeshDev m, StateDev s, FlowPhys
 }
 
 // h = the enthalpy solve's solution, T = T(h, u), Mach (cflow_solver.cpp:462-472).
-__global__ void __launch_bounds__(kBlock) k_take_h(StateDev s, Gas g, const double* __restrict__ x, int n) {
+__global__ void __launch_bounds__(kBlock) k_take_h(StateDev s, Gas g, const double* __restrict__ x, const int* cperm,
+                                                   int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double h = x[i];
+    const double h = x[cperm ? cperm[i] : i];
     s.h[i] = h;
     const double T = g.temperature(h, s.u + 3 * i);
     s.T[i] = T;
@@ -640,9 +674,16 @@ struct ocsl_flow_s {
     // faces as owner are contiguous: device face k is the caller's face
     // fperm[k].  The host mirror keeps the caller's order.
     std::vector<int> fperm;
-    size_t asm_lds = 0;  // LDS bytes of one assembly block: its longest run of rows
-    DBuf<int> d_fperm;
+    // Cells: device cell k is the caller's cell cperm[k] (ciperm the inverse);
+    // cell_perm false when that is the identity.  Per-cell state and mesh
+    // arrays are stored in the device order; interior faces follow their
+    // (device) left cell; boundary faces keep the caller's order.
+    std::vector<int> cperm, ciperm;
+    bool cell_perm = false;
+    size_t asm_lds = 0;  // LDS bytes of one assembly block
+    DBuf<int> d_fperm, d_cperm, d_rstart, d_rlen;
     DBuf<double> d_fstage;  // caller-order staging of per-face state
+    DBuf<double> d_cstage;  // caller-order staging of per-cell state
     bool dev = true;  // support loops on the device, state resident there
     // device mesh
     DBuf<double> d_vol, d_fS, d_fe, d_fdist, d_fksi, d_fgeo, d_falpha, d_bS, d_barea, d_bksi, d_bperp, d_be, d_balpha,
@@ -667,7 +708,7 @@ struct ocsl_flow_s {
                 d_fe.p,     d_fdist.p,    d_fksi.p,    d_fgeo.p,   d_falpha.p,  d_bL.p,       d_bmark.p,
                 d_bS.p,     d_barea.p,    d_bksi.p,    d_bperp.p,  d_be.p,      d_balpha.p,   d_bdist.p,
                 d_cf_sp.p,  d_cf_code.p,  d_cf_pos.p,  d_cb_ptr.p, d_cb_face.p, d_row_ptr.p,  d_dpos.p,
-                d_cf_nb.p,  d_cf_geo.p,   d_cf_ksi.p};
+                d_cf_nb.p,  d_cf_geo.p,   d_cf_ksi.p,  cell_perm ? d_cperm.p : nullptr, d_rstart.p, d_rlen.p};
     }
     StateDev state_view() const {
         return {d_u.p, d_u_old.p, d_rho.p, d_rho_old.p, d_p.p, d_p_old.p, d_pc.p, d_gp.p, d_T.p, d_h.p,
@@ -712,13 +753,13 @@ void down(std::vector<T>& h, const DBuf<T>& d, hipStream_t s) {
     if (!h.empty()) OCSL_HIP(hipMemcpyAsync(h.data(), d.p, sizeof(T) * h.size(), hipMemcpyDeviceToHost, s));
 }
 
-// Face data between the caller's face order and the handle's: device face k
-// is the caller's face fperm[k]; c components per face.
+// Per-face or per-cell data between the caller's order and the handle's:
+// device item k is the caller's item perm[k]; c components per item.
 template <bool kToHandle>
-__global__ void k_face_perm(int F, int c, const int* __restrict__ fperm, const double* __restrict__ in,
-                            double* __restrict__ out) {
+__global__ void k_perm_items(int F, int c, const int* __restrict__ perm, const double* __restrict__ in,
+                             double* __restrict__ out) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < F; k += gridDim.x * blockDim.x) {
-        const size_t f = (size_t)fperm[k];
+        const size_t f = (size_t)perm[k];
         for (int q = 0; q < c; ++q) {
             if (kToHandle) out[(size_t)k * c + q] = in[f * c + q];
             else out[f * c + q] = in[(size_t)k * c + q];
@@ -744,7 +785,7 @@ void up_f(ocsl_flow_s* fl, DBuf<double>& d, const std::vector<double>& h, hipStr
     d.alloc(h.size());
     if (h.empty() || F == 0) return;
     fl->d_fstage.upload(h.data(), h.size(), s);
-    k_face_perm<true><<<grid_for(F), kBlock, 0, s>>>(F, (int)(h.size() / F), fl->d_fperm.p, fl->d_fstage.p, d.p);
+    k_perm_items<true><<<grid_for(F), kBlock, 0, s>>>(F, (int)(h.size() / F), fl->d_fperm.p, fl->d_fstage.p, d.p);
     OCSL_HIP(hipGetLastError());
 }
 
@@ -754,9 +795,37 @@ void down_f(ocsl_flow_s* fl, std::vector<double>& h, const DBuf<double>& d, hipS
     const int F = (int)fl->fperm.size();
     if (h.empty() || F == 0) return;
     fl->d_fstage.alloc(h.size());
-    k_face_perm<false><<<grid_for(F), kBlock, 0, s>>>(F, (int)(h.size() / F), fl->d_fperm.p, d.p, fl->d_fstage.p);
+    k_perm_items<false><<<grid_for(F), kBlock, 0, s>>>(F, (int)(h.size() / F), fl->d_fperm.p, d.p, fl->d_fstage.p);
     OCSL_HIP(hipGetLastError());
     OCSL_HIP(hipMemcpyAsync(h.data(), fl->d_fstage.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+}
+
+// Per-cell state (1 or 3 components per cell) between the host mirror, in the
+// caller's order, and the device, in the handle's.
+void up_c(ocsl_flow_s* fl, DBuf<double>& d, const std::vector<double>& h, hipStream_t s) {
+    if (!fl->cell_perm) {
+        up(d, h, s);
+        return;
+    }
+    const int n = fl->mesh.n;
+    d.alloc(h.size());
+    if (h.empty()) return;
+    fl->d_cstage.upload(h.data(), h.size(), s);
+    k_perm_items<true><<<grid_for(n), kBlock, 0, s>>>(n, (int)(h.size() / n), fl->d_cperm.p, fl->d_cstage.p, d.p);
+    OCSL_HIP(hipGetLastError());
+}
+
+void down_c(ocsl_flow_s* fl, std::vector<double>& h, const DBuf<double>& d, hipStream_t s) {
+    if (!fl->cell_perm) {
+        down(h, d, s);
+        return;
+    }
+    const int n = fl->mesh.n;
+    if (h.empty()) return;
+    fl->d_cstage.alloc(h.size());
+    k_perm_items<false><<<grid_for(n), kBlock, 0, s>>>(n, (int)(h.size() / n), fl->d_cperm.p, d.p, fl->d_cstage.p);
+    OCSL_HIP(hipGetLastError());
+    OCSL_HIP(hipMemcpyAsync(h.data(), fl->d_cstage.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
 }
 
 inline const double* v3(const std::vector<double>& a, int i) { return a.data() + 3 * i; }
@@ -937,23 +1006,23 @@ void set_bcs(ocsl_flow_s* fl) {
 void upload_cells(ocsl_flow_s* fl, int eq) {
     HostFlow& s = fl->st;
     hipStream_t st = fl->solver->stream;
-    up(fl->d_T, s.T, st);
-    up(fl->d_rho, s.rho, st);
-    up(fl->d_dt, s.dt, st);
+    up_c(fl, fl->d_T, s.T, st);
+    up_c(fl, fl->d_rho, s.rho, st);
+    up_c(fl, fl->d_dt, s.dt, st);
     up_f(fl, fl->d_f_mdot, s.f_mdot, st);
     up(fl->d_b_mdot, s.b_mdot, st);
     if (eq <= OCSL_EQ_MOMENTUM_Z) {
-        up(fl->d_u, s.u, st);
-        up(fl->d_u_old, s.u_old, st);
-        up(fl->d_rho_old, s.rho_old, st);
-        up(fl->d_gp, s.gp, st);
-        up(fl->d_ap, s.ap, st);
-        up(fl->d_ap_vec, s.ap_vec, st);
+        up_c(fl, fl->d_u, s.u, st);
+        up_c(fl, fl->d_u_old, s.u_old, st);
+        up_c(fl, fl->d_rho_old, s.rho_old, st);
+        up_c(fl, fl->d_gp, s.gp, st);
+        up_c(fl, fl->d_ap, s.ap, st);
+        up_c(fl, fl->d_ap_vec, s.ap_vec, st);
         up_f(fl, fl->d_f_stress, s.f_stress, st);
         up(fl->d_b_u, s.b_u, st);
     } else if (eq == OCSL_EQ_PRESSURE) {
-        up(fl->d_rho_old, s.rho_old, st);
-        up(fl->d_pc, s.pc, st);
+        up_c(fl, fl->d_rho_old, s.rho_old, st);
+        up_c(fl, fl->d_pc, s.pc, st);
         up_f(fl, fl->d_f_rho, s.f_rho, st);
         up_f(fl, fl->d_f_qy, s.f_qy, st);
         up(fl->d_b_T, s.b_T, st);
@@ -961,11 +1030,11 @@ void upload_cells(ocsl_flow_s* fl, int eq) {
         up(fl->d_b_qy, s.b_qy, st);
         up(fl->d_b_super, s.b_super, st);
     } else {
-        up(fl->d_h, s.h, st);
-        up(fl->d_h_old, s.h_old, st);
-        up(fl->d_rho_old, s.rho_old, st);
-        up(fl->d_p, s.p, st);
-        up(fl->d_p_old, s.p_old, st);
+        up_c(fl, fl->d_h, s.h, st);
+        up_c(fl, fl->d_h_old, s.h_old, st);
+        up_c(fl, fl->d_rho_old, s.rho_old, st);
+        up_c(fl, fl->d_p, s.p, st);
+        up_c(fl, fl->d_p_old, s.p_old, st);
         up(fl->d_b_h, s.b_h, st);
         up(fl->d_b_T, s.b_T, st);
     }
@@ -975,22 +1044,22 @@ void upload_cells(ocsl_flow_s* fl, int eq) {
 void upload_all(ocsl_flow_s* fl) {
     const HostFlow& s = fl->st;
     hipStream_t st = fl->solver->stream;
-    up(fl->d_u, s.u, st);
-    up(fl->d_u_old, s.u_old, st);
-    up(fl->d_rho, s.rho, st);
-    up(fl->d_rho_old, s.rho_old, st);
-    up(fl->d_p, s.p, st);
-    up(fl->d_p_old, s.p_old, st);
-    up(fl->d_pc, s.pc, st);
-    up(fl->d_gp, s.gp, st);
-    up(fl->d_gpc, s.gpc, st);
-    up(fl->d_T, s.T, st);
-    up(fl->d_h, s.h, st);
-    up(fl->d_h_old, s.h_old, st);
-    up(fl->d_ap, s.ap, st);
-    up(fl->d_ap_vec, s.ap_vec, st);
-    up(fl->d_dt, s.dt, st);
-    up(fl->d_mach, s.mach, st);
+    up_c(fl, fl->d_u, s.u, st);
+    up_c(fl, fl->d_u_old, s.u_old, st);
+    up_c(fl, fl->d_rho, s.rho, st);
+    up_c(fl, fl->d_rho_old, s.rho_old, st);
+    up_c(fl, fl->d_p, s.p, st);
+    up_c(fl, fl->d_p_old, s.p_old, st);
+    up_c(fl, fl->d_pc, s.pc, st);
+    up_c(fl, fl->d_gp, s.gp, st);
+    up_c(fl, fl->d_gpc, s.gpc, st);
+    up_c(fl, fl->d_T, s.T, st);
+    up_c(fl, fl->d_h, s.h, st);
+    up_c(fl, fl->d_h_old, s.h_old, st);
+    up_c(fl, fl->d_ap, s.ap, st);
+    up_c(fl, fl->d_ap_vec, s.ap_vec, st);
+    up_c(fl, fl->d_dt, s.dt, st);
+    up_c(fl, fl->d_mach, s.mach, st);
     up_f(fl, fl->d_f_mdot, s.f_mdot, st);
     up_f(fl, fl->d_f_rho, s.f_rho, st);
     up_f(fl, fl->d_f_qy, s.f_qy, st);
@@ -1010,22 +1079,22 @@ void upload_all(ocsl_flow_s* fl) {
 void download_all(ocsl_flow_s* fl) {
     HostFlow& s = fl->st;
     hipStream_t st = fl->solver->stream;
-    down(s.u, fl->d_u, st);
-    down(s.u_old, fl->d_u_old, st);
-    down(s.rho, fl->d_rho, st);
-    down(s.rho_old, fl->d_rho_old, st);
-    down(s.p, fl->d_p, st);
-    down(s.p_old, fl->d_p_old, st);
-    down(s.pc, fl->d_pc, st);
-    down(s.gp, fl->d_gp, st);
-    down(s.gpc, fl->d_gpc, st);
-    down(s.T, fl->d_T, st);
-    down(s.h, fl->d_h, st);
-    down(s.h_old, fl->d_h_old, st);
-    down(s.ap, fl->d_ap, st);
-    down(s.ap_vec, fl->d_ap_vec, st);
-    down(s.dt, fl->d_dt, st);
-    down(s.mach, fl->d_mach, st);
+    down_c(fl, s.u, fl->d_u, st);
+    down_c(fl, s.u_old, fl->d_u_old, st);
+    down_c(fl, s.rho, fl->d_rho, st);
+    down_c(fl, s.rho_old, fl->d_rho_old, st);
+    down_c(fl, s.p, fl->d_p, st);
+    down_c(fl, s.p_old, fl->d_p_old, st);
+    down_c(fl, s.pc, fl->d_pc, st);
+    down_c(fl, s.gp, fl->d_gp, st);
+    down_c(fl, s.gpc, fl->d_gpc, st);
+    down_c(fl, s.T, fl->d_T, st);
+    down_c(fl, s.h, fl->d_h, st);
+    down_c(fl, s.h_old, fl->d_h_old, st);
+    down_c(fl, s.ap, fl->d_ap, st);
+    down_c(fl, s.ap_vec, fl->d_ap_vec, st);
+    down_c(fl, s.dt, fl->d_dt, st);
+    down_c(fl, s.mach, fl->d_mach, st);
     down_f(fl, s.f_mdot, fl->d_f_mdot, st);
     down_f(fl, s.f_rho, fl->d_f_rho, st);
     down_f(fl, s.f_qy, fl->d_f_qy, st);
@@ -1051,8 +1120,13 @@ void launch_assemble(ocsl_flow_s* fl, int d) {
     const StateDev s = fl->state_view();
     const FlowPhys ph = fl->phys();
     double *val = fl->d_val.p, *b = fl->d_b.p, *x0 = fl->d_x0.p;
-#define OCSL_ASM(V, BL, TR) \
-    k_assemble<kEq, V, BL, TR><<<grid, kBlock, fl->asm_lds, st>>>(m, s, ph, d, val, b, x0)
+#define OCSL_ASM(V, BL, TR)                                                                       \
+    do {                                                                                          \
+        if (fl->cell_perm)                                                                        \
+            k_assemble<kEq, V, BL, TR, true><<<grid, kBlock, fl->asm_lds, st>>>(m, s, ph, d, val, b, x0);  \
+        else                                                                                      \
+            k_assemble<kEq, V, BL, TR, false><<<grid, kBlock, fl->asm_lds, st>>>(m, s, ph, d, val, b, x0); \
+    } while (0)
     const int key = (p.viscous ? 4 : 0) + (p.blended ? 2 : 0) + (p.transient ? 1 : 0);
     switch (key) {
         case 0: OCSL_ASM(false, false, false); break;
@@ -1077,8 +1151,8 @@ void assemble(ocsl_flow_s* fl, int eq) {
     else if (eq == OCSL_EQ_PRESSURE) launch_assemble<kPressure>(fl, 0);
     else launch_assemble<kEnthalpy>(fl, 0);
     if (eq <= OCSL_EQ_MOMENTUM_Z && !fl->dev) {
-        down(fl->st.ap, fl->d_ap, st);
-        down(fl->st.ap_vec, fl->d_ap_vec, st);
+        down_c(fl, fl->st.ap, fl->d_ap, st);
+        down_c(fl, fl->st.ap_vec, fl->d_ap_vec, st);
     }
     OCSL_HIP(hipStreamSynchronize(st));
     ocsl_solver* h = fl->solver;
@@ -1095,8 +1169,8 @@ void assemble(ocsl_flow_s* fl, int eq) {
 void viscous_stress(ocsl_flow_s* fl) {
     hipStream_t st = fl->solver->stream;
     if (!fl->dev) {
-        up(fl->d_u, fl->st.u, st);
-        up(fl->d_T, fl->st.T, st);
+        up_c(fl, fl->d_u, fl->st.u, st);
+        up_c(fl, fl->d_T, fl->st.T, st);
     }
     const int F = fl->mesh.F;
     if (F > 0)
@@ -1151,7 +1225,7 @@ void iterate_dev(ocsl_flow_s* fl) {
         assemble(fl, OCSL_EQ_MOMENTUM_X + d);
         solve(fl, fl->prm.method_momentum);
         x = fl->solver->x_own.p;
-        k_take_u<<<grid(n), kBlock, 0, st>>>(s, x, d, n);
+        k_take_u<<<grid(n), kBlock, 0, st>>>(s, x, m.cperm, d, n);
     }
     if (F) k_face_mdot<<<grid(F), kBlock, 0, st>>>(m, s, blend);
     if (B) k_bc_mdot<<<grid(B), kBlock, 0, st>>>(m, s, ph);
@@ -1159,7 +1233,7 @@ void iterate_dev(ocsl_flow_s* fl) {
     assemble(fl, OCSL_EQ_PRESSURE);
     solve(fl, fl->prm.method_pressure);
     x = fl->solver->x_own.p;
-    k_take_pc<<<grid(n), kBlock, 0, st>>>(s, x, n);
+    k_take_pc<<<grid(n), kBlock, 0, st>>>(s, x, m.cperm, n);
     if (B) k_bp_after_p<<<grid(B), kBlock, 0, st>>>(m, s, ph);
     k_press_corr_grad<<<grid(n), kBlock, 0, st>>>(m, s, ph);
     if (F + B) k_correct_mdot<<<grid(F + B), kBlock, 0, st>>>(m, s, ph, blend);
@@ -1169,7 +1243,7 @@ void iterate_dev(ocsl_flow_s* fl) {
     assemble(fl, OCSL_EQ_ENTHALPY);
     solve(fl, fl->prm.method_enthalpy);
     x = fl->solver->x_own.p;
-    k_take_h<<<grid(n), kBlock, 0, st>>>(s, fl->gas, x, n);
+    k_take_h<<<grid(n), kBlock, 0, st>>>(s, fl->gas, x, m.cperm, n);
     OCSL_HIP(hipGetLastError());
 }
 
@@ -1337,8 +1411,8 @@ void alloc_state(ocsl_flow_s* fl) {
     s.b_super.assign(B, 0);
     hipStream_t st = fl->solver->stream;
     up_f(fl, fl->d_f_stress, s.f_stress, st);
-    up(fl->d_ap, s.ap, st);
-    up(fl->d_ap_vec, s.ap_vec, st);
+    up_c(fl, fl->d_ap, s.ap, st);
+    up_c(fl, fl->d_ap_vec, s.ap_vec, st);
     for (auto* d : {&fl->d_u, &fl->d_u_old, &fl->d_gp}) d->alloc(3 * (size_t)n);
     for (auto* d : {&fl->d_rho, &fl->d_rho_old, &fl->d_p, &fl->d_p_old, &fl->d_pc, &fl->d_T, &fl->d_h, &fl->d_h_old,
                     &fl->d_dt})
@@ -1464,18 +1538,54 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
             dpos[i] = find((int)i, (int)i);
             OCSL_REQUIRE(dpos[i] >= 0, OCSL_ERR_STRUCTURE, "row without diagonal");
         }
-        int span = 1;  // at most kBlock * kMaxRow entries: 32 KB
-        for (size_t i0 = 0; i0 < n; i0 += kBlock) span = std::max(span, rp32[std::min(n, i0 + kBlock)] - rp32[i0]);
-        fl->asm_lds = sizeof(double) * (size_t)span;
+        // the device's cell order (ocsl_flow_cell_order)
+        OCSL_REQUIRE(prm->cell_order >= OCSL_CELL_ORDER_AUTO && prm->cell_order <= OCSL_CELL_ORDER_LOCALITY,
+                     OCSL_ERR_ARG, "bad cell order");
+        fl->cperm.resize(n);
+        std::iota(fl->cperm.begin(), fl->cperm.end(), 0);
+        fl->cell_perm = false;
+        if (prm->cell_order != OCSL_CELL_ORDER_CALLER && n > 1) {
+            std::vector<int> rcm = rcm_order(fl->solver->host);  // new -> old
+            bool use = prm->cell_order == OCSL_CELL_ORDER_LOCALITY;
+            if (!use) {
+                // mean distance of face neighbours in each numbering
+                std::vector<int> inv(n);
+                for (size_t k = 0; k < n; ++k) inv[rcm[k]] = (int)k;
+                double dc = 0.0, dr = 0.0;
+                for (size_t f = 0; f < F; ++f) {
+                    dc += std::abs(m.fL[f] - m.fR[f]);
+                    dr += std::abs(inv[m.fL[f]] - inv[m.fR[f]]);
+                }
+                use = dc > 2.0 * dr;
+            }
+            if (use) {
+                fl->cperm = rcm;
+                fl->cell_perm = true;
+            }
+        }
+        const std::vector<int>& cperm = fl->cperm;
+        std::vector<int>& ciperm = fl->ciperm;
+        ciperm.resize(n);
+        for (size_t k = 0; k < n; ++k) ciperm[cperm[k]] = (int)k;
+        if (fl->cell_perm) {
+            fl->asm_lds = sizeof(double) * (size_t)kBlock * kMaxRow;  // one column of slots per thread
+        } else {
+            int span = 1;  // at most kBlock * kMaxRow entries: 32 KB
+            for (size_t i0 = 0; i0 < n; i0 += kBlock)
+                span = std::max(span, rp32[std::min(n, i0 + kBlock)] - rp32[i0]);
+            fl->asm_lds = sizeof(double) * (size_t)span;
+        }
         fl->fperm.resize(F);
         std::iota(fl->fperm.begin(), fl->fperm.end(), 0);
-        std::stable_sort(fl->fperm.begin(), fl->fperm.end(), [&](int x, int y) { return m.fL[x] < m.fL[y]; });
+        std::stable_sort(fl->fperm.begin(), fl->fperm.end(),
+                         [&](int x, int y) { return ciperm[m.fL[x]] < ciperm[m.fL[y]]; });
         up(fl->d_fperm, fl->fperm, fl->solver->stream);
         OCSL_HIP(hipStreamSynchronize(fl->solver->stream));
         std::vector<int> finv(F);
         for (size_t k = 0; k < F; ++k) finv[fl->fperm[k]] = (int)k;
         // each cell's faces in the caller's ascending face order (the order
-        // the reference's face loop adds them in), coded with device indices
+        // the reference's face loop adds them in), coded with device indices;
+        // cells, neighbours and faces in the device order
         std::vector<std::vector<int>> cf(n), cb(n);
         for (size_t f = 0; f < F; ++f) {
             cf[m.fL[f]].push_back((int)f << 1);
@@ -1484,21 +1594,22 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
         for (size_t b = 0; b < B; ++b) cb[m.bL[b]].push_back((int)b);
         std::vector<int> cf_ptr(n + 1, 0), cf_code, cf_pos, cf_nb, cb_ptr(n + 1, 0), cb_face;
         std::vector<double> cf_geo, cf_ksi;
-        for (size_t i = 0; i < n; ++i) {
-            for (int code : cf[i]) {
+        for (size_t k = 0; k < n; ++k) {
+            const int c = cperm[k];
+            for (int code : cf[c]) {
                 const int f = code >> 1;
                 const int other = (code & 1) ? m.fL[f] : m.fR[f];
-                const int pos = find((int)i, other);
+                const int pos = find(c, other);
                 OCSL_REQUIRE(pos >= 0, OCSL_ERR_STRUCTURE, "face neighbour missing from the matrix row");
                 cf_code.push_back((finv[f] << 1) | (code & 1));
                 cf_pos.push_back(pos);
-                cf_nb.push_back(other);
+                cf_nb.push_back(ciperm[other]);
                 cf_geo.push_back(m.fgeo[f]);
                 cf_ksi.push_back(m.fksi[f]);
             }
-            cf_ptr[i + 1] = (int)cf_code.size();
-            for (int b : cb[i]) cb_face.push_back(b);
-            cb_ptr[i + 1] = (int)cb_face.size();
+            cf_ptr[k + 1] = (int)cf_code.size();
+            for (int b : cb[c]) cb_face.push_back(b);
+            cb_ptr[k + 1] = (int)cb_face.size();
         }
         // the sliced form of the cell-face lists (MeshDev::cf_sp)
         const size_t nsl = (n + 63) / 64;
@@ -1529,9 +1640,27 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
             cf_ksi.swap(sk);
         }
         hipStream_t s = fl->solver->stream;
-        up(fl->d_vol, m.vol, s);
-        up_mesh_f(fl, fl->d_fL, m.fL, s);
-        up_mesh_f(fl, fl->d_fR, m.fR, s);
+        // cell-indexed mesh data and cell indices in the device order
+        std::vector<double> vol_d(n);
+        std::vector<int> dpos_d(n), rstart(n), rlen(n), fL_d(F), fR_d(F), bL_d(B);
+        for (size_t k = 0; k < n; ++k) {
+            const int c = cperm[k];
+            vol_d[k] = m.vol[c];
+            dpos_d[k] = dpos[c];
+            rstart[k] = rp32[c];
+            rlen[k] = rp32[c + 1] - rp32[c];
+        }
+        for (size_t f = 0; f < F; ++f) {
+            fL_d[f] = ciperm[m.fL[f]];
+            fR_d[f] = ciperm[m.fR[f]];
+        }
+        for (size_t b = 0; b < B; ++b) bL_d[b] = ciperm[m.bL[b]];
+        up(fl->d_cperm, cperm, s);
+        up(fl->d_rstart, rstart, s);
+        up(fl->d_rlen, rlen, s);
+        up(fl->d_vol, vol_d, s);
+        up_mesh_f(fl, fl->d_fL, fL_d, s);
+        up_mesh_f(fl, fl->d_fR, fR_d, s);
         up_mesh_f(fl, fl->d_fS, m.fS, s);
         up_mesh_f(fl, fl->d_fe, m.fe, s);
         up_mesh_f(fl, fl->d_fdist, m.fdist, s);
@@ -1541,7 +1670,7 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
         up(fl->d_be, m.be, s);
         up(fl->d_balpha, m.balpha, s);
         up(fl->d_bdist, m.bdist, s);
-        up(fl->d_bL, m.bL, s);
+        up(fl->d_bL, bL_d, s);
         up(fl->d_bmark, m.bmark, s);
         up(fl->d_bS, m.bS, s);
         up(fl->d_barea, m.barea, s);
@@ -1556,7 +1685,7 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
         up(fl->d_cb_ptr, cb_ptr, s);
         up(fl->d_cb_face, cb_face, s);
         up(fl->d_row_ptr, rp32, s);
-        up(fl->d_dpos, dpos, s);
+        up(fl->d_dpos, dpos_d, s);
         fl->d_val.alloc(nnz);
         fl->d_b.alloc(n);
         fl->d_x0.alloc(n);
@@ -1747,7 +1876,7 @@ ocsl_status ocsl_flow_time_assemble(ocsl_flow fl, int equation, int reps, double
             OCSL_HIP(hipMemcpyAsync(fl->d_ap.p, ap_saved.p, sizeof(double) * fl->mesh.n, hipMemcpyDeviceToDevice,
                                     h->stream));
         else
-            up(fl->d_ap, fl->st.ap, h->stream);
+            up_c(fl, fl->d_ap, fl->st.ap, h->stream);
         OCSL_HIP(hipStreamSynchronize(h->stream));
     });
 }
@@ -1776,6 +1905,12 @@ ocsl_status ocsl_flow_run(ocsl_flow fl, int steps, int inner) {
             update_timestep(fl);
         }
     });
+}
+
+ocsl_status ocsl_flow_cell_order_get(ocsl_flow fl, int32_t* order) {
+    if (!fl || !order) return OCSL_ERR_ARG;
+    std::copy(fl->cperm.begin(), fl->cperm.end(), order);
+    return OCSL_OK;
 }
 
 ocsl_status ocsl_flow_solve_history(ocsl_flow fl, double* out, int64_t cap, int64_t* len) {

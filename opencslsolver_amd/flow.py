@@ -22,6 +22,8 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 # ocsl_bc_type (enums.h:38 order)
 EMPTY, SYMMETRY, WALL, VELOCITY_INLET, SUPERSONIC_INLET, PRESSURE_OUTLET = range(6)
 MOMENTUM_X, MOMENTUM_Y, MOMENTUM_Z, PRESSURE, ENTHALPY = range(5)
+# ocsl_flow_cell_order: how the device stores cells
+CELL_ORDER_AUTO, CELL_ORDER_CALLER, CELL_ORDER_LOCALITY = range(3)
 # ocsl_flow_support: where the SIMPLE step's support loops run
 SUPPORT_DEVICE, SUPPORT_HOST = range(2)
 MAX_BC = 16
@@ -56,7 +58,7 @@ class Params(ctypes.Structure):
                 ("t2", ctypes.c_double), ("n_bc", ctypes.c_int), ("bc", BC * MAX_BC),
                 ("method_momentum", ctypes.c_int), ("method_pressure", ctypes.c_int),
                 ("method_enthalpy", ctypes.c_int), ("mode", ctypes.c_int), ("precision", ctypes.c_int),
-                ("support", ctypes.c_int), ("solve_options", _abi.Options)]
+                ("support", ctypes.c_int), ("solve_options", _abi.Options), ("cell_order", ctypes.c_int)]
 
 
 CELL_FIELDS = {"u": 3, "u_old": 3, "rho": 1, "rho_old": 1, "p": 1, "p_old": 1, "p_corr": 1, "grad_p": 3,
@@ -87,6 +89,7 @@ SIGNATURES = {
     "ocsl_flow_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "ocsl_flow_solve_history": (ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_int64,
                                                ctypes.POINTER(ctypes.c_int64)]),
+    "ocsl_flow_cell_order_get": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
 }
 
 
@@ -259,6 +262,12 @@ class Flow:
         _check(self._lib.ocsl_flow_solve_history(self._h, None, 0, ctypes.byref(ln)))
         out = np.empty(ln.value)
         _check(self._lib.ocsl_flow_solve_history(self._h, out.ctypes.data_as(_f64p), ln.value, ctypes.byref(ln)))
+        return out
+
+    def cell_order(self) -> np.ndarray:
+        """The device's cell order: entry k is the caller's cell stored k-th."""
+        out = np.empty(self.n, dtype=np.int32)
+        _check(self._lib.ocsl_flow_cell_order_get(self._h, out.ctypes.data_as(_i32p)))
         return out
 
 

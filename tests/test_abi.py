@@ -3,6 +3,8 @@
 import ctypes
 import os
 import re
+import shutil
+import subprocess
 
 import pytest
 
@@ -40,18 +42,28 @@ def test_flow_signatures_cover_header():
     assert sorted(flow.SIGNATURES) == _declared("ocsl_flow.h")
 
 
-def test_flow_struct_layouts():
-    """ctypes mirrors of the flow structs have the C sizes (checked against
-    offsets computed from the header's field list)."""
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="no host compiler")
+def test_flow_struct_layouts(tmp_path):
+    """ctypes mirrors of the ABI structs have the C compiler's sizes and field
+    offsets (a C program over the headers prints them)."""
     import ctypes
 
     from opencslsolver_amd import _abi, flow
 
-    assert ctypes.sizeof(flow.BC) == 3 * 4 + 4 + 5 * 8
-    assert ctypes.sizeof(flow.Mesh) == 3 * 8 + 19 * 8
-    assert ctypes.sizeof(flow.State) == 29 * 8
-    assert flow.Params.solve_options.offset % 8 == 0
-    assert ctypes.sizeof(flow.Params) == flow.Params.solve_options.offset + ctypes.sizeof(_abi.Options)
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "ocsl_flow.h"\n'
+        'int main(void) {\n'
+        '  printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ocsl_bc), sizeof(ocsl_mesh), sizeof(ocsl_flow_state),\n'
+        '         sizeof(ocsl_flow_params), offsetof(ocsl_flow_params, solve_options),\n'
+        '         offsetof(ocsl_flow_params, cell_order), sizeof(ocsl_options), sizeof(ocsl_report));\n'
+        '  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == [ctypes.sizeof(flow.BC), ctypes.sizeof(flow.Mesh), ctypes.sizeof(flow.State),
+                   ctypes.sizeof(flow.Params), flow.Params.solve_options.offset, flow.Params.cell_order.offset,
+                   ctypes.sizeof(_abi.Options), ctypes.sizeof(_abi.Report)]
 
 
 def test_no_device_is_reported_not_fatal():
