@@ -260,6 +260,12 @@ ocsl_status ocsl_tune_cg(ocsl_handle h, int precision, int iterations, ocsl_opti
  * ocsl_tune_cg results); all-zero fields clear it. */
 ocsl_status ocsl_set_tuned_cg(ocsl_handle h, int precision, const ocsl_options* tuned);
 
+/* REFERENCE-mode solves (ocsl_CG_solve, ocsl_ILU_BICGSTAB_solve and
+ * ocsl_solve in REFERENCE mode) replay their whole device work as one hipGraph
+ * once a solve with the same right-hand-side address has run (on: 1, the
+ * default).  0 issues every launch each time; results are bitwise the same. */
+ocsl_status ocsl_set_reference_graphs(ocsl_handle h, int on);
+
 /* HIP stream of the handle (a hipStream_t), for callers that pass device
  * pointers produced on another stream. */
 void* ocsl_get_stream(ocsl_handle h);

@@ -99,6 +99,7 @@ SIGNATURES = {
     "ocsl_gemv": (ctypes.c_int, [c_handle, ctypes.c_double, ctypes.c_void_p, ctypes.c_double,
                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "ocsl_ILU_decompose": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
+    "ocsl_set_reference_graphs": (ctypes.c_int, [c_handle, ctypes.c_int]),
     "ocsl_ILU_decompose_from": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_int]),
     "ocsl_ILU_forsub": (ctypes.c_int, [c_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),

@@ -493,4 +493,12 @@ ocsl_status ocsl_ILU_decompose(ocsl_handle h, int sweeps, double* m_out, ocsl_wh
 
 void* ocsl_get_stream(ocsl_handle h) { return h ? (void*)h->stream : nullptr; }
 
+ocsl_status ocsl_set_reference_graphs(ocsl_handle h, int on) {
+    if (!h || (on != 0 && on != 1)) return OCSL_ERR_ARG;
+    return guarded([&] {
+        sync(h);
+        h->ref_graphs = on != 0;
+    });
+}
+
 }  // extern "C"

@@ -98,12 +98,24 @@ void cg_reference(ocsl_solver* h, ocsl_report* rep) {
     const double* beta = field(offsetof(Scalars, beta));
     const int gd = red_blocks(kBlock * 4, n);
     const int gv = grid_for(n);
-    Scalars init{};
-    OCSL_HIP(hipMemcpyAsync(S, &init, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
+    const Scalars init{};
     OCSL_HIP(hipEventRecord(h->ev0, h->stream));
+    // the solve's device work: replayed as one graph after its first run.  r
+    // and p rotate through the iterations, so the buffers are picked anew at
+    // each issue.
+    double* const r_first = r;
+    double* const rn_first = rn;
+    double* const p_first = p;
+    double* const pn_first = pn;
+    auto body = [&] {
+    r = r_first;
+    rn = rn_first;
+    p = p_first;
+    pn = pn_first;
+    k_set_scalars<<<1, 64, 0, h->stream>>>(S, init);
     // r = b - A x ; p = r
     launch_spmv(h, s, -1.0, x, 1.0, b, r);
-    OCSL_HIP(hipMemcpyAsync(p, r, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+    k_permute_in<double><<<grid_for(n), kBlock, 0, h->stream>>>(n, nullptr, r, p);
     for (int it = 0; it < 4; ++it) {
         launch_spmv(h, s, 1.0, p, 0.0, nullptr, q);
         DotArgs<2> d2{{r, p}, {r, q}, {0, 0}};
@@ -121,6 +133,8 @@ void cg_reference(ocsl_solver* h, ocsl_report* rep) {
         std::swap(p, pn);
     }
     OCSL_HIP(hipGetLastError());
+    };
+    run_reference(h, 2, b, body);
     OCSL_HIP(hipEventRecord(h->ev1, h->stream));
     // the one read-back of the solve: iterations, residual, history
     OCSL_HIP(hipMemcpyAsync(h->S_host, S, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));

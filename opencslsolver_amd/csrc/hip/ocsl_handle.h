@@ -159,7 +159,9 @@ struct ocsl_solver {
     ocsl::DBuf<double> grad_own;
 
     // hipGraph cache of iteration chunks (cg_graph_key, bicgstab_graph_key)
+    // and of whole REFERENCE-mode solves (reference_graph_key)
     std::map<ocsl::GraphKey, hipGraphExec_t> graphs;
+    bool ref_graphs = true;  // REFERENCE solves replay a graph after their first run
 
     ocsl::RedScratch red() const { return {red_partial.p}; }
 };
@@ -195,6 +197,38 @@ inline T* work_buf(ocsl_solver* h, DBuf<T>& b, size_t count) {
         b.alloc(count);
     }
     return b.p;
+}
+
+// A REFERENCE-mode solve's device work (`body` issues it on the handle's
+// stream): run eagerly the first time a right-hand-side address is seen (the
+// run allocates whatever the solve allocates lazily), and captured then as a
+// graph that later solves with the same address replay.  Returns after the
+// work is issued.
+template <class F>
+void run_reference(ocsl_solver* h, int family, const void* b, F&& body) {
+    if (!h->ref_graphs) {
+        body();
+        return;
+    }
+    const GraphKey key = reference_graph_key(family, b);
+    auto it = h->graphs.find(key);
+    if (it != h->graphs.end()) {
+        OCSL_HIP(hipGraphLaunch(it->second, h->stream));
+        return;
+    }
+    body();
+    // at most a few right-hand-side buffers per family are kept
+    int same = 0;
+    for (const auto& kv : h->graphs) same += kv.first.a == key.a;
+    if (same >= 4) drop_graphs(h);
+    hipGraph_t gr;
+    OCSL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    OCSL_HIP(hipStreamEndCapture(h->stream, &gr));
+    hipGraphExec_t ex;
+    OCSL_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+    OCSL_HIP(hipGraphDestroy(gr));
+    h->graphs.emplace(key, ex);
 }
 
 // Calls f(std::integral_constant<int, R>) for the SELL-P rows per lane R.

@@ -45,6 +45,12 @@ __global__ void k_permute_out(int n, const int* __restrict__ perm, const T* __re
         out[perm ? perm[i] : i] = in[i];
 }
 
+// *S = v, from a kernel: a solve that may be replayed as a graph sets its
+// scalars without a host-to-device copy.
+[[maybe_unused]] static __global__ void k_set_scalars(Scalars* S, Scalars v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *S = v;
+}
+
 template <class T>
 __global__ void k_fill(int n, T v, T* __restrict__ out) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = v;

@@ -165,6 +165,15 @@ inline GraphKey cg_graph_key(int kind, const CgLaunch& L, int chunk, int rpl) {
     return {a, chunk * 8 + rpl, 0};
 }
 
+// REFERENCE-mode solves (family 1: ILU_BICGSTAB_solve, 2: CG_solve): the
+// whole solve is one graph; it bakes in the right-hand side's address, the
+// one caller buffer a solve reads that can change between solves (bit 50
+// set: no key of the other families).
+inline GraphKey reference_graph_key(int family, const void* b) {
+    if (family < 1 || family > 2) throw Error{OCSL_ERR_ARG, "graph key: reference family"};
+    return {(int64_t)1 << 50 | family, 0, (uint64_t)(uintptr_t)b};
+}
+
 // Tolerance BiCGSTAB: its iteration form (< 2^40), the chunk length and the
 // damped-Jacobi weight of its triangular sweeps, which the captured sweeps
 // take as an argument.

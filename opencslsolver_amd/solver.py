@@ -166,6 +166,11 @@ class Solver:
         _check(st, allow_warn=True)
         return X, [r.as_dict() for r in reps]
 
+    def set_reference_graphs(self, on: bool) -> None:
+        """REFERENCE-mode solves replay one hipGraph per right-hand-side address
+        after their first run (default), or issue every launch each time."""
+        _check(self._lib.ocsl_set_reference_graphs(self._h, 1 if on else 0))
+
     def CG_solve(self) -> float:
         """CMatrix_solver::CG_solve (cmatrix_solver.cpp:635-680)."""
         res = ctypes.c_double()

@@ -164,6 +164,10 @@ void check_keys() {
     for (int64_t kind : {(int64_t)0, (int64_t)16, (int64_t)1 << 20, ((int64_t)1 << 40) - 1})
         for (double w : {0.9, 0.8, 1.0})
             if (!keys.insert(bicgstab_graph_key(kind, 32, w)).second) fail("BiCGSTAB key collides");
+    static double b0[4], b1[4];
+    for (int fam : {1, 2})
+        for (const void* b : {(const void*)b0, (const void*)b1, (const void*)nullptr})
+            if (!keys.insert(reference_graph_key(fam, b)).second) fail("REFERENCE key collides");
     for (int kind : {-1, 256}) {
         bool threw = false;
         try {
