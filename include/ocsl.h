@@ -218,6 +218,17 @@ ocsl_status ocsl_vec_product(int64_t n, const double* x, const double* y, double
 ocsl_status ocsl_vec_axpy(int64_t n, double a, const double* x, double b, const double* y, double* out,
                           ocsl_where where);
 
+/* ocsl_vec_ABSsum without any allocation (no counterpart in the reference:
+ * for callers that cannot allocate -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  a: n doubles in device
+ * memory; out: one double in device memory, written when `stream` (a
+ * hipStream_t; NULL: the null stream) reaches the call; nothing is copied to
+ * the host and nothing waits.  The partial sums live in a fixed device array
+ * of the library (16 KiB per device, held until the process ends) shared by
+ * every call on that device: calls that overlap on one device race.  n from 0
+ * to 2^31 - 1; the same result, bit for bit, as ocsl_vec_ABSsum. */
+ocsl_status ocsl_vec_ABSsum_noalloc(int64_t n, const double* a, double* out, void* stream);
+
 /* Measurement hook (no counterpart in the reference): launches the tolerance-
  * mode CG SpMV kernel (q = A p with its fused dot products) `reps` times on
  * the handle's stream between two HIP events and returns its average duration

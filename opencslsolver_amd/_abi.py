@@ -105,6 +105,7 @@ SIGNATURES = {
     "ocsl_ILU_forsub": (ctypes.c_int, [c_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "ocsl_ILU_backsub": (ctypes.c_int, [c_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "ocsl_vec_norm": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _f64p]),
+    "ocsl_vec_ABSsum_noalloc": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "ocsl_vec_sum": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, _f64p]),
     "ocsl_vec_ABSsum": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, _f64p]),
     "ocsl_vec_product": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

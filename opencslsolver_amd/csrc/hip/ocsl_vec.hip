@@ -130,6 +130,32 @@ ocsl_status vec_elementwise(int64_t n, double a, const double* x, double b, cons
     });
 }
 
+// ---------------------------------------------------------------- no allocation
+// Partial sums of ocsl_vec_ABSsum_noalloc: one per reduction block (at most
+// kMaxRedBlocks), in a file-scope device array instead of an allocation.
+// Each device holds its own copy at its own address, looked up once per
+// device with hipGetSymbolAddress; host code uses only those addresses.
+constexpr int kNoAllocPartials = kMaxRedBlocks;  // 16 KiB
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+double* scratch_for_current_device() {
+    int dev = 0;
+    OCSL_HIP(hipGetDevice(&dev));
+    OCSL_REQUIRE(dev >= 0 && dev < kMaxDevices, OCSL_ERR_ARG, "device index beyond the supported 64");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        double* ptr = nullptr;
+        OCSL_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+        OCSL_REQUIRE(ptr, OCSL_ERR_HIP, "no device address for the reduction scratch");
+        g_scratch[dev] = ptr;
+    }
+    return g_scratch[dev];
+}
+
 }  // namespace
 }  // namespace ocsl
 
@@ -147,6 +173,37 @@ ocsl_status ocsl_vec_sum(int64_t n, const double* a, ocsl_where where, double* o
 
 ocsl_status ocsl_vec_ABSsum(int64_t n, const double* a, ocsl_where where, double* out) {
     return vec_reduce(n, a, nullptr, 2, where, out);
+}
+
+ocsl_status ocsl_vec_ABSsum_noalloc(int64_t n, const double* a, double* out, void* stream) {
+    if (n < 0 || n > 0x7fffffff) {
+        set_last_error("ocsl_vec_ABSsum_noalloc: n must be 0 .. 2^31 - 1");
+        return OCSL_ERR_ARG;
+    }
+    if (!out || (n > 0 && !a)) {
+        set_last_error("ocsl_vec_ABSsum_noalloc: null pointer");
+        return OCSL_ERR_ARG;
+    }
+    try {
+        double* part = scratch_for_current_device();
+        hipStream_t st = (hipStream_t)stream;
+        // the same launch as ocsl_vec_ABSsum: the same partial sums, added in
+        // the same order; every partial read is written first by this call
+        const int g = n > 0 ? red_blocks(kBlock * 4, n) : 1;
+        OCSL_REQUIRE(g <= kNoAllocPartials, OCSL_ERR_ARG, "ocsl_vec_ABSsum_noalloc: too many reduction blocks");
+        DotArgs<1> d{{a}, {a}, {2}};
+        const RedScratch red{part};
+        k_dots<1><<<g, kBlock, 0, st>>>((int)n, d, red);
+        k_finalize<1, FinStore<1>><<<1, kBlock, 0, st>>>(red, g, FinStore<1>{out});
+        OCSL_HIP(hipGetLastError());
+        return OCSL_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what);
+        return e.status;
+    } catch (...) {
+        set_last_error("unexpected exception");
+        return OCSL_ERR_ARG;
+    }
 }
 
 ocsl_status ocsl_vec_axpy(int64_t n, double a, const double* x, double b, const double* y, double* out,
