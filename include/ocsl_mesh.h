@@ -87,6 +87,23 @@ typedef struct ocsl_vtk_fields {
  * (cpost_processing.cpp:26-127, 260-333). */
 int ocsl_mesh_write_vtk(ocsl_mesh m, const char* path, const ocsl_vtk_fields* fields, int precision);
 
+/* Boundary-face fields for ocsl_mesh_write_boundary_vtk, one value (vectors:
+ * three) per boundary face in ocsl_mesh_bc_faces order; NULL leaves a field
+ * out. */
+typedef struct ocsl_vtk_bc_fields {
+    const double* mach;        /* the Mach number of the face's cell */
+    const double* wall_stress; /* [B][3]: CFlow_solver::calc_wall_stress
+                                  (cflow_solver_supply.cpp:223-239) */
+} ocsl_vtk_bc_fields;
+
+/* Legacy-VTK unstructured grid of the mesh's boundary faces: all nodes, the
+ * boundary faces as cells (types 5 / 9), then per face Area[m2], normal,
+ * Marker, area_vector, center and the given fields, with the reference's
+ * names, keywords and order; numbers %.{precision}g (6: the reference's text).
+ * Reference: Cpost_processing::write_boundary_VTK_file /
+ * write_boundary_VTK_data (cpost_processing.cpp:335-401, 129-193). */
+int ocsl_mesh_write_boundary_vtk(ocsl_mesh m, const char* path, const ocsl_vtk_bc_fields* fields, int precision);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,4 +58,5 @@ def mesh_lib() -> ctypes.CDLL:
     lib.ocsl_mesh_faces.argtypes = [p] + [p] * 9
     lib.ocsl_mesh_bc_faces.argtypes = [p] + [p] * 9
     lib.ocsl_mesh_write_vtk.argtypes = [p, c.c_char_p, p, c.c_int]
+    lib.ocsl_mesh_write_boundary_vtk.argtypes = [p, c.c_char_p, p, c.c_int]
     return lib

@@ -641,4 +641,67 @@ int ocsl_mesh_write_vtk(ocsl_mesh m, const char* path, const ocsl_vtk_fields* fi
     return 0;
 }
 
+// Cpost_processing::write_boundary_VTK_file (cpost_processing.cpp:335-401)
+// and write_boundary_VTK_data (:129-193): the mesh's nodes, its boundary
+// faces as VTK cells (nodes in the stored order, types 5 / 9), then per face
+// its area, unit normal, marker, area vector and centre, and the flow
+// fields.
+int ocsl_mesh_write_boundary_vtk(ocsl_mesh m, const char* path, const ocsl_vtk_bc_fields* fields, int precision) {
+    if (!m || !path || precision < 1 || precision > 17) {
+        g_err = "bad argument";
+        return -1;
+    }
+    FILE* f = std::fopen(path, "w");
+    if (!f) {
+        g_err = std::string("cannot open ") + path + " for writing";
+        return -2;
+    }
+    const int p = precision;
+    const size_t B = m->boundary.size();
+    std::fprintf(f, "# vtk DataFile Version 1.0\nSolver Data File\nASCII\nDATASET UNSTRUCTURED_GRID\nPOINTS %zu double\n",
+                 m->nodes.size());
+    for (const V3& x : m->nodes) std::fprintf(f, "%.*g %.*g %.*g\n", p, x.v[0], p, x.v[1], p, x.v[2]);
+    size_t list = 0;
+    for (int q : m->boundary) list += 1 + (size_t)m->faces[(size_t)q].nn;
+    std::fprintf(f, "CELLS %zu %zu\n", B, list);
+    for (int q : m->boundary) {
+        const Face& fc = m->faces[(size_t)q];
+        std::fprintf(f, "%d", fc.nn);
+        for (int k = 0; k < fc.nn; ++k) std::fprintf(f, " %d", fc.node[k]);
+        std::fputc('\n', f);
+    }
+    std::fprintf(f, "CELL_TYPES %zu\n", B);
+    for (int q : m->boundary) std::fprintf(f, "%d\n", m->faces[(size_t)q].nn == 3 ? 5 : 9);
+    std::fprintf(f, "CELL_DATA %zu\n", B);
+    auto vec3 = [&](const V3& a) { std::fprintf(f, "%.*g %.*g %.*g\n", p, a.v[0], p, a.v[1], p, a.v[2]); };
+    std::fprintf(f, "SCALARS Area[m2] double 1\nLOOKUP_TABLE default\n");
+    for (int q : m->boundary) std::fprintf(f, "%.*g\n", p, m->faces[(size_t)q].area);
+    std::fprintf(f, "VECTORS normal double\n");
+    for (int q : m->boundary) vec3(m->faces[(size_t)q].normal);
+    std::fprintf(f, "SCALARS Marker double 1\nLOOKUP_TABLE default\n");
+    for (int q : m->boundary) std::fprintf(f, "%d\n", m->faces[(size_t)q].marker);
+    std::fprintf(f, "Vectors area_vector double\n");
+    for (int q : m->boundary) {
+        const Face& fc = m->faces[(size_t)q];
+        vec3(fc.area * fc.normal);
+    }
+    std::fprintf(f, "Vectors center double\n");
+    for (int q : m->boundary) vec3(m->faces[(size_t)q].center);
+    if (fields && fields->mach) {
+        std::fprintf(f, "SCALARS MACH double 1\nLOOKUP_TABLE default\n");
+        for (size_t b = 0; b < B; ++b) std::fprintf(f, "%.*g\n", p, fields->mach[b]);
+    }
+    if (fields && fields->wall_stress) {
+        std::fprintf(f, "Vectors Stress_vector double\n");
+        const double* w = fields->wall_stress;
+        for (size_t b = 0; b < B; ++b) std::fprintf(f, "%.*g %.*g %.*g\n", p, w[3 * b], p, w[3 * b + 1], p, w[3 * b + 2]);
+    }
+    const bool ok = !std::ferror(f);
+    if (std::fclose(f) != 0 || !ok) {
+        g_err = std::string("write to ") + path + " failed";
+        return -2;
+    }
+    return 0;
+}
+
 }  // extern "C"

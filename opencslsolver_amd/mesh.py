@@ -141,6 +141,76 @@ def write_vtk(su2, path: str, fields: dict | None = None, precision: int = 6) ->
         lib.ocsl_mesh_destroy(h)
 
 
+class _VtkBcFields(ctypes.Structure):
+    _fields_ = [("mach", ctypes.c_void_p), ("wall_stress", ctypes.c_void_p)]
+
+
+def wall_stress(fvm: dict, state: dict, bcs, viscous: bool, sutherland) -> np.ndarray:
+    """Wall shear stress per boundary face [B, 3], as
+    CFlow_solver::calc_wall_stress computes it (cflow_solver_supply.cpp:223-239):
+    on a no-slip wall of a viscous run, mu(T_L) / perp * (u_L - (u_L . n) n)
+    with n = S / |S| and u_L, T_L the face's cell; zero on every other face.
+    `fvm` holds bc_left, bc_marker, bc_area_vector, bc_area, bc_perp (the keys
+    read_su2 returns); `bcs` one dict per marker with "type" and "no_slip"
+    (the flow module's boundary conditions)."""
+    from .flow import WALL
+
+    left = np.asarray(fvm["bc_left"], np.int64)
+    marker = np.asarray(fvm["bc_marker"], np.int64)
+    out = np.zeros((left.size, 3))
+    if not viscous:
+        return out
+    on = np.array([bc.get("type") == WALL and bool(bc.get("no_slip", 0)) for bc in bcs], bool)[marker]
+    if not on.any():
+        return out
+    u = np.asarray(state["u"], np.float64).reshape(-1, 3)[left[on]]
+    T = np.asarray(state["T"], np.float64)[left[on]]
+    n = (1.0 / np.asarray(fvm["bc_area"], np.float64)[on])[:, None] * np.asarray(fvm["bc_area_vector"])[on]
+    un = (u[:, 0] * n[:, 0] + u[:, 1] * n[:, 1]) + u[:, 2] * n[:, 2]
+    par = u - un[:, None] * n
+    mu = sutherland[0] * np.power(T, 1.5)
+    mu = mu / (T + sutherland[1])
+    out[on] = (mu / np.asarray(fvm["bc_perp"], np.float64)[on])[:, None] * par
+    return out
+
+
+def write_boundary_vtk(su2, path: str, fields: dict | None = None, precision: int = 6) -> None:
+    """Writes the boundary faces of `su2` (SU2 text, or a path to an SU2 file)
+    as a legacy-VTK file, as Cpost_processing::write_boundary_VTK_file does
+    (cpost_processing.cpp:129-193, 335-401): one VTK cell per boundary face,
+    marker by marker in file order, with area, unit normal, marker index, area
+    vector and centre; `fields` may add "mach" [B] (the Mach number of each
+    face's cell) and "wall_stress" [B, 3] (``wall_stress``), which a flow run
+    writes."""
+    lib = mesh_lib()
+    h = ctypes.c_void_p()
+    if isinstance(su2, (bytes, str)) and ("\n" in (su2 if isinstance(su2, str) else su2.decode(errors="ignore"))):
+        data = su2.encode() if isinstance(su2, str) else bytes(su2)
+        st = lib.ocsl_mesh_parse_su2(data, len(data), ctypes.byref(h))
+    else:
+        st = lib.ocsl_mesh_read_su2(str(su2).encode(), ctypes.byref(h))
+    if st != 0:
+        raise MeshError(f"SU2 mesh: {lib.ocsl_mesh_last_error().decode()} (status {st})")
+    try:
+        nb = ctypes.c_int64()
+        lib.ocsl_mesh_sizes(h, None, None, None, ctypes.byref(nb), None)
+        keep, f = [], _VtkBcFields()
+        for k, v in (fields or {}).items():
+            if k not in ("mach", "wall_stress"):
+                raise ValueError(f"unknown boundary VTK field {k!r}")
+            a = np.ascontiguousarray(v, dtype=np.float64)
+            if a.size != nb.value * (3 if k == "wall_stress" else 1):
+                raise ValueError(f"field {k!r} has {a.size} values for {nb.value} boundary faces")
+            keep.append(a)
+            setattr(f, k, a.ctypes.data)
+        st = lib.ocsl_mesh_write_boundary_vtk(h, str(path).encode(), ctypes.byref(f) if fields else None,
+                                              int(precision))
+        if st != 0:
+            raise MeshError(f"VTK: {lib.ocsl_mesh_last_error().decode()} (status {st})")
+    finally:
+        lib.ocsl_mesh_destroy(h)
+
+
 # ------------------------------------------------------------------ writer
 SU2_TET, SU2_HEX, SU2_PRISM, SU2_TRI, SU2_QUAD = 10, 12, 13, 5, 9
 

@@ -119,4 +119,28 @@ void ref_last_run_dump(double* nodes, int32_t* cell_type, int32_t* cell_nodes, d
     }
 }
 
+// The boundary faces of the last run as Cpost_processing::write_boundary_VTK_file
+// and write_boundary_VTK_data see them (cpost_processing.cpp:129-193,
+// 335-401): per face of BC_face_global_indexes, its node count, nodes in the
+// stored order and marker; per boundary face of the flow solver, the Mach
+// number of its cell and CFlow_solver::calc_wall_stress
+// (cflow_solver_supply.cpp:223-239).
+void ref_last_run_bc_size(int64_t* n_bc) { *n_bc = (int64_t)root.geometry->BC_face_global_indexes.size(); }
+
+void ref_last_run_bc_dump(int32_t* face_nn, int32_t* face_nodes, int32_t* marker, double* mach, double* stress) {
+    const Cgeometry& g = *root.geometry;
+    for (size_t q = 0; q < g.BC_face_global_indexes.size(); ++q) {
+        const auto& f = g.faces[(size_t)g.BC_face_global_indexes[q]];
+        face_nn[q] = (int32_t)f.i_nodes.size();
+        for (int k = 0; k < 4; ++k) face_nodes[4 * q + k] = k < (int)f.i_nodes.size() ? f.i_nodes[(size_t)k] : -1;
+        marker[q] = f.boundary_marker;
+    }
+    auto& fl = *root.Flow_solver;
+    for (size_t q = 0; q < fl.BC_faces.size(); ++q) {
+        mach[q] = fl.BC_faces[q].left_cell_ptr->Mach;
+        const Svector w = fl.calc_wall_stress(root.FVM->BC_faces[q], fl.BC_faces[q], root.Fluid_properties);
+        put3(stress, q, w);
+    }
+}
+
 }  // extern "C"

@@ -161,9 +161,10 @@ class RefFlowMesh(ctypes.Structure):
 EMPTY, SYMMETRY, WALL, VELOCITY_INLET, SUPERSONIC_INLET, PRESSURE_OUTLET = range(6)
 
 
-def flow_run(cfg, max_dump, max_state):
+def flow_run(cfg, max_dump, max_state, keep_vtk=False, variant=""):
     """Runs the reference flow solver on its 1D mesh; returns the mesh, the
-    systems and flow states of the first solves, and the final fields."""
+    systems and flow states of the first solves, and the final fields.
+    variant "ieee" runs the build without contraction or fast-math."""
     c = RefFlowConfig()
     for k, v in cfg.items():
         if k in ("u0",):
@@ -180,7 +181,7 @@ def flow_run(cfg, max_dump, max_state):
             setattr(c, k, v)
     n = c.n_cells
     F, B, nnz0 = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-    oracle.ref().ref_flow_sizes(n, ctypes.c_double(c.lx), ctypes.c_double(c.ly), ctypes.c_double(c.lz),
+    oracle.ref(variant).ref_flow_sizes(n, ctypes.c_double(c.lx), ctypes.c_double(c.ly), ctypes.c_double(c.lz),
                                 ctypes.byref(F), ctypes.byref(B), ctypes.byref(nnz0))
     F, B = F.value, B.value
     nnz_cap = 5 * n
@@ -217,13 +218,17 @@ def flow_run(cfg, max_dump, max_state):
     with tempfile.TemporaryDirectory() as tmp:
         os.chdir(tmp)
         try:
-            cnt = oracle.ref().ref_flow_run(
+            cnt = oracle.ref(variant).ref_flow_run(
                 ctypes.byref(c), ctypes.c_int64(hist_cap), hist.ctypes.data_as(_f), max_dump,
                 ctypes.c_int64(nnz_cap), rp.ctypes.data_as(_u), col.ctypes.data_as(_u), ctypes.byref(nnz),
                 dv.ctypes.data_as(_f), db.ctypes.data_as(_f), dx0.ctypes.data_as(_f), dx.ctypes.data_as(_f),
                 dm.ctypes.data_as(_ip), max_state, ctypes.byref(st), ctypes.byref(mesh),
                 p.ctypes.data_as(_f), u.ctypes.data_as(_f), T.ctypes.data_as(_f), rho.ctypes.data_as(_f),
                 h.ctypes.data_as(_f))
+            vtk = {}
+            if keep_vtk:  # the files Croot::start_solver writes at the end (croot.cpp:235-240)
+                for name in ("solution.vtk", "solution_BC.vtk"):
+                    vtk[name] = open(os.path.join(tmp, name), "rb").read()
         finally:
             os.chdir(cwd)
     assert cnt > 0, cnt
@@ -233,6 +238,8 @@ def flow_run(cfg, max_dump, max_state):
                p=p, u=u.reshape(n, 3), T=T, rho=rho, h=h)
     out.update({"state_" + k: v[:max_state] for k, v in st_arr.items()})
     out.update({"mesh_" + k: v for k, v in mesh_arr.items()})
+    if keep_vtk:
+        out["vtk"] = vtk
     return out
 
 

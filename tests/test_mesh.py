@@ -229,3 +229,110 @@ def test_vtk_text_equals_the_reference_file(tmp_path):
     path = str(tmp_path / "mine.vtk")
     mesh.write_vtk(text, path, {k: d[k] for k in mesh.VTK_FIELDS}, precision=6)
     assert open(path, "rb").read() == d["vtk_text"].tobytes()
+
+
+def _fixture_su2(d):
+    """SU2 text of a VTK fixture's mesh: its cells, and its boundary faces
+    listed marker by marker in the reference's own boundary-face order."""
+    elems = [(int(t), [int(v) for v in ns if v >= 0]) for t, ns in zip(d["cell_type"], d["cell_nodes"])]
+    markers = []
+    for m in range(int(d["bc_marker"].max()) + 1):
+        sel = np.flatnonzero(d["bc_marker"] == m)
+        markers.append((f"m{m}", [(mesh.SU2_QUAD if d["bc_nn"][b] == 4 else mesh.SU2_TRI,
+                                   [int(v) for v in d["bc_nodes"][b][:d["bc_nn"][b]]]) for b in sel]))
+    return mesh.write_su2_text(d["nodes"], elems, markers)
+
+
+def _tokens_within_last_digit(mine: bytes, ref: bytes):
+    """The two texts hold the same words, and each number differs by at most
+    one unit in its last printed digit (a value at an exact decimal tie,
+    0.2265625 printed to six digits, goes either way on a one-ulp
+    difference)."""
+    a, b = mine.decode().split(), ref.decode().split()
+    assert len(a) == len(b)
+    moved = 0
+    for x, y in zip(a, b):
+        if x == y:
+            continue
+        fx, fy = float(x), float(y)  # a non-number raises: words must match
+        mant = y.split("e")[0].lstrip("-")
+        unit = 10.0 ** -(len(mant.split(".")[1]) if "." in mant else 0)
+        exp = float("1e" + y.split("e")[1]) if "e" in y else 1.0
+        assert abs(fx - fy) <= 1.0000001 * unit * exp, (x, y)
+        moved += 1
+    return moved
+
+
+@pytest.mark.parametrize("case", ["sod40", "duct64_ieee", "duct64"])
+def test_vtk_files_equal_the_reference_files(case, tmp_path):
+    """Both files the reference writes at the end of a run, solution.vtk and
+    solution_BC.vtk, from a 40-cell inviscid Sod run (slip walls) and a
+    64-cell viscous duct run (velocity inlet, no-slip walls, outlet); fixtures
+    from the reference build.  The cell file from the recorded cell fields;
+    the boundary-face file once from the reference's own per-face Mach and
+    wall stress and once with the wall stress computed by mesh.wall_stress
+    from the recorded cell velocity and temperature.  Against the reference
+    compiled without contraction (duct64_ieee) and on the Sod mesh the text is
+    the same byte for byte; the reference's own -ffast-math build rounds a few
+    duct cell and face centres one ulp the other way, which moves the six-digit
+    text of values at exact ties (0.2265625) by one in the last digit and
+    nothing else."""
+    from opencslsolver_amd import flow
+
+    d = np.load(os.path.join(GOLDEN, f"vtk_{case}.npz"))
+    same = (lambda a, b: a == b) if case != "duct64" else (lambda a, b: _tokens_within_last_digit(a, b) <= 24)
+    # the boundary faces are grouped by marker in the reference's order
+    assert np.all(np.diff(d["bc_marker"]) >= 0)
+    text = _fixture_su2(d)
+    path = str(tmp_path / "cells.vtk")
+    mesh.write_vtk(text, path, {k: d[k] for k in mesh.VTK_FIELDS}, precision=6)
+    assert same(open(path, "rb").read(), d["vtk_text"].tobytes())
+    bc_text = d["vtk_bc_text"].tobytes()
+    path = str(tmp_path / "bc.vtk")
+    mesh.write_boundary_vtk(text, path, {"mach": d["bc_mach"], "wall_stress": d["bc_stress"]}, precision=6)
+    assert same(open(path, "rb").read(), bc_text)
+    fvm = mesh.parse_su2(text)
+    assert np.array_equal(d["mach"][fvm["bc_left"]], d["bc_mach"])
+    if case.startswith("duct64"):
+        bcs = [{"type": flow.VELOCITY_INLET}, {"type": flow.WALL, "no_slip": 1}, {"type": flow.PRESSURE_OUTLET}]
+        viscous = True
+    else:
+        bcs = [{"type": flow.WALL}] * 3
+        viscous = False
+    ws = mesh.wall_stress(fvm, {"u": d["velocity"], "T": d["temperature"]}, bcs, viscous, (1.458e-6, 110.4))
+    scale = np.abs(d["bc_stress"]).max()
+    assert (scale > 0) == viscous
+    assert np.abs(ws - d["bc_stress"]).max() <= 1e-14 * max(scale, 1e-300)
+    mesh.write_boundary_vtk(text, path, {"mach": d["mach"][fvm["bc_left"]], "wall_stress": ws}, precision=6)
+    assert same(open(path, "rb").read(), bc_text)
+
+
+def test_last_digit_comparison_catches_real_differences():
+    """The fast-build comparison accepts a move of one in the last digit only."""
+    assert _tokens_within_last_digit(b"CELLS 2\n0.226563 1e-05\n", b"CELLS 2\n0.226562 1e-05\n") == 1
+    for mine in (b"CELLS 2\n0.226564 1e-05\n", b"CELLS 2\n0.226562 3e-05\n", b"CELL 2\n0.226562 1e-05\n"):
+        with pytest.raises((AssertionError, ValueError)):
+            _tokens_within_last_digit(mine, b"CELLS 2\n0.226562 1e-05\n")
+
+
+def test_boundary_vtk_without_flow_fields_and_errors(tmp_path):
+    """Without flow fields the boundary file stops after the face centres
+    (the reference writes Mach and wall stress for flow problems only); wrong
+    sizes, unknown fields and an unwritable path are errors."""
+    d = np.load(os.path.join(GOLDEN, "vtk_duct64_ieee.npz"))
+    text = _fixture_su2(d)
+    path = str(tmp_path / "bc.vtk")
+    mesh.write_boundary_vtk(text, path)
+    body = open(path, "rb").read()
+    full = d["vtk_bc_text"].tobytes()
+    assert full.startswith(body) and body.endswith(b"\n") and b"MACH" not in body
+    assert full[len(body):].startswith(b"SCALARS MACH double 1\n")
+    B = d["bc_mach"].size
+    with pytest.raises(ValueError):
+        mesh.write_boundary_vtk(text, path, {"mach": np.zeros(B - 1)})
+    with pytest.raises(ValueError):
+        mesh.write_boundary_vtk(text, path, {"wall_stress": np.zeros(B)})
+    with pytest.raises(ValueError):
+        mesh.write_boundary_vtk(text, path, {"stress": np.zeros(3 * B)})
+    with pytest.raises(mesh.MeshError):
+        mesh.write_boundary_vtk(text, "/nonexistent/dir/x.vtk")
