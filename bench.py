@@ -45,29 +45,93 @@ def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(a, b, iters_full, budget_s=12.0):
-    """Oracle CG (the reference's loop structure, its compile flags) on the host
-    cores: time a bounded number of iterations on the same system and scale to
-    the iteration count of a full solve."""
+def host_threads():
+    """Host threads for the CPU baselines: the CPUs this process may run on,
+    at most 16 (the host CPU share that comes with one GPU on this machine;
+    more threads than that share only time-slice against each other)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_model():
+    """The host CPU's model name and logical CPU count, from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            names = [ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs)" if names else None
+    except OSError:
+        return None
+
+
+def _ref_cg_child(n, threads, calls):
+    """Child-process leg of the CPU baseline: the reference's own CG_solve
+    (built from its sources for the build host's instruction set, so it runs
+    apart from this process) on the same seeded system; prints JSON."""
+    import oracle
+    from opencslsolver_amd import gen
+
+    a, b, _ = gen.poisson3d(n, seed_mesh=1, seed_rhs=2, sigma=0.5, leak=1e-6)
+    call, setup = oracle.ref_cg_timing(a, b, np.zeros(a.n), threads, calls)
+    print(json.dumps({"call_s": call, "setup_s": setup}))
+
+
+def cpu_baseline(a, b, n, iters_full, budget_s=12.0):
+    """The headline solve on the host cores, two ways.
+
+    reference: the reference's own CG_solve (cmatrix_solver.cpp:635-680, built
+    from its sources in oracle/_ref with its own flags and OpenMP), which stops
+    after 4 iterations by design: the median call and its set-up are timed and
+    one iteration = (call - set-up) / 4; its iteration computes A p twice and
+    five dot products.  port: the oracle's textbook FP64 CG (one A p per
+    iteration) with the reference's flags, the median of 5 chunks of
+    iterations.  Both per-iteration costs are scaled to the iterations of the
+    full FP64 solve: a full solve is not run, and the extrapolation is stated
+    in the line."""
+    import subprocess
+
     import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     oracle.set_threads(threads)
     t0 = time.perf_counter()
     oracle.cg_tol(a, b, tol=0.0, maxit=5, fast=True)
     per_it = (time.perf_counter() - t0) / 5.0
-    k = int(max(10, min(2000, budget_s / max(per_it, 1e-6))))
-    t0 = time.perf_counter()
-    oracle.cg_tol(a, b, tol=0.0, maxit=k, fast=True)
-    per_it = (time.perf_counter() - t0) / k
-    return {
-        "value": per_it * iters_full * 1e3,
-        "unit": "ms",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"{k} FP64 CG iterations of the same 128^3 system on {threads} host threads "
-                  f"({per_it * 1e3:.2f} ms/iteration), scaled to the {iters_full} iterations of a full solve",
-    }
+    k = int(max(10, min(400, budget_s / 5.0 / max(per_it, 1e-6))))
+    chunks = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        oracle.cg_tol(a, b, tol=0.0, maxit=k, fast=True)
+        chunks.append((time.perf_counter() - t0) / k)
+    port_it = float(np.median(chunks))
+    port = {"value": round(port_it * iters_full * 1e3, 1), "ms_per_iteration": round(port_it * 1e3, 3),
+            "sample": f"median of 5 chunks of {k} FP64 CG iterations of the same system"}
+    ref_it = None
+    if oracle.ref_available():
+        code = "import sys; sys.path.insert(0, sys.argv[1]); import bench; bench._ref_cg_child(*map(int, sys.argv[2:]))"
+        r = subprocess.run([sys.executable, "-c", code, ROOT, str(n), str(threads), "7"],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode == 0:
+            d = json.loads(r.stdout.strip().splitlines()[-1])
+            ref_it = (d["call_s"] - d["setup_s"]) / 4.0
+            ref_call = d
+        else:
+            _log(f"bench: reference CG unavailable here (exit {r.returncode}): {r.stderr[-300:]}")
+    common = {"unit": "ms", "cores": threads, "cpu_model": cpu_model(),
+              "iterations_scaled_to": iters_full}
+    if ref_it is None:
+        return {"value": port["value"], "kind": "port", **common,
+                "sample": f"{port['sample']} on {threads} host threads ({port['ms_per_iteration']} ms/iteration), "
+                          f"scaled to the {iters_full} iterations of the full FP64 solve"}
+    return {"value": round(ref_it * iters_full * 1e3, 1), "kind": "reference", **common,
+            "sample": f"the reference's CG_solve on {threads} OpenMP threads, median of 7 calls of its fixed 4 "
+                      f"iterations ({ref_call['call_s'] * 1e3:.2f} ms per call, {ref_call['setup_s'] * 1e3:.2f} ms "
+                      f"of it set-up): {ref_it * 1e3:.3f} ms/iteration, scaled to the {iters_full} iterations "
+                      f"of the full FP64 solve",
+            "ms_per_iteration": round(ref_it * 1e3, 3),
+            "port": port, "reference_over_port_per_iteration": round(ref_it / port_it, 3)}
 
 
 def measured_traffic(kernel):
@@ -179,7 +243,7 @@ def headline(args, out):
     cpu = None
     if not args.no_cpu_baseline:
         _log("bench: CPU baseline")
-        cpu = cpu_baseline(a, b, r64["iterations"])
+        cpu = cpu_baseline(a, b, n, r64["iterations"])
 
     kname, kname64 = dom_m["kernel"], dom_64["kernel"]
     triad = stream_triad()
@@ -421,7 +485,7 @@ def assembly(args, out):
     del mesh_n, rp_n, col_n
     cpu = None
     if not args.no_cpu_baseline:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_threads()
         oracle.set_threads(threads)
         t0 = time.perf_counter()
         for eq in range(5):
@@ -492,19 +556,27 @@ def mesh_read(args, out):
         if not args.no_cpu_baseline and oracle.ref_available():
             # a child process: the reference build targets the build host's
             # instruction set, and a failure there must not end this run
-            code = ("import sys, time; sys.path.insert(0, sys.argv[1]); import oracle; t = time.perf_counter(); "
-                    "oracle.ref_su2_arrays(sys.argv[2]); print(time.perf_counter() - t)")
+            code = ("import sys, json; sys.path.insert(0, sys.argv[1]); import oracle; "
+                    "print(json.dumps(oracle.ref_su2_stage_times(sys.argv[2])))")
             r = subprocess.run([sys.executable, "-c", code, ROOT, os.path.join(tmp, "box32.su2")],
                                capture_output=True, text=True, timeout=300)
             if r.returncode == 0:
-                ref_s = float(r.stdout.strip().splitlines()[-1])
+                ref_s = json.loads(r.stdout.strip().splitlines()[-1])
             else:
                 _log(f"bench: reference mesh reader unavailable here (exit {r.returncode})")
         if ref_s is not None:
-            cpu = {"value": round(ref_s * 1e3, 1), "unit": "ms for the 32^3 file", "cores": 1, "kind": "reference",
-                   "sample": "the reference's Read_SU2_mesh, create_FVM_from_geometry and define_matrix_structure "
-                             "(built from its sources) on the 32^3 file",
-                   "speedup_32": round(ref_s / res[32][0], 1)}
+            # like for like: the port's read_su2 parses, connects and computes
+            # the geometry and face factors, as Read_SU2_mesh and
+            # create_FVM_from_geometry do; the matrix structure is a separate
+            # step on both sides (the port's is the device structure builder)
+            like = ref_s[0] + ref_s[1]
+            cpu = {"value": round(like * 1e3, 1), "unit": "ms for the 32^3 file", "cores": 1, "kind": "reference",
+                   "sample": "the reference's Read_SU2_mesh + create_FVM_from_geometry (built from its sources, "
+                             "timed inside the library, its progress output included) on the 32^3 file",
+                   "stages_ms": {"Read_SU2_mesh": round(ref_s[0] * 1e3, 1),
+                                 "create_FVM_from_geometry": round(ref_s[1] * 1e3, 1),
+                                 "define_matrix_structure": round(ref_s[2] * 1e3, 1)},
+                   "speedup_32": round(like / res[32][0], 1)}
     t64 = res[64]
     out["mesh"] = {"workload": f"SU2 hexahedral box, 64^3 cells ({t64[1]} cells, {t64[2]} interior faces, "
                                f"{t64[3]} boundary faces), random numbering (seed 31): read, connect, geometry, "

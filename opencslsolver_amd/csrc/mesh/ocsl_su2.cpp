@@ -17,6 +17,8 @@
 #include "ocsl_mesh.h"
 
 #include <algorithm>
+#include <cerrno>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -125,8 +127,9 @@ struct Text {
         }
         return false;
     }
-    // the token copied into a scratch buffer ended there, not at its size
-    bool at_end(size_t n) const { return p + n >= end || std::strchr(" \t\n\r\v\f", p[n]); }
+    // the token copied into a scratch buffer ended there, not at its size (a
+    // NUL byte is not a separator: strchr would match the terminator)
+    bool at_end(size_t n) const { return p + n >= end || (p[n] != '\0' && std::strchr(" \t\n\r\v\f", p[n])); }
     void skip_ws() {
         while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r' || *p == '\v' || *p == '\f')) ++p;
     }
@@ -137,10 +140,18 @@ struct Text {
         while (p + n < end && n < sizeof(buf) - 1 && !std::strchr(" \t\n\r\v\f", p[n])) buf[n] = p[n], ++n;
         buf[n] = 0;
         char* q = nullptr;
+        errno = 0;
         const long long v = std::strtoll(buf, &q, 10);
-        if (n == 0 || q != buf + n || !at_end(n)) fail(-3, std::string("expected an integer (") + what + ")");
+        if (n == 0 || q != buf + n || !at_end(n) || errno == ERANGE)
+            fail(-3, std::string("expected an integer (") + what + ")");
         p += n;
         return v;
+    }
+    // a node number: 0 .. INT_MAX, range-checked before it is narrowed
+    int index(const char* what) {
+        const long long v = integer(what);
+        if (v < 0 || v > INT_MAX) fail(-3, std::string(what) + " out of range");
+        return (int)v;
     }
     double real(const char* what) {
         skip_ws();
@@ -149,8 +160,12 @@ struct Text {
         while (p + n < end && n < sizeof(buf) - 1 && !std::strchr(" \t\n\r\v\f", p[n])) buf[n] = p[n], ++n;
         buf[n] = 0;
         char* q = nullptr;
+        errno = 0;
         const double v = std::strtod(buf, &q);
         if (n == 0 || q != buf + n || !at_end(n)) fail(-3, std::string("expected a number (") + what + ")");
+        // overflow and inf / nan spellings; an underflow to a subnormal or zero is a value
+        if (!std::isfinite(v) || (errno == ERANGE && std::fabs(v) > 1.0))
+            fail(-3, std::string("non-finite number (") + what + ")");
         p += n;
         return v;
     }
@@ -378,7 +393,7 @@ void parse(ocsl_mesh_s& m, const char* text, size_t len) {
         else if (code == 12) c.type = kHex, nn = 8;
         else fail(-3, "unsupported element type " + std::to_string(code) + " (tetrahedra, prisms, hexahedra only)");
         int in[8];
-        for (int k = 0; k < nn; ++k) in[k] = (int)t.integer("element node");
+        for (int k = 0; k < nn; ++k) in[k] = t.index("element node");
         if (c.type == kHex) {
             // SU2 lists a hexahedron's bottom and top faces around; the cell
             // tables take nodes 2/3 and 6/7 swapped (cgeometry_supply.h:266-276)
@@ -415,12 +430,12 @@ void parse(ocsl_mesh_s& m, const char* text, size_t len) {
             const long long code = t.integer("boundary element type");
             int nodes[4];
             if (code == 5) {
-                for (int k = 0; k < 3; ++k) nodes[k] = (int)t.integer("boundary node");
+                for (int k = 0; k < 3; ++k) nodes[k] = t.index("boundary node");
                 add_bc_face(m, nc, nodes, 3, (int)mk);
             } else if (code == 9) {
                 // quadrilateral read as 0, 1, 3, 2 (cgeometry_supply.h:416-421)
                 const int map[4] = {0, 1, 3, 2};
-                for (int k = 0; k < 4; ++k) nodes[map[k]] = (int)t.integer("boundary node");
+                for (int k = 0; k < 4; ++k) nodes[map[k]] = t.index("boundary node");
                 add_bc_face(m, nc, nodes, 4, (int)mk);
             } else {
                 fail(-3, "unsupported boundary element type " + std::to_string(code));

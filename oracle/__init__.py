@@ -258,6 +258,37 @@ def ref_su2_arrays(path: str) -> dict:
     return out
 
 
+def ref_su2_stage_times(path: str) -> tuple:
+    """Seconds of the reference's three mesh set-up stages on one SU2 file,
+    timed inside the reference library (so its loading is not counted):
+    Read_SU2_mesh, create_FVM_from_geometry, define_matrix_structure.  Its
+    progress lines go to stderr."""
+    import sys
+
+    r = ref()
+    sz = (ctypes.c_int64 * 5)()
+    st = (ctypes.c_double * 3)()
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        r.ref_su2_read_timed(str(path).encode(), sz, st)
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+    return tuple(st)
+
+
+def ref_cg_timing(A, b, x0, threads: int, calls: int = 5) -> tuple:
+    """Median seconds of one call of the reference's CG_solve (its fixed 4
+    iterations, cmatrix_solver.cpp:635-680) and of that call's set-up alone,
+    with `threads` OpenMP threads (ref_driver.cpp)."""
+    call, setup = ctypes.c_double(), ctypes.c_double()
+    ref().ref_cg_timing(*_csr(A), _p(A.val), _p(b), _p(np.ascontiguousarray(x0, np.float64)),
+                        int(threads), int(calls), ctypes.byref(call), ctypes.byref(setup))
+    return call.value, setup.value
+
+
 def ref_cg_solve(A, b, x0, variant=""):
     x = np.array(x0, dtype=np.float64)
     res = ref(variant).ref_cg_solve(*_csr(A), _p(A.val), _p(b), _p(x))

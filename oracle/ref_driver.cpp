@@ -9,6 +9,8 @@
 // routine with one OpenMP thread and copies the results out.
 #include <omp.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -101,6 +103,41 @@ double ref_abssum(int64_t n, const double* a) {
     CMatrix_solver::Cvector va((int)n);
     va.value.assign(a, a + n);
     return CMatrix_solver::ABSsum(va);
+}
+
+// Times the reference's CG_solve (cmatrix_solver.cpp:635-680) as a caller
+// sees it, with `threads` OpenMP threads: `calls` calls on one system, each the
+// reference's fixed 4 iterations from the caller's x (x reset before each call,
+// so no call stops early), and as many runs of the call's own set-up alone
+// (its five work vectors, r = b - A x, p = r).  Medians in seconds; one
+// iteration costs (call - setup) / 4.
+void ref_cg_timing(int64_t n, const uint64_t* rp, const uint64_t* col, const double* val,
+                   const double* b, const double* x0, int threads, int calls, double* call_s,
+                   double* setup_s) {
+    using clk = std::chrono::steady_clock;
+    CMatrix_solver s = make_solver(n, rp, col, val, b, x0);
+    omp_set_num_threads(threads > 0 ? threads : 1);
+    std::vector<double> tc, ts;
+    for (int k = 0; k < calls; ++k) {
+        s.x_vector.value.assign(x0, x0 + n);
+        const auto t0 = clk::now();
+        (void)s.CG_solve();
+        tc.push_back(std::chrono::duration<double>(clk::now() - t0).count());
+        const auto t1 = clk::now();
+        {
+            CMatrix_solver::Cvector r((int)n), r_next((int)n), p((int)n), p_next((int)n), dum((int)n);
+            s.gemv(-1.0, s.matrix, s.x_vector, 1.0, s.b_vector, r);
+            p = r;
+        }
+        ts.push_back(std::chrono::duration<double>(clk::now() - t1).count());
+    }
+    auto median = [](std::vector<double> v) {
+        std::sort(v.begin(), v.end());
+        return v.empty() ? 0.0 : v[v.size() / 2];
+    };
+    *call_s = median(tc);
+    *setup_s = median(ts);
+    omp_set_num_threads(1);
 }
 
 }  // extern "C"

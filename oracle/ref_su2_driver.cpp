@@ -7,6 +7,7 @@
 // CFVM::create_FVM_from_geometry (cfvm.cpp:100-170) and
 // CFVM::define_matrix_structure (cfvm.cpp:74-98) on one file and keeps the
 // result; ref_su2_copy copies its arrays out in the reference's order.
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -25,13 +26,25 @@ std::shared_ptr<CMatrix_solver> g_ms;
 
 extern "C" {
 
-// sizes: n_cells, n_faces, n_bc, nnz, n_markers
-int ref_su2_read(const char* path, int64_t* sizes) {
+// sizes: n_cells, n_faces, n_bc, nnz, n_markers.  stage_s, when given,
+// receives the seconds of the three stages: Read_SU2_mesh,
+// create_FVM_from_geometry, define_matrix_structure.
+int ref_su2_read_timed(const char* path, int64_t* sizes, double* stage_s) {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
     root.geometry = std::make_shared<Cgeometry>(Read_SU2_mesh(path));
+    const auto t1 = clk::now();
     g_fvm = std::make_shared<CFVM>();
     root.FVM = g_fvm;
     g_fvm->create_FVM_from_geometry();
+    const auto t2 = clk::now();
     g_fvm->define_matrix_structure(g_ms);
+    const auto t3 = clk::now();
+    if (stage_s) {
+        stage_s[0] = std::chrono::duration<double>(t1 - t0).count();
+        stage_s[1] = std::chrono::duration<double>(t2 - t1).count();
+        stage_s[2] = std::chrono::duration<double>(t3 - t2).count();
+    }
     sizes[0] = (int64_t)g_fvm->cells.size();
     sizes[1] = (int64_t)g_fvm->faces.size();
     sizes[2] = (int64_t)g_fvm->BC_faces.size();
@@ -39,6 +52,8 @@ int ref_su2_read(const char* path, int64_t* sizes) {
     sizes[4] = (int64_t)root.geometry->n_boundary;
     return 0;
 }
+
+int ref_su2_read(const char* path, int64_t* sizes) { return ref_su2_read_timed(path, sizes, nullptr); }
 
 static void put3(double* d, size_t i, Svector v) {
     for (int k = 0; k < 3; ++k) d[3 * i + k] = v[k];

@@ -118,6 +118,9 @@ def test_file_and_memory_agree(tmp_path):
         assert np.array_equal(a[k], b[k]), k
 
 
+_TET = "NDIME= 3\nNELEM= 1\n10 0 1 2 3 0\nNPOIN= 4\n0 0 0 0\n1 0 0 1\n0 1 0 2\n0 0 1 3\nNMARK= 0\n"
+
+
 @pytest.mark.parametrize("text,needle", [
     ("NDIME= 3\nNELEM= 1\n14 0 1 2 3 4 0\n", "unsupported element type 14"),
     ("NDIME= 3\nNELEM= 1\n10 0 1 2 3 0\n", "NPOIN"),
@@ -126,10 +129,30 @@ def test_file_and_memory_agree(tmp_path):
      "NMARK= 1\nMARKER_TAG= w\nMARKER_ELEMS= 1\n5 0 1 4\n", "share no cell"),
     ("NDIME= 3\nNELEM= 1\n10 0 1 x 3 0\n", "integer"),
     ("NDIME= 3\nNELEM= 1\n10 0 1 " + "2" * 80 + " 3 0\n", "integer"),
+    # 2^32 + 3 would narrow to node 3
+    (_TET.replace("10 0 1 2 3 0", "10 0 1 2 4294967299 0"), "element node out of range"),
+    (_TET.replace("10 0 1 2 3 0", "10 0 1 2 -1 0"), "element node out of range"),
+    (_TET.replace("NMARK= 0\n", "NMARK= 1\nMARKER_TAG= w\nMARKER_ELEMS= 1\n5 0 1 4294967298\n"),
+     "boundary node out of range"),
+    # strtoll overflow (ERANGE) would give LLONG_MAX
+    (_TET.replace("10 0 1 2 3 0", "10 0 1 2 99999999999999999999 0"), "integer"),
+    (_TET.replace("1 0 0 1\n", "1e999 0 0 1\n"), "non-finite"),
+    (_TET.replace("1 0 0 1\n", "inf 0 0 1\n"), "non-finite"),
+    (_TET.replace("1 0 0 1\n", "nan 0 0 1\n"), "non-finite"),
+    # a NUL byte is not a separator
+    (_TET.replace("10 0 1 2 3 0", "10 0 1 2 3\x00 0"), "integer"),
+    (_TET.replace("1 0 0 1\n", "1\x00 0 0 1\n"), "number"),
 ])
 def test_malformed_input_is_rejected(text, needle):
     with pytest.raises(mesh.MeshError, match=needle):
         mesh.parse_su2(text)
+
+
+def test_tiny_tet_text_is_valid():
+    """The base text of the malformed cases parses; an underflowing coordinate
+    (a subnormal value) is a value, not an error."""
+    assert mesh.parse_su2(_TET)["cell_volume"].shape == (1,)
+    assert mesh.parse_su2(_TET.replace("0 0 0 0\n", "1e-320 0 0 0\n"))["cell_volume"].shape == (1,)
 
 
 def test_missing_file():
