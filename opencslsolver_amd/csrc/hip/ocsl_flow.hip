@@ -357,8 +357,8 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
 // rows lie together in the caller's value array, so they are staged as laid
 // out there (sentinel slots stay 0) and written out as one coalesced run.
 // Cells in an order of their own (kPerm): each thread's row sits in a column
-// of LDS slots and is written to the caller's row of its cell; b and x0 go
-// to the caller's index of the cell.
+// of LDS slots; the block then writes the rows to the caller's rows of its
+// cells, eight lanes to a row; b and x0 go to the caller's index of the cell.
 template <int kEq, bool kVisc, bool kBlend, bool kTransient, bool kPerm>
 __global__ void __launch_bounds__(kBlock) k_assemble(MeshDev m, StateDev s, FlowPhys ph, int d,
                                                      double* __restrict__ val, double* __restrict__ b,
@@ -375,12 +375,26 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshDev m, StateDev s, Flow
         __syncthreads();
         for (int k = threadIdx.x; k < span; k += blockDim.x) val[base + k] = rows[k];
     } else {
-        if (i >= m.n) return;
-        const int t = threadIdx.x, nt = blockDim.x;
-        const int len = m.rlen[i], rs = m.rstart[i];
-        for (int q = 0; q < len; ++q) rows[q * nt + t] = 0.0;
-        assemble_cell<kEq, kVisc, kBlend, kTransient>(m, s, ph, d, i, m.cperm[i], RowAcc{rows + t, nt}, b, x0);
-        for (int q = 0; q < len; ++q) val[rs + q] = rows[q * nt + t];
+        // slot q of thread t's row at rows[q * (nt + 1) + t]: conflict-free
+        // both for the per-thread accumulation and for the write-out below
+        __shared__ int s_rs[kBlock], s_len[kBlock];
+        const int t = threadIdx.x, nt = blockDim.x, ld = nt + 1;
+        int len = 0, rs = 0;
+        if (i < m.n) {
+            len = m.rlen[i];
+            rs = m.rstart[i];
+            for (int q = 0; q < len; ++q) rows[q * ld + t] = 0.0;
+            assemble_cell<kEq, kVisc, kBlend, kTransient>(m, s, ph, d, i, m.cperm[i], RowAcc{rows + t, ld}, b, x0);
+        }
+        s_rs[t] = rs;
+        s_len[t] = len;
+        __syncthreads();
+        // eight lanes per row: each store writes eight rows' contiguous runs
+        // instead of one slot of 64 scattered rows
+        for (int r = t >> 3; r < nt; r += nt >> 3) {
+            const int L = s_len[r], base = s_rs[r];
+            for (int q = t & 7; q < L; q += 8) val[base + q] = rows[q * ld + r];
+        }
     }
 }
 
@@ -1568,7 +1582,7 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
         ciperm.resize(n);
         for (size_t k = 0; k < n; ++k) ciperm[cperm[k]] = (int)k;
         if (fl->cell_perm) {
-            fl->asm_lds = sizeof(double) * (size_t)kBlock * kMaxRow;  // one column of slots per thread
+            fl->asm_lds = sizeof(double) * (size_t)(kBlock + 1) * kMaxRow;  // one padded column of slots per thread
         } else {
             int span = 1;  // at most kBlock * kMaxRow entries: 32 KB
             for (size_t i0 = 0; i0 < n; i0 += kBlock)
