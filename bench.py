@@ -381,6 +381,11 @@ def batch(args, out):
         "iterations": [r["iterations"] for r in reps],
         "max_fp64_relres": max(rel),
         "sequential_sum_ms": round(seq, 3),
+        # algorithmic bytes of every system's iterations and replacements
+        # (the single-system split-CG count with x updated every iteration)
+        # over the wall time of the iteration loop
+        "algorithmic_GB": round(sum(r["bytes"] for r in reps) / 1e9, 3),
+        "algorithmic_GBps": round(sum(r["bytes"] for r in reps) / (reps[0]["solve_ms"] * 1e-3) / 1e9, 1),
         "launch": dict(shape, us_per_iteration=round(us_it, 2)),
         "tuning": [[t["rows_per_lane"], t["block_size"], round(t["us_per_iteration"], 2)] for t in table],
     }

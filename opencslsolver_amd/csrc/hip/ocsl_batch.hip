@@ -1071,6 +1071,17 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
                 r.residual = tr;
                 r.true_relres = tr;
                 r.solve_ms = total_ms;
+                // algorithmic bytes, as the single-system split CG counts
+                // them with the solution updated every iteration: SpMV
+                // (off-diagonal values and columns, diagonal, slice pointers,
+                // p, r read, q written), update (x read and written, p, r, q
+                // read, r, p written); a replacement reads the FP64 matrix,
+                // x, b and writes r
+                const double n = (double)bt->n_sys[k], sv = mixed ? 4.0 : 8.0, cb = bt->compact ? 2.0 : 4.0;
+                const double off = (double)bt->csr[k].col.size() - n, sl = 4.0 * (bt->ns_sys[k] + 1);
+                const double it = off * (sv + cb) + sl + n * (9.0 * sv + 16.0);
+                const double res = off * (8.0 + cb) + sl + n * (8.0 + 8.0 + 8.0 + sv);
+                r.bytes = (double)r.iterations * it + (double)r.replacements * res;
             }
             if (latency_ms) latency_ms[k] = lat[k] >= 0.0 ? lat[k] : total_ms;
         }
