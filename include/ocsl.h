@@ -81,7 +81,19 @@ typedef struct {
                             3 = one kernel, gathering from memory; 0 = tuned, else 2 */
     int x_period;        /* MIXED split CG: iterations per update of the FP64 solution, 2, 4 or 8;
                             0 = 8 */
+    int bicgstab_order;  /* ocsl_bicgstab_order: row order of the TOLERANCE ILU(0)-BiCGSTAB; default input */
 } ocsl_options;
+
+/* Row order in which TOLERANCE-mode ILU(0)-BiCGSTAB factors and solves
+ * (REFERENCE mode always keeps the input order, as the reference does).
+ * INPUT: the caller's numbering (the factor CMatrix_solver::ILU_decompose
+ * builds).  COLOUR: rows renumbered by a greedy colouring of the
+ * symmetrised row graph (reverse Cuthill-McKee order within a colour): no two
+ * rows of a colour are coupled, so each triangular solve has one level per
+ * colour and the SpMV gathers stay local; the ILU(0) factor is that of the
+ * renumbered matrix, so iterates differ from INPUT's while the residual
+ * target is the same.  b, x and the results stay in the caller's numbering. */
+typedef enum { OCSL_BICGSTAB_ORDER_INPUT = 0, OCSL_BICGSTAB_ORDER_COLOUR = 1 } ocsl_bicgstab_order;
 
 typedef struct {
     int status;          /* ocsl_status of the solve */
@@ -276,6 +288,12 @@ ocsl_status ocsl_set_tuned_cg(ocsl_handle h, int precision, const ocsl_options* 
  * once a solve with the same right-hand-side address has run (on: 1, the
  * default).  0 issues every launch each time; results are bitwise the same. */
 ocsl_status ocsl_set_reference_graphs(ocsl_handle h, int on);
+
+/* The row order of OCSL_BICGSTAB_ORDER_COLOUR solves (no counterpart in the
+ * reference): order[i] = the input row placed at position i (n entries;
+ * NULL to query only), *n_colours = the colours, i.e. the levels of each
+ * triangular solve.  Built on first use and kept by the handle. */
+ocsl_status ocsl_colour_order(ocsl_handle h, int* order, int* n_colours);
 
 /* HIP stream of the handle (a hipStream_t), for callers that pass device
  * pointers produced on another stream. */

@@ -30,6 +30,8 @@ HOST = 0
 DEVICE = 1
 TRI_EXACT = 0
 TRI_JACOBI = 1
+BICGSTAB_ORDER_INPUT = 0
+BICGSTAB_ORDER_COLOUR = 1
 
 
 class Options(ctypes.Structure):
@@ -50,6 +52,7 @@ class Options(ctypes.Structure):
         ("rows_per_lane", ctypes.c_int),
         ("cg_kernels", ctypes.c_int),
         ("x_period", ctypes.c_int),
+        ("bicgstab_order", ctypes.c_int),
     ]
 
 
@@ -100,6 +103,7 @@ SIGNATURES = {
                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "ocsl_ILU_decompose": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
     "ocsl_set_reference_graphs": (ctypes.c_int, [c_handle, ctypes.c_int]),
+    "ocsl_colour_order": (ctypes.c_int, [c_handle, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "ocsl_ILU_decompose_from": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_int]),
     "ocsl_ILU_forsub": (ctypes.c_int, [c_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),

@@ -219,6 +219,7 @@ void ocsl_default_options(ocsl_options* o) {
     o->rows_per_lane = 0;
     o->cg_kernels = 0;
     o->x_period = 0;
+    o->bicgstab_order = OCSL_BICGSTAB_ORDER_INPUT;
 }
 
 int ocsl_device_count(void) {
@@ -269,6 +270,7 @@ ocsl_status ocsl_destroy(ocsl_handle h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
     if (h->adj) ocsl_destroy(h->adj);
+    if (h->col_h) ocsl_destroy(h->col_h);
     if (h->S_host) (void)hipHostFree(h->S_host);
     if (h->dscal_host) (void)hipHostFree(h->dscal_host);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -492,6 +494,11 @@ ocsl_status ocsl_ILU_decompose(ocsl_handle h, int sweeps, double* m_out, ocsl_wh
 }
 
 void* ocsl_get_stream(ocsl_handle h) { return h ? (void*)h->stream : nullptr; }
+
+ocsl_status ocsl_colour_order(ocsl_handle h, int* order, int* n_colours) {
+    if (!h) return OCSL_ERR_ARG;
+    return guarded([&] { colour_order_get(h, order, n_colours); });
+}
 
 ocsl_status ocsl_set_reference_graphs(ocsl_handle h, int on) {
     if (!h || (on != 0 && on != 1)) return OCSL_ERR_ARG;

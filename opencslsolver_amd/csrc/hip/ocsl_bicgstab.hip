@@ -13,6 +13,7 @@
 //                          pass serves all of them), FP64 or MIXED (FP32
 //                          storage, FP64 arithmetic, FP64 solution and
 //                          residual replacement).
+#include <algorithm>
 #include <cfloat>
 #include <cstddef>
 #include <cmath>
@@ -908,9 +909,164 @@ __global__ void k_interleave(int n, int k, const T* __restrict__ in, T* __restri
         }
 }
 
+// k column blocks of n between two row orders: gather out[c n + i] =
+// in[c n + perm[i]], or scatter out[c n + perm[i]] = in[c n + i].
+__global__ void k_perm_cols(int n, int k, const int* __restrict__ perm, const double* __restrict__ in,
+                            double* __restrict__ out, int gather) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int j = perm[i];
+        for (int c = 0; c < k; ++c) {
+            if (gather) out[(size_t)c * n + i] = in[(size_t)c * n + j];
+            else out[(size_t)c * n + j] = in[(size_t)c * n + i];
+        }
+    }
+}
+
+namespace {
+
+// Greedy colouring of the symmetrised row graph (A and A^T), rows visited in
+// reverse Cuthill-McKee order, each taking the smallest colour none of its
+// coloured neighbours has; rows renumbered colour by colour, in visiting
+// order within a colour.  Returns new -> old.
+std::vector<int> colour_order(const HostCSR& a, int* n_colours) {
+    const int n = a.n;
+    std::vector<int> ptr(n + 1, 0);
+    for (int i = 0; i < n; ++i)
+        for (int p = a.rp[i]; p < a.rp[i + 1]; ++p)
+            if (a.col[p] != i) {
+                ++ptr[i + 1];
+                ++ptr[a.col[p] + 1];
+            }
+    for (int i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int> adj(ptr[n]), fill(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; ++i)
+        for (int p = a.rp[i]; p < a.rp[i + 1]; ++p) {
+            const int j = a.col[p];
+            if (j != i) {
+                adj[fill[i]++] = j;
+                adj[fill[j]++] = i;
+            }
+        }
+    const std::vector<int> visit = rcm_order(a);
+    std::vector<int> colour(n, -1), seen;
+    int nc = 0;
+    for (int v : visit) {
+        // colours of v's neighbours are < its degree + 1
+        seen.assign((size_t)(ptr[v + 1] - ptr[v]) + 1, 0);
+        for (int q = ptr[v]; q < ptr[v + 1]; ++q) {
+            const int c = colour[adj[q]];
+            if (c >= 0 && c < (int)seen.size()) seen[c] = 1;
+        }
+        int c = 0;
+        while (seen[c]) ++c;
+        colour[v] = c;
+        nc = std::max(nc, c + 1);
+    }
+    std::vector<int> start(nc + 1, 0), order(n);
+    for (int v = 0; v < n; ++v) ++start[colour[v] + 1];
+    for (int c = 0; c < nc; ++c) start[c + 1] += start[c];
+    for (int v : visit) order[start[colour[v]]++] = v;
+    *n_colours = nc;
+    return order;
+}
+
+// The handle of the colour-renumbered matrix P A P^T (built once): its rows
+// in the new order, each row's entries by ascending new column; the value
+// map points every entry at its slot in the caller's value array.
+ocsl_solver* coloured(ocsl_solver* h) {
+    if (h->col_h) return h->col_h;
+    const HostCSR& a = h->host;
+    const int n = a.n;
+    int nc = 0;
+    const std::vector<int> order = colour_order(a, &nc);
+    std::vector<int> inv(n);
+    for (int i = 0; i < n; ++i) inv[order[i]] = i;
+    std::vector<uint64_t> rp(n + 1, 0), col;
+    std::vector<int> map;
+    col.reserve(a.col.size());
+    map.reserve(a.col.size());
+    std::vector<std::pair<int, int>> row;
+    for (int i = 0; i < n; ++i) {
+        const int o = order[i];
+        row.clear();
+        for (int p = a.rp[o]; p < a.rp[o + 1]; ++p) row.emplace_back(inv[a.col[p]], a.src[p]);
+        std::sort(row.begin(), row.end());
+        for (const auto& e : row) {
+            col.push_back((uint64_t)e.first);
+            map.push_back(e.second);
+        }
+        rp[i + 1] = col.size();
+    }
+    ocsl_handle t = nullptr;
+    const ocsl_status st = ocsl_create(&t, n, rp.data(), col.data());
+    if (st != OCSL_OK) throw Error{st, std::string("colour-ordered handle: ") + ocsl_last_error()};
+    h->col_h = t;
+    h->n_colours = nc;
+    h->col_map.upload(map.data(), map.size(), h->stream);
+    h->col_perm.upload(order.data(), order.size(), h->stream);
+    h->col_epoch = ~0ull;
+    sync(h);
+    return t;
+}
+
+}  // namespace
+
+void bicgstab_multi(ocsl_solver* h, bool mixed, const ocsl_options& o, int k, const double* B_dev,
+                    double* X_dev, ocsl_report* reps);
+
+// The colour order and its colour count (ocsl_colour_order).
+void colour_order_get(ocsl_solver* h, int* order, int* n_colours) {
+    (void)coloured(h);
+    if (order) {
+        OCSL_HIP(hipMemcpy(order, h->col_perm.p, sizeof(int) * h->host.n, hipMemcpyDeviceToHost));
+    }
+    if (n_colours) *n_colours = h->n_colours;
+}
+
+// TOLERANCE ILU(0)-BiCGSTAB in the colour order: values gathered into the
+// renumbered handle (when they changed), b and x permuted in, the solve run
+// there, x permuted back.  Reports and history are the inner solve's.
+static void bicgstab_coloured(ocsl_solver* h, bool mixed, const ocsl_options& o, int k, const double* B_dev,
+                              double* X_dev, ocsl_report* reps) {
+    ocsl_solver* t = coloured(h);
+    const int n = h->host.n;
+    const int64_t nr = (int64_t)h->host.col.size();
+    sync(h);  // the caller's values, b and x are complete
+    if (h->col_epoch != h->val_epoch) {
+        t->val_in_own.alloc(nr);
+        k_gather_vals<double><<<grid_for(nr), kBlock, 0, t->stream>>>(nr, h->col_map.p, h->val_in, t->val_in_own.p);
+        OCSL_HIP(hipGetLastError());
+        t->val_in = t->val_in_own.p;
+        t->have_vals = true;
+        t->val_epoch++;
+        t->csr_val_valid = t->csr_val32_valid = false;
+        h->col_epoch = h->val_epoch;
+    }
+    const size_t cnt = (size_t)n * k;
+    h->col_b.alloc(cnt);
+    h->col_x.alloc(cnt);
+    k_perm_cols<<<grid_for(n), kBlock, 0, t->stream>>>(n, k, h->col_perm.p, B_dev, h->col_b.p, 1);
+    k_perm_cols<<<grid_for(n), kBlock, 0, t->stream>>>(n, k, h->col_perm.p, X_dev, h->col_x.p, 1);
+    OCSL_HIP(hipGetLastError());
+    ocsl_options oi = o;
+    oi.bicgstab_order = OCSL_BICGSTAB_ORDER_INPUT;
+    bicgstab_multi(t, mixed, oi, k, h->col_b.p, h->col_x.p, reps);
+    k_perm_cols<<<grid_for(n), kBlock, 0, t->stream>>>(n, k, h->col_perm.p, h->col_x.p, X_dev, 0);
+    OCSL_HIP(hipGetLastError());
+    sync(t);
+    h->hist_host = t->hist_host;
+    h->hist_len = t->hist_len;
+}
+
 void bicgstab_multi(ocsl_solver* h, bool mixed, const ocsl_options& o, int k, const double* B_dev,
                     double* X_dev, ocsl_report* reps) {
     OCSL_REQUIRE(o.ilu_sweeps >= 1, OCSL_ERR_ARG, "ilu_sweeps must be >= 1");
+    OCSL_REQUIRE(o.bicgstab_order == OCSL_BICGSTAB_ORDER_INPUT || o.bicgstab_order == OCSL_BICGSTAB_ORDER_COLOUR,
+                 OCSL_ERR_ARG, "bad bicgstab_order");
+    if (o.bicgstab_order == OCSL_BICGSTAB_ORDER_COLOUR) {
+        bicgstab_coloured(h, mixed, o, k, B_dev, X_dev, reps);
+        return;
+    }
     const int n = h->host.n;
     Work& w = h->work;
     double* b_il = work_buf(h, w.d[14], (size_t)n * k);

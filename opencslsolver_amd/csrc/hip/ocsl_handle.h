@@ -158,6 +158,14 @@ struct ocsl_solver {
     uint64_t adj_epoch = ~0ull;
     ocsl::DBuf<double> grad_own;
 
+    // colour-ordered ILU(0)-BiCGSTAB: handle of the renumbered matrix, its
+    // value map into the caller's slots, new -> old rows, staged vectors
+    ocsl_solver* col_h = nullptr;
+    ocsl::DBuf<int> col_map, col_perm;
+    uint64_t col_epoch = ~0ull;
+    int n_colours = 0;
+    ocsl::DBuf<double> col_b, col_x;
+
     // hipGraph cache of iteration chunks (cg_graph_key, bicgstab_graph_key)
     // and of whole REFERENCE-mode solves (reference_graph_key)
     std::map<ocsl::GraphKey, hipGraphExec_t> graphs;
@@ -257,6 +265,7 @@ void bicgstab_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_
 // The reference's ILU sweeps from M^0 = m0 (device, real-entry order), or
 // from A when m0 is null; the result lands in h->ilu_m.
 void ilu_decompose_device(ocsl_solver* h, int sweeps, bool want32, const double* m0 = nullptr);
+void colour_order_get(ocsl_solver* h, int* order, int* n_colours);
 void bicgstab_multi(ocsl_solver* h, bool mixed, const ocsl_options& o, int k, const double* B_dev,
                     double* X_dev, ocsl_report* reps);
 

@@ -171,6 +171,14 @@ class Solver:
         after their first run (default), or issue every launch each time."""
         _check(self._lib.ocsl_set_reference_graphs(self._h, 1 if on else 0))
 
+    def colour_order(self):
+        """(order, n_colours) of colour-ordered BiCGSTAB solves: order[i] is the
+        input row at position i."""
+        nc = ctypes.c_int()
+        out = np.empty(self.n, np.int32)
+        _check(self._lib.ocsl_colour_order(self._h, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nc)))
+        return out, nc.value
+
     def CG_solve(self) -> float:
         """CMatrix_solver::CG_solve (cmatrix_solver.cpp:635-680)."""
         res = ctypes.c_double()

@@ -149,7 +149,9 @@ def test_convdiff_96cubed_three_rhs_against_oracle():
     iterations, FP64 refinement).  Every solution's relative residual is
     recomputed by a host SpMV; each is compared with the oracle's FP64
     solution of the same right-hand side, and each MIXED solution with the
-    FP64 one (errors printed per right-hand side)."""
+    FP64 one (errors printed per right-hand side).  The same in the colour
+    order (OCSL_BICGSTAB_ORDER_COLOUR): residuals within 1e-10, solutions
+    within 1e-9 of the oracle's."""
     oracle.set_threads(8)
     a, _ = gen.convdiff3d(96, seed_perm=3)
     assert (a.n, a.nnz) == (884736, 6137856)
@@ -158,13 +160,19 @@ def test_convdiff_96cubed_three_rhs_against_oracle():
     X = {}
     with Solver(a.row_ptr, a.col) as s:
         s.set_values(a.val)
-        for prec in ("fp64", "mixed"):
-            X[prec], reps = s.solve_multi(B, precision=prec, tol=1e-10)
-            for k in range(3):
-                assert reps[k]["status"] == 0, (prec, k, reps[k])
-                relres = np.linalg.norm(B[k] - a.matvec(X[prec][k])) / np.linalg.norm(B[k])
-                parity.check(f"convdiff96 {prec} rhs{k} host relres", relres, 1e-10)
+        for order, tag in ((0, ""), (1, " colour")):
+            for prec in ("fp64", "mixed"):
+                X[prec + tag], reps = s.solve_multi(B, precision=prec, tol=1e-10, bicgstab_order=order)
+                for k in range(3):
+                    assert reps[k]["status"] == 0, (prec, tag, k, reps[k])
+                    relres = np.linalg.norm(B[k] - a.matvec(X[prec + tag][k])) / np.linalg.norm(B[k])
+                    parity.check(f"convdiff96{tag} {prec} rhs{k} host relres", relres, 1e-10)
     for k in range(3):
+        # the colour order factors the renumbered matrix: another
+        # preconditioner, the same residual target
+        for prec in ("fp64", "mixed"):
+            parity.check(f"convdiff96 colour {prec} rhs{k} vs oracle fp64",
+                         parity.rel(X[prec + " colour"][k], ref[k][1]), 1e-9)
         it_o, x_o, _, _ = ref[k]
         parity.check(f"convdiff96 fp64 rhs{k} vs oracle fp64", parity.rel(X["fp64"][k], x_o), 1e-10)
         err = np.linalg.norm(X["mixed"][k] - X["fp64"][k]) / np.linalg.norm(X["fp64"][k])
@@ -172,6 +180,100 @@ def test_convdiff_96cubed_three_rhs_against_oracle():
         # diagonally dominant (rho V / dt), so their distance is of that order
         parity.check(f"convdiff96 mixed rhs{k} error vs fp64", float(err), 1e-9)
         parity.check(f"convdiff96 mixed rhs{k} vs oracle fp64", parity.rel(X["mixed"][k], x_o), 1e-9)
+
+
+def _renumbered(a, order):
+    """P A P^T for new -> old `order`: rows in the new order, each row's
+    entries by ascending new column (host numpy)."""
+    n = a.n
+    inv = np.empty(n, np.int64)
+    inv[order] = np.arange(n)
+    rp = a.row_ptr.astype(np.int64)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    nr, nc = inv[rows], inv[a.col.astype(np.int64)]
+    key = np.lexsort((nc, nr))
+    counts = np.bincount(nr, minlength=n)
+    out_rp = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    return CSR(out_rp, nc[key].astype(np.uint64), a.val[key].copy())
+
+
+def _lower_depth(a):
+    """Levels of the lower triangle's dependency graph (host)."""
+    n = a.n
+    rows = np.repeat(np.arange(n), np.diff(a.row_ptr.astype(np.int64)))
+    cols = a.col.astype(np.int64)
+    m = cols < rows
+    r, c = rows[m], cols[m]
+    lev = np.zeros(n, np.int64)
+    while True:
+        new = lev.copy()
+        np.maximum.at(new, r, lev[c] + 1)
+        if np.array_equal(new, lev):
+            return int(lev.max()) + 1
+        lev = new
+
+
+@pytest.mark.parametrize("n", [10, 24])
+def test_colour_order_solves_the_renumbered_system(n):
+    """OCSL_BICGSTAB_ORDER_COLOUR: the order the handle reports is a
+    permutation whose renumbered matrix has a lower triangle of exactly
+    n_colours levels (no two rows of a colour coupled; 2 colours on the
+    7-point pattern, which is bipartite).  The FP64 solves of three
+    right-hand sides equal the oracle's ILU(0)-BiCGSTAB run on that
+    renumbered matrix (1e-10, same iteration count); MIXED lands within 1e-9
+    of FP64; every residual, recomputed on the host in the caller's
+    numbering, is within the 1e-10 target; solutions of the input-order solve
+    agree to 1e-9."""
+    a, _ = gen.convdiff3d(n, seed_perm=3)
+    B = np.stack([gen.uniform(sd, a.n) for sd in (4, 5, 6)])
+    with Solver(a.row_ptr, a.col) as s:
+        s.set_values(a.val)
+        order, nc = s.colour_order()
+        assert np.array_equal(np.sort(order), np.arange(a.n))
+        ap = _renumbered(a, order)
+        assert nc == 2 and _lower_depth(ap) == nc
+        X = {}
+        for prec in ("fp64", "mixed"):
+            X[prec], reps = s.solve_multi(B, precision=prec, tol=1e-10, bicgstab_order=1)
+            for k in range(3):
+                assert reps[k]["status"] == 0, (prec, k, reps[k])
+                relres = np.linalg.norm(B[k] - a.matvec(X[prec][k])) / np.linalg.norm(B[k])
+                parity.check(f"colour{n} {prec} rhs{k} host relres", relres, 1e-10)
+            if prec == "fp64":
+                its = [r["iterations"] for r in reps]
+        X_in, _ = s.solve_multi(B, precision="fp64", tol=1e-10)
+    for k in range(3):
+        it_o, x_o, _, _ = oracle.bicgstab_tol(ap, B[k][order], tol=1e-10)
+        x_back = np.empty(a.n)
+        x_back[order] = x_o
+        assert its[k] == it_o, (k, its[k], it_o)
+        parity.check(f"colour{n} fp64 rhs{k} vs oracle on P A P^T", parity.rel(X["fp64"][k], x_back), 1e-10)
+        parity.check(f"colour{n} mixed rhs{k} vs fp64", parity.rel(X["mixed"][k], X["fp64"][k]), 1e-9)
+        parity.check(f"colour{n} fp64 rhs{k} vs input order", parity.rel(X["fp64"][k], X_in[k]), 1e-9)
+
+
+def test_colour_order_single_rhs_and_history():
+    """ocsl_solve in the colour order: the same solution as the one-column
+    multi-RHS solve, a history of iterations + 1 entries starting at the
+    initial relative residual 1, and a later input-order solve on the same
+    handle unaffected (equal to a fresh handle's)."""
+    a, _ = gen.convdiff3d(12, seed_perm=5, dt=1.0)
+    b = gen.uniform(7, a.n)
+    with _solver(a, b) as s:
+        rep = s.solve("ilu_bicgstab", "tolerance", "fp64", tol=1e-10, bicgstab_order=1)
+        x1 = s.get_x()
+        h = s.history()
+        X, _ = s.solve_multi(b[None, :], precision="fp64", tol=1e-10, bicgstab_order=1)
+        s.set_x(np.zeros(a.n))
+        s.solve("ilu_bicgstab", "tolerance", "fp64", tol=1e-10)
+        x_in = s.get_x()
+    assert rep["status"] == 0 and rep["iterations"] >= 1
+    assert np.array_equal(x1, X[0])
+    assert h.size == rep["iterations"] + 1
+    assert abs(h[0] - 1.0) <= 1e-12 and np.all(np.isfinite(h))  # x0 = 0: ||r0|| / ||b|| = 1
+    with _solver(a, b) as s:
+        s.solve("ilu_bicgstab", "tolerance", "fp64", tol=1e-10)
+        assert np.array_equal(s.get_x(), x_in)
 
 
 def test_graph_cache_follows_the_jacobi_weight():
