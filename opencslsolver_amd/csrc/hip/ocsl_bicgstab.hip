@@ -1018,7 +1018,8 @@ void bicgstab_multi(ocsl_solver* h, bool mixed, const ocsl_options& o, int k, co
 void colour_order_get(ocsl_solver* h, int* order, int* n_colours) {
     (void)coloured(h);
     if (order) {
-        OCSL_HIP(hipMemcpy(order, h->col_perm.p, sizeof(int) * h->host.n, hipMemcpyDeviceToHost));
+        OCSL_HIP(hipMemcpyAsync(order, h->col_perm.p, sizeof(int) * h->host.n, hipMemcpyDeviceToHost, h->stream));
+        sync(h);
     }
     if (n_colours) *n_colours = h->n_colours;
 }
