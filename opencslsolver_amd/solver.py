@@ -233,6 +233,24 @@ class Solver:
                                             _abi.HOST))
         return out
 
+    def forsub(self, m, b) -> np.ndarray:
+        """CMatrix_solver::ILU_forsub with the factor m (slot order): y = L^-1 b."""
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        y = np.empty(self.n)
+        _check(self._lib.ocsl_ILU_forsub(self._h, m.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p),
+                                         y.ctypes.data_as(ctypes.c_void_p), _abi.HOST))
+        return y
+
+    def backsub(self, m, y) -> np.ndarray:
+        """CMatrix_solver::ILU_backsub with the factor m (slot order): x = U^-1 y."""
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        x = np.empty(self.n)
+        _check(self._lib.ocsl_ILU_backsub(self._h, m.ctypes.data_as(ctypes.c_void_p), y.ctypes.data_as(ctypes.c_void_p),
+                                          x.ctypes.data_as(ctypes.c_void_p), _abi.HOST))
+        return x
+
     def time_cg_spmv(self, precision="fp64", reorder=1, reps=50, rows_per_lane=0):
         """(average ms, algorithmic bytes) of one CG SpMV launch on the handle's stream."""
         ms, by = ctypes.c_double(), ctypes.c_double()

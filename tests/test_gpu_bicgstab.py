@@ -276,6 +276,24 @@ def test_colour_order_single_rhs_and_history():
         assert np.array_equal(s.get_x(), x_in)
 
 
+@pytest.mark.parametrize("n,seed", [(12, 3), (48, 3), (64, 31)])
+def test_exact_triangular_solves_match_oracle(n, seed):
+    """ocsl_ILU_forsub / ocsl_ILU_backsub with a 3-sweep ILU(0) factor against
+    the oracle's sequential substitutions: 12^3 runs every level in one
+    workgroup; 48^3 and 64^3 (randomly numbered, 19-24 levels) launch one
+    kernel per level.  Same terms in the same order per row: 1e-13."""
+    a, _ = gen.convdiff3d(n, seed_perm=seed)
+    b = gen.uniform(4, a.n)
+    m = oracle.ilu_decompose(a, 3)
+    with Solver(a.row_ptr, a.col) as s:
+        s.set_values(a.val)
+        y = s.forsub(m, b)
+        x = s.backsub(m, y)
+    y_o = oracle.forsub(a, m, b)
+    parity.check(f"forsub {n}^3", parity.rel(y, y_o), 1e-13)
+    parity.check(f"backsub {n}^3", parity.rel(x, oracle.backsub(a, m, y_o)), 1e-13)
+
+
 def test_graph_cache_follows_the_jacobi_weight():
     """TOLERANCE BiCGSTAB with damped-Jacobi triangular sweeps replays its
     iteration chunks as graphs whose sweeps take the weight as an argument: a
