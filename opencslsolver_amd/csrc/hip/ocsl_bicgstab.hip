@@ -113,11 +113,44 @@ __device__ __forceinline__ void tri_row_level(int k, LevelTri t, const VM* __res
     }
 }
 
+// tri_row_level in two parts, so that a level kernel can load its first
+// row's position data together with the halt flag instead of after it.
+struct TriPos {
+    int i, p0, p1;
+    double dd;
+};
+template <class VM>
+__device__ __forceinline__ TriPos tri_pos(int k, LevelTri t, const VM* __restrict__ dv, bool lower) {
+    return {t.rows[k], t.eptr[k], t.eptr[k + 1], lower ? 1.0 : (double)dv[k]};
+}
+template <class VM, class X, int NR>
+__device__ __forceinline__ void tri_row_at(const TriPos& q, LevelTri t, const VM* __restrict__ ev, const X* in,
+                                           X* out, bool lower) {
+    double sum[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) sum[c] = 0.0;
+    for (int p = q.p0; p < q.p1; ++p) {
+        const double mv = (double)ev[p];
+        const int j = t.ecol[p];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) sum[c] += mv * (double)out[(size_t)j * NR + c];
+    }
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+        const double v = (double)in[(size_t)q.i * NR + c] - sum[c];
+        out[(size_t)q.i * NR + c] = (X)(lower ? v : v / q.dd);
+    }
+}
+
 template <class VM, class X, int NR>
 __global__ void k_tri_level(LevelTri t, int begin, int count, const VM* __restrict__ ev, const VM* __restrict__ dv,
                             const X* in, X* out, int lower, const Scalars* S) {
+    const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
+    TriPos q{0, 0, 0, 1.0};
+    if (k0 < count) q = tri_pos<VM>(begin + k0, t, dv, lower != 0);
     if (S && S->halt) return;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x)
+    if (k0 < count) tri_row_at<VM, X, NR>(q, t, ev, in, out, lower != 0);
+    for (int k = k0 + gridDim.x * blockDim.x; k < count; k += gridDim.x * blockDim.x)
         tri_row_level<VM, X, NR>(begin + k, t, ev, dv, in, out, lower != 0);
 }
 
@@ -127,8 +160,15 @@ template <class VM>
 __global__ void k_tri_level_pair(LevelTri t, int begin, int count, const VM* __restrict__ ev,
                                  const VM* __restrict__ dv, const double* in0, double* out0, const double* in1,
                                  double* out1, int lower, const Scalars* S) {
+    const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
+    TriPos q{0, 0, 0, 1.0};
+    if (k0 < count) q = tri_pos<VM>(begin + k0, t, dv, lower != 0);
     if (S && S->halt) return;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+    if (k0 < count) {
+        tri_row_at<VM, double, 1>(q, t, ev, in0, out0, lower != 0);
+        tri_row_at<VM, double, 1>(q, t, ev, in1, out1, lower != 0);
+    }
+    for (int k = k0 + gridDim.x * blockDim.x; k < count; k += gridDim.x * blockDim.x) {
         tri_row_level<VM, double, 1>(begin + k, t, ev, dv, in0, out0, lower != 0);
         tri_row_level<VM, double, 1>(begin + k, t, ev, dv, in1, out1, lower != 0);
     }
