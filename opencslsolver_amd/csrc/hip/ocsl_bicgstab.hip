@@ -188,6 +188,101 @@ __global__ void __launch_bounds__(kTriBlock) k_tri_block(LevelTri t, const int* 
     }
 }
 
+// The same solves from the padded-column copy (Levels::ell_w): a row's
+// columns and factor values sit at fixed offsets from its position, so they
+// are loaded in the same round as its row index, and only the solution
+// values remain to gather.  Terms added in the same order as tri_row_level.
+struct LevelEll {
+    const int* rows;
+    const int* cnt;
+    const int* ecol;  // w * n
+    int n, w;
+};
+struct EllRow {
+    int i, c;
+    double dd;
+    int j[kTriEll];
+    double v[kTriEll];
+};
+template <class VM>
+__device__ __forceinline__ void ell_load(int k, const LevelEll& t, const VM* __restrict__ ev,
+                                         const VM* __restrict__ dv, bool lower, EllRow& r) {
+    r.i = t.rows[k];
+    r.c = t.cnt[k];
+    r.dd = lower ? 1.0 : (double)dv[k];
+#pragma unroll
+    for (int e = 0; e < kTriEll; ++e) {
+        r.j[e] = 0;
+        r.v[e] = 0.0;
+        if (e < t.w) {
+            r.j[e] = t.ecol[(size_t)e * t.n + k];
+            r.v[e] = (double)ev[(size_t)e * t.n + k];
+        }
+    }
+}
+template <class X, int NR>
+__device__ __forceinline__ void ell_apply(const EllRow& r, const X* in, X* out, bool lower) {
+    double sum[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) sum[c] = 0.0;
+#pragma unroll
+    for (int e = 0; e < kTriEll; ++e)
+        if (e < r.c) {
+#pragma unroll
+            for (int c = 0; c < NR; ++c) sum[c] += r.v[e] * (double)out[(size_t)r.j[e] * NR + c];
+        }
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+        const double v = (double)in[(size_t)r.i * NR + c] - sum[c];
+        out[(size_t)r.i * NR + c] = (X)(lower ? v : v / r.dd);
+    }
+}
+
+template <class VM, class X, int NR>
+__global__ void k_tri_level_ell(LevelEll t, int begin, int count, const VM* __restrict__ ev,
+                                const VM* __restrict__ dv, const X* in, X* out, int lower, const Scalars* S) {
+    const int k0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    EllRow r;
+    if (k0 < count) ell_load<VM>(begin + k0, t, ev, dv, lower != 0, r);
+    if (S && S->halt) return;
+    if (k0 < count) ell_apply<X, NR>(r, in, out, lower != 0);
+    for (int k = k0 + stride; k < count; k += stride) {
+        ell_load<VM>(begin + k, t, ev, dv, lower != 0, r);
+        ell_apply<X, NR>(r, in, out, lower != 0);
+    }
+}
+
+template <class VM>
+__global__ void k_tri_level_pair_ell(LevelEll t, int begin, int count, const VM* __restrict__ ev,
+                                     const VM* __restrict__ dv, const double* in0, double* out0, const double* in1,
+                                     double* out1, int lower, const Scalars* S) {
+    const int k0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    EllRow r;
+    if (k0 < count) ell_load<VM>(begin + k0, t, ev, dv, lower != 0, r);
+    if (S && S->halt) return;
+    for (int k = k0; k < count; k += stride) {
+        if (k != k0) ell_load<VM>(begin + k, t, ev, dv, lower != 0, r);
+        ell_apply<double, 1>(r, in0, out0, lower != 0);
+        ell_apply<double, 1>(r, in1, out1, lower != 0);
+    }
+}
+
+template <class VM, class X, int NR>
+__global__ void __launch_bounds__(kTriBlock) k_tri_block_ell(LevelEll t, const int* __restrict__ lptr, int n_levels,
+                                                             const VM* __restrict__ ev, const VM* __restrict__ dv,
+                                                             const X* in, X* out, int lower, const Scalars* S) {
+    if (S && S->halt) return;
+    EllRow r;
+    for (int l = 0; l < n_levels; ++l) {
+        const int b = lptr[l], e = lptr[l + 1];
+        for (int k = b + threadIdx.x; k < e; k += kTriBlock) {
+            ell_load<VM>(k, t, ev, dv, lower != 0, r);
+            ell_apply<X, NR>(r, in, out, lower != 0);
+        }
+        __syncthreads();
+    }
+}
+
 // Factor values into level order: ev[e] = m[esrc[e]], dv[k] = m[dslot[k]].
 template <class VM>
 __global__ void k_level_gather(int ne, const int* __restrict__ esrc, int n, const int* __restrict__ dslot,
@@ -262,11 +357,13 @@ void level_values(ocsl_solver* h, Levels& L, const VM* m, const VM** ev, const V
         epoch = &L.epoch32;
     }
     const int n = h->host.n;
-    const int ne = (int)L.esrc.n;
+    // the padded-column copy when there is one, else the row-packed entries
+    const int ne = L.ell_w ? L.ell_w * n : (int)L.esrc.n;
+    const int* esrc = L.ell_w ? L.esw.p : L.esrc.p;
     if (*from != (const void*)m || *epoch != h->ilu_epoch || !e->p) {
         e->alloc(std::max(ne, 1));
         d->alloc(n);
-        k_level_gather<VM><<<grid_for(std::max(ne, n)), kBlock, 0, h->stream>>>(ne, L.esrc.p, n, L.dslot.p, m, e->p,
+        k_level_gather<VM><<<grid_for(std::max(ne, n)), kBlock, 0, h->stream>>>(ne, esrc, n, L.dslot.p, m, e->p,
                                                                                  d->p);
         OCSL_HIP(hipGetLastError());
         *from = (const void*)m;
@@ -283,6 +380,20 @@ void tri_exact(ocsl_solver* h, const VM* m, const X* in, X* out, bool lower, con
     const int n = h->host.n;
     const VM *ev, *dv;
     level_values<VM>(h, L, m, &ev, &dv);
+    if (L.ell_w) {
+        const LevelEll te{L.rows.p, L.cnt.p, L.ecw.p, n, L.ell_w};
+        if (use_block_tri(L, n)) {
+            k_tri_block_ell<VM, X, NR><<<1, kTriBlock, 0, h->stream>>>(te, L.ptr.p, L.n_levels, ev, dv, in, out, lower,
+                                                                       S);
+        } else {
+            for (int l = 0; l < L.n_levels; ++l) {
+                const int b = L.ptr_host[l], cnt = L.ptr_host[l + 1] - b;
+                k_tri_level_ell<VM, X, NR><<<grid_for(cnt), kBlock, 0, h->stream>>>(te, b, cnt, ev, dv, in, out, lower,
+                                                                                     S);
+            }
+        }
+        return;
+    }
     const LevelTri t{L.rows.p, L.eptr.p, L.ecol.p};
     if (use_block_tri(L, n)) {
         k_tri_block<VM, X, NR><<<1, kTriBlock, 0, h->stream>>>(t, L.ptr.p, L.n_levels, ev, dv, in, out, lower, S);
@@ -308,6 +419,15 @@ void tri_exact_pair(ocsl_solver* h, const VM* m, const double* in0, double* out0
     }
     const VM *ev, *dv;
     level_values<VM>(h, L, m, &ev, &dv);
+    if (L.ell_w) {
+        const LevelEll te{L.rows.p, L.cnt.p, L.ecw.p, n, L.ell_w};
+        for (int l = 0; l < L.n_levels; ++l) {
+            const int b = L.ptr_host[l], cnt = L.ptr_host[l + 1] - b;
+            k_tri_level_pair_ell<VM><<<grid_for(cnt), kBlock, 0, h->stream>>>(te, b, cnt, ev, dv, in0, out0, in1, out1,
+                                                                              lower, S);
+        }
+        return;
+    }
     const LevelTri t{L.rows.p, L.eptr.p, L.ecol.p};
     for (int l = 0; l < L.n_levels; ++l) {
         const int b = L.ptr_host[l], cnt = L.ptr_host[l + 1] - b;

@@ -70,9 +70,18 @@ inline SellPView<float> SellP::view<float>() const {
 }
 
 // Level schedule of a triangular solve: rows grouped by dependency depth.
+// Rows with at most this many entries on one side of the diagonal keep their
+// level-ordered entries in padded columns (Levels::ell_w).
+constexpr int kTriEll = 8;
+
 struct Levels {
     int n_levels = 0;
     int max_width = 0;
+    // Padded-column copy of the entries when every row has at most kTriEll
+    // (ell_w = that maximum, else 0): entry e of position k at e * n + k,
+    // cnt[k] of them used; esw the factor slots (the diagonal's in padding).
+    int ell_w = 0;
+    DBuf<int> cnt, ecw, esw;
     std::vector<int> ptr_host;  // n_levels + 1
     DBuf<int> rows, ptr;
     // The direction's entries in level order: position k (row rows[k]) owns

@@ -340,6 +340,28 @@ void build_levels(const HostCSR& a, bool lower, Levels& L, hipStream_t st) {
     L.ecol.upload(ecol.data(), ecol.size(), st);
     L.esrc.upload(esrc.data(), esrc.size(), st);
     L.dslot.upload(dslot.data(), dslot.size(), st);
+    // padded columns, when every row is short
+    int w = 0;
+    for (int k = 0; k < n; ++k) w = std::max(w, eptr[k + 1] - eptr[k]);
+    L.ell_w = (w >= 1 && w <= kTriEll) ? w : 0;
+    if (L.ell_w) {
+        std::vector<int> cnt(n), ecw((size_t)w * n, 0), esw((size_t)w * n);
+        for (int k = 0; k < n; ++k) {
+            cnt[k] = eptr[k + 1] - eptr[k];
+            for (int e = 0; e < w; ++e) {
+                const size_t at = (size_t)e * n + k;
+                if (e < cnt[k]) {
+                    ecw[at] = ecol[eptr[k] + e];
+                    esw[at] = esrc[eptr[k] + e];
+                } else {
+                    esw[at] = dslot[k];
+                }
+            }
+        }
+        L.cnt.upload(cnt.data(), cnt.size(), st);
+        L.ecw.upload(ecw.data(), ecw.size(), st);
+        L.esw.upload(esw.data(), esw.size(), st);
+    }
     L.from64 = L.from32 = nullptr;
     L.epoch64 = L.epoch32 = -1;
     OCSL_HIP(hipStreamSynchronize(st));
