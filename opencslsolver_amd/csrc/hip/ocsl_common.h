@@ -353,15 +353,28 @@ __device__ __forceinline__ void store_rows(T* __restrict__ a, int row0, int n, c
 // at R = 1, 2 at R = 2, and 1 at R = 4, where 2 would take the MIXED SpMV
 // from 78 to 96 registers (6 to 5 waves per SIMD) and measured 21.3 against
 // 20.4 us at 128^3.
-template <class V, bool kDiff, int R, class G, int kPairs = (R == 1 ? 4 : (R == 4 ? 1 : 2))>
-__device__ __forceinline__ void sellp_rows_g(const SellPView<V>& A, int s, int lane, int row0, const G& gather,
-                                             const double (&xi)[R], double (&acc)[R]) {
-    const int base = A.slice_ptr[s];
-    const int np = (A.slice_ptr[s + 1] - base) / (128 * R);  // pairs per row
+// A slice's start, its pairs per row and the lane's diagonals: the loads that
+// open sellp_rows_g, separable so that a kernel can issue them early.
+template <int R>
+struct SellpHead {
+    int base, np;
     double dg[R];
-    load_rows_p<R>(A.diag, row0, dg);  // diagonals are allocated to whole slices
+};
+template <class V, int R>
+__device__ __forceinline__ SellpHead<R> sellp_head(const SellPView<V>& A, int s, int row0) {
+    SellpHead<R> h;
+    h.base = A.slice_ptr[s];
+    h.np = (A.slice_ptr[s + 1] - h.base) / (128 * R);  // pairs per row
+    load_rows_p<R>(A.diag, row0, h.dg);                // diagonals are allocated to whole slices
+    return h;
+}
+
+template <class V, bool kDiff, int R, class G, int kPairs = (R == 1 ? 4 : (R == 4 ? 1 : 2))>
+__device__ __forceinline__ void sellp_rows_gh(const SellPView<V>& A, const SellpHead<R>& hd, int lane, int row0,
+                                              const G& gather, const double (&xi)[R], double (&acc)[R]) {
+    const int base = hd.base, np = hd.np;
 #pragma unroll
-    for (int j = 0; j < R; ++j) acc[j] = dg[j] * xi[j];
+    for (int j = 0; j < R; ++j) acc[j] = hd.dg[j] * xi[j];
     for (int t0 = 0; t0 < np; t0 += kPairs) {
         int c[kPairs][2 * R];
         V v[kPairs][2 * R];
@@ -406,6 +419,12 @@ __device__ __forceinline__ void sellp_rows_g(const SellPView<V>& A, int s, int l
                     }
             }
     }
+}
+
+template <class V, bool kDiff, int R, class G, int kPairs = (R == 1 ? 4 : (R == 4 ? 1 : 2))>
+__device__ __forceinline__ void sellp_rows_g(const SellPView<V>& A, int s, int lane, int row0, const G& gather,
+                                             const double (&xi)[R], double (&acc)[R]) {
+    sellp_rows_gh<V, kDiff, R, G, kPairs>(A, sellp_head<V, R>(A, s, row0), lane, row0, gather, xi, acc);
 }
 
 template <class V, class X, bool kDiff, int R>

@@ -597,14 +597,29 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_xp(CgArgs<V, X> c, in
 template <class V, class X, int R>
 __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int first) {
     Scalars* S = c.S;
-    if (S->done) {
+    // the wave's first slice opens with its loads issued beside the stop
+    // flag's, not after it
+    const int done = S->done;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int g = gridDim.x;
+    const int lb = xcd_block(blockIdx.x, g);
+    const SliceRange sr = block_slices(lb, g, c.A.n_slices);
+    const int s0 = sr.s0, s1 = sr.s1;
+    const int sf = s0 + (int)(threadIdx.x / kWave);
+    double pf[R] = {};
+    SellpHead<R> hf{};
+    if (sf < s1) {
+        const int row0 = sf * kWave * R + lane * R;
+        load_rows_p<R>(c.p, row0, pf);
+        hf = sellp_head<V, R>(c.A, sf, row0);
+    }
+    if (done) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             S->halt = 1;
             S->lag = 0;
         }
         return;
     }
-    const int g = gridDim.x;
     if (blockIdx.x == 0) {
         const double rr = block_sum_partials(c.red, kSlotRR, S->rr_blocks);
         if (threadIdx.x == 0) {
@@ -627,16 +642,21 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int fi
             }
         }
     }
-    const int lane = threadIdx.x & (kWave - 1);
-    const int lb = xcd_block(blockIdx.x, g);
-    const SliceRange sr = block_slices(lb, g, c.A.n_slices);
-    const int s0 = sr.s0, s1 = sr.s1;
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
+    for (int s = sf; s < s1; s += (int)(blockDim.x / kWave)) {
         const int row0 = s * kWave * R + lane * R;
         double pi[R], qd[R], rv[R];
-        load_rows_p<R>(c.p, row0, pi);
-        sellp_rows<V, X, sizeof(V) == 4, R>(c.A, s, lane, row0, c.p, pi, qd);
+        SellpHead<R> hd;
+        if (s == sf) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) pi[j] = pf[j];
+            hd = hf;
+        } else {
+            load_rows_p<R>(c.p, row0, pi);
+            hd = sellp_head<V, R>(c.A, s, row0);
+        }
+        const X* px = c.p;
+        sellp_rows_gh<V, sizeof(V) == 4, R>(c.A, hd, lane, row0, [px](int cc) { return (double)px[cc]; }, pi, qd);
         load_rows_p<R>(c.r, row0, rv);
         X qv[R];
 #pragma unroll
@@ -705,11 +725,9 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
     const int nv = n / N;
     const int stride = g * blockDim.x;
     const int v0 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (S->done) {
-        // a period cut after its update kJ - 1: add the pending steps
-        if (kJ > 0 && S->lag) cgm_flush_pending<V, X, kJ>(c, S, n);
-        return;
-    }
+    // the stop flag, the partial sums and the first vector are all in flight
+    // before the flag is tested
+    const int done = S->done;
     constexpr int kSlots[3] = {kSlotPQ, kSlotRQ, kSlotQQ};
     const int gs[3] = {c.g_spmv, c.g_spmv, c.g_spmv};
     double d3[3];
@@ -730,6 +748,11 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_update(CgArgs<V, X> c, int 
         vload(c.p, vs, pv);
         vload(c.r, vs, rv);
         vload(c.q, vs, qv);
+    }
+    if (done) {
+        // a period cut after its update kJ - 1: add the pending steps
+        if (kJ > 0 && S->lag) cgm_flush_pending<V, X, kJ>(c, S, n);
+        return;
     }
     ps.finish(c.red, kSlots, gs, d3);
     const double pq = d3[0], rq = d3[1], qq = d3[2];
