@@ -355,26 +355,26 @@ __device__ __forceinline__ void store_rows(T* __restrict__ a, int row0, int n, c
 // 20.4 us at 128^3.
 // A slice's start, its pairs per row and the lane's diagonals: the loads that
 // open sellp_rows_g, separable so that a kernel can issue them early.
-template <int R>
+template <class V, int R>
 struct SellpHead {
     int base, np;
-    double dg[R];
+    V dg[R];  // as stored (fewer registers while held)
 };
 template <class V, int R>
-__device__ __forceinline__ SellpHead<R> sellp_head(const SellPView<V>& A, int s, int row0) {
-    SellpHead<R> h;
+__device__ __forceinline__ SellpHead<V, R> sellp_head(const SellPView<V>& A, int s, int row0) {
+    SellpHead<V, R> h;
     h.base = A.slice_ptr[s];
     h.np = (A.slice_ptr[s + 1] - h.base) / (128 * R);  // pairs per row
-    load_rows_p<R>(A.diag, row0, h.dg);                // diagonals are allocated to whole slices
+    load_rows_tp<R>(A.diag, row0, h.dg);               // diagonals are allocated to whole slices
     return h;
 }
 
 template <class V, bool kDiff, int R, class G, int kPairs = (R == 1 ? 4 : (R == 4 ? 1 : 2))>
-__device__ __forceinline__ void sellp_rows_gh(const SellPView<V>& A, const SellpHead<R>& hd, int lane, int row0,
+__device__ __forceinline__ void sellp_rows_gh(const SellPView<V>& A, const SellpHead<V, R>& hd, int lane, int row0,
                                               const G& gather, const double (&xi)[R], double (&acc)[R]) {
     const int base = hd.base, np = hd.np;
 #pragma unroll
-    for (int j = 0; j < R; ++j) acc[j] = hd.dg[j] * xi[j];
+    for (int j = 0; j < R; ++j) acc[j] = (double)hd.dg[j] * xi[j];
     for (int t0 = 0; t0 < np; t0 += kPairs) {
         int c[kPairs][2 * R];
         V v[kPairs][2 * R];

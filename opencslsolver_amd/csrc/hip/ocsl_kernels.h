@@ -606,11 +606,11 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int fi
     const SliceRange sr = block_slices(lb, g, c.A.n_slices);
     const int s0 = sr.s0, s1 = sr.s1;
     const int sf = s0 + (int)(threadIdx.x / kWave);
-    double pf[R] = {};
-    SellpHead<R> hf{};
+    X pf[R] = {};
+    SellpHead<V, R> hf{};
     if (sf < s1) {
         const int row0 = sf * kWave * R + lane * R;
-        load_rows_p<R>(c.p, row0, pf);
+        load_rows_tp<R>(c.p, row0, pf);
         hf = sellp_head<V, R>(c.A, sf, row0);
     }
     if (done) {
@@ -643,20 +643,14 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int fi
         }
     }
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int s = sf; s < s1; s += (int)(blockDim.x / kWave)) {
+    const int step = (int)(blockDim.x / kWave);
+    for (int s = sf; s < s1; s += step) {
         const int row0 = s * kWave * R + lane * R;
         double pi[R], qd[R], rv[R];
-        SellpHead<R> hd;
-        if (s == sf) {
 #pragma unroll
-            for (int j = 0; j < R; ++j) pi[j] = pf[j];
-            hd = hf;
-        } else {
-            load_rows_p<R>(c.p, row0, pi);
-            hd = sellp_head<V, R>(c.A, s, row0);
-        }
+        for (int j = 0; j < R; ++j) pi[j] = (double)pf[j];
         const X* px = c.p;
-        sellp_rows_gh<V, sizeof(V) == 4, R>(c.A, hd, lane, row0, [px](int cc) { return (double)px[cc]; }, pi, qd);
+        sellp_rows_gh<V, sizeof(V) == 4, R>(c.A, hf, lane, row0, [px](int cc) { return (double)px[cc]; }, pi, qd);
         load_rows_p<R>(c.r, row0, rv);
         X qv[R];
 #pragma unroll
@@ -670,6 +664,12 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cgm_spmv(CgArgs<V, X> c, int fi
             }
         }
         store_rows<R>(c.q, row0, c.A.n, qv);
+        // the next slice's opening loads
+        if (s + step < s1) {
+            const int row1 = (s + step) * kWave * R + lane * R;
+            load_rows_tp<R>(c.p, row1, pf);
+            hf = sellp_head<V, R>(c.A, s + step, row1);
+        }
     }
     const double pq1[1] = {acc[0]};
     block_partials<1>(pq1, c.red, kSlotPQ);
