@@ -380,25 +380,37 @@ struct FinCgResidual {
 template <class V, class X, int R>
 __global__ void __launch_bounds__(kCgMaxBlock) k_cg_spmv(CgArgs<V, X> c) {
     Scalars* S = c.S;
-    if (S->done) {
+    // the wave's first slice opens with its loads issued beside the stop
+    // flag's, not after it; each later slice's at the end of the one before
+    const int done = S->done;
+    const int g = gridDim.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int lb = xcd_block(blockIdx.x, g);
+    const SliceRange sr = block_slices(lb, g, c.A.n_slices);
+    const int s1 = sr.s1, sf = sr.s0 + (int)(threadIdx.x / kWave), step = (int)(blockDim.x / kWave);
+    X pf[R] = {};
+    SellpHead<V, R> hf{};
+    if (sf < s1) {
+        const int row0 = sf * kWave * R + lane * R;
+        load_rows_tp<R>(c.p, row0, pf);
+        hf = sellp_head<V, R>(c.A, sf, row0);
+    }
+    if (done) {
         if (blockIdx.x == 0 && threadIdx.x == 0) S->halt = 1;
         return;
     }
-    const int g = gridDim.x;
     if (blockIdx.x == 0) {
         const double rr = block_sum_partials(c.red, kSlotRR, S->rr_blocks);
         if (threadIdx.x == 0) S->rr = rr;
     }
-    const int lane = threadIdx.x & (kWave - 1);
-    const int lb = xcd_block(blockIdx.x, g);
-    const SliceRange sr = block_slices(lb, g, c.A.n_slices);
-    const int s0 = sr.s0, s1 = sr.s1;
     double acc[1] = {0.0};
-    for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += (int)(blockDim.x / kWave)) {
+    for (int s = sf; s < s1; s += step) {
         const int row0 = s * kWave * R + lane * R;
         double pi[R], qd[R];
-        load_rows_p<R>(c.p, row0, pi);
-        sellp_rows<V, X, sizeof(V) == 4, R>(c.A, s, lane, row0, c.p, pi, qd);
+#pragma unroll
+        for (int j = 0; j < R; ++j) pi[j] = (double)pf[j];
+        const X* px = c.p;
+        sellp_rows_gh<V, sizeof(V) == 4, R>(c.A, hf, lane, row0, [px](int cc) { return (double)px[cc]; }, pi, qd);
         X qv[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) {
@@ -406,6 +418,11 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_spmv(CgArgs<V, X> c) {
             if (row0 + j < c.A.n) acc[0] += pi[j] * (double)qv[j];
         }
         store_rows<R>(c.q, row0, c.A.n, qv);
+        if (s + step < s1) {
+            const int row1 = (s + step) * kWave * R + lane * R;
+            load_rows_tp<R>(c.p, row1, pf);
+            hf = sellp_head<V, R>(c.A, s + step, row1);
+        }
     }
     block_partials<1>(acc, c.red, kSlotPQ);
 }
@@ -422,7 +439,7 @@ template <class V, class X>
 __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_r(CgArgs<V, X> c, int n) {
     constexpr int N = Vec16<X>::N;
     Scalars* S = c.S;
-    if (S->done) return;
+    const int done = S->done;  // tested once the first loads are in flight
     const int nv = n / N;
     const int stride = gridDim.x * blockDim.x;
     const int v0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -439,6 +456,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_r(CgArgs<V, X> c, int
         vload(c.r, v, rv[k]);
         vload(c.q, v, qv[k]);
     }
+    if (done) return;
     ps.finish(c.red, kSl, gs, pqs);
     const double pq = pqs[0];
     if (!cg_pq_ok(pq)) {
@@ -493,7 +511,7 @@ template <class V, class X>
 __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_xp(CgArgs<V, X> c, int n) {
     constexpr int N = Vec16<X>::N;
     Scalars* S = c.S;
-    if (S->halt) return;
+    const int halt = S->halt;  // tested once the first loads are in flight
     const int g = gridDim.x;
     const int nv = n / N;
     const int stride = g * blockDim.x;
@@ -513,6 +531,7 @@ __global__ void __launch_bounds__(kCgMaxBlock) k_cg_update_xp(CgArgs<V, X> c, in
         vload(c.p, v, pv[k]);
         vload(c.r, v, rv[k]);
     }
+    if (halt) return;
     ps.finish(c.red, kSl, gs, sums);
     const double pq = sums[0];
     if (!cg_pq_ok(pq)) return;
