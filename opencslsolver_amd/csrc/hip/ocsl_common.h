@@ -135,15 +135,22 @@ __device__ __forceinline__ void block_partials(const double (&v)[NR], RedScratch
 
 template <int NR, class Fin>
 __global__ void __launch_bounds__(kBlock) k_finalize(RedScratch red, int G, Fin fin, int slot0 = 0) {
-    if (!fin.active()) return;
     __shared__ double sh[NR][kWavesPerBlock];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    // every slot's partials in one pass (each thread's sum of a slot still
+    // runs over i ascending), loads in flight before the finaliser's own
+    // state is read
+    double s[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[r] = 0.0;
+    for (int i = tid; i < G; i += kBlock)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) s[r] += red.partial[(slot0 + r) * kMaxRedBlocks + i];
+    if (!fin.active()) return;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        double s = 0.0;
-        for (int i = tid; i < G; i += kBlock) s += red.partial[(slot0 + r) * kMaxRedBlocks + i];
-        s = wave_sum(s);
-        if (lane == 0) sh[r][wid] = s;
+        const double t = wave_sum(s[r]);
+        if (lane == 0) sh[r][wid] = t;
     }
     __syncthreads();
     if (tid == 0) {
