@@ -236,6 +236,40 @@ int ref_flow_sizes(int n_cells, double lx, double ly, double lz, int64_t* n_face
     return 0;
 }
 
+// The run settings of a RefFlowConfig common to every mesh source.
+static void flow_settings(const RefFlowConfig* cf, Cconfiguration& c) {
+    c.openmp_number_of_threads = 1;
+    c.eproblem_type = enums::Eproblem_type::Fluid_flow;
+    c.eflow_type_visc = cf->visc_type ? enums::Eflow_type_visc::Inviscid : enums::Eflow_type_visc::Viscous;
+    c.etimestepping_type = cf->steady ? enums::Etimestepping_type::Steady : enums::Etimestepping_type::Transient;
+    c.ematrix_solver_type = cf->solver_type ? enums::Ematrix_solver_type::ILU_BICSTAB : enums::Ematrix_solver_type::CG;
+    c.econv_flux_type = cf->conv_type ? enums::Econv_flux_type::Blended_central : enums::Econv_flux_type::First_order_upwind;
+    c.implicit_underrelax = cf->implicit_underrelax;
+    c.implicit_underrelax_max = cf->implicit_underrelax_max;
+    c.underrelax_ramp_iteration = cf->ramp_iterations;
+    c.global_delta_t = cf->dt;
+    c.inital_condition->einitial_condition_type =
+        cf->init_shock_tube ? enums::Einitial_condition_type::Shock_tube : enums::Einitial_condition_type::Uniform;
+    c.inital_condition->Pressure = cf->p0;
+    c.inital_condition->Temperature = cf->T0;
+    c.inital_condition->Pressure2 = cf->p2;
+    c.inital_condition->Temperature2 = cf->T2;
+    for (int d = 0; d < 3; ++d) c.inital_condition->Velocity.val[d] = cf->u0[d];
+    c.max_iteration = cf->steps;
+    c.max_inner_iteration = cf->inner;
+}
+
+// One boundary condition of the configuration.
+static void flow_bc(Cconfiguration& c, int m, int type, int no_slip, int const_T, double T, double p, const double* u) {
+    auto& bc = c.boundary_conditions[m];
+    bc.eflow_BC_type = (enums::Eflow_BC_type)type;
+    bc.ewall_BC_type = no_slip ? enums::Ewall_BC_type::no_slip : enums::Ewall_BC_type::slip;
+    bc.ethermal_BC_type = const_T ? enums::Ethermal_BC_type::constant_temperature : enums::Ethermal_BC_type::adiabatic;
+    bc.Temperature = T;
+    bc.Pressure = p;
+    for (int d = 0; d < 3; ++d) bc.Velocity.val[d] = u[d];
+}
+
 int64_t ref_flow_run(const RefFlowConfig* cf, int64_t hist_cap, double* res_hist, int max_dump, int64_t nnz_cap,
                      uint64_t* rp_out, uint64_t* col_out, int64_t* nnz_out, double* dump_val, double* dump_b,
                      double* dump_x0, double* dump_x, int* dump_method, int max_state, RefFlowState* state,
@@ -339,6 +373,48 @@ int64_t ref_flow_run(const RefFlowConfig* cf, int64_t hist_cap, double* res_hist
             mesh->bc_ksi[b] = bc.diffusion_ksi_A;
             mesh->bc_perp[b] = bc.perp_dist;
         }
+    }
+    return g_rec.count;
+}
+
+// The same run on a mesh read from an SU2 file (Emesh_source::from_file,
+// croot.cpp:72-76) with n_bc boundary conditions, one per marker in file
+// order; cf's 1D geometry and its three boundary conditions are not used.
+// Outputs: the per-solve residual history, the matrix structure and the
+// final fields of the n_cells cells.
+int64_t ref_flow_run_file(const RefFlowConfig* cf, const char* path, int n_bc, const int* bc_type,
+                          const int* bc_no_slip, const int* bc_const_T, const double* bc_T, const double* bc_p,
+                          const double* bc_u, int64_t n_cells, int64_t hist_cap, double* res_hist, int64_t nnz_cap,
+                          uint64_t* rp_out, uint64_t* col_out, int64_t* nnz_out, double* p, double* u, double* T,
+                          double* rho, double* h) {
+    Cconfiguration c;
+    flow_settings(cf, c);
+    c.emesh_source = enums::Emesh_source::from_file;
+    c.mesh_filename = path;
+    c.set_BC_number(n_bc);
+    for (int m = 0; m < n_bc; ++m)
+        flow_bc(c, m, bc_type[m], bc_no_slip[m], bc_const_T[m], bc_T[m], bc_p[m], bc_u + 3 * m);
+    g_rec = Recorder();
+    g_rec.solver_mode = cf->solver_mode;
+    g_rec.hist_cap = hist_cap;
+    g_rec.res_hist = res_hist;
+    g_rec.n = (int)n_cells;
+    g_rec.nnz_cap = nnz_cap;
+    root.run_config(c);
+    const auto& cells = root.Flow_solver->cells;
+    if ((int64_t)cells.size() != n_cells) return -2;
+    const CMatrix_solver& S = *root.Matrix_solver;
+    const int64_t nnz = (int64_t)S.matrix.value.size();
+    *nnz_out = nnz;
+    if (nnz > nnz_cap) return -1;
+    std::memcpy(rp_out, S.matrix.row_ptr.data(), sizeof(uint64_t) * (n_cells + 1));
+    std::memcpy(col_out, S.matrix.column_index.data(), sizeof(uint64_t) * nnz);
+    for (int64_t i = 0; i < n_cells; ++i) {
+        p[i] = cells[i].Pressure;
+        for (int d = 0; d < 3; ++d) u[3 * i + d] = cells[i].Velocity.val[d];
+        T[i] = cells[i].Temperature;
+        rho[i] = cells[i].Density;
+        h[i] = cells[i].Enthalpy;
     }
     return g_rec.count;
 }

@@ -359,3 +359,25 @@ def test_boundary_vtk_without_flow_fields_and_errors(tmp_path):
         mesh.write_boundary_vtk(text, path, {"stress": np.zeros(3 * B)})
     with pytest.raises(mesh.MeshError):
         mesh.write_boundary_vtk(text, "/nonexistent/dir/x.vtk")
+
+
+@pytest.mark.parametrize("kind", ["hex", "prism", "tet"])
+def test_su2_flow_fixture_structure(kind):
+    """The SU2 flow fixtures' reference matrix structure (define_matrix_structure
+    on the reference's reading of the file) equals the oracle's restatement on
+    this reader's arrays for the same text, and the fixture's meshes are the
+    sizes the generator gives."""
+    import json
+
+    import oracle
+
+    d = np.load(os.path.join(GOLDEN, f"flow_su2_{kind}.npz"))
+    meta = json.loads(d["meta"].tobytes())
+    text = d["su2_text"].tobytes().decode()
+    assert text == mesh.box_su2(meta["n"], kind, h=0.01, seed=meta["seed"])
+    m = mesh.parse_su2(text)
+    n = m["cell_volume"].shape[0]
+    assert n == meta["n_cells"] and d["row_ptr"].shape == (n + 1,)
+    rp, col = oracle.define_matrix_structure(n, m["face_left"], m["face_right"], m["bc_left"])
+    assert np.array_equal(rp, d["row_ptr"]) and np.array_equal(col, d["col"])
+
