@@ -477,12 +477,16 @@ def assembly(args, out):
               b_u=rng.normal(size=(B, 3)), b_supersonic=np.zeros(B, np.int32))
     names = ["momentum_x", "momentum_y", "momentum_z", "pressure", "enthalpy"]
 
-    def timed(m, r, c, label):
+    order_names = {flow.CELL_ORDER_CURVE: "Hilbert curve through the cell centres",
+                   flow.CELL_ORDER_LOCALITY: "reverse Cuthill-McKee"}
+
+    def timed(m, r, c, label, cell_order=flow.CELL_ORDER_AUTO):
         res = {}
+        prm.cell_order = cell_order
         with flow.Flow(m, r, c, prm) as fl:
             order = fl.cell_order()
             res["device_cell_order"] = ("caller" if np.array_equal(order, np.arange(order.size))
-                                        else "locality (reverse Cuthill-McKee)")
+                                        else order_names.get(cell_order, "Hilbert curve through the cell centres"))
             fl.set_state(**st)
             for eq, name in enumerate(names):
                 ms, by = fl.time_assemble(eq, reps=10)
@@ -496,9 +500,12 @@ def assembly(args, out):
                         res[name]["traffic_over_algorithmic"] = round(tr / by, 2)
         return res
 
-    res = timed(mesh, rp, col, "permuted cells")
+    # AUTO keeps the permuted mesh's cells along the Hilbert curve
+    res = timed(mesh, rp, col, "permuted cells, device curve order")
+    # the same mesh with the cells kept in reverse Cuthill-McKee order
+    res_rcm = timed(mesh, rp, col, "permuted cells, device locality order", flow.CELL_ORDER_LOCALITY)
     # the same loops with the generator's lexicographic cell numbering, where
-    # a cell's neighbours sit near it in memory
+    # a cell's neighbours sit near it in memory (AUTO keeps it)
     mesh_n, rp_n, col_n = gen.hex_mesh(n)
     res_nat = timed(mesh_n, rp_n, col_n, "lexicographic cells")
     del mesh_n, rp_n, col_n
@@ -513,16 +520,22 @@ def assembly(args, out):
         cpu = {"value": round(cpu_ms, 1), "unit": "ms for the five equations", "cores": threads, "kind": "port",
                "sample": "one assembly of each equation on the same mesh and state (the oracle's restatement of "
                          "the reference loops, OpenMP with atomic adds into linearly searched CSR slots)"}
+    def short(r):
+        return {k: (v if not isinstance(v, dict) else
+                    {kk: v[kk] for kk in ("avg_us", "frac", "traffic_over_algorithmic") if kk in v})
+                for k, v in r.items()}
+
+    def total_us(r):
+        return round(sum(v["avg_us"] for v in r.values() if isinstance(v, dict)), 2)
+
     out["assembly"] = {"workload": f"face/boundary/cell coefficient loops, {n}^3 hex mesh (N={N}, faces={F}, "
                                    f"boundary faces={B}), viscous, blended, transient, permuted cells",
                        "kernel": "k_assemble", "gpu": res,
-                       "gpu_total_us": round(sum(v["avg_us"] for v in res.values() if isinstance(v, dict)), 2),
-                       "gpu_natural_order": {k: (v if not isinstance(v, dict) else
-                                                 {kk: v[kk] for kk in ("avg_us", "frac", "traffic_over_algorithmic")
-                                                  if kk in v})
-                                             for k, v in res_nat.items()},
-                       "gpu_natural_order_total_us": round(sum(v["avg_us"] for v in res_nat.values()
-                                                               if isinstance(v, dict)), 2),
+                       "gpu_total_us": total_us(res),
+                       "gpu_rcm_order": short(res_rcm),
+                       "gpu_rcm_order_total_us": total_us(res_rcm),
+                       "gpu_natural_order": short(res_nat),
+                       "gpu_natural_order_total_us": total_us(res_nat),
                        "cpu_baseline": cpu}
 
 

@@ -106,11 +106,19 @@ typedef struct {
  * left cell) for the kernels that gather over a cell's neighbours.  The
  * caller's numbering, rows, right-hand side and solution order are unchanged
  * whatever the choice, and so are all results, bit for bit: each cell's sums
- * run over its faces in the caller's face order either way.  AUTO (default):
- * a reverse Cuthill-McKee order when the caller's numbering puts face
- * neighbours more than twice as far apart on average, else the caller's.
- * CALLER / LOCALITY force one. */
-typedef enum { OCSL_CELL_ORDER_AUTO = 0, OCSL_CELL_ORDER_CALLER = 1, OCSL_CELL_ORDER_LOCALITY = 2 } ocsl_flow_cell_order;
+ * run over its faces in the caller's face order either way.  LOCALITY: the
+ * reverse Cuthill-McKee order of the matrix graph.  CURVE: the cells sorted
+ * along a 3-D Hilbert curve through their centres (rebuilt from the faces'
+ * centre distance vectors), which keeps most face neighbours of a workgroup's
+ * cells inside it.  AUTO (default): CURVE when the caller's numbering puts
+ * face neighbours more than twice as far apart on average, else the
+ * caller's.  CALLER forces the caller's. */
+typedef enum {
+    OCSL_CELL_ORDER_AUTO = 0,
+    OCSL_CELL_ORDER_CALLER = 1,
+    OCSL_CELL_ORDER_LOCALITY = 2,
+    OCSL_CELL_ORDER_CURVE = 3
+} ocsl_flow_cell_order;
 
 /* The device cell order chosen at create: order[k] = the caller's cell stored
  * k-th (n_cells entries). */

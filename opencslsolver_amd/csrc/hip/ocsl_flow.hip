@@ -26,6 +26,7 @@
 #include "ocsl_flow.h"
 #include "ocsl_handle.h"
 #include "ocsl_kernels.h"
+#include "ocsl_order.h"
 
 namespace ocsl {
 
@@ -1553,18 +1554,22 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
             OCSL_REQUIRE(dpos[i] >= 0, OCSL_ERR_STRUCTURE, "row without diagonal");
         }
         // the device's cell order (ocsl_flow_cell_order)
-        OCSL_REQUIRE(prm->cell_order >= OCSL_CELL_ORDER_AUTO && prm->cell_order <= OCSL_CELL_ORDER_LOCALITY,
+        OCSL_REQUIRE(prm->cell_order >= OCSL_CELL_ORDER_AUTO && prm->cell_order <= OCSL_CELL_ORDER_CURVE,
                      OCSL_ERR_ARG, "bad cell order");
         fl->cperm.resize(n);
         std::iota(fl->cperm.begin(), fl->cperm.end(), 0);
         fl->cell_perm = false;
         if (prm->cell_order != OCSL_CELL_ORDER_CALLER && n > 1) {
-            std::vector<int> rcm = rcm_order(fl->solver->host);  // new -> old
-            bool use = prm->cell_order == OCSL_CELL_ORDER_LOCALITY;
+            // new -> old
+            std::vector<int> ord = prm->cell_order == OCSL_CELL_ORDER_LOCALITY
+                                       ? rcm_order(fl->solver->host)
+                                       : curve_order((int)n, (int)F, m.fL.data(), m.fR.data(), m.fe.data(),
+                                                     m.fdist.data());
+            bool use = prm->cell_order != OCSL_CELL_ORDER_AUTO;
             if (!use) {
                 // mean distance of face neighbours in each numbering
                 std::vector<int> inv(n);
-                for (size_t k = 0; k < n; ++k) inv[rcm[k]] = (int)k;
+                for (size_t k = 0; k < n; ++k) inv[ord[k]] = (int)k;
                 double dc = 0.0, dr = 0.0;
                 for (size_t f = 0; f < F; ++f) {
                     dc += std::abs(m.fL[f] - m.fR[f]);
@@ -1573,7 +1578,7 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
                 use = dc > 2.0 * dr;
             }
             if (use) {
-                fl->cperm = rcm;
+                fl->cperm = std::move(ord);
                 fl->cell_perm = true;
             }
         }

@@ -23,7 +23,7 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 EMPTY, SYMMETRY, WALL, VELOCITY_INLET, SUPERSONIC_INLET, PRESSURE_OUTLET = range(6)
 MOMENTUM_X, MOMENTUM_Y, MOMENTUM_Z, PRESSURE, ENTHALPY = range(5)
 # ocsl_flow_cell_order: how the device stores cells
-CELL_ORDER_AUTO, CELL_ORDER_CALLER, CELL_ORDER_LOCALITY = range(3)
+CELL_ORDER_AUTO, CELL_ORDER_CALLER, CELL_ORDER_LOCALITY, CELL_ORDER_CURVE = range(4)
 # ocsl_flow_support: where the SIMPLE step's support loops run
 SUPPORT_DEVICE, SUPPORT_HOST = range(2)
 MAX_BC = 16

@@ -245,9 +245,9 @@ def test_cell_order_changes_no_bit(support):
     """The device's cell order (ocsl_flow_cell_order) moves data, not
     arithmetic: on a randomly numbered 9^3 hex mesh with every boundary kind,
     two SIMPLE steps give bitwise the same fields, fluxes and per-solve
-    residuals whether the device keeps the caller's order or a locality
-    order; so does one assembly of each equation.  AUTO picks the locality
-    order here."""
+    residuals whether the device keeps the caller's order, the reverse
+    Cuthill-McKee order or the Hilbert-curve order; so does one assembly of
+    each equation.  AUTO picks the curve order here."""
     rng = np.random.default_rng(21)
     n = 9
     mesh, rp, col = gen_hex(n, perm=rng.permutation(n ** 3))
@@ -257,7 +257,8 @@ def test_cell_order_changes_no_bit(support):
            dict(type=flow.SYMMETRY), dict(type=flow.SUPERSONIC_INLET, velocity=(400.0, 0.0, 0.0), pressure=1e5,
                                           temperature=300.0)]
     out = {}
-    for order in (flow.CELL_ORDER_CALLER, flow.CELL_ORDER_LOCALITY, flow.CELL_ORDER_AUTO):
+    orders = (flow.CELL_ORDER_CALLER, flow.CELL_ORDER_LOCALITY, flow.CELL_ORDER_CURVE, flow.CELL_ORDER_AUTO)
+    for order in orders:
         prm = flow.default_params(viscous=1, blended=1, transient=1, dt=1e-4, implicit_underrelax=0.6, p0=1e5,
                                   t0=300.0, u0=(2.0, 0.0, 0.0), bcs=bcs, mode=1, method_pressure=0, support=support,
                                   solve_options=dict(tol=1e-12, max_iter=2000), cell_order=order)
@@ -268,30 +269,33 @@ def test_cell_order_changes_no_bit(support):
             fl.initialize()
             fl.run(2, 2)
             out[order] = (co, systems, fl.get_state(), fl.solve_history())
-    caller, local, auto = (out[o] for o in (flow.CELL_ORDER_CALLER, flow.CELL_ORDER_LOCALITY, flow.CELL_ORDER_AUTO))
+    caller, local, curve, auto = (out[o] for o in orders)
     assert np.array_equal(caller[0], np.arange(n ** 3))
-    assert not np.array_equal(local[0], np.arange(n ** 3))
-    assert np.array_equal(np.sort(local[0]), np.arange(n ** 3))
-    assert np.array_equal(auto[0], local[0])
-    for eq in range(5):
-        for k in range(3):
-            assert np.array_equal(caller[1][eq][k], local[1][eq][k]), (eq, k)
-    for key, v in caller[2].items():
-        assert np.array_equal(v, local[2][key], equal_nan=True), key
-    assert np.array_equal(caller[3], local[3])
+    for other in (local, curve):
+        assert not np.array_equal(other[0], np.arange(n ** 3))
+        assert np.array_equal(np.sort(other[0]), np.arange(n ** 3))
+    assert not np.array_equal(local[0], curve[0])
+    assert np.array_equal(auto[0], curve[0])
+    for other in (local, curve):
+        for eq in range(5):
+            for k in range(3):
+                assert np.array_equal(caller[1][eq][k], other[1][eq][k]), (eq, k)
+        for key, v in caller[2].items():
+            assert np.array_equal(v, other[2][key], equal_nan=True), key
+        assert np.array_equal(caller[3], other[3])
 
 
 def test_cell_order_on_the_shock_tube():
     """The 1D shock tube (the fixtures' mesh, numbered along the tube): AUTO
-    keeps the caller's order; a forced locality order gives the same two
-    steps bit for bit."""
+    keeps the caller's order; a forced reverse Cuthill-McKee or Hilbert-curve
+    order gives the same two steps bit for bit."""
     d = np.load(os.path.join(GOLDEN, "sod_split_2steps.npz"))
     mesh = np.load(os.path.join(GOLDEN, "flow_sod1000_mesh.npz"))
     cfg = dict(FLOW_META["sod64"])
     cfg.update(n_cells=SOD_META["n_cells"], dt=SOD_META["dt"], inner=SOD_META["inner"], solver_type=0,
                solver_mode=1)
     res = {}
-    for order in (flow.CELL_ORDER_AUTO, flow.CELL_ORDER_LOCALITY):
+    for order in (flow.CELL_ORDER_AUTO, flow.CELL_ORDER_LOCALITY, flow.CELL_ORDER_CURVE):
         prm = params_for(cfg)
         prm.cell_order = order
         with flow.Flow(flow.mesh_from_npz(mesh), d["row_ptr"], d["col"], prm) as fl:
@@ -300,6 +304,7 @@ def test_cell_order_on_the_shock_tube():
             fl.run(2, SOD_META["inner"])
             res[order] = (co, fl.get_state(), fl.solve_history())
     assert np.array_equal(res[flow.CELL_ORDER_AUTO][0], np.arange(SOD_META["n_cells"]))
-    for key, v in res[flow.CELL_ORDER_AUTO][1].items():
-        assert np.array_equal(v, res[flow.CELL_ORDER_LOCALITY][1][key], equal_nan=True), key
-    assert np.array_equal(res[flow.CELL_ORDER_AUTO][2], res[flow.CELL_ORDER_LOCALITY][2], equal_nan=True)
+    for order in (flow.CELL_ORDER_LOCALITY, flow.CELL_ORDER_CURVE):
+        for key, v in res[flow.CELL_ORDER_AUTO][1].items():
+            assert np.array_equal(v, res[order][1][key], equal_nan=True), (order, key)
+        assert np.array_equal(res[flow.CELL_ORDER_AUTO][2], res[order][2], equal_nan=True), order
