@@ -665,6 +665,27 @@ def structure(args, out):
     }
 
 
+def tri_solves(args, out):
+    """Exact ILU(0) triangular solves (ILU_forsub / ILU_backsub of the 3-sweep
+    factor, level-scheduled) on randomly numbered convection-diffusion
+    systems (about 20 level sets: one launch per level) and a lexicographic
+    one (one level per diagonal plane: one workgroup for the whole solve)."""
+    from opencslsolver_amd import gen
+    from opencslsolver_amd.solver import Solver
+
+    res = {}
+    for n, permute in ((64, True), (96, True), (96, False)):
+        a, _ = gen.convdiff3d(n, seed_perm=3, permute=permute)
+        key = f"{n}^3 {'permuted' if permute else 'lexicographic'}"
+        _log(f"bench: triangular solves, {key}")
+        with Solver(a.row_ptr, a.col) as s:
+            s.set_values(a.val)
+            res[key] = {"lower_us": round(s.time_tri_solve(True, reps=20) * 1e3, 2),
+                        "upper_us": round(s.time_tri_solve(False, reps=20) * 1e3, 2)}
+    out["tri_solves"] = {"workload": "exact L and U solves of the 3-sweep ILU(0) factor of the convection-diffusion "
+                                     "system (HIP events, 20 solves each)", **res}
+
+
 def flow_step(args, out):
     """One SIMPLE iteration of the flow solver (three momentum solves with
     ILU(0)-BiCGSTAB, the pressure correction with CG, enthalpy; REFERENCE-mode
@@ -714,9 +735,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--sections", default="adjoint,multi,batch,assembly,flow,structure,mesh",
-                    help="side workloads to add to the headline: adjoint, multi, batch, assembly, flow, "
-                         "structure (or none)")
+    ap.add_argument("--sections", default="adjoint,multi,batch,assembly,tri,flow,structure,mesh",
+                    help="side workloads to add to the headline: adjoint, multi, batch, assembly, tri, flow, "
+                         "structure, mesh (or none)")
     ap.add_argument("--flow-n", type=int, default=64, help="hex mesh size of the flow-step section")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reuse-tuning", action="store_true",
@@ -757,6 +778,8 @@ def main():
         batch(args, out)
     if "assembly" in args.sections:
         assembly(args, out)
+    if "tri" in args.sections:
+        tri_solves(args, out)
     if "flow" in args.sections:
         flow_step(args, out)
     if "structure" in args.sections:

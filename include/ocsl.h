@@ -289,6 +289,13 @@ ocsl_status ocsl_set_tuned_cg(ocsl_handle h, int precision, const ocsl_options* 
  * default).  0 issues every launch each time; results are bitwise the same. */
 ocsl_status ocsl_set_reference_graphs(ocsl_handle h, int on);
 
+/* Measurement hook (no counterpart in the reference): the exact lower
+ * (lower = 1) or upper (0) solve of the handle's 3-sweep ILU(0) factor
+ * (ILU_forsub / ILU_backsub, cmatrix_solver.cpp:553-633, level-scheduled),
+ * `reps` times between two HIP events on the handle's stream; *avg_ms
+ * receives the average per solve.  Values must be set. */
+ocsl_status ocsl_time_tri_solve(ocsl_handle h, int lower, int reps, double* avg_ms);
+
 /* The row order of OCSL_BICGSTAB_ORDER_COLOUR solves (no counterpart in the
  * reference): order[i] = the input row placed at position i (n entries;
  * NULL to query only), *n_colours = the colours, i.e. the levels of each

@@ -1362,3 +1362,27 @@ extern "C" ocsl_status ocsl_ILU_decompose_from(ocsl_handle h, int sweeps, const 
         sync(h);
     });
 }
+
+extern "C" ocsl_status ocsl_time_tri_solve(ocsl_handle h, int lower, int reps, double* avg_ms) {
+    if (!h || !avg_ms || reps <= 0 || (lower != 0 && lower != 1)) return OCSL_ERR_ARG;
+    return tri_guard([&] {
+        OCSL_REQUIRE(h->have_vals, OCSL_ERR_STATE, "values not set");
+        const int n = h->host.n;
+        ensure_csr_values(h, false);
+        ensure_levels(h);
+        ilu_decompose_device(h, 3, false);
+        DBuf<double> in, out;
+        in.alloc((size_t)n);
+        out.alloc((size_t)n);
+        k_fill<double><<<grid_for(n), kBlock, 0, h->stream>>>(n, 1.0, in.p);
+        OCSL_HIP(hipGetLastError());
+        tri_exact<double, double, 1>(h, h->ilu_m.p, in.p, out.p, lower != 0, nullptr);  // warm: level values
+        OCSL_HIP(hipEventRecord(h->ev0, h->stream));
+        for (int r = 0; r < reps; ++r) tri_exact<double, double, 1>(h, h->ilu_m.p, in.p, out.p, lower != 0, nullptr);
+        OCSL_HIP(hipEventRecord(h->ev1, h->stream));
+        sync(h);
+        float ms = 0.f;
+        OCSL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *avg_ms = ms / reps;
+    });
+}
