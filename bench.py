@@ -481,6 +481,7 @@ def assembly(args, out):
                    flow.CELL_ORDER_LOCALITY: "reverse Cuthill-McKee"}
 
     def timed(m, r, c, label, cell_order=flow.CELL_ORDER_AUTO):
+        _log(f"bench: assembly, {label}")
         res = {}
         prm.cell_order = cell_order
         with flow.Flow(m, r, c, prm) as fl:
