@@ -29,10 +29,10 @@ __global__ void k_ilu_sweep(int nnz, const double* __restrict__ a, const double*
                             const int* __restrict__ pk, const int* __restrict__ pq,
                             const int* __restrict__ den) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += gridDim.x * blockDim.x) {
-        double dum = 0.0;
-        for (int t = pptr[p]; t < pptr[p + 1]; ++t) dum += old[pk[t]] * old[pq[t]];
+        double tmp = 0.0;
+        for (int t = pptr[p]; t < pptr[p + 1]; ++t) tmp += old[pk[t]] * old[pq[t]];
         const int d = den[p];
-        nw[p] = d >= 0 ? (a[p] - dum) / old[d] : a[p] - dum;
+        nw[p] = d >= 0 ? (a[p] - tmp) / old[d] : a[p] - tmp;
     }
 }
 
@@ -469,7 +469,7 @@ static void apply_precond(ocsl_solver* h, const VM* m, const X* in, X* tmp, X* t
 
 // ------------------------------------------------------------------ reference mode
 // Device-resident scalars of the REFERENCE BiCGSTAB: the dot products land
-// in S->aux (0: r0.r, 1: r0.v, 2: r.r, 3: b.b, 4: dum.dum2, 5: dum.dum), a
+// in S->aux (0: r0.r, 1: r0.v, 2: r.r, 3: b.b, 4: (L^-1 t).(L^-1 s), 5: (L^-1 t).(L^-1 t)), a
 // one-thread kernel forms alpha, the residual and omega with the reference's
 // formulas, and the vector updates read their coefficients from S.  The
 // reference's early exit (residual below 1e-20, before x = h) sets S->halt,
@@ -497,8 +497,8 @@ void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
     ensure_sell_values(h, s, true, false);
     Work& w = h->work;
     for (int i = 0; i < 11; ++i) work_buf(h, w.d[i], n);
-    double *r = w.d[0].p, *p = w.d[1].p, *v = w.d[2].p, *r0 = w.d[3].p, *sv = w.d[4].p, *dum = w.d[5].p;
-    double *hh = w.d[6].p, *t = w.d[7].p, *y = w.d[8].p, *z = w.d[9].p, *dum2 = w.d[10].p;
+    double *r = w.d[0].p, *p = w.d[1].p, *v = w.d[2].p, *r0 = w.d[3].p, *sv = w.d[4].p, *tmp = w.d[5].p;
+    double *hh = w.d[6].p, *t = w.d[7].p, *y = w.d[8].p, *z = w.d[9].p, *lts = w.d[10].p;
     double* x = h->x_own.p;
     const double* b = h->b_dev;
     const int gv = grid_for(n);
@@ -545,10 +545,10 @@ void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
     k_permute_in<double><<<gv, kBlock, 0, h->stream>>>(n, nullptr, r, r0);
     for (int it = 0; it < 3; ++it) {
         dot1(r0, r, 0);                                                // rhon
-        upd(1.0, nullptr, p, -1.0, omega, v, dum);                     // p - omega v
-        upd(1.0, nullptr, r, 1.0, omega, dum, p);                      // p = r + omega (p - omega v)
-        tri_exact<double, double, 1>(h, m, p, dum, true, S);
-        tri_exact<double, double, 1>(h, m, dum, y, false, S);
+        upd(1.0, nullptr, p, -1.0, omega, v, tmp);                     // p - omega v
+        upd(1.0, nullptr, r, 1.0, omega, tmp, p);                      // p = r + omega (p - omega v)
+        tri_exact<double, double, 1>(h, m, p, tmp, true, S);
+        tri_exact<double, double, 1>(h, m, tmp, y, false, S);
         spmv(1.0, y, 0.0, nullptr, v);
         dot1(r0, v, 1);
         k_bref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 0, it);    // alpha
@@ -556,11 +556,11 @@ void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
         dot2(r, r, b, b, 2);
         k_bref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 1, it);    // residual, early exit
         upd(-1.0, alpha, v, 1.0, nullptr, r, sv);                      // s = r - alpha v
-        tri_exact<double, double, 1>(h, m, sv, dum, true, S);
-        tri_exact<double, double, 1>(h, m, dum, z, false, S);
+        tri_exact<double, double, 1>(h, m, sv, tmp, true, S);
+        tri_exact<double, double, 1>(h, m, tmp, z, false, S);
         spmv(1.0, z, 0.0, nullptr, t);
-        tri_exact_pair<double>(h, m, t, dum, sv, dum2, true, S);       // L^-1 t, L^-1 s
-        dot2(dum, dum2, dum, dum, 4);
+        tri_exact_pair<double>(h, m, t, tmp, sv, lts, true, S);       // L^-1 t, L^-1 s
+        dot2(tmp, lts, tmp, tmp, 4);
         k_bref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 2, it);    // omega
         upd(1.0, nullptr, hh, 1.0, omega, z, x);                       // x = h + omega z
         upd(-1.0, omega, t, 1.0, nullptr, sv, r);                      // r = s - omega t

@@ -217,19 +217,19 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
             rhs += side == 0 ? -mdot : mdot;
             const double Tj = s.T[j], qj = s.pc[j];
             const double Tf = conv_interp<kBlend>(geo, mdot, side ? Tj : own_T, side ? own_T : Tj);
-            double dum = gas.dp_drho(Tf);
-            dum = dum / s.f_rho[f];
-            dum *= mdot;
-            conv_row<kBlend>(a, rhs, side, dp, op, dum, side ? qj : own_q, side ? own_q : qj, geo);
-            const double avgD = s.f_qy[f] * s.f_rho[f];
-            diff_row(a, dp, op, avgD);
+            double conv_w = gas.dp_drho(Tf);
+            conv_w = conv_w / s.f_rho[f];
+            conv_w *= mdot;
+            conv_row<kBlend>(a, rhs, side, dp, op, conv_w, side ? qj : own_q, side ? own_q : qj, geo);
+            const double diff_w = s.f_qy[f] * s.f_rho[f];
+            diff_row(a, dp, op, diff_w);
         } else {
             const double Tj = s.T[j], qj = s.h[j];
             conv_row<kBlend>(a, rhs, side, dp, op, mdot, side ? qj : own_q, side ? own_q : qj, geo);
             const double Tf = face_avg(geo, side ? Tj : own_T, side ? own_T : Tj);
-            double avgD = gas.viscosity(Tf);
-            avgD = avgD / gas.pr * m.cf_ksi[e];
-            diff_row(a, dp, op, avgD);
+            double diff_w = gas.viscosity(Tf);
+            diff_w = diff_w / gas.pr * m.cf_ksi[e];
+            diff_row(a, dp, op, diff_w);
         }
     }
     // boundary faces (cflow_solver_cell_face_loops.cpp:77-132, 229-297, 357-421)
@@ -246,15 +246,15 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
                         for (int k = 0; k < 3; ++k) nrm[k] = inv * m.bS[3 * bf + k];
                         double D = gas.viscosity(s.T[i]);
                         D *= m.barea[bf] / m.bperp[bf];
-                        double dum = D * (1.0 - (nrm[d] * nrm[d]));
-                        add_at(a, dp, dum);
-                        dum = 0.0;
+                        const double a_wall = D * (1.0 - (nrm[d] * nrm[d]));
+                        add_at(a, dp, a_wall);
+                        double tang = 0.0;
                         for (int k = d + 1; k < d + 3; ++k) {
-                            const int k_ = k % 3;
-                            dum += s.u[3 * i + k_] * nrm[k_] * nrm[d];
+                            const int kk = k % 3;
+                            tang += s.u[3 * i + kk] * nrm[kk] * nrm[d];
                         }
-                        dum = D * dum;
-                        rhs += dum;
+                        tang = D * tang;
+                        rhs += tang;
                     }
                 }
             }
@@ -266,36 +266,36 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
         } else if constexpr (kEq == kPressure) {
             if (bc.type == OCSL_BC_VELOCITY_INLET) {
                 rhs += -bm;
-                double dum = gas.dp_drho(s.b_T[bf]) / s.rho[i];
-                dum *= bm;
-                if (!isnan(dum)) add_at(a, dp, dum);  // Convective_flux_neumann_zero_BC adds any non-NaN flux
+                double term = gas.dp_drho(s.b_T[bf]) / s.rho[i];
+                term *= bm;
+                if (!isnan(term)) add_at(a, dp, term);  // Convective_flux_neumann_zero_BC adds any non-NaN flux
             }
             if (bc.type == OCSL_BC_SUPERSONIC_INLET) {
                 rhs += -bm;
-                const double avgD = s.b_qy[bf] * s.b_rho[bf];
-                add_at(a, dp, avgD);
-                rhs += avgD * 0.0;
+                const double diff_w = s.b_qy[bf] * s.b_rho[bf];
+                add_at(a, dp, diff_w);
+                rhs += diff_w * 0.0;
             }
             if (bc.type == OCSL_BC_PRESSURE_OUTLET) {
                 rhs += -bm;
                 if (s.b_super[bf]) {
-                    double dum = gas.dp_drho(s.T[i]) / s.rho[i];
-                    dum *= bm;
-                    if (!isnan(dum)) add_at(a, dp, dum);
+                    double term = gas.dp_drho(s.T[i]) / s.rho[i];
+                    term *= bm;
+                    if (!isnan(term)) add_at(a, dp, term);
                 } else {
-                    const double avgD = s.b_qy[bf] * s.rho[i];
-                    add_at(a, dp, avgD);
-                    rhs += avgD * 0.0;
+                    const double diff_w = s.b_qy[bf] * s.rho[i];
+                    add_at(a, dp, diff_w);
+                    rhs += diff_w * 0.0;
                 }
             }
         } else {
             if (bc.type == OCSL_BC_VELOCITY_INLET || bc.type == OCSL_BC_SUPERSONIC_INLET) {
                 const double v = bm * s.b_h[bf];
                 rhs += -v;
-                double avgD = gas.viscosity(s.b_T[bf]);
-                avgD *= gas.pr * m.bksi[bf];  // mu * Pr: the reference's boundary form
-                add_at(a, dp, avgD);
-                rhs += avgD * s.b_h[bf];
+                double diff_w = gas.viscosity(s.b_T[bf]);
+                diff_w *= gas.pr * m.bksi[bf];  // mu * Pr: the reference's boundary form
+                add_at(a, dp, diff_w);
+                rhs += diff_w * s.b_h[bf];
             }
             if (bc.type == OCSL_BC_PRESSURE_OUTLET) {
                 if (!isnan(bm)) add_at(a, dp, bm);
@@ -306,15 +306,15 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
     const double V = m.vol[i];
     double diag = a[dp];
     if constexpr (kEq == kMomentum) {
-        double dum = 0.0, dum2 = 0.0;
+        double term = 0.0, term2 = 0.0;
         if constexpr (kTransient) {
-            dum = V / s.dt[i];
-            dum2 = dum * s.rho[i];
-            diag += dum2;
-            dum2 = dum * (s.u_old[3 * i + d] * s.rho_old[i]);
+            term = V / s.dt[i];
+            term2 = term * s.rho[i];
+            diag += term2;
+            term2 = term * (s.u_old[3 * i + d] * s.rho_old[i]);
         }
-        dum2 = dum2 - s.gp[3 * i + d] * V;
-        rhs += dum2;
+        term2 = term2 - s.gp[3 * i + d] * V;
+        rhs += term2;
         if constexpr (!kTransient) {
             const double dd = diag;
             diag = dd / ph.urf;
@@ -325,23 +325,23 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
         x0[oi] = s.u[3 * i + d];
     } else if constexpr (kEq == kPressure) {
         if constexpr (kTransient) {
-            const double dum = V / s.dt[i];
-            double dum2 = dum * (s.rho[i] - s.rho_old[i]);
-            rhs += -dum2;
-            dum2 = dum * gas.dp_drho(s.T[i]);
-            diag += dum2;
+            const double term = V / s.dt[i];
+            double term2 = term * (s.rho[i] - s.rho_old[i]);
+            rhs += -term2;
+            term2 = term * gas.dp_drho(s.T[i]);
+            diag += term2;
         } else {
             diag = diag / ph.urf;
         }
         x0[oi] = s.pc[i];
     } else {
         if constexpr (kTransient) {
-            const double dum = V / s.dt[i];
-            double dum2 = dum * s.rho[i];
-            diag += dum2;
-            dum2 = dum * (s.h_old[i] * s.rho_old[i]);
-            dum2 = dum2 + (s.p[i] - s.p_old[i]) / s.dt[i] * V;
-            rhs += dum2;
+            const double term = V / s.dt[i];
+            double term2 = term * s.rho[i];
+            diag += term2;
+            term2 = term * (s.h_old[i] * s.rho_old[i]);
+            term2 = term2 + (s.p[i] - s.p_old[i]) / s.dt[i] * V;
+            rhs += term2;
         } else {
             const double dd = diag;
             diag = dd / ph.urf;

@@ -125,9 +125,9 @@ __global__ void __launch_bounds__(kBlock) k_sell_axpby(SellView<V> A, double a, 
     const int nwaves = gridDim.x * kWavesPerBlock;
     for (int s = wave; s < A.n_slices; s += nwaves) {
         const int row = s * kWave + lane;
-        const double dum = sell_row<V, double, double>(A, s, lane, x);
+        const double ax = sell_row<V, double, double>(A, s, lane, x);
         if (row < A.n) {
-            double o = (a == 1.0) ? dum : a * dum;
+            double o = (a == 1.0) ? ax : a * ax;
             if (z) o += b * z[row];
             out[row] = o;
         }
