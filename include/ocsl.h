@@ -79,8 +79,8 @@ typedef struct {
                             with each block's row window of the direction staged in LDS);
                             2 = split (SpMV + update for MIXED, three kernels for FP64);
                             3 = one kernel, gathering from memory; 0 = tuned, else 2 */
-    int x_period;        /* MIXED split CG: iterations per update of the FP64 solution, 2, 4 or 8;
-                            0 = 8 */
+    int x_period;        /* MIXED split CG and MIXED batch: iterations per update of the FP64
+                            solution, 2, 4 or 8 (the batch also takes 1); 0 = 8 */
     int bicgstab_order;  /* ocsl_bicgstab_order: row order of the TOLERANCE ILU(0)-BiCGSTAB; default input */
 } ocsl_options;
 

@@ -20,6 +20,7 @@
 #include <chrono>
 #include <cmath>
 #include <numeric>
+#include <tuple>
 
 #include "ocsl_handle.h"
 #include "ocsl_kernels.h"
@@ -30,12 +31,22 @@ namespace ocsl {
 // one SpMV workgroup with one 16-byte vector per thread and round.
 constexpr int kBtUpdateBlock = 256;
 
+// Most iterations per FP64 update of x (the period of the MIXED solves).
+constexpr int kBtMaxPeriod = 8;
+
 struct SysScalars {
     double rr, bb, r_ref, tol2, alpha, beta, replace_ratio;
     int iter, max_iter, done, halt, need_update, breakdown, nonfinite, replacements;
     int verified;     // convergence confirmed on the true FP64 residual (MIXED)
     int replace_now;  // the next SpMV kernel replaces r instead (set by the update kernel)
     int replaced;     // this iteration's SpMV kernel replaced r (set by the SpMV kernel)
+    // x period (see k_bt_update): the steps of the current period, whether
+    // position j's update left x holding every step up to j, the direction
+    // buffer of the iteration at which the system stopped, and the stop of
+    // this iteration (its update adds the pending steps)
+    double alpha_q[kBtMaxPeriod];
+    int added[kBtMaxPeriod];
+    int pdir, flush_now;
 };
 
 // partials of the true r.r of an in-iteration replacement (the batch's
@@ -56,7 +67,7 @@ struct BtArgs {
     SellPView<V> A;  // concatenated; n = padded total rows
     BatchMap map;
     X* r;
-    X* p;
+    X* p;      // this iteration's direction: P_j
     X* q;
     double* x;
     SysScalars* sys;
@@ -64,6 +75,11 @@ struct BtArgs {
     int slice_rows;        // 64 R
     SellPView<double> A64; // FP64 values (the residual replacement)
     const double* b;
+    X* pb;          // direction buffers P_0 .. P_{M-1}, pstride elements apart
+    int64_t pstride;
+    int M;          // iterations per FP64 update of x
+    int j;          // this iteration's position in the period
+    X* p_alt;       // the next direction: P_{(j + 1) mod M}
 };
 
 // Sum of system k's partials of one slot; identical in all its blocks.
@@ -152,18 +168,25 @@ __device__ __host__ __forceinline__ bool bt_wants_residual(const SysScalars& S) 
     return S.done && !S.verified && !S.nonfinite && !S.breakdown;
 }
 
-// r = b - A x (FP64) for the systems being (re)started; p = r on the first
-// call; partials of r.r and b.b.  One wave per slice of 64 R rows; every row
-// of the concatenation exists (systems are padded to whole slices, padding
-// rows are zero rows with zero right-hand side).
+// r = b - A x (FP64) for the systems being (re)started; p = r (in P_0) on
+// the first call; partials of r.r and b.b.  One wave per slice of 64 R rows;
+// every row of the concatenation exists (systems are padded to whole slices,
+// padding rows are zero rows with zero right-hand side).  A system that goes
+// on iterating after this pass resumes at period position phi (>= 0): its
+// direction moves from the buffer it stopped at (pdir) to P_phi.
 template <class X, int R>
 __global__ void __launch_bounds__(1024) k_bt_residual(SellPView<double> A, BatchMap map, const double* __restrict__ b,
-                                                    const double* __restrict__ x, X* r, X* p, SysScalars* sys,
-                                                    RedScratchBig red, int first) {
+                                                    const double* __restrict__ x, X* r, X* pb, int64_t pstride,
+                                                    int phi, SysScalars* sys, RedScratchBig red, int first) {
     const int gb = map.block();
     const int k = map.bsys[gb];
     if (!first && !bt_wants_residual(sys[k])) return;
     const int lane = threadIdx.x & (kWave - 1);
+    X* p = pb;  // P_0
+    const int pdir = sys[k].pdir;
+    const bool move = !first && phi >= 0 && pdir != phi;
+    const X* p_from = pb + (int64_t)pdir * pstride;
+    X* p_to = pb + (int64_t)(phi > 0 ? phi : 0) * pstride;
     double acc[2] = {0.0, 0.0};
     for (int s = map.bs0[gb] + (int)(threadIdx.x / kWave); s < map.bs1[gb];
          s += (int)(blockDim.x / kWave)) {
@@ -182,12 +205,19 @@ __global__ void __launch_bounds__(1024) k_bt_residual(SellPView<double> A, Batch
         }
         store_rows<R>(r, row0, A.n, rv);
         if (first) store_rows<R>(p, row0, A.n, rv);
+        if (move) {
+            X pv[R];
+            load_rows_tp<R>(p_from, row0, pv);
+            store_rows<R>(p_to, row0, A.n, pv);
+        }
     }
     bt_partials<2>(acc, red, kSlotRR, gb);
 }
 
 // One block per system: bb, rr, the convergence test on the true residual.
-__global__ void k_bt_fin_residual(BatchMap map, SysScalars* sys, RedScratchBig red, int first, int m) {
+// A system that goes on iterating resumes at period position phi: the
+// positions before it hold no pending step.
+__global__ void k_bt_fin_residual(BatchMap map, SysScalars* sys, RedScratchBig red, int first, int m, int phi) {
     const int k = blockIdx.x;
     if (k >= m) return;
     SysScalars& S = sys[k];
@@ -209,6 +239,8 @@ __global__ void k_bt_fin_residual(BatchMap map, SysScalars* sys, RedScratchBig r
         S.need_update = 0;
         S.replace_now = 0;
         S.replaced = 0;
+        S.flush_now = 0;
+        for (int i = 0; i < kBtMaxPeriod; ++i) S.added[i] = i < phi ? 1 : 0;
     }
 }
 
@@ -220,8 +252,12 @@ __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
     const int gb = c.map.block();
     const int k = c.map.bsys[gb];
     SysScalars* S = c.sys + k;
-    if (S->done) {
-        if (gb == c.map.bptr[k] && threadIdx.x == 0) S->halt = 1;
+    if (S->done || S->breakdown || S->nonfinite) {
+        if (gb == c.map.bptr[k] && threadIdx.x == 0) {
+            S->halt = 1;
+            S->done = 1;       // a breakdown found by the last update stops the system here
+            S->flush_now = 0;  // the update after the stop has added the pending steps
+        }
         return;
     }
     const int lane = threadIdx.x & (kWave - 1);
@@ -253,6 +289,12 @@ __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
             const double scale = S->bb > 0.0 ? S->bb : 1.0;
             if (!(rr > S->tol2 * scale) || S->iter >= S->max_iter) S->done = 1;
             if (!isfinite(rr)) { S->nonfinite = 1; S->done = 1; }
+            if (S->done) {
+                // this iteration's update adds the period's pending steps;
+                // the direction stays in P_j
+                S->flush_now = 1;
+                S->pdir = c.j;
+            }
             S->need_update = sqrt(rr) < S->replace_ratio * S->r_ref;
             S->replaced = 0;
         }
@@ -282,18 +324,72 @@ __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
     bt_partials<2>(rq_qq, c.red, kSlotRQ, gb);
 }
 
-// K2: alpha, beta per system; x += alpha p; r -= alpha q; p = r + beta p; r.r
+// x += alpha_q[lo] P_lo + ... + alpha_q[hi-1] P_{hi-1} over the block's rows
+// (16-byte vectors v0..v1 of X): the pending steps of a period cut short.
+template <class V, class X>
+__device__ __forceinline__ void bt_add_pending(const BtArgs<V, X>& c, const SysScalars* S, int v0, int v1, int lo,
+                                               int hi) {
+    constexpr int N = Vec16<X>::N;
+    if (lo >= hi) return;
+    for (int v = v0 + (int)threadIdx.x; v < v1; v += (int)blockDim.x) {
+        double xe[N];
+        dload<N>(c.x, v, xe);
+        for (int i = lo; i < hi; ++i) {
+            const double ai = S->alpha_q[i];
+            X pp[N];
+            vload(c.pb + (int64_t)i * c.pstride, v, pp);
+#pragma unroll
+            for (int e = 0; e < N; ++e) xe[e] += ai * (double)pp[e];
+        }
+        dstore<N>(c.x, v, xe);
+    }
+}
+
+// K2: alpha, beta per system; r -= alpha q; p' = r + beta p (into P_{j+1});
+// partials of r.r.
+// x period: the FP64 x takes the steps only at the period's last position
+// (j = M - 1), before an FP64 residual replacement (which reads x), and when
+// the system stops (a breakdown here, or a stop found by the next SpMV,
+// after which this kernel adds the steps).  Every step is one FP64 addition
+// in iteration order, so x is bitwise the x of an update every iteration;
+// x moves 16 bytes per row once per period instead of every iteration.
+// Position j's update records its step (alpha_q[j]) and whether x then holds
+// every step up to j (added[j]); the pending steps at position j are those
+// after the last position i < j with added[i].  A position writes only its
+// own entries and reads only earlier ones, so no kernel reads what it writes.
 template <class V, class X>
 __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
     constexpr int N = Vec16<X>::N;
     const int gb = c.map.block();
     const int k = c.map.bsys[gb];
     SysScalars* S = c.sys + k;
-    if (S->done) return;
+    const int j = c.j;
+    // the block's rows are whole slices: 16-byte vectors v0..v1
+    const int v0 = c.map.bs0[gb] * (c.slice_rows / N), v1 = c.map.bs1[gb] * (c.slice_rows / N);
+    int lo = 0;  // first pending position of the period
+    for (int i = j - 1; i >= 0; --i)
+        if (S->added[i]) {
+            lo = i + 1;
+            break;
+        }
+    // (done is set only by other kernels; the breakdown flags this kernel
+    // may set stop the system at the next SpMV)
+    if (S->done) {
+        if (S->flush_now) bt_add_pending(c, S, v0, v1, lo, j);
+        return;
+    }
     const int b0 = c.map.bptr[k], b1 = c.map.bptr[k + 1];
     if (S->replaced) {
-        // the replaced residual's r.r partials become the next rr; its norm
-        // the reference of the next replacement test
+        // no step at this position (x already holds every earlier step): the
+        // direction moves on to P_{j+1}; the replaced residual's r.r partials
+        // become the next rr, its norm the reference of the next replacement
+        // test
+        if (c.p_alt != c.p)
+            for (int v = v0 + (int)threadIdx.x; v < v1; v += (int)blockDim.x) {
+                X pe[N];
+                vload(c.p, v, pe);
+                vstore(c.p_alt, v, pe);
+            }
         if (threadIdx.x == 0)
             c.red.partial[(size_t)kSlotRR * c.red.stride + gb] = c.red.partial[(size_t)kBtSlotTR * c.red.stride + gb];
         if (gb == b0) {
@@ -303,22 +399,23 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
                 S->replacements++;
                 S->replace_now = 0;
                 S->need_update = 0;
-                if (!isfinite(tr)) { S->nonfinite = 1; S->done = 1; }
+                S->added[j] = 1;
+                if (!isfinite(tr)) S->nonfinite = 1;
             }
         }
         return;
     }
+    // x takes the steps at the period's end and before a replacement
+    const bool add = j == c.M - 1 || S->need_update;
     constexpr int kSl[3] = {kSlotPQ, kSlotRQ, kSlotQQ};
     SysSums<3> ps;
     ps.issue(c.red, kSl, b0, b1);
-    // the block's rows are whole slices: 16-byte vectors, the first one in
-    // flight while the partial sums are re-added
-    const int v0 = c.map.bs0[gb] * (c.slice_rows / N), v1 = c.map.bs1[gb] * (c.slice_rows / N);
+    // the first vector in flight while the partial sums are re-added
     const int va = v0 + (int)threadIdx.x;
     double xa[N];
     X pa[N], ra[N], qa[N];
     if (va < v1) {
-        dload<N>(c.x, va, xa);
+        if (add) dload<N>(c.x, va, xa);
         vload(c.p, va, pa);
         vload(c.r, va, ra);
         vload(c.q, va, qa);
@@ -327,10 +424,11 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
     ps.finish(c.red, kSl, b0, b1, d3);
     const double pq = d3[0], rq = d3[1], qq = d3[2];
     if (!(pq != 0.0 && isfinite(pq))) {
+        // x keeps the period's earlier steps; the next SpMV stops the system
+        bt_add_pending(c, S, v0, v1, lo, j);
         if (gb == b0 && threadIdx.x == 0) {
             S->breakdown = (pq == 0.0);
             S->nonfinite = !isfinite(pq);
-            S->done = 1;
         }
         return;
     }
@@ -339,26 +437,40 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
     double rr_next = rr - 2.0 * alpha * rq + alpha * alpha * qq;
     if (!(rr_next > 0.0)) rr_next = 0.0;
     const double beta = rr_next / rr;
+    // the pending steps' sizes, in registers (static indices)
+    double al[kBtMaxPeriod];
+#pragma unroll
+    for (int i = 0; i < kBtMaxPeriod; ++i) al[i] = (add && i >= lo && i < j) ? S->alpha_q[i] : 0.0;
     double acc[1] = {0.0};
     auto body = [&](int v, double (&xe)[N], const X (&pe)[N], const X (&re)[N], const X (&qe)[N]) {
         X rn[N], pn[N];
 #pragma unroll
         for (int e = 0; e < N; ++e) {
-            const double p0 = (double)pe[e];
-            xe[e] += alpha * p0;
             rn[e] = (X)((double)re[e] - alpha * (double)qe[e]);
-            pn[e] = (X)((double)rn[e] + beta * p0);
+            pn[e] = (X)((double)rn[e] + beta * (double)pe[e]);
             acc[0] += (double)rn[e] * (double)rn[e];
         }
-        dstore<N>(c.x, v, xe);
+        if (add) {
+#pragma unroll
+            for (int i = 0; i < kBtMaxPeriod; ++i)
+                if (i >= lo && i < j) {
+                    X pp[N];
+                    vload(c.pb + (int64_t)i * c.pstride, v, pp);
+#pragma unroll
+                    for (int e = 0; e < N; ++e) xe[e] += al[i] * (double)pp[e];
+                }
+#pragma unroll
+            for (int e = 0; e < N; ++e) xe[e] += alpha * (double)pe[e];
+            dstore<N>(c.x, v, xe);
+        }
         vstore(c.r, v, rn);
-        vstore(c.p, v, pn);
+        vstore(c.p_alt, v, pn);
     };
     if (va < v1) body(va, xa, pa, ra, qa);
     for (int v = va + (int)blockDim.x; v < v1; v += (int)blockDim.x) {
         double xe[N];
         X pe[N], re[N], qe[N];
-        dload<N>(c.x, v, xe);
+        if (add) dload<N>(c.x, v, xe);
         vload(c.p, v, pe);
         vload(c.r, v, re);
         vload(c.q, v, qe);
@@ -370,6 +482,8 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
         S->beta = beta;
         ++S->iter;
         S->replace_now = S->need_update;
+        S->alpha_q[j] = alpha;
+        S->added[j] = add ? 1 : 0;
     }
 }
 
@@ -400,16 +514,19 @@ struct ocsl_batch_s {
     bool compact = true;
     int64_t total = 0;
     DBuf<double> val_in, val64, diag64, b_in, b_glob, x_glob;
+    DBuf<double> x_flat;                 // the solutions in the callers' order (staging of x0 and x)
     DBuf<float> val32, diag32;
     std::vector<int64_t> val_off;        // offset of system k's values in val_in
     int64_t n_flat = 0;                  // sum of the systems' rows
     bool have_vals = false, have_b = false;
     DBuf<double> d[4];
     DBuf<float> f[4];
+    DBuf<float> pb32;                    // MIXED direction buffers P_0 .. P_{M-1}, n_tot apart
     DBuf<double> red_partial;
     DBuf<SysScalars> sys;
     SysScalars* sys_host = nullptr;
-    std::map<int64_t, hipGraphExec_t> graphs;  // iteration chunks by (precision, chunk, launch size, compacted)
+    // iteration chunks by (precision, chunk, launch size, compacted, x period, first position)
+    std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> graphs;
     ~ocsl_batch_s() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         if (sys_host) (void)hipHostFree(sys_host);
@@ -651,15 +768,23 @@ void bt_set_launch(ocsl_batch bt, int rpl, int block) {
     OCSL_HIP(hipStreamSynchronize(bt->stream));
 }
 
-// Krylov vectors of the precision (zeroed: padding rows stay zero) and fresh
-// per-system scalars for a solve to tol within max_iter iterations.
-void bt_prepare(ocsl_batch bt, bool mixed, double tol, int max_iter, double replace_ratio) {
+// Iterations per FP64 update of x: the MIXED option's 1, 2, 4 or 8 (other
+// values: 8); FP64 updates x every iteration.
+int bt_period(bool mixed, const ocsl_options& o) {
+    if (!mixed) return 1;
+    return o.x_period == 1 || o.x_period == 2 || o.x_period == 4 ? o.x_period : kBtMaxPeriod;
+}
+
+// Krylov vectors of the precision (zeroed: padding rows stay zero; MIXED:
+// M direction buffers) and fresh per-system scalars for a solve to tol
+// within max_iter iterations.
+void bt_prepare(ocsl_batch bt, bool mixed, double tol, int max_iter, double replace_ratio, int M) {
     hipStream_t s = bt->stream;
     const int N = bt->n_tot, m = bt->m;
-    const bool remap = mixed ? (!bt->f[0].p || bt->f[0].n < (size_t)N) : (!bt->d[0].p || bt->d[0].n < (size_t)N);
-    if (remap) bt_drop_graphs(bt);
+    const void* before[4] = {bt->f[0].p, bt->f[2].p, bt->pb32.p, bt->d[0].p};
     for (int i = 0; i < 3; ++i) {
         if (mixed) {
+            if (i == 1) continue;  // the directions live in pb32
             bt->f[i].alloc(N);
             OCSL_HIP(hipMemsetAsync(bt->f[i].p, 0, sizeof(float) * N, s));
         } else {
@@ -667,6 +792,12 @@ void bt_prepare(ocsl_batch bt, bool mixed, double tol, int max_iter, double repl
             OCSL_HIP(hipMemsetAsync(bt->d[i].p, 0, sizeof(double) * N, s));
         }
     }
+    if (mixed) {
+        bt->pb32.alloc((size_t)M * N);
+        OCSL_HIP(hipMemsetAsync(bt->pb32.p, 0, sizeof(float) * (size_t)M * N, s));
+    }
+    const void* after[4] = {bt->f[0].p, bt->f[2].p, bt->pb32.p, bt->d[0].p};
+    if (!std::equal(before, before + 4, after)) bt_drop_graphs(bt);  // graphs hold the buffers' addresses
     std::vector<SysScalars> init(m);
     for (auto& S : init) {
         S = SysScalars{};
@@ -679,45 +810,66 @@ void bt_prepare(ocsl_batch bt, bool mixed, double tol, int max_iter, double repl
 }
 
 // The batch's kernel launches in one precision over the current layout.
+// The batch's kernel launches in one precision over the current layout,
+// with x updated once per M iterations (bt_prepare allocates the M direction
+// buffers first).
 struct BtLaunch {
     ocsl_batch bt;
     bool mixed;
     int UB;
+    int M;
     BtArgs<float, float> c32;
     BtArgs<double, double> c64;
-    BtLaunch(ocsl_batch b, bool mx)
-        : bt(b), mixed(mx), UB(b->upd_block(mx)),
-          c32{b->view32(), b->map(), b->f[0].p, b->f[1].p, b->f[2].p, b->x_glob.p, b->sys.p, b->red(), kWave * b->rpl,
-              b->view64(), b->b_glob.p},
+    BtLaunch(ocsl_batch b, bool mx, int period)
+        : bt(b), mixed(mx), UB(b->upd_block(mx)), M(period),
+          c32{b->view32(), b->map(), b->f[0].p, b->pb32.p, b->f[2].p, b->x_glob.p, b->sys.p, b->red(), kWave * b->rpl,
+              b->view64(), b->b_glob.p, b->pb32.p, b->n_tot, period, 0, b->pb32.p},
           c64{b->view64(), b->map(), b->d[0].p, b->d[1].p, b->d[2].p, b->x_glob.p, b->sys.p, b->red(),
-              kWave * b->rpl, b->view64(), b->b_glob.p} {}
+              kWave * b->rpl, b->view64(), b->b_glob.p, b->d[1].p, b->n_tot, 1, 0, b->d[1].p} {
+        OCSL_REQUIRE(mixed ? (M >= 1 && M <= kBtMaxPeriod && b->pb32.n >= (size_t)M * b->n_tot) : M == 1,
+                     OCSL_ERR_STATE, "batch direction buffers do not match the x period");
+    }
     void set_map(const BatchMap& mp) { c32.map = c64.map = mp; }
-    // r = b - A x (and p = r when first) for the systems of the map's blocks
-    void residual(int first, const BatchMap& mp, int grid) {
+    // r = b - A x (and p = r when first) for the systems of the map's blocks;
+    // a system that goes on iterating resumes at period position phi
+    // (phi < 0: none does)
+    void residual(int first, const BatchMap& mp, int grid, int phi) {
         hipStream_t s = bt->stream;
         with_rpl(bt->rpl, [&](auto RC) {
             constexpr int R = decltype(RC)::value;
             if (mixed)
                 k_bt_residual<float, R><<<grid, bt->block, 0, s>>>(bt->view64(), mp, bt->b_glob.p, bt->x_glob.p,
-                                                                   c32.r, c32.p, bt->sys.p, bt->red(), first);
+                                                                   c32.r, c32.pb, c32.pstride, phi, bt->sys.p,
+                                                                   bt->red(), first);
             else
                 k_bt_residual<double, R><<<grid, bt->block, 0, s>>>(bt->view64(), mp, bt->b_glob.p, bt->x_glob.p,
-                                                                    c64.r, c64.p, bt->sys.p, bt->red(), first);
+                                                                    c64.r, c64.pb, c64.pstride, phi, bt->sys.p,
+                                                                    bt->red(), first);
         });
-        k_bt_fin_residual<<<bt->m, 256, 0, s>>>(bt->map(), bt->sys.p, bt->red(), first, bt->m);
+        k_bt_fin_residual<<<bt->m, 256, 0, s>>>(bt->map(), bt->sys.p, bt->red(), first, bt->m, phi);
         OCSL_HIP(hipGetLastError());
     }
-    // one CG iteration of every unfinished system of the map's blocks
-    void iter(int grid) {
+    // one CG iteration, at period position j, of every unfinished system of
+    // the map's blocks
+    void iter(int grid, int j) {
         hipStream_t s = bt->stream;
+        auto at = [&](auto c) {
+            c.j = j;
+            c.p = c.pb + (int64_t)j * c.pstride;
+            c.p_alt = c.pb + (int64_t)((j + 1) % c.M) * c.pstride;
+            return c;
+        };
+        OCSL_REQUIRE(j >= 0 && j < M, OCSL_ERR_ARG, "period position out of range");
         with_rpl(bt->rpl, [&](auto RC) {
             constexpr int R = decltype(RC)::value;
             if (mixed) {
-                k_bt_spmv<float, float, R><<<grid, bt->block, 0, s>>>(c32);
-                k_bt_update<float, float><<<grid, UB, 0, s>>>(c32);
+                const BtArgs<float, float> c = at(c32);
+                k_bt_spmv<float, float, R><<<grid, bt->block, 0, s>>>(c);
+                k_bt_update<float, float><<<grid, UB, 0, s>>>(c);
             } else {
-                k_bt_spmv<double, double, R><<<grid, bt->block, 0, s>>>(c64);
-                k_bt_update<double, double><<<grid, UB, 0, s>>>(c64);
+                const BtArgs<double, double> c = at(c64);
+                k_bt_spmv<double, double, R><<<grid, bt->block, 0, s>>>(c);
+                k_bt_update<double, double><<<grid, UB, 0, s>>>(c);
             }
         });
     }
@@ -736,13 +888,14 @@ double bt_time_iterations(ocsl_batch bt, bool mixed, int iters) {
         OCSL_HIP(hipEventCreate(&e1));
         for (int rep = 0; rep < 2; ++rep) {
             OCSL_HIP(hipMemsetAsync(bt->x_glob.p, 0, sizeof(double) * bt->n_tot, s));
-            bt_prepare(bt, mixed, 0.0, 1 << 30, 0.0);
-            BtLaunch L(bt, mixed);
-            L.residual(1, bt->map(), bt->nblocks);
+            const int M = mixed ? kBtMaxPeriod : 1;  // the solve's default period
+            bt_prepare(bt, mixed, 0.0, 1 << 30, 0.0, M);
+            BtLaunch L(bt, mixed, M);
+            L.residual(1, bt->map(), bt->nblocks, 0);
             if (!ex) {
                 hipGraph_t gr;
                 OCSL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                for (int i = 0; i < iters; ++i) L.iter(bt->nblocks);
+                for (int i = 0; i < iters; ++i) L.iter(bt->nblocks, i % M);
                 OCSL_HIP(hipStreamEndCapture(s, &gr));
                 const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(gr);
@@ -942,19 +1095,24 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
         const bool mixed = precision == OCSL_MIXED;
         hipStream_t s = bt->stream;
         const int m = bt->m, N = bt->n_tot, G = bt->nblocks;
-        DBuf<double> flat;
+        DBuf<double>& flat = bt->x_flat;
+        const double* x_before = bt->x_glob.p;
         bt->x_glob.alloc(N);
+        if (x_before != bt->x_glob.p) bt_drop_graphs(bt);  // graphs hold x's address
         bt_flat_in(bt, x, where, flat, bt->x_glob.p);
-        bt_prepare(bt, mixed, o.tol, o.max_iter, o.replace_ratio);
+        const int M = bt_period(mixed, o);
+        bt_prepare(bt, mixed, o.tol, o.max_iter, o.replace_ratio, M);
         // launched blocks: all, or (compacted) the blocks of unfinished systems
         int G_cur = G;
         bool comp = false;
-        BtLaunch L(bt, mixed);
-        auto residual = [&](int first, bool all) {
+        BtLaunch L(bt, mixed, M);
+        // iterations launched so far: the next one sits at period position
+        // launched mod M
+        int64_t launched = 0;
+        auto residual = [&](int first, bool all, int phi) {
             const bool c = comp && !all;
-            L.residual(first, bt->map(c), c ? G_cur : G);
+            L.residual(first, bt->map(c), c ? G_cur : G, phi);
         };
-        auto iter = [&] { L.iter(G_cur); };
         auto read_sys = [&] {
             OCSL_HIP(hipMemcpyAsync(bt->sys_host, bt->sys.p, sizeof(SysScalars) * m, hipMemcpyDeviceToHost, s));
             OCSL_HIP(hipStreamSynchronize(s));
@@ -987,11 +1145,10 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
             L.set_map(bt->map(true));
         };
         OCSL_HIP(hipEventRecord(bt->ev0, s));
-        residual(1, true);
+        residual(1, true, 0);
         read_sys();
         stamp();
         const int chunk = o.check_every;
-        int64_t launched = 0;
         if (bt->graphs.size() > 64) bt_drop_graphs(bt);
         while (true) {
             bool all = true;
@@ -1002,13 +1159,17 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
             compact();
             // graphs hold the launch size and map; the active list is read
             // through its (fixed) device address when the graph runs
-            const int64_t key = (mixed ? 1 : 0) + 2 * (int64_t)chunk + ((int64_t)G_cur << 24) + (comp ? (1ll << 56) : 0);
+            const int j0 = (int)(launched % M);
+            const auto key = std::make_tuple(mixed ? 1 : 0, chunk, G_cur, comp ? 1 : 0, M, j0);
+            auto chunk_iters = [&] {
+                for (int i = 0; i < chunk; ++i) L.iter(G_cur, (int)((j0 + i) % M));
+            };
             if (o.use_graph) {
                 auto it = bt->graphs.find(key);
                 if (it == bt->graphs.end()) {
                     hipGraph_t gr;
                     OCSL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                    for (int i = 0; i < chunk; ++i) iter();
+                    chunk_iters();
                     OCSL_HIP(hipStreamEndCapture(s, &gr));
                     hipGraphExec_t ex;
                     OCSL_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
@@ -1017,7 +1178,7 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
                 }
                 OCSL_HIP(hipGraphLaunch(it->second, s));
             } else {
-                for (int i = 0; i < chunk; ++i) iter();
+                chunk_iters();
                 OCSL_HIP(hipGetLastError());
             }
             launched += chunk;
@@ -1026,7 +1187,7 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
                 bool any = false;
                 for (int k = 0; k < m; ++k) any = any || bt_wants_residual(bt->sys_host[k]);
                 if (any) {
-                    residual(0, false);
+                    residual(0, false, (int)(launched % M));
                     read_sys();
                 }
             }
@@ -1051,7 +1212,7 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
             bt->sys_host[k].breakdown = 0;
         }
         OCSL_HIP(hipMemcpyAsync(bt->sys.p, bt->sys_host, sizeof(SysScalars) * m, hipMemcpyHostToDevice, s));
-        residual(0, true);
+        residual(0, true, -1);
         read_sys();
         for (int k = 0; k < m; ++k) {
             const SysScalars& S = bt->sys_host[k];
@@ -1072,14 +1233,16 @@ ocsl_status ocsl_batch_solve(ocsl_batch bt, int precision, const ocsl_options* o
                 r.true_relres = tr;
                 r.solve_ms = total_ms;
                 // algorithmic bytes, as the single-system split CG counts
-                // them with the solution updated every iteration: SpMV
-                // (off-diagonal values and columns, diagonal, slice pointers,
-                // p, r read, q written), update (x read and written, p, r, q
-                // read, r, p written); a replacement reads the FP64 matrix,
-                // x, b and writes r
+                // them: SpMV (off-diagonal values and columns, diagonal,
+                // slice pointers, p, r read, q written), update (p, r, q
+                // read, r, p written, and x read and written once per period
+                // of M iterations with the period's M - 1 earlier
+                // directions); a replacement reads the FP64 matrix, x, b and
+                // writes r
                 const double n = (double)bt->n_sys[k], sv = mixed ? 4.0 : 8.0, cb = bt->compact ? 2.0 : 4.0;
                 const double off = (double)bt->csr[k].col.size() - n, sl = 4.0 * (bt->ns_sys[k] + 1);
-                const double it = off * (sv + cb) + sl + n * (9.0 * sv + 16.0);
+                const double xb = (16.0 + (M - 1) * sv) / M;
+                const double it = off * (sv + cb) + sl + n * (9.0 * sv + xb);
                 const double res = off * (8.0 + cb) + sl + n * (8.0 + 8.0 + 8.0 + sv);
                 r.bytes = (double)r.iterations * it + (double)r.replacements * res;
             }

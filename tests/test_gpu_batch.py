@@ -145,6 +145,55 @@ def test_batch_mixed_replacement_independent_of_check_every(check_every):
         assert abs(reps[k]["iterations"] - ref[k]["iterations"]) <= 3, (k, reps[k], ref[k])
 
 
+@pytest.mark.parametrize("check_every", [1, 5, 32])
+def test_batch_x_period_is_bitwise_neutral(check_every):
+    # x takes the period's steps at its end, before each FP64 residual
+    # replacement and when a system stops, one FP64 addition per step in
+    # iteration order: every solution, iteration count and replacement count
+    # equals that of an update every iteration (x_period 1), for chunk
+    # lengths that cut periods anywhere.  The systems include near-null-mode
+    # pressure systems (several replacements) and a zero right-hand side.
+    sysl = [gen.poisson3d(n, seed_mesh=100, seed_rhs=101)[0] for n in (3, 5, 7, 13)]
+    sysl += [a for a, _ in _systems([9, 17, 24], seed0=300)]
+    bs = [gen.uniform(70 + k, a.n) for k, a in enumerate(sysl)]
+    bs[5] = np.zeros_like(bs[5])
+    runs = {}
+    with Batch([(a.row_ptr, a.col) for a in sysl]) as bt:
+        bt.set_values([a.val for a in sysl])
+        bt.set_rhs(bs)
+        for m in (1, 2, 8):
+            runs[m] = bt.solve("mixed", tol=1e-8, check_every=check_every, max_iter=3000, x_period=m)
+    xs1, reps1, _ = runs[1]
+    assert sum(r["replacements"] for r in reps1) >= 4
+    for m in (2, 8):
+        xs, reps, _ = runs[m]
+        for k in range(len(sysl)):
+            assert reps[k]["iterations"] == reps1[k]["iterations"], (m, k, reps[k], reps1[k])
+            assert reps[k]["replacements"] == reps1[k]["replacements"], (m, k)
+            assert np.array_equal(xs[k], xs1[k]), (m, k, np.max(np.abs(xs[k] - xs1[k])))
+    for k, (a, b) in enumerate(zip(sysl, bs)):
+        assert reps1[k]["status"] == 0, (k, reps1[k])
+        nb = np.linalg.norm(b)
+        assert np.linalg.norm(b - a.matvec(xs1[k])) <= 1.05e-8 * nb, k
+
+
+def test_batch_x_period_at_the_iteration_cap():
+    # systems stopped by max_iter in the middle of a period: x holds every
+    # step taken, as with an update every iteration
+    sysl = [a for a, _ in _systems([10, 19, 27], seed0=400)]
+    bs = [gen.uniform(80 + k, a.n) for k, a in enumerate(sysl)]
+    out = {}
+    with Batch([(a.row_ptr, a.col) for a in sysl]) as bt:
+        bt.set_values([a.val for a in sysl])
+        bt.set_rhs(bs)
+        for m in (1, 8):
+            out[m] = bt.solve("mixed", tol=1e-12, check_every=4, max_iter=21, x_period=m)
+    for k in range(len(sysl)):
+        assert out[8][1][k]["iterations"] == out[1][1][k]["iterations"] == 21
+        assert np.array_equal(out[8][0][k], out[1][0][k]), k
+        assert np.linalg.norm(out[1][0][k]) > 0
+
+
 def test_batch_full_plan_64_systems():
     """The full uneven batch: 64 pressure systems of n^3 cells, n log-uniform
     in [22, 80] from seed 7 (about 7.5 M cells), MIXED CG to 1e-8 in one
