@@ -377,7 +377,7 @@ def batch(args, out):
         total = (time.perf_counter() - t0) * 1e3
     rel = [float(np.linalg.norm(b - a.matvec(x)) / np.linalg.norm(b)) for (a, b), x in zip(systems, xs)]
     seq = 0.0
-    for a, b in systems:
+    for a, b in ([] if args.skip_batch_sequential else systems):
         with Solver(a.row_ptr, a.col) as s:
             s.set_values(a.val)
             s.set_rhs(b)
@@ -394,7 +394,7 @@ def batch(args, out):
         "per_system_ms": [round(float(v), 3) for v in lat],
         "iterations": [r["iterations"] for r in reps],
         "max_fp64_relres": max(rel),
-        "sequential_sum_ms": round(seq, 3),
+        "sequential_sum_ms": None if args.skip_batch_sequential else round(seq, 3),
         # algorithmic bytes of every system's iterations and replacements
         # (the single-system split-CG count with x updated every iteration)
         # over the wall time of the iteration loop
@@ -745,6 +745,8 @@ def main():
                     help="apply the cached tuned launch when present instead of tuning again")
     ap.add_argument("--skip-headline", action="store_true",
                     help="run only the side sections (for profiling them); the JSON line then has no value")
+    ap.add_argument("--skip-batch-sequential", action="store_true",
+                    help="batch section: leave out the same systems solved one by one")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the measured kernels, --steps times each (for rocprofv3)")
     args = ap.parse_args()

@@ -252,6 +252,19 @@ __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
     const int gb = c.map.block();
     const int k = c.map.bsys[gb];
     SysScalars* S = c.sys + k;
+    // the wave's first slice opens with its loads (direction rows, slice
+    // start, diagonals) issued beside those of the system's flags, not after
+    // them; each later slice's at the end of the one before
+    const int lane = threadIdx.x & (kWave - 1);
+    const int s1 = c.map.bs1[gb], step = (int)(blockDim.x / kWave);
+    const int sf = c.map.bs0[gb] + (int)(threadIdx.x / kWave);
+    X pf[R] = {};
+    SellpHead<V, R> hf{};
+    if (sf < s1) {
+        const int row0 = sf * kWave * R + lane * R;
+        load_rows_tp<R>(c.p, row0, pf);
+        hf = sellp_head<V, R>(c.A, sf, row0);
+    }
     if (S->done || S->breakdown || S->nonfinite) {
         if (gb == c.map.bptr[k] && threadIdx.x == 0) {
             S->halt = 1;
@@ -260,7 +273,6 @@ __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
         }
         return;
     }
-    const int lane = threadIdx.x & (kWave - 1);
     if (S->replace_now) {
         double acc[1] = {0.0};
         for (int s = c.map.bs0[gb] + (int)(threadIdx.x / kWave); s < c.map.bs1[gb];
@@ -300,12 +312,14 @@ __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
         }
     }
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int s = c.map.bs0[gb] + (int)(threadIdx.x / kWave); s < c.map.bs1[gb]; s += (int)(blockDim.x / kWave)) {
+    for (int s = sf; s < s1; s += step) {
         const int row0 = s * kWave * R + lane * R;
         double pi[R], rv[R], qd[R];
-        load_rows_p<R>(c.p, row0, pi);
+#pragma unroll
+        for (int j = 0; j < R; ++j) pi[j] = (double)pf[j];
+        const X* px = c.p;
+        sellp_rows_gh<V, sizeof(V) == 4, R>(c.A, hf, lane, row0, [px](int cc) { return (double)px[cc]; }, pi, qd);
         load_rows_p<R>(c.r, row0, rv);
-        sellp_rows<V, X, sizeof(V) == 4, R>(c.A, s, lane, row0, c.p, pi, qd);
         X qv[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) {
@@ -315,6 +329,11 @@ __global__ void __launch_bounds__(1024) k_bt_spmv(BtArgs<V, X> c) {
             acc[2] += (double)qv[j] * (double)qv[j];
         }
         store_rows<R>(c.q, row0, c.A.n, qv);
+        if (s + step < s1) {
+            const int row1 = (s + step) * kWave * R + lane * R;
+            load_rows_tp<R>(c.p, row1, pf);
+            hf = sellp_head<V, R>(c.A, s + step, row1);
+        }
     }
     // p.q, then r.q and q.q in slots of their own: the r.r slot is being read
     // by the first block of each system during this kernel
@@ -366,6 +385,21 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
     const int j = c.j;
     // the block's rows are whole slices: 16-byte vectors v0..v1
     const int v0 = c.map.bs0[gb] * (c.slice_rows / N), v1 = c.map.bs1[gb] * (c.slice_rows / N);
+    const int b0 = c.map.bptr[k], b1 = c.map.bptr[k + 1];
+    // the SpMV's partial sums, then the first vector, in flight before the
+    // system's flags are tested (vector loads complete in issue order, so the
+    // re-add waits for the partials only)
+    constexpr int kSl[3] = {kSlotPQ, kSlotRQ, kSlotQQ};
+    SysSums<3> ps;
+    ps.issue(c.red, kSl, b0, b1);
+    const int va = v0 + (int)threadIdx.x;
+    double xa[N];
+    X pa[N], ra[N], qa[N];
+    if (va < v1) {
+        vload(c.p, va, pa);
+        vload(c.r, va, ra);
+        vload(c.q, va, qa);
+    }
     int lo = 0;  // first pending position of the period
     for (int i = j - 1; i >= 0; --i)
         if (S->added[i]) {
@@ -378,7 +412,6 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
         if (S->flush_now) bt_add_pending(c, S, v0, v1, lo, j);
         return;
     }
-    const int b0 = c.map.bptr[k], b1 = c.map.bptr[k + 1];
     if (S->replaced) {
         // no step at this position (x already holds every earlier step): the
         // direction moves on to P_{j+1}; the replaced residual's r.r partials
@@ -407,19 +440,7 @@ __global__ void __launch_bounds__(1024) k_bt_update(BtArgs<V, X> c) {
     }
     // x takes the steps at the period's end and before a replacement
     const bool add = j == c.M - 1 || S->need_update;
-    constexpr int kSl[3] = {kSlotPQ, kSlotRQ, kSlotQQ};
-    SysSums<3> ps;
-    ps.issue(c.red, kSl, b0, b1);
-    // the first vector in flight while the partial sums are re-added
-    const int va = v0 + (int)threadIdx.x;
-    double xa[N];
-    X pa[N], ra[N], qa[N];
-    if (va < v1) {
-        if (add) dload<N>(c.x, va, xa);
-        vload(c.p, va, pa);
-        vload(c.r, va, ra);
-        vload(c.q, va, qa);
-    }
+    if (add && va < v1) dload<N>(c.x, va, xa);
     double d3[3];
     ps.finish(c.red, kSl, b0, b1, d3);
     const double pq = d3[0], rq = d3[1], qq = d3[2];
