@@ -111,12 +111,25 @@ struct MeshDev {
     const int *cperm, *rstart, *rlen;
 };
 
+struct V3 {
+    double v[3];
+};
+
+// The 3-vectors of cells (u, u_old, gp, gpc, ap_vec) and of interior faces
+// (f_stress) are stored as three planes, component k of item i at
+// k * stride + i (cs cells, fs faces): a momentum component's kernels read
+// one plane, not every third element of interleaved triples.  Boundary-face
+// vectors (b_u) stay interleaved.
 struct StateDev {
     double *u, *u_old, *rho, *rho_old, *p, *p_old, *pc, *gp, *T, *h, *h_old, *ap, *ap_vec, *dt;
     double *f_mdot, *f_rho, *f_qy, *f_stress;
     double *b_T, *b_rho, *b_mdot, *b_h, *b_qy, *b_u;
     int* b_super;
     double *gpc, *mach, *b_p, *b_p_out;
+    int cs, fs;  // plane strides: cells, interior faces
+    __device__ __forceinline__ V3 vec3(const double* a, int i) const {
+        return {{a[i], a[cs + i], a[2 * cs + i]}};
+    }
 };
 
 enum { kMomentum = 0, kPressure = 1, kEnthalpy = 2 };
@@ -188,7 +201,7 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
     const int dp = m.dpos[i];
     // this cell's own values, the same for each of its faces
     double own_q = 0.0, own_T = 0.0;
-    if constexpr (kEq == kMomentum) own_q = s.u[3 * i + d];
+    if constexpr (kEq == kMomentum) own_q = s.u[d * s.cs + i];
     else if constexpr (kEq == kPressure) own_q = s.pc[i];
     else own_q = s.h[i];
     if constexpr (kEq != kMomentum || kVisc) own_T = s.T[i];
@@ -202,7 +215,7 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
         const double geo = m.cf_geo[e];
         const double mdot = s.f_mdot[f];
         if constexpr (kEq == kMomentum) {
-            const double qj = s.u[3 * j + d];
+            const double qj = s.u[d * s.cs + j];
             conv_row<kBlend>(a, rhs, side, dp, op, mdot, side ? qj : own_q, side ? own_q : qj, geo);
             if constexpr (kVisc) {
                 const double Tj = s.T[j];
@@ -210,7 +223,7 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
                 double D = gas.viscosity(Tf);
                 D = D * m.cf_ksi[e];
                 diff_row(a, dp, op, D);
-                const double st = s.f_stress[3 * f + d];
+                const double st = s.f_stress[d * s.fs + f];
                 rhs += side == 0 ? st : -st;
             }
         } else if constexpr (kEq == kPressure) {
@@ -251,7 +264,7 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
                         double tang = 0.0;
                         for (int k = d + 1; k < d + 3; ++k) {
                             const int kk = k % 3;
-                            tang += s.u[3 * i + kk] * nrm[kk] * nrm[d];
+                            tang += s.u[kk * s.cs + i] * nrm[kk] * nrm[d];
                         }
                         tang = D * tang;
                         rhs += tang;
@@ -311,18 +324,18 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
             term = V / s.dt[i];
             term2 = term * s.rho[i];
             diag += term2;
-            term2 = term * (s.u_old[3 * i + d] * s.rho_old[i]);
+            term2 = term * (s.u_old[d * s.cs + i] * s.rho_old[i]);
         }
-        term2 = term2 - s.gp[3 * i + d] * V;
+        term2 = term2 - s.gp[d * s.cs + i] * V;
         rhs += term2;
         if constexpr (!kTransient) {
             const double dd = diag;
             diag = dd / ph.urf;
-            rhs += (1.0 - ph.urf) / ph.urf * dd * s.u_old[3 * i + d];
+            rhs += (1.0 - ph.urf) / ph.urf * dd * s.u_old[d * s.cs + i];
         }
         s.ap[i] += diag / 3.0;
-        s.ap_vec[3 * i + d] = diag;
-        x0[oi] = s.u[3 * i + d];
+        s.ap_vec[d * s.cs + i] = diag;
+        x0[oi] = s.u[d * s.cs + i];
     } else if constexpr (kEq == kPressure) {
         if constexpr (kTransient) {
             const double term = V / s.dt[i];
@@ -409,7 +422,7 @@ __global__ void __launch_bounds__(kBlock) k_visc_stress(MeshDev m, StateDev s, G
     const double dist = m.fdist[f];
     double du[3], e[3];
     for (int k = 0; k < 3; ++k) {
-        du[k] = s.u[3 * R + k] - s.u[3 * L + k];
+        du[k] = s.u[k * s.cs + R] - s.u[k * s.cs + L];
         e[k] = m.fe[3 * f + k];
     }
     double M[3][3];
@@ -424,7 +437,7 @@ __global__ void __launch_bounds__(kBlock) k_visc_stress(MeshDev m, StateDev s, G
     for (int r = 0; r < 3; ++r) {
         double acc = 0.0;
         for (int c = 0; c < 3; ++c) acc += (mu * M[r][c]) * m.fS[3 * f + c];
-        s.f_stress[3 * f + r] = acc;
+        s.f_stress[r * s.fs + f] = acc;
     }
 }
 
@@ -450,15 +463,15 @@ __global__ void __launch_bounds__(kBlock) k_set_bcs(MeshDev m, StateDev s, FlowP
     }
     if (t == OCSL_BC_PRESSURE_OUTLET) {
         s.b_T[b] = s.T[L];
-        for (int k = 0; k < 3; ++k) s.b_u[3 * b + k] = s.u[3 * L + k];
+        for (int k = 0; k < 3; ++k) s.b_u[3 * b + k] = s.u[k * s.cs + L];
         s.b_rho[b] = g.density(s.b_p[b], s.T[L]);
-        s.b_h[b] = g.enthalpy(s.T[L], s.u + 3 * L);
+        s.b_h[b] = g.enthalpy(s.T[L], s.vec3(s.u, L).v);
         // both branches mark the face supersonic (cflow_solver.cpp:208-217)
         s.b_super[b] = 1;
         if ((s.b_mdot[b] / s.rho[L] / m.barea[b]) >= g.sound(s.T[L])) s.b_p[b] = s.p[L];
         else s.b_p[b] = s.b_p_out[b];
     }
-    if (t == OCSL_BC_WALL) s.b_p[b] = s.p[L] + Gas::dot3(s.gp + 3 * L, m.be + 3 * b) * m.bdist[b];
+    if (t == OCSL_BC_WALL) s.b_p[b] = s.p[L] + Gas::dot3(s.vec3(s.gp, L).v, m.be + 3 * b) * m.bdist[b];
 }
 
 // calc_press_grad (cflow_solver_supply.cpp:132-175) gathered per cell: the
@@ -494,7 +507,7 @@ __global__ void __launch_bounds__(kBlock) k_press_grad(MeshDev m, StateDev s, Fl
             continue;
         for (int k = 0; k < 3; ++k) gv[k] = gv[k] + pv * m.bS[3 * b + k];
     }
-    for (int k = 0; k < 3; ++k) s.gp[3 * i + k] = (1.0 / m.vol[i]) * gv[k];
+    for (int k = 0; k < 3; ++k) s.gp[k * s.cs + i] = (1.0 / m.vol[i]) * gv[k];
 }
 
 // calc_press_corr_grad (cflow_solver_supply.cpp:177-214), gathered per cell.
@@ -521,7 +534,7 @@ __global__ void __launch_bounds__(kBlock) k_press_corr_grad(MeshDev m, StateDev 
             (s.b_super[b] && t == OCSL_BC_PRESSURE_OUTLET))
             for (int k = 0; k < 3; ++k) gv[k] = gv[k] + s.pc[i] * m.bS[3 * b + k];
     }
-    for (int k = 0; k < 3; ++k) s.gpc[3 * i + k] = (1.0 / m.vol[i]) * gv[k];
+    for (int k = 0; k < 3; ++k) s.gpc[k * s.cs + i] = (1.0 / m.vol[i]) * gv[k];
 }
 
 // calc_face_mdot, interior faces (cflow_solver_supply.cpp:25-46).
@@ -534,10 +547,10 @@ __global__ void __launch_bounds__(kBlock) k_face_mdot(MeshDev m, StateDev s, int
     s.f_qy[f] = qy;
     double pgf[3], vf[3], dvec[3];
     for (int k = 0; k < 3; ++k) {
-        pgf[k] = geo * s.gp[3 * L + k];
-        pgf[k] += (1.0 - geo) * s.gp[3 * R + k];
-        vf[k] = geo * s.u[3 * L + k];
-        vf[k] += (1.0 - geo) * s.u[3 * R + k];
+        pgf[k] = geo * s.gp[k * s.cs + L];
+        pgf[k] += (1.0 - geo) * s.gp[k * s.cs + R];
+        vf[k] = geo * s.u[k * s.cs + L];
+        vf[k] += (1.0 - geo) * s.u[k * s.cs + R];
         dvec[k] = m.fdist[f] * m.fe[3 * f + k];
     }
     const double YF = qy * (s.p[R] - s.p[L] - Gas::dot3(pgf, dvec));
@@ -560,15 +573,15 @@ __global__ void __launch_bounds__(kBlock) k_bc_mdot(MeshDev m, StateDev s, FlowP
         s.b_qy[b] = (m.vol[L]) / (s.ap[L]) * Gas::dot3(m.balpha + 3 * b, m.bS + 3 * b);
     if (t == OCSL_BC_PRESSURE_OUTLET) {
         if (s.b_super[b]) {
-            s.b_mdot[b] = g.density(s.p[L], s.T[L]) * Gas::dot3(s.u + 3 * L, m.bS + 3 * b);
+            s.b_mdot[b] = g.density(s.p[L], s.T[L]) * Gas::dot3(s.vec3(s.u, L).v, m.bS + 3 * b);
             s.b_qy[b] = 1.0;
         } else {
             const double qy = (m.vol[L]) / (s.ap[L]) * Gas::dot3(m.balpha + 3 * b, m.bS + 3 * b);
             s.b_qy[b] = qy;
             double dvec[3];
             for (int k = 0; k < 3; ++k) dvec[k] = m.bdist[b] * m.be[3 * b + k];
-            const double YF = qy * (s.b_p[b] - s.p[L] - 0.5 * Gas::dot3(s.gp + 3 * L, dvec));
-            double mdot = Gas::dot3(s.u + 3 * L, m.bS + 3 * b) - YF;
+            const double YF = qy * (s.b_p[b] - s.p[L] - 0.5 * Gas::dot3(s.vec3(s.gp, L).v, dvec));
+            double mdot = Gas::dot3(s.vec3(s.u, L).v, m.bS + 3 * b) - YF;
             mdot *= s.b_rho[b];
             s.b_mdot[b] = mdot;
         }
@@ -601,7 +614,7 @@ __global__ void __launch_bounds__(kBlock) k_correct_mdot(MeshDev m, StateDev s, 
 __global__ void __launch_bounds__(kBlock) k_take_u(StateDev s, const double* __restrict__ x, const int* cperm, int d,
                                                    int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) s.u[3 * i + d] = x[cperm ? cperm[i] : i];
+    if (i < n) s.u[d * s.cs + i] = x[cperm ? cperm[i] : i];
 }
 
 // p' = 0.5 x, p += p' (cflow_solver.cpp:350-357).
@@ -631,9 +644,9 @@ __global__ void __launch_bounds__(kBlock) k_rho_u(MeshDev m, StateDev s, FlowPhy
     s.rho[i] = ph.gas.density(s.p[i], s.T[i]);
     const double a = kRelaxU * (m.vol[i]);
     for (int k = 0; k < 3; ++k) {
-        double den = s.ap_vec[3 * i + k];
+        double den = s.ap_vec[k * s.cs + i];
         if (den == 0.0) den = 2.2250738585072014e-308;  // DBL_MIN
-        s.u[3 * i + k] -= a * (s.gpc[3 * i + k] / den);
+        s.u[k * s.cs + i] -= a * (s.gpc[k * s.cs + i] / den);
     }
 }
 
@@ -655,9 +668,10 @@ __global__ void __launch_bounds__(kBlock) k_take_h(StateDev s, Gas g, const doub
     if (i >= n) return;
     const double h = x[cperm ? cperm[i] : i];
     s.h[i] = h;
-    const double T = g.temperature(h, s.u + 3 * i);
+    const V3 ui = s.vec3(s.u, i);
+    const double T = g.temperature(h, ui.v);
     s.T[i] = T;
-    s.mach[i] = Gas::mag3(s.u + 3 * i) / g.sound(T);
+    s.mach[i] = Gas::mag3(ui.v) / g.sound(T);
 }
 
 }  // namespace ocsl
@@ -729,7 +743,7 @@ struct ocsl_flow_s {
         return {d_u.p, d_u_old.p, d_rho.p, d_rho_old.p, d_p.p, d_p_old.p, d_pc.p, d_gp.p, d_T.p, d_h.p,
                 d_h_old.p, d_ap.p, d_ap_vec.p, d_dt.p, d_f_mdot.p, d_f_rho.p, d_f_qy.p, d_f_stress.p,
                 d_b_T.p, d_b_rho.p, d_b_mdot.p, d_b_h.p, d_b_qy.p, d_b_u.p, d_b_super.p,
-                d_gpc.p, d_mach.p, d_b_p.p, d_b_p_out.p};
+                d_gpc.p, d_mach.p, d_b_p.p, d_b_p_out.p, mesh.n, mesh.F};
     }
     FlowPhys phys() const {
         FlowPhys ph{};
@@ -769,15 +783,17 @@ void down(std::vector<T>& h, const DBuf<T>& d, hipStream_t s) {
 }
 
 // Per-face or per-cell data between the caller's order and the handle's:
-// device item k is the caller's item perm[k]; c components per item.
+// device item k is the caller's item perm[k] (perm null: item k); c
+// components per item, interleaved on the caller's side, in planes of F
+// items on the device.
 template <bool kToHandle>
 __global__ void k_perm_items(int F, int c, const int* __restrict__ perm, const double* __restrict__ in,
                              double* __restrict__ out) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < F; k += gridDim.x * blockDim.x) {
-        const size_t f = (size_t)perm[k];
+        const size_t f = perm ? (size_t)perm[k] : (size_t)k;
         for (int q = 0; q < c; ++q) {
-            if (kToHandle) out[(size_t)k * c + q] = in[f * c + q];
-            else out[f * c + q] = in[(size_t)k * c + q];
+            if (kToHandle) out[(size_t)q * F + k] = in[f * c + q];
+            else out[f * c + q] = in[(size_t)q * F + k];
         }
     }
 }
@@ -816,29 +832,32 @@ void down_f(ocsl_flow_s* fl, std::vector<double>& h, const DBuf<double>& d, hipS
 }
 
 // Per-cell state (1 or 3 components per cell) between the host mirror, in the
-// caller's order, and the device, in the handle's.
+// caller's order with interleaved vectors, and the device, in the handle's
+// order with vectors in planes.
 void up_c(ocsl_flow_s* fl, DBuf<double>& d, const std::vector<double>& h, hipStream_t s) {
-    if (!fl->cell_perm) {
+    const int n = fl->mesh.n;
+    if (!fl->cell_perm && h.size() <= (size_t)n) {
         up(d, h, s);
         return;
     }
-    const int n = fl->mesh.n;
     d.alloc(h.size());
     if (h.empty()) return;
     fl->d_cstage.upload(h.data(), h.size(), s);
-    k_perm_items<true><<<grid_for(n), kBlock, 0, s>>>(n, (int)(h.size() / n), fl->d_cperm.p, fl->d_cstage.p, d.p);
+    k_perm_items<true><<<grid_for(n), kBlock, 0, s>>>(n, (int)(h.size() / n), fl->cell_perm ? fl->d_cperm.p : nullptr,
+                                                       fl->d_cstage.p, d.p);
     OCSL_HIP(hipGetLastError());
 }
 
 void down_c(ocsl_flow_s* fl, std::vector<double>& h, const DBuf<double>& d, hipStream_t s) {
-    if (!fl->cell_perm) {
+    const int n = fl->mesh.n;
+    if (!fl->cell_perm && h.size() <= (size_t)n) {
         down(h, d, s);
         return;
     }
-    const int n = fl->mesh.n;
     if (h.empty()) return;
     fl->d_cstage.alloc(h.size());
-    k_perm_items<false><<<grid_for(n), kBlock, 0, s>>>(n, (int)(h.size() / n), fl->d_cperm.p, d.p, fl->d_cstage.p);
+    k_perm_items<false><<<grid_for(n), kBlock, 0, s>>>(n, (int)(h.size() / n), fl->cell_perm ? fl->d_cperm.p : nullptr,
+                                                        d.p, fl->d_cstage.p);
     OCSL_HIP(hipGetLastError());
     OCSL_HIP(hipMemcpyAsync(h.data(), fl->d_cstage.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
 }
