@@ -96,15 +96,13 @@ struct MeshDev {
     // Per cell, its interior faces in ascending caller face order, sliced 64
     // cells (one wave) to a slice: entry k of cell i is at cf_sp[i / 64] +
     // 64 k + i % 64, so each step of the face loop is one contiguous load per
-    // wave; code = f << 1 | side, or -1 after the cell's last face; pos = the
-    // neighbour's slot position in the row.
-    const int *cf_sp, *cf_code, *cf_pos;
+    // wave; code = f << 5 | pos << 1 | side (pos: the neighbour's slot
+    // position in the row), or -1 after the cell's last face.
+    const int *cf_sp, *cf_code;
     const int *cb_ptr, *cb_face;           // per cell: boundary faces
     const int *row_ptr, *dpos;             // caller layout (int32), diagonal position in the row
-    // per cell-face entry, beside cf_code: the cell across the face and the
-    // face's static weights
+    // per cell-face entry, beside cf_code: the cell across the face
     const int* cf_nb;
-    const double *cf_geo, *cf_ksi;
     // Cells stored in an order of their own (cperm != nullptr): device cell
     // k is the caller's cell cperm[k], whose row starts at rstart[k] in the
     // caller's value array and holds rlen[k] slots.
@@ -210,9 +208,9 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
     for (int e = m.cf_sp[i >> 6] + (i & 63), e1 = m.cf_sp[(i >> 6) + 1]; e < e1; e += 64) {
         const int code = m.cf_code[e];
         if (code < 0) break;
-        const int f = code >> 1, side = code & 1, op = m.cf_pos[e];
+        const int f = code >> 5, side = code & 1, op = (code >> 1) & (kMaxRow - 1);
         const int j = m.cf_nb[e];
-        const double geo = m.cf_geo[e];
+        const double geo = m.fgeo[f];
         const double mdot = s.f_mdot[f];
         if constexpr (kEq == kMomentum) {
             const double qj = s.u[d * s.cs + j];
@@ -221,7 +219,7 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
                 const double Tj = s.T[j];
                 const double Tf = face_avg(geo, side ? Tj : own_T, side ? own_T : Tj);
                 double D = gas.viscosity(Tf);
-                D = D * m.cf_ksi[e];
+                D = D * m.fksi[f];
                 diff_row(a, dp, op, D);
                 const double st = s.f_stress[d * s.fs + f];
                 rhs += side == 0 ? st : -st;
@@ -241,7 +239,7 @@ __device__ __forceinline__ void assemble_cell(const MeshDev& m, const StateDev& 
             conv_row<kBlend>(a, rhs, side, dp, op, mdot, side ? qj : own_q, side ? own_q : qj, geo);
             const double Tf = face_avg(geo, side ? Tj : own_T, side ? own_T : Tj);
             double diff_w = gas.viscosity(Tf);
-            diff_w = diff_w / gas.pr * m.cf_ksi[e];
+            diff_w = diff_w / gas.pr * m.fksi[f];
             diff_row(a, dp, op, diff_w);
         }
     }
@@ -485,7 +483,7 @@ __global__ void __launch_bounds__(kBlock) k_press_grad(MeshDev m, StateDev s, Fl
     for (int e = m.cf_sp[i >> 6] + (i & 63), e1 = m.cf_sp[(i >> 6) + 1]; e < e1; e += 64) {
         const int code = m.cf_code[e];
         if (code < 0) break;
-        const int f = code >> 1, j = m.cf_nb[e];
+        const int f = code >> 5, j = m.cf_nb[e];
         const double ap_j = s.ap[j], p_j = s.p[j];
         const double apL = code & 1 ? ap_j : ap_i, apR = code & 1 ? ap_i : ap_j;
         const double pL = code & 1 ? p_j : p_i, pR = code & 1 ? p_i : p_j;
@@ -519,8 +517,8 @@ __global__ void __launch_bounds__(kBlock) k_press_corr_grad(MeshDev m, StateDev 
     for (int e = m.cf_sp[i >> 6] + (i & 63), e1 = m.cf_sp[(i >> 6) + 1]; e < e1; e += 64) {
         const int code = m.cf_code[e];
         if (code < 0) break;
-        const int f = code >> 1;
-        const double geo = m.cf_geo[e], pc_j = s.pc[m.cf_nb[e]];
+        const int f = code >> 5;
+        const double geo = m.fgeo[f], pc_j = s.pc[m.cf_nb[e]];
         double fp = geo * (code & 1 ? pc_j : pc_i);
         fp += (1.0 - geo) * (code & 1 ? pc_i : pc_j);
         for (int k = 0; k < 3; ++k) {
@@ -717,9 +715,8 @@ struct ocsl_flow_s {
     // device mesh
     DBuf<double> d_vol, d_fS, d_fe, d_fdist, d_fksi, d_fgeo, d_falpha, d_bS, d_barea, d_bksi, d_bperp, d_be, d_balpha,
         d_bdist;
-    DBuf<int> d_fL, d_fR, d_bL, d_bmark, d_cf_sp, d_cf_code, d_cf_pos, d_cb_ptr, d_cb_face, d_row_ptr, d_dpos;
+    DBuf<int> d_fL, d_fR, d_bL, d_bmark, d_cf_sp, d_cf_code, d_cb_ptr, d_cb_face, d_row_ptr, d_dpos;
     DBuf<int> d_cf_nb;
-    DBuf<double> d_cf_geo, d_cf_ksi;
     // device state
     DBuf<double> d_u, d_u_old, d_rho, d_rho_old, d_p, d_p_old, d_pc, d_gp, d_T, d_h, d_h_old, d_ap, d_ap_vec, d_dt;
     DBuf<double> d_f_mdot, d_f_rho, d_f_qy, d_f_stress;
@@ -736,8 +733,8 @@ struct ocsl_flow_s {
         return {mesh.n,     mesh.F,       mesh.B,      d_vol.p,    d_fL.p,      d_fR.p,       d_fS.p,
                 d_fe.p,     d_fdist.p,    d_fksi.p,    d_fgeo.p,   d_falpha.p,  d_bL.p,       d_bmark.p,
                 d_bS.p,     d_barea.p,    d_bksi.p,    d_bperp.p,  d_be.p,      d_balpha.p,   d_bdist.p,
-                d_cf_sp.p,  d_cf_code.p,  d_cf_pos.p,  d_cb_ptr.p, d_cb_face.p, d_row_ptr.p,  d_dpos.p,
-                d_cf_nb.p,  d_cf_geo.p,   d_cf_ksi.p,  cell_perm ? d_cperm.p : nullptr, d_rstart.p, d_rlen.p};
+                d_cf_sp.p,  d_cf_code.p,  d_cb_ptr.p, d_cb_face.p, d_row_ptr.p,  d_dpos.p,
+                d_cf_nb.p,  cell_perm ? d_cperm.p : nullptr, d_rstart.p, d_rlen.p};
     }
     StateDev state_view() const {
         return {d_u.p, d_u_old.p, d_rho.p, d_rho_old.p, d_p.p, d_p_old.p, d_pc.p, d_gp.p, d_T.p, d_h.p,
@@ -1630,20 +1627,19 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
             cf[m.fR[f]].push_back(((int)f << 1) | 1);
         }
         for (size_t b = 0; b < B; ++b) cb[m.bL[b]].push_back((int)b);
-        std::vector<int> cf_ptr(n + 1, 0), cf_code, cf_pos, cf_nb, cb_ptr(n + 1, 0), cb_face;
-        std::vector<double> cf_geo, cf_ksi;
+        // code = f << 5 | pos << 1 | side: faces below 2^26, positions below kMaxRow = 16
+        static_assert(kMaxRow == 16, "the cell-face code holds the slot position in 4 bits");
+        OCSL_REQUIRE(F < ((size_t)1 << 26), OCSL_ERR_ARG, "more than 2^26 interior faces");
+        std::vector<int> cf_ptr(n + 1, 0), cf_code, cf_nb, cb_ptr(n + 1, 0), cb_face;
         for (size_t k = 0; k < n; ++k) {
             const int c = cperm[k];
             for (int code : cf[c]) {
                 const int f = code >> 1;
                 const int other = (code & 1) ? m.fL[f] : m.fR[f];
                 const int pos = find(c, other);
-                OCSL_REQUIRE(pos >= 0, OCSL_ERR_STRUCTURE, "face neighbour missing from the matrix row");
-                cf_code.push_back((finv[f] << 1) | (code & 1));
-                cf_pos.push_back(pos);
+                OCSL_REQUIRE(pos >= 0 && pos < kMaxRow, OCSL_ERR_STRUCTURE, "face neighbour missing from the matrix row");
+                cf_code.push_back((finv[f] << 5) | (pos << 1) | (code & 1));
                 cf_nb.push_back(ciperm[other]);
-                cf_geo.push_back(m.fgeo[f]);
-                cf_ksi.push_back(m.fksi[f]);
             }
             cf_ptr[k + 1] = (int)cf_code.size();
             for (int b : cb[c]) cb_face.push_back(b);
@@ -1660,22 +1656,15 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
         }
         {
             const size_t tot = (size_t)cf_sp[nsl];
-            std::vector<int> sc(tot, -1), sp(tot, 0), sn(tot, 0);
-            std::vector<double> sg(tot, 0.0), sk(tot, 0.0);
+            std::vector<int> sc(tot, -1), sn(tot, 0);
             for (size_t i = 0; i < n; ++i)
                 for (int k = 0; k < cf_ptr[i + 1] - cf_ptr[i]; ++k) {
                     const size_t e = (size_t)cf_sp[i / 64] + 64 * (size_t)k + i % 64, q = (size_t)cf_ptr[i] + k;
                     sc[e] = cf_code[q];
-                    sp[e] = cf_pos[q];
                     sn[e] = cf_nb[q];
-                    sg[e] = cf_geo[q];
-                    sk[e] = cf_ksi[q];
                 }
             cf_code.swap(sc);
-            cf_pos.swap(sp);
             cf_nb.swap(sn);
-            cf_geo.swap(sg);
-            cf_ksi.swap(sk);
         }
         hipStream_t s = fl->solver->stream;
         // cell-indexed mesh data and cell indices in the device order
@@ -1716,10 +1705,7 @@ ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64
         up(fl->d_bperp, m.bperp, s);
         up(fl->d_cf_sp, cf_sp, s);
         up(fl->d_cf_code, cf_code, s);
-        up(fl->d_cf_pos, cf_pos, s);
         up(fl->d_cf_nb, cf_nb, s);
-        up(fl->d_cf_geo, cf_geo, s);
-        up(fl->d_cf_ksi, cf_ksi, s);
         up(fl->d_cb_ptr, cb_ptr, s);
         up(fl->d_cb_face, cb_face, s);
         up(fl->d_row_ptr, rp32, s);
