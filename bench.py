@@ -687,6 +687,125 @@ def tri_solves(args, out):
                                      "system (HIP events, 20 solves each)", **res}
 
 
+# SIMPLE run on an SU2 box, the same settings for this package and for the
+# reference's own run of the file: viscous, blended, transient dt 1e-4, CG
+# for the pressure correction and ILU(0)-BiCGSTAB for the rest, one boundary
+# condition per marker of mesh.box_su2 (x-: velocity inlet, x+: pressure
+# outlet, y-: no-slip wall, y+: slip wall, z-/z+: symmetry)
+FLOW_SU2_STEPS = 3
+FLOW_SU2_SETTINGS = dict(steps=FLOW_SU2_STEPS, inner=1, dt=1e-4, steady=0, implicit_underrelax=0.4,
+                         implicit_underrelax_max=0.8, ramp_iterations=100, solver_type=1, solver_mode=1, conv_type=1,
+                         visc_type=0, init_shock_tube=0, p0=1e5, T0=300.0, u0=(2.0, 0.0, 0.0), p2=0.0, T2=0.0)
+FLOW_SU2_BCS = dict(bc_type=[3, 5, 2, 2, 1, 1], bc_no_slip=[0, 0, 1, 0, 0, 0], bc_const_T=[0] * 6,
+                    bc_T=[300.0, 0.0, 0.0, 0.0, 0.0, 0.0], bc_p=[0.0, 1e5, 0.0, 0.0, 0.0, 0.0],
+                    bc_u=[[3.0, 0.5, -0.2]] + [[0.0, 0.0, 0.0]] * 5)
+
+
+def _flow_su2_text(n):
+    from opencslsolver_amd import mesh
+
+    return mesh.box_su2(n, "hex", h=1.0 / n, seed=31)
+
+
+def _ref_flow_child(n, threads, out_path):
+    """Child-process leg of the flow-step CPU baseline: the reference's own
+    SIMPLE run of the SU2 box (built from its sources for the build host's
+    instruction set), timed by its own per-iteration clock; writes the final
+    fields to out_path and prints JSON."""
+    import tempfile
+
+    import oracle
+
+    text = _flow_su2_text(n)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "box.su2")
+        with open(path, "w") as f:
+            f.write(text)
+        os.chdir(tmp)  # the reference writes its VTK files into the working directory
+        sys.stdout.flush()
+        saved = os.dup(1)
+        os.dup2(2, 1)  # and its progress lines to stdout
+        try:
+            r = oracle.ref_flow_time_file(path, FLOW_SU2_SETTINGS, FLOW_SU2_BCS, threads, n ** 3)
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+    np.savez(out_path, **{k: r[k] for k in ("p", "u", "T", "rho", "h")})
+    print(json.dumps({k: r[k] for k in ("iter_us", "total_s", "solves")}))
+
+
+def flow_su2_vs_reference(args, n=32):
+    """The SIMPLE step on an n^3 hexahedral SU2 box in random numbering, read
+    by this package's reader (matrix structure from the device builder), run
+    for FLOW_SU2_STEPS iterations from the initial state; then timed over
+    further iterations.  Beside it, the reference's own run of the same file
+    on the host cores, timed by its own per-iteration clock, and the largest
+    relative difference of the final fields between the two runs."""
+    import subprocess
+    import tempfile
+
+    import torch
+
+    import oracle
+    from opencslsolver_amd import flow, mesh
+    from opencslsolver_amd.solver import build_structure
+
+    _log(f"bench: SIMPLE iterations on a {n}^3 SU2 box, beside the reference")
+    m = mesh.parse_su2(_flow_su2_text(n))
+    N = m["cell_volume"].shape[0]
+    rp, col = build_structure(N, m["face_left"], m["face_right"], m["bc_left"])
+    b = FLOW_SU2_BCS
+    bcs = [dict(type=b["bc_type"][k], no_slip=b["bc_no_slip"][k], const_temperature=b["bc_const_T"][k],
+                temperature=b["bc_T"][k], pressure=b["bc_p"][k], velocity=tuple(b["bc_u"][k]))
+           for k in range(len(b["bc_type"]))]
+    st = FLOW_SU2_SETTINGS
+    prm = flow.default_params(viscous=1, blended=1, transient=1, dt=st["dt"],
+                              implicit_underrelax=st["implicit_underrelax"],
+                              implicit_underrelax_max=st["implicit_underrelax_max"],
+                              ramp_iterations=st["ramp_iterations"], p0=st["p0"], t0=st["T0"], u0=st["u0"], bcs=bcs,
+                              method_momentum=1, method_pressure=0, method_enthalpy=1)
+    with flow.Flow(m, rp, col, prm) as fl:
+        fl.initialize()
+        fl.run(FLOW_SU2_STEPS, 1)
+        gpu = fl.get_state()
+        iters = max(3, args.steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fl.run(iters, 1)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / iters
+    res = {"workload": f"SIMPLE iteration, {n}^3 hexahedral SU2 box in random numbering (seed 31, N={N}), inlet/"
+                       "outlet/no-slip and slip walls/symmetry, viscous, blended, transient dt 1e-4, REFERENCE-mode "
+                       "solves (BiCGSTAB momentum and enthalpy, CG pressure)",
+           "gpu_ms_per_iteration": round(ms, 3)}
+    if args.no_cpu_baseline or not oracle.ref_available():
+        return res
+    threads = host_threads()
+    with tempfile.TemporaryDirectory() as tmp:
+        out_path = os.path.join(tmp, "ref_fields.npz")
+        code = ("import sys; sys.path.insert(0, sys.argv[1]); import bench; "
+                "bench._ref_flow_child(int(sys.argv[2]), int(sys.argv[3]), sys.argv[4])")
+        r = subprocess.run([sys.executable, "-c", code, ROOT, str(n), str(threads), out_path],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            res["cpu_baseline"] = {"error": r.stderr.strip().splitlines()[-1:]}
+            return res
+        d = json.loads(r.stdout.strip().splitlines()[-1])
+        ref = np.load(out_path)
+        diff = {f: float(np.max(np.abs(gpu[f] - ref[f])) / max(np.max(np.abs(ref[f])), 1e-300))
+                for f in ("p", "u", "T", "rho", "h")}
+    ref_ms = d["iter_us"] * 1e-3
+    res["cpu_baseline"] = {"value": round(ref_ms, 2), "unit": "ms per SIMPLE iteration", "cores": threads,
+                           "kind": "reference", "cpu_model": cpu_model(),
+                           "sample": f"the reference's own run of the same SU2 file ({FLOW_SU2_STEPS} iterations "
+                                     f"from the initial state, {d['solves']} solves), timed by its per-iteration "
+                                     f"clock (Croot::start_solver); whole run incl. reading and set-up "
+                                     f"{d['total_s']:.2f} s"}
+    res["speedup_gpu_vs_reference"] = round(ref_ms / ms, 1)
+    res["fields_max_rel_diff_after_3_iterations"] = diff
+    return res
+
+
 def flow_step(args, out):
     """One SIMPLE iteration of the flow solver (three momentum solves with
     ILU(0)-BiCGSTAB, the pressure correction with CG, enthalpy; REFERENCE-mode
@@ -727,7 +846,8 @@ def flow_step(args, out):
                                     "(BiCGSTAB momentum and enthalpy, CG pressure)",
                         **res,
                         "speedup_device_vs_host_support": round(res["host_support"]["ms_per_iteration"]
-                                                                / res["device"]["ms_per_iteration"], 2)}
+                                                                / res["device"]["ms_per_iteration"], 2),
+                        "su2_box": flow_su2_vs_reference(args)}
 
 
 def main():

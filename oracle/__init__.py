@@ -73,6 +73,59 @@ def ref(variant: str = "") -> ctypes.CDLL:
     return r
 
 
+class RefFlowConfig(ctypes.Structure):
+    """Run settings of the reference drivers' flow runs (RefFlowConfig in
+    ref_sod_driver.cpp)."""
+    _fields_ = [("n_cells", ctypes.c_int), ("lx", ctypes.c_double), ("ly", ctypes.c_double),
+                ("lz", ctypes.c_double), ("steps", ctypes.c_int), ("inner", ctypes.c_int),
+                ("dt", ctypes.c_double), ("steady", ctypes.c_int), ("implicit_underrelax", ctypes.c_double),
+                ("implicit_underrelax_max", ctypes.c_double), ("ramp_iterations", ctypes.c_int),
+                ("solver_type", ctypes.c_int), ("solver_mode", ctypes.c_int), ("conv_type", ctypes.c_int),
+                ("visc_type", ctypes.c_int), ("init_shock_tube", ctypes.c_int), ("p0", ctypes.c_double),
+                ("T0", ctypes.c_double), ("u0", ctypes.c_double * 3), ("p2", ctypes.c_double),
+                ("T2", ctypes.c_double), ("bc_type", ctypes.c_int * 3), ("bc_no_slip", ctypes.c_int * 3),
+                ("bc_const_T", ctypes.c_int * 3), ("bc_T", ctypes.c_double * 3), ("bc_p", ctypes.c_double * 3),
+                ("bc_u", (ctypes.c_double * 3) * 3)]
+
+
+def ref_flow_config(settings: dict) -> RefFlowConfig:
+    c = RefFlowConfig()
+    for k, v in settings.items():
+        if k == "u0":
+            for d in range(3):
+                c.u0[d] = v[d]
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def ref_flow_time_file(path: str, settings: dict, bcs: dict, threads: int, n_cells: int) -> dict:
+    """The reference's SIMPLE run on the SU2 file at `path` (one boundary
+    condition per marker: bcs holds bc_type, bc_no_slip, bc_const_T, bc_T,
+    bc_p, bc_u lists), on `threads` OpenMP threads, timed by the reference's
+    own per-iteration clock (ref_sod_driver.cpp's ref_flow_time_file).  The
+    reference writes its VTK files into the current directory.  Returns
+    iter_us, total_s, the number of solves and the final fields."""
+    c = ref_flow_config(settings)
+    c.n_cells = n_cells
+    nb = len(bcs["bc_type"])
+    i32 = lambda a: (ctypes.c_int * nb)(*a)  # noqa: E731
+    f64 = lambda a: (ctypes.c_double * len(a))(*a)  # noqa: E731
+    p, T, rho, h = (np.zeros(n_cells) for _ in range(4))
+    u = np.zeros(3 * n_cells)
+    it_us, tot_s = ctypes.c_double(), ctypes.c_double()
+    r = ref()
+    r.ref_flow_time_file.restype = ctypes.c_int64
+    cnt = r.ref_flow_time_file(ctypes.byref(c), path.encode(), nb, i32(bcs["bc_type"]), i32(bcs["bc_no_slip"]),
+                               i32(bcs["bc_const_T"]), f64(bcs["bc_T"]), f64(bcs["bc_p"]),
+                               f64([v for u3 in bcs["bc_u"] for v in u3]), int(threads), _i64(n_cells),
+                               ctypes.byref(it_us), ctypes.byref(tot_s), _p(p), _p(u), _p(T), _p(rho), _p(h))
+    if cnt <= 0:
+        raise RuntimeError(f"ref_flow_time_file returned {cnt}")
+    return dict(iter_us=it_us.value, total_s=tot_s.value, solves=int(cnt), p=p, u=u.reshape(n_cells, 3), T=T,
+                rho=rho, h=h)
+
+
 def set_threads(t: int) -> None:
     lib().or_set_threads(int(t))
     lib(True).or_set_threads(int(t))

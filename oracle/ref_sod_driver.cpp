@@ -11,6 +11,7 @@
 
 #include <cstdint>
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -409,6 +410,45 @@ int64_t ref_flow_run_file(const RefFlowConfig* cf, const char* path, int n_bc, c
     if (nnz > nnz_cap) return -1;
     std::memcpy(rp_out, S.matrix.row_ptr.data(), sizeof(uint64_t) * (n_cells + 1));
     std::memcpy(col_out, S.matrix.column_index.data(), sizeof(uint64_t) * nnz);
+    for (int64_t i = 0; i < n_cells; ++i) {
+        p[i] = cells[i].Pressure;
+        for (int d = 0; d < 3; ++d) u[3 * i + d] = cells[i].Velocity.val[d];
+        T[i] = cells[i].Temperature;
+        rho[i] = cells[i].Density;
+        h[i] = cells[i].Enthalpy;
+    }
+    return g_rec.count;
+}
+
+// The same run on `threads` OpenMP threads, timed by the reference itself:
+// Croot::start_solver accumulates each inner iteration's wall time in
+// total_elapsed_iteration_time (croot.cpp:126-219; the time-step update and
+// the VTK output are outside it).  iter_us receives the average per inner
+// iteration, total_s the whole run (mesh reading, FVM, matrix structure,
+// initialisation, iterations, VTK files).  Outputs as ref_flow_run_file:
+// the final fields of the n_cells cells.  Returns the number of solves.
+int64_t ref_flow_time_file(const RefFlowConfig* cf, const char* path, int n_bc, const int* bc_type,
+                           const int* bc_no_slip, const int* bc_const_T, const double* bc_T, const double* bc_p,
+                           const double* bc_u, int threads, int64_t n_cells, double* iter_us, double* total_s,
+                           double* p, double* u, double* T, double* rho, double* h) {
+    Cconfiguration c;
+    flow_settings(cf, c);
+    c.openmp_number_of_threads = threads > 0 ? threads : 1;
+    c.emesh_source = enums::Emesh_source::from_file;
+    c.mesh_filename = path;
+    c.set_BC_number(n_bc);
+    for (int m = 0; m < n_bc; ++m)
+        flow_bc(c, m, bc_type[m], bc_no_slip[m], bc_const_T[m], bc_T[m], bc_p[m], bc_u + 3 * m);
+    g_rec = Recorder();
+    g_rec.solver_mode = cf->solver_mode;
+    const unsigned long long before = root.total_elapsed_iteration_time;
+    const auto t0 = std::chrono::steady_clock::now();
+    root.run_config(c);
+    *total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const int inner = c.etimestepping_type == enums::Etimestepping_type::Steady ? 1 : cf->inner;
+    *iter_us = (double)(root.total_elapsed_iteration_time - before) / ((double)cf->steps * inner);
+    const auto& cells = root.Flow_solver->cells;
+    if ((int64_t)cells.size() != n_cells) return -2;
     for (int64_t i = 0; i < n_cells; ++i) {
         p[i] = cells[i].Pressure;
         for (int d = 0; d < 3; ++d) u[3 * i + d] = cells[i].Velocity.val[d];

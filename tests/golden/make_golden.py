@@ -126,17 +126,7 @@ def sod_cases():
         json.dump(meta, f, indent=1)
 
 
-class RefFlowConfig(ctypes.Structure):
-    _fields_ = [("n_cells", ctypes.c_int), ("lx", ctypes.c_double), ("ly", ctypes.c_double),
-                ("lz", ctypes.c_double), ("steps", ctypes.c_int), ("inner", ctypes.c_int),
-                ("dt", ctypes.c_double), ("steady", ctypes.c_int), ("implicit_underrelax", ctypes.c_double),
-                ("implicit_underrelax_max", ctypes.c_double), ("ramp_iterations", ctypes.c_int),
-                ("solver_type", ctypes.c_int), ("solver_mode", ctypes.c_int), ("conv_type", ctypes.c_int),
-                ("visc_type", ctypes.c_int), ("init_shock_tube", ctypes.c_int), ("p0", ctypes.c_double),
-                ("T0", ctypes.c_double), ("u0", ctypes.c_double * 3), ("p2", ctypes.c_double),
-                ("T2", ctypes.c_double), ("bc_type", ctypes.c_int * 3), ("bc_no_slip", ctypes.c_int * 3),
-                ("bc_const_T", ctypes.c_int * 3), ("bc_T", ctypes.c_double * 3), ("bc_p", ctypes.c_double * 3),
-                ("bc_u", (ctypes.c_double * 3) * 3)]
+RefFlowConfig = oracle.RefFlowConfig
 
 
 STATE_CELL = ("u", "u_old", "rho", "rho_old", "p", "p_old", "p_corr", "grad_p", "T", "h", "h_old", "ap",
