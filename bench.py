@@ -675,14 +675,17 @@ def tri_solves(args, out):
     from opencslsolver_amd.solver import Solver
 
     res = {}
-    for n, permute in ((64, True), (96, True), (96, False)):
+    for n, permute in ((32, True), (48, True), (64, True), (96, True), (96, False)):
         a, _ = gen.convdiff3d(n, seed_perm=3, permute=permute)
         key = f"{n}^3 {'permuted' if permute else 'lexicographic'}"
         _log(f"bench: triangular solves, {key}")
         with Solver(a.row_ptr, a.col) as s:
             s.set_values(a.val)
             res[key] = {"lower_us": round(s.time_tri_solve(True, reps=20) * 1e3, 2),
-                        "upper_us": round(s.time_tri_solve(False, reps=20) * 1e3, 2)}
+                        "upper_us": round(s.time_tri_solve(False, reps=20) * 1e3, 2),
+                        # the two forms the solvers choose between
+                        "lower_us_one_workgroup": round(s.time_tri_solve(True, reps=20, form=1) * 1e3, 2),
+                        "lower_us_launch_per_level": round(s.time_tri_solve(True, reps=20, form=2) * 1e3, 2)}
     out["tri_solves"] = {"workload": "exact L and U solves of the 3-sweep ILU(0) factor of the convection-diffusion "
                                      "system (HIP events, 20 solves each)", **res}
 

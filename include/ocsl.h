@@ -295,6 +295,10 @@ ocsl_status ocsl_set_reference_graphs(ocsl_handle h, int on);
  * `reps` times between two HIP events on the handle's stream; *avg_ms
  * receives the average per solve.  Values must be set. */
 ocsl_status ocsl_time_tri_solve(ocsl_handle h, int lower, int reps, double* avg_ms);
+/* The same for one form of the solve (form 0: as the solvers choose it;
+ * 1: one workgroup for the whole solve, a barrier per level; 2: one launch
+ * per level). */
+ocsl_status ocsl_time_tri_solve_form(ocsl_handle h, int lower, int form, int reps, double* avg_ms);
 
 /* The row order of OCSL_BICGSTAB_ORDER_COLOUR solves (no counterpart in the
  * reference): order[i] = the input row placed at position i (n entries;

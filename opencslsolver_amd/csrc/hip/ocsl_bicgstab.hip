@@ -333,8 +333,15 @@ void ensure_levels(ocsl_solver* h) {
     h->levels_built = true;
 }
 
-bool use_block_tri(const Levels& L, int n) {
-    return n <= 65536 || (int64_t)L.n_levels * 256 > n;
+// One workgroup for the whole solve when its levels average fewer than 1024
+// rows, else one launch per level.  Measured (lower solves of the 3-sweep
+// factor of the convection-diffusion systems): a launch costs ~4 us per
+// level whatever its size, a workgroup level ~3 ns per row; randomly
+// numbered 32^3 (about 2,000 rows per level) 62 us per level launch against
+// 98 us in one workgroup, lexicographic 96^3 (3,100) 1.07 against 4.8 ms.
+bool use_block_tri(const ocsl_solver* h, const Levels& L, int n) {
+    if (h->tri_form != 0) return h->tri_form == 1;
+    return (int64_t)L.n_levels * 1024 > n;
 }
 
 // The factor m's values in L's level order, gathered once per factor
@@ -382,7 +389,7 @@ void tri_exact(ocsl_solver* h, const VM* m, const X* in, X* out, bool lower, con
     level_values<VM>(h, L, m, &ev, &dv);
     if (L.ell_w) {
         const LevelEll te{L.rows.p, L.cnt.p, L.ecw.p, n, L.ell_w};
-        if (use_block_tri(L, n)) {
+        if (use_block_tri(h, L, n)) {
             k_tri_block_ell<VM, X, NR><<<1, kTriBlock, 0, h->stream>>>(te, L.ptr.p, L.n_levels, ev, dv, in, out, lower,
                                                                        S);
         } else {
@@ -395,7 +402,7 @@ void tri_exact(ocsl_solver* h, const VM* m, const X* in, X* out, bool lower, con
         return;
     }
     const LevelTri t{L.rows.p, L.eptr.p, L.ecol.p};
-    if (use_block_tri(L, n)) {
+    if (use_block_tri(h, L, n)) {
         k_tri_block<VM, X, NR><<<1, kTriBlock, 0, h->stream>>>(t, L.ptr.p, L.n_levels, ev, dv, in, out, lower, S);
     } else {
         for (int l = 0; l < L.n_levels; ++l) {
@@ -412,7 +419,7 @@ void tri_exact_pair(ocsl_solver* h, const VM* m, const double* in0, double* out0
                     bool lower, const Scalars* S) {
     Levels& L = lower ? h->lev_fwd : h->lev_bwd;
     const int n = h->host.n;
-    if (use_block_tri(L, n)) {
+    if (use_block_tri(h, L, n)) {
         tri_exact<VM, double, 1>(h, m, in0, out0, lower, S);
         tri_exact<VM, double, 1>(h, m, in1, out1, lower, S);
         return;
@@ -1364,7 +1371,17 @@ extern "C" ocsl_status ocsl_ILU_decompose_from(ocsl_handle h, int sweeps, const 
 }
 
 extern "C" ocsl_status ocsl_time_tri_solve(ocsl_handle h, int lower, int reps, double* avg_ms) {
-    if (!h || !avg_ms || reps <= 0 || (lower != 0 && lower != 1)) return OCSL_ERR_ARG;
+    return ocsl_time_tri_solve_form(h, lower, 0, reps, avg_ms);
+}
+
+extern "C" ocsl_status ocsl_time_tri_solve_form(ocsl_handle h, int lower, int form, int reps, double* avg_ms) {
+    if (!h || !avg_ms || reps <= 0 || (lower != 0 && lower != 1) || form < 0 || form > 2) return OCSL_ERR_ARG;
+    struct Restore {
+        ocsl_solver* h;
+        int form;
+        ~Restore() { h->tri_form = form; }
+    } restore{h, h->tri_form};
+    h->tri_form = form;
     return tri_guard([&] {
         OCSL_REQUIRE(h->have_vals, OCSL_ERR_STATE, "values not set");
         const int n = h->host.n;

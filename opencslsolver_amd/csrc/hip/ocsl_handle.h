@@ -145,6 +145,10 @@ struct ocsl_solver {
 
     ocsl::Levels lev_fwd, lev_bwd;
     bool levels_built = false;
+    // form of the exact triangular solves: 0 chosen by size and depth, 1 one
+    // workgroup for the whole solve, 2 one launch per level (the timing hook
+    // sets it for the duration of a measurement)
+    int tri_form = 0;
     ocsl::IluTables ilu;
     bool ilu_built = false;
     ocsl::DBuf<double> ilu_m, ilu_old;

@@ -171,11 +171,12 @@ class Solver:
         after their first run (default), or issue every launch each time."""
         _check(self._lib.ocsl_set_reference_graphs(self._h, 1 if on else 0))
 
-    def time_tri_solve(self, lower=True, reps=20) -> float:
+    def time_tri_solve(self, lower=True, reps=20, form=0) -> float:
         """Average ms of one exact triangular solve of the 3-sweep ILU(0)
-        factor (HIP events on the handle's stream)."""
+        factor (HIP events on the handle's stream); form 0 as the solvers
+        choose it, 1 one workgroup, 2 one launch per level."""
         ms = ctypes.c_double()
-        _check(self._lib.ocsl_time_tri_solve(self._h, 1 if lower else 0, reps, ctypes.byref(ms)))
+        _check(self._lib.ocsl_time_tri_solve_form(self._h, 1 if lower else 0, int(form), reps, ctypes.byref(ms)))
         return ms.value
 
     def colour_order(self):
