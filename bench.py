@@ -675,17 +675,21 @@ def tri_solves(args, out):
     from opencslsolver_amd.solver import Solver
 
     res = {}
+    cases = [("1000-row chain (one row per level)", gen.chain1d(1000, 5))]
     for n, permute in ((32, True), (48, True), (64, True), (96, True), (96, False)):
-        a, _ = gen.convdiff3d(n, seed_perm=3, permute=permute)
-        key = f"{n}^3 {'permuted' if permute else 'lexicographic'}"
+        cases.append((f"{n}^3 {'permuted' if permute else 'lexicographic'}",
+                      gen.convdiff3d(n, seed_perm=3, permute=permute)[0]))
+    for key, a in cases:
         _log(f"bench: triangular solves, {key}")
         with Solver(a.row_ptr, a.col) as s:
             s.set_values(a.val)
             res[key] = {"lower_us": round(s.time_tri_solve(True, reps=20) * 1e3, 2),
                         "upper_us": round(s.time_tri_solve(False, reps=20) * 1e3, 2),
-                        # the two forms the solvers choose between
+                        # the forms the solvers choose between
                         "lower_us_one_workgroup": round(s.time_tri_solve(True, reps=20, form=1) * 1e3, 2),
                         "lower_us_launch_per_level": round(s.time_tri_solve(True, reps=20, form=2) * 1e3, 2)}
+            if a.n <= 1000:
+                res[key]["lower_us_serial"] = round(s.time_tri_solve(True, reps=20, form=3) * 1e3, 2)
     out["tri_solves"] = {"workload": "exact L and U solves of the 3-sweep ILU(0) factor of the convection-diffusion "
                                      "system (HIP events, 20 solves each)", **res}
 

@@ -289,15 +289,23 @@ ocsl_status ocsl_set_tuned_cg(ocsl_handle h, int precision, const ocsl_options* 
  * default).  0 issues every launch each time; results are bitwise the same. */
 ocsl_status ocsl_set_reference_graphs(ocsl_handle h, int on);
 
+/* Form of the exact triangular solves (ILU_forsub / ILU_backsub and the
+ * BiCGSTAB preconditioner; no counterpart in the reference, whose solves are
+ * sequential loops): 0 (default) chosen from the level schedule; 1 one
+ * workgroup, a barrier per level; 2 one launch per level; 3 serial, one
+ * thread over the rows staged in LDS (systems whose level-ordered factor fits
+ * in 64 KB; otherwise as 2).  Every form adds the same terms in the same
+ * order: results are bitwise the same. */
+ocsl_status ocsl_set_tri_form(ocsl_handle h, int form);
+
 /* Measurement hook (no counterpart in the reference): the exact lower
  * (lower = 1) or upper (0) solve of the handle's 3-sweep ILU(0) factor
  * (ILU_forsub / ILU_backsub, cmatrix_solver.cpp:553-633, level-scheduled),
  * `reps` times between two HIP events on the handle's stream; *avg_ms
  * receives the average per solve.  Values must be set. */
 ocsl_status ocsl_time_tri_solve(ocsl_handle h, int lower, int reps, double* avg_ms);
-/* The same for one form of the solve (form 0: as the solvers choose it;
- * 1: one workgroup for the whole solve, a barrier per level; 2: one launch
- * per level). */
+/* The same for one form of the solve (the forms of ocsl_set_tri_form; form
+ * 3 is an argument error when the system does not fit it). */
 ocsl_status ocsl_time_tri_solve_form(ocsl_handle h, int lower, int form, int reps, double* avg_ms);
 
 /* The row order of OCSL_BICGSTAB_ORDER_COLOUR solves (no counterpart in the

@@ -104,6 +104,7 @@ SIGNATURES = {
     "ocsl_ILU_decompose": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
     "ocsl_set_reference_graphs": (ctypes.c_int, [c_handle, ctypes.c_int]),
     "ocsl_time_tri_solve": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "ocsl_set_tri_form": (ctypes.c_int, [c_handle, ctypes.c_int]),
     "ocsl_time_tri_solve_form": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                 ctypes.POINTER(ctypes.c_double)]),
     "ocsl_colour_order": (ctypes.c_int, [c_handle, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),

@@ -508,4 +508,13 @@ ocsl_status ocsl_set_reference_graphs(ocsl_handle h, int on) {
     });
 }
 
+ocsl_status ocsl_set_tri_form(ocsl_handle h, int form) {
+    if (!h || form < 0 || form > 3) return OCSL_ERR_ARG;
+    return guarded([&] {
+        sync(h);
+        if (form != h->tri_form) drop_graphs(h);  // captured solves hold the form they were captured with
+        h->tri_form = form;
+    });
+}
+
 }  // extern "C"

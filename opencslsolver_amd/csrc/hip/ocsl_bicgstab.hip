@@ -17,6 +17,7 @@
 #include <cfloat>
 #include <cstddef>
 #include <cmath>
+#include <type_traits>
 
 #include "ocsl_handle.h"
 #include "ocsl_kernels.h"
@@ -283,6 +284,188 @@ __global__ void __launch_bounds__(kTriBlock) k_tri_block_ell(LevelEll t, const i
     }
 }
 
+// Deep, narrow schedules (levels of a few rows: the chains of 1D meshes):
+// the workgroup stages the level-ordered factor and the input in LDS, then
+// one thread walks the positions in order -- every row's dependencies sit at
+// earlier positions -- and keeps the row it just solved in registers, so a
+// chain costs one dependent multiply-add per row instead of a workgroup
+// barrier and a memory round trip per level.  The terms are added in the
+// order of ell_apply and each result is rounded to X before it is read
+// again, so results are bitwise those of the level kernels.
+constexpr int kTriSerialBlock = 1024;  // threads that stage the arrays
+constexpr int kTriSerialLds = 65536;   // LDS bytes of a serial solve, at most
+inline size_t tri_serial_lds(int n, int w, int nr) {
+    return (size_t)n * (16 * (size_t)nr + 12 * (size_t)w + 16);
+}
+template <class VM, class X, int NR, bool kLower, int kW>
+__global__ void __launch_bounds__(kTriSerialBlock) k_tri_serial_ell(LevelEll t, const VM* __restrict__ ev,
+                                                                    const VM* __restrict__ dv, const X* in, X* out,
+                                                                    const Scalars* S) {
+    extern __shared__ double lds[];
+    if (S && S->halt) return;
+    const int n = t.n, w = t.w;
+    double* xs = lds;               // n NR: the solution, by row
+    double* xin = xs + n * NR;      // n NR: the input, by position
+    double* vv = xin + n * NR;      // w n: factor values, entry e of position k at e n + k
+    double* vd = vv + w * n;        // n: diagonal by position (upper)
+    int* ri = (int*)(vd + n);       // n: row of each position
+    int* rc = ri + n;               // n: entries of each position
+    int* rj = rc + n;               // w n: columns, as vv
+    for (int q = threadIdx.x; q < w * n; q += blockDim.x) {
+        vv[q] = (double)ev[q];
+        rj[q] = t.ecol[q];
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int i = t.rows[k];
+        ri[k] = i;
+        rc[k] = t.cnt[k];
+        vd[k] = kLower ? 1.0 : (double)dv[k];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) xin[k * NR + c] = (double)in[(size_t)i * NR + c];
+    }
+    __syncthreads();
+    // A chain: position 0 has no entry, every later position one, on the row
+    // of the position before it (the factors of 1D meshes).
+    int chain = 1;
+    if constexpr (kW == 1 && NR == 1) {
+        for (int k = threadIdx.x; k < n; k += blockDim.x)
+            if (rc[k] != (k > 0 ? 1 : 0) || (k > 0 && rj[k] != ri[k - 1])) chain = 0;
+        chain = __syncthreads_and(chain);
+    } else {
+        chain = 0;
+    }
+    if (chain && threadIdx.x == 0) {
+        // y_k = x_k - m_k y_{k-1} (upper: then / d_k), one dependent
+        // multiply-add per row; the terms as in the general loop below
+        constexpr int kC = 8;
+        double pv;
+        {
+            const double v = xin[0];
+            const X r0 = (X)(kLower ? v : v / vd[0]);
+            pv = (double)r0;
+            xs[ri[0]] = pv;
+        }
+        int k0 = 1;
+        for (; k0 + kC <= n; k0 += kC) {
+            double xv[kC], mv[kC], dd[kC];
+            int rr[kC];
+#pragma unroll
+            for (int u = 0; u < kC; ++u) {
+                xv[u] = xin[k0 + u];
+                mv[u] = vv[k0 + u];
+                dd[u] = kLower ? 1.0 : vd[k0 + u];
+                rr[u] = ri[k0 + u];
+            }
+#pragma unroll
+            for (int u = 0; u < kC; ++u) {
+                double sum = 0.0;
+                sum += mv[u] * pv;
+                const double v = xv[u] - sum;
+                const X r = (X)(kLower ? v : v / dd[u]);
+                pv = (double)r;
+                xs[rr[u]] = pv;
+            }
+        }
+        for (; k0 < n; ++k0) {
+            double sum = 0.0;
+            sum += vv[k0] * pv;
+            const double v = xin[k0] - sum;
+            const X r = (X)(kLower ? v : v / vd[k0]);
+            pv = (double)r;
+            xs[ri[k0]] = pv;
+        }
+    }
+    if (!chain && threadIdx.x == 0) {
+        // Rows in chunks: a chunk's position data is read from LDS at once,
+        // so one LDS round trip serves kChunk rows, each of which then waits
+        // only for the multiply-adds (and, upper, the division) of the row
+        // before it.
+        constexpr int kChunk = (kW == 1 && NR <= 2) ? 8 : 2;
+        struct Row {
+            int i, c;
+            double d, x[NR];
+            int j[kW];
+            double v[kW];
+        };
+        auto load = [&](int k, Row& r) {
+            r.i = ri[k];
+            r.c = rc[k];
+            r.d = kLower ? 1.0 : vd[k];
+#pragma unroll
+            for (int c = 0; c < NR; ++c) r.x[c] = xin[k * NR + c];
+#pragma unroll
+            for (int e = 0; e < kW; ++e) {
+                const bool in_w = kW == 1 || e < w;
+                r.j[e] = in_w ? rj[e * n + k] : 0;
+                r.v[e] = in_w ? vv[e * n + k] : 0.0;
+            }
+        };
+        int prev = -1;
+        double pv[NR];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) pv[c] = 0.0;
+        auto solve = [&](const Row& r) {
+            double sum[NR];
+#pragma unroll
+            for (int c = 0; c < NR; ++c) sum[c] = 0.0;
+#pragma unroll
+            for (int e = 0; e < kW; ++e)
+                if (e < r.c) {
+                    const int j = r.j[e];
+                    if (j == prev) {
+#pragma unroll
+                        for (int c = 0; c < NR; ++c) sum[c] += r.v[e] * pv[c];
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < NR; ++c) sum[c] += r.v[e] * xs[j * NR + c];
+                    }
+                }
+#pragma unroll
+            for (int c = 0; c < NR; ++c) {
+                const double v = r.x[c] - sum[c];
+                const X res = (X)(kLower ? v : v / r.d);
+                pv[c] = (double)res;
+                xs[r.i * NR + c] = (double)res;
+            }
+            prev = r.i;
+        };
+        for (int k0 = 0; k0 < n; k0 += kChunk) {
+            Row r[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                if (k0 + u < n) load(k0 + u, r[u]);
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                if (k0 + u < n) solve(r[u]);
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < n * NR; q += blockDim.x) out[q] = (X)xs[q];
+}
+
+// Launches the serial form: lower or upper, the one-entry chain or up to
+// kTriEll entries per row.
+template <class VM, class X, int NR>
+void launch_tri_serial(const LevelEll& te, const VM* ev, const VM* dv, const X* in, X* out, bool lower,
+                       const Scalars* S, hipStream_t st) {
+    const size_t lds = tri_serial_lds(te.n, te.w, NR);
+    auto go = [&](auto L, auto W) {
+        k_tri_serial_ell<VM, X, NR, decltype(L)::value, decltype(W)::value>
+            <<<1, kTriSerialBlock, lds, st>>>(te, ev, dv, in, out, S);
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    using W1 = std::integral_constant<int, 1>;
+    using W8 = std::integral_constant<int, kTriEll>;
+    if (lower) {
+        if (te.w == 1) go(T{}, W1{});
+        else go(T{}, W8{});
+    } else {
+        if (te.w == 1) go(F{}, W1{});
+        else go(F{}, W8{});
+    }
+}
+
 // Factor values into level order: ev[e] = m[esrc[e]], dv[k] = m[dslot[k]].
 template <class VM>
 __global__ void k_level_gather(int ne, const int* __restrict__ esrc, int n, const int* __restrict__ dslot,
@@ -344,6 +527,16 @@ bool use_block_tri(const ocsl_solver* h, const Levels& L, int n) {
     return (int64_t)L.n_levels * 1024 > n;
 }
 
+// The serial form (k_tri_serial_ell): padded-column entries, the staged
+// arrays within kTriSerialLds, and levels of 8 rows or fewer on average
+// (each serial row costs ~50 cycles of LDS and multiply-add latency, a
+// workgroup level ~0.35 us).
+bool use_serial_tri(const ocsl_solver* h, const Levels& L, int n, int nr) {
+    if (!L.ell_w || tri_serial_lds(n, L.ell_w, nr) > (size_t)kTriSerialLds) return false;
+    if (h->tri_form != 0) return h->tri_form == 3;
+    return (int64_t)L.n_levels * 8 >= n;
+}
+
 // The factor m's values in L's level order, gathered once per factor
 // (address) and decomposition.
 template <class VM>
@@ -389,7 +582,9 @@ void tri_exact(ocsl_solver* h, const VM* m, const X* in, X* out, bool lower, con
     level_values<VM>(h, L, m, &ev, &dv);
     if (L.ell_w) {
         const LevelEll te{L.rows.p, L.cnt.p, L.ecw.p, n, L.ell_w};
-        if (use_block_tri(h, L, n)) {
+        if (use_serial_tri(h, L, n, NR)) {
+            launch_tri_serial<VM, X, NR>(te, ev, dv, in, out, lower, S, h->stream);
+        } else if (use_block_tri(h, L, n)) {
             k_tri_block_ell<VM, X, NR><<<1, kTriBlock, 0, h->stream>>>(te, L.ptr.p, L.n_levels, ev, dv, in, out, lower,
                                                                        S);
         } else {
@@ -419,7 +614,7 @@ void tri_exact_pair(ocsl_solver* h, const VM* m, const double* in0, double* out0
                     bool lower, const Scalars* S) {
     Levels& L = lower ? h->lev_fwd : h->lev_bwd;
     const int n = h->host.n;
-    if (use_block_tri(h, L, n)) {
+    if (use_serial_tri(h, L, n, 1) || use_block_tri(h, L, n)) {
         tri_exact<VM, double, 1>(h, m, in0, out0, lower, S);
         tri_exact<VM, double, 1>(h, m, in1, out1, lower, S);
         return;
@@ -1375,7 +1570,7 @@ extern "C" ocsl_status ocsl_time_tri_solve(ocsl_handle h, int lower, int reps, d
 }
 
 extern "C" ocsl_status ocsl_time_tri_solve_form(ocsl_handle h, int lower, int form, int reps, double* avg_ms) {
-    if (!h || !avg_ms || reps <= 0 || (lower != 0 && lower != 1) || form < 0 || form > 2) return OCSL_ERR_ARG;
+    if (!h || !avg_ms || reps <= 0 || (lower != 0 && lower != 1) || form < 0 || form > 3) return OCSL_ERR_ARG;
     struct Restore {
         ocsl_solver* h;
         int form;
@@ -1387,6 +1582,8 @@ extern "C" ocsl_status ocsl_time_tri_solve_form(ocsl_handle h, int lower, int fo
         const int n = h->host.n;
         ensure_csr_values(h, false);
         ensure_levels(h);
+        OCSL_REQUIRE(form != 3 || use_serial_tri(h, lower ? h->lev_fwd : h->lev_bwd, n, 1), OCSL_ERR_ARG,
+                     "the serial form does not fit this system");
         ilu_decompose_device(h, 3, false);
         DBuf<double> in, out;
         in.alloc((size_t)n);

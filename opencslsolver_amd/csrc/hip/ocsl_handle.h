@@ -145,9 +145,10 @@ struct ocsl_solver {
 
     ocsl::Levels lev_fwd, lev_bwd;
     bool levels_built = false;
-    // form of the exact triangular solves: 0 chosen by size and depth, 1 one
-    // workgroup for the whole solve, 2 one launch per level (the timing hook
-    // sets it for the duration of a measurement)
+    // form of the exact triangular solves: 0 chosen from the level schedule,
+    // 1 one workgroup for the whole solve, 2 one launch per level, 3 serial
+    // over the rows staged in LDS (ocsl_set_tri_form; the timing hook sets it
+    // for the duration of a measurement)
     int tri_form = 0;
     ocsl::IluTables ilu;
     bool ilu_built = false;

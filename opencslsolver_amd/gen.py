@@ -94,6 +94,27 @@ def poisson3d(n: int, seed_mesh: int = 1, seed_rhs: int | None = 2, sigma: float
     return CSR(rp, col, val), b, perm
 
 
+def chain1d(n: int, seed: int) -> CSR:
+    """Nonsymmetric, diagonally dominant tridiagonal matrix (a 1D
+    convection-diffusion row, the shape of the shock-tube systems): every
+    level of its ILU(0) triangular solves holds one row.  Off-diagonals
+    -1 - U[0,1) (lower) and -0.5 - U[0,1) (upper), diagonal 4 + U[0,1), row by
+    row from seed."""
+    rng = np.random.default_rng(seed)
+    rp, col, val = [0], [], []
+    for i in range(n):
+        if i > 0:
+            col.append(i - 1)
+            val.append(-1.0 - rng.random())
+        col.append(i)
+        val.append(4.0 + rng.random())
+        if i + 1 < n:
+            col.append(i + 1)
+            val.append(-0.5 - rng.random())
+        rp.append(len(col))
+    return CSR(np.array(rp, np.uint64), np.array(col, np.uint64), np.array(val))
+
+
 def convdiff3d(n: int, seed_perm: int = 3, rho: float = 1.2, dt: float = 1e-3,
                peclet: float = 10.0, permute: bool = True, sentinels: bool = False):
     """Taylor-Green upwind convection-diffusion system; returns (A, perm)."""

@@ -171,6 +171,12 @@ class Solver:
         after their first run (default), or issue every launch each time."""
         _check(self._lib.ocsl_set_reference_graphs(self._h, 1 if on else 0))
 
+    def set_tri_form(self, form: int = 0):
+        """Form of the exact triangular solves: 0 chosen from the level
+        schedule, 1 one workgroup, 2 one launch per level, 3 serial over the
+        rows staged in LDS; the results are bitwise the same."""
+        _check(self._lib.ocsl_set_tri_form(self._h, int(form)))
+
     def time_tri_solve(self, lower=True, reps=20, form=0) -> float:
         """Average ms of one exact triangular solve of the 3-sweep ILU(0)
         factor (HIP events on the handle's stream); form 0 as the solvers

@@ -294,6 +294,57 @@ def test_exact_triangular_solves_match_oracle(n, seed):
     parity.check(f"backsub {n}^3", parity.rel(x, oracle.backsub(a, m, y_o)), 1e-13)
 
 
+@pytest.mark.parametrize("kind", ["chain", "box"])
+def test_triangular_solve_forms_agree_bitwise(kind):
+    """ocsl_ILU_forsub / ocsl_ILU_backsub in every form: chosen from the
+    schedule (0), one workgroup (1), one launch per level (2), serial over the
+    rows staged in LDS (3).  Each adds the same terms in the same order, so
+    the results are bitwise equal.  The chain (1000 rows, one per level) takes
+    the serial form by default; the randomly numbered 8^3 box (about 8
+    levels of 64 rows) one workgroup."""
+    a = gen.chain1d(1000, 5) if kind == "chain" else gen.convdiff3d(8, seed_perm=3)[0]
+    b = gen.uniform(6, a.n)
+    m = oracle.ilu_decompose(a, 3)
+    out = {}
+    with Solver(a.row_ptr, a.col) as s:
+        s.set_values(a.val)
+        for form in (0, 1, 2, 3):
+            s.set_tri_form(form)
+            y = s.forsub(m, b)
+            out[form] = (y, s.backsub(m, y))
+    for form in (1, 2, 3):
+        assert np.array_equal(out[form][0], out[0][0]), form
+        assert np.array_equal(out[form][1], out[0][1]), form
+    y_o = oracle.forsub(a, m, b)
+    parity.check(f"forsub {kind}", parity.rel(out[0][0], y_o), 1e-13)
+    parity.check(f"backsub {kind}", parity.rel(out[0][1], oracle.backsub(a, m, y_o)), 1e-13)
+
+
+def test_serial_triangular_form_inside_the_solvers():
+    """The serial form inside the REFERENCE ILU(0)-BiCGSTAB (FP64, one vector)
+    and the TOLERANCE MIXED solve of three right-hand sides (FP32 factor,
+    three interleaved vectors) on a 1000-row chain: bitwise the results of
+    one launch per level."""
+    a = gen.chain1d(1000, 7)
+    B = np.stack([gen.uniform(sd, a.n) for sd in (4, 5, 6)])
+    res = {}
+    for form in (0, 2):
+        with Solver(a.row_ptr, a.col) as s:
+            s.set_values(a.val)
+            s.set_tri_form(form)
+            s.set_rhs(B[0])
+            s.set_x(np.zeros(a.n))
+            r = s.ILU_BICGSTAB_solve()
+            x = s.get_x()
+            X, reps = s.solve_multi(B, precision="mixed", tol=1e-10)
+            res[form] = (r, x, X, [q["iterations"] for q in reps])
+    assert res[0][0] == res[2][0]
+    assert np.array_equal(res[0][1], res[2][1])
+    assert np.array_equal(res[0][2], res[2][2]) and res[0][3] == res[2][3]
+    for k in range(3):
+        assert np.linalg.norm(B[k] - a.matvec(res[0][2][k])) <= 1e-10 * np.linalg.norm(B[k])
+
+
 def test_graph_cache_follows_the_jacobi_weight():
     """TOLERANCE BiCGSTAB with damped-Jacobi triangular sweeps replays its
     iteration chunks as graphs whose sweeps take the weight as an argument: a
