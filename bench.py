@@ -17,8 +17,11 @@ already resident in HBM.  The JSON line also carries the side workloads:
   sampled outer product dL/dA on the CSR pattern) on the 128^3 system;
 * assembly -- the GPU coefficient assembly of each equation on a 128^3 hex
   mesh, beside the oracle's host restatement of the reference loops;
+* tri -- exact triangular solves of ILU(0) factors in each of their forms;
 * flow -- SIMPLE iterations of the flow solver on a 64^3 hex mesh, support
-  loops on the device (state resident) and on the host;
+  loops on the device (state resident) and on the host, and on a 32^3 SU2
+  box beside the reference's own run of the same file;
+* sod -- the 1000-cell shock tube, 200 steps, beside the reference's own run;
 * structure -- the CSR pattern of the 128^3 mesh built on the device;
 * mesh -- SU2 reading, connectivity and geometry of a 64^3 box on the host.
 
@@ -813,6 +816,100 @@ def flow_su2_vs_reference(args, n=32):
     return res
 
 
+def _ref_sod_child(steps, threads, out_path):
+    """Child-process leg of the shock-tube CPU baseline: the reference's own
+    run (built from its sources for the build host's instruction set) timed
+    by its own per-iteration clock; writes the final fields to out_path and
+    prints JSON."""
+    import tempfile
+
+    import oracle
+
+    with tempfile.TemporaryDirectory() as tmp:
+        os.chdir(tmp)  # the reference writes its VTK files into the working directory
+        sys.stdout.flush()
+        saved = os.dup(1)
+        os.dup2(2, 1)  # and its progress lines to stdout
+        try:
+            r = oracle.ref_sod_time(steps, threads)
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+    np.savez(out_path, **{k: r[k] for k in ("p", "u", "T", "rho", "h")})
+    print(json.dumps({k: r[k] for k in ("iter_us", "total_s", "solves")}))
+
+
+def sod(args, out):
+    """The reference's CPU-runnable case: the 1D shock tube, 1000 cells over
+    1 m (the reference's own 1D mesh, recorded in the fixture
+    flow_sod1000_mesh.npz), slip walls, empty side faces, inviscid, blended
+    central convection, dt 1e-6, 200 steps of 10 inner iterations, CG for the
+    pressure correction and REFERENCE ILU(0)-BiCGSTAB for momentum and
+    enthalpy, the state resident on the device.  Beside it the reference's
+    own run of the same case on the host, timed by its per-iteration clock,
+    and the largest relative difference of the final fields."""
+    import subprocess
+    import tempfile
+
+    import torch
+
+    import oracle
+    from opencslsolver_amd import flow
+
+    steps, inner = 200, 10
+    _log(f"bench: shock tube, {steps} steps x {inner} inner iterations")
+    mz = np.load(os.path.join(ROOT, "tests", "golden", "flow_sod1000_mesh.npz"))
+    bcs = [dict(type=flow.WALL, no_slip=0), dict(type=flow.EMPTY), dict(type=flow.WALL, no_slip=0)]
+    prm = flow.default_params(viscous=0, blended=1, transient=1, dt=1e-6, implicit_underrelax=0.4,
+                              implicit_underrelax_max=0.8, ramp_iterations=100, shock_tube=1, p0=1e5, t0=348.4,
+                              u0=(0.0, 0.0, 0.0), p2=1e4, t2=278.7, bcs=bcs, method_momentum=1, method_pressure=0,
+                              method_enthalpy=1)
+    with flow.Flow(flow.mesh_from_npz(mz), mz["row_ptr"], mz["col"], prm) as fl:
+        fl.initialize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fl.run(steps, inner)
+        torch.cuda.synchronize()
+        gpu_s = time.perf_counter() - t0
+        gpu = fl.get_state()
+    res = {"workload": f"1D shock tube, 1000 cells, {steps} steps x {inner} inner iterations (dt 1e-6), CG for the "
+                       "pressure correction, REFERENCE ILU(0)-BiCGSTAB for momentum and enthalpy",
+           "gpu_s": round(gpu_s, 3), "gpu_ms_per_iteration": round(gpu_s * 1e3 / (steps * inner), 3)}
+    if args.no_cpu_baseline or not oracle.ref_available():
+        out["sod"] = res
+        return
+    legs = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for threads in sorted({1, host_threads()}):
+            out_path = os.path.join(tmp, f"ref_{threads}.npz")
+            code = ("import sys; sys.path.insert(0, sys.argv[1]); import bench; "
+                    "bench._ref_sod_child(int(sys.argv[2]), int(sys.argv[3]), sys.argv[4])")
+            r = subprocess.run([sys.executable, "-c", code, ROOT, str(steps), str(threads), out_path],
+                               capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                legs[threads] = {"error": r.stderr.strip().splitlines()[-1:]}
+                continue
+            d = json.loads(r.stdout.strip().splitlines()[-1])
+            ref = np.load(out_path)
+            d["fields_max_rel_diff"] = {f: float(np.max(np.abs(gpu[f] - ref[f])) / max(np.max(np.abs(ref[f])), 1e-300))
+                                        for f in ("p", "u", "T", "rho", "h")}
+            legs[threads] = d
+    ok = {t: d for t, d in legs.items() if "iter_us" in d}
+    if ok:
+        best = min(ok, key=lambda t: ok[t]["iter_us"])
+        d = ok[best]
+        res["cpu_baseline"] = {"value": round(d["iter_us"] * 1e-3, 3), "unit": "ms per SIMPLE iteration",
+                               "cores": best, "kind": "reference", "cpu_model": cpu_model(),
+                               "sample": f"the reference's own run of the same case ({d['solves']} solves), timed "
+                                         f"by its per-iteration clock (Croot::start_solver); the faster of 1 and "
+                                         f"{host_threads()} OpenMP threads; whole run {d['total_s']:.2f} s",
+                               "by_threads_ms_per_iteration": {str(t): round(v["iter_us"] * 1e-3, 3)
+                                                               for t, v in ok.items()}}
+        res["speedup_gpu_vs_reference"] = round(d["iter_us"] * 1e-3 / res["gpu_ms_per_iteration"], 2)
+        res["fields_max_rel_diff_vs_reference_1_thread"] = ok.get(1, d)["fields_max_rel_diff"]
+    out["sod"] = res
+
+
 def flow_step(args, out):
     """One SIMPLE iteration of the flow solver (three momentum solves with
     ILU(0)-BiCGSTAB, the pressure correction with CG, enthalpy; REFERENCE-mode
@@ -863,8 +960,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--sections", default="adjoint,multi,batch,assembly,tri,flow,structure,mesh",
-                    help="side workloads to add to the headline: adjoint, multi, batch, assembly, tri, flow, "
+    ap.add_argument("--sections", default="adjoint,multi,batch,assembly,tri,flow,sod,structure,mesh",
+                    help="side workloads to add to the headline: adjoint, multi, batch, assembly, tri, flow, sod, "
                          "structure, mesh (or none)")
     ap.add_argument("--flow-n", type=int, default=64, help="hex mesh size of the flow-step section")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -912,6 +1009,8 @@ def main():
         tri_solves(args, out)
     if "flow" in args.sections:
         flow_step(args, out)
+    if "sod" in args.sections:
+        sod(args, out)
     if "structure" in args.sections:
         structure(args, out)
     if "mesh" in args.sections:

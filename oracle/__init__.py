@@ -126,6 +126,50 @@ def ref_flow_time_file(path: str, settings: dict, bcs: dict, threads: int, n_cel
                 rho=rho, h=h)
 
 
+# The shock tube of the reference fixtures (ref_sod_run): 1000 cells over 1 m,
+# Ly = Lz = 0.1, slip walls at both ends, empty side faces, inviscid, blended
+# central convection, transient dt 1e-6, 10 inner iterations per step.
+SOD_SETTINGS = dict(n_cells=1000, lx=1.0, ly=0.1, lz=0.1, dt=1e-6, p_left=1e5, t_left=348.4, p_right=1e4,
+                    t_right=278.7, conv_type=1, visc_type=1, inner=10)
+
+
+def ref_sod_time(steps: int, threads: int, solver_type: int = 0, solver_mode: int = 1) -> dict:
+    """The reference's shock-tube run (SOD_SETTINGS; solver_mode 1: CG for the
+    pressure correction, ILU(0)-BiCGSTAB otherwise) on `threads` OpenMP
+    threads, timed by the reference's own per-iteration clock.  The reference
+    writes its VTK files into the current directory.  Returns iter_us,
+    total_s, the number of solves and the final fields."""
+    import time
+
+    c = SOD_SETTINGS
+    n = c["n_cells"]
+    nnz_cap = 5 * n
+    hist = np.zeros(1)
+    rp, col = np.zeros(n + 1, np.uint64), np.zeros(nnz_cap, np.uint64)
+    nnz = ctypes.c_int64()
+    dummy = np.zeros(1)
+    dm = np.zeros(1, np.int32)
+    p, T, rho, h = (np.zeros(n) for _ in range(4))
+    u = np.zeros(3 * n)
+    r = ref()
+    r.ref_last_iteration_us.restype = ctypes.c_double
+    r.ref_set_threads(int(threads))
+    t0 = time.perf_counter()
+    try:
+        cnt = r.ref_sod_run(n, _d(c["lx"]), _d(c["ly"]), _d(c["lz"]), steps, c["inner"], _d(c["dt"]), solver_type,
+                            solver_mode, c["conv_type"], c["visc_type"], _d(c["p_left"]), _d(c["t_left"]),
+                            _d(c["p_right"]), _d(c["t_right"]), _i64(0), _p(hist), 0, _i64(nnz_cap),
+                            _p(rp, _u64p), _p(col, _u64p), ctypes.byref(nnz), _p(dummy), _p(dummy), _p(dummy),
+                            _p(dummy), _p(dm, _ip), _p(p), _p(u), _p(T), _p(rho), _p(h))
+    finally:
+        r.ref_set_threads(1)
+    total = time.perf_counter() - t0
+    if cnt <= 0:
+        raise RuntimeError(f"ref_sod_run returned {cnt}")
+    return dict(iter_us=r.ref_last_iteration_us(), total_s=total, solves=int(cnt), p=p, u=u.reshape(n, 3), T=T,
+                rho=rho, h=h)
+
+
 def set_threads(t: int) -> None:
     lib().or_set_threads(int(t))
     lib(True).or_set_threads(int(t))

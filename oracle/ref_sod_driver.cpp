@@ -54,6 +54,13 @@ struct Recorder {
 };
 Recorder g_rec;
 
+// OpenMP threads of the shock-tube runs (1 for the fixtures: the reference's
+// reductions and atomics then run in a fixed order) and the reference's own
+// per-inner-iteration clock of the last run (Croot::start_solver
+// accumulates each inner iteration's wall time in total_elapsed_iteration_time).
+int g_threads = 1;
+double g_last_iter_us = 0.0;
+
 }  // namespace
 
 extern "C" void __real__ZN14CMatrix_solver5solveEv(CMatrix_solver* self);
@@ -121,6 +128,11 @@ extern "C" void __wrap__ZN14CMatrix_solver5solveEv(CMatrix_solver* self) {
 
 extern "C" {
 
+// OpenMP threads of the next ref_sod_run (default 1) and the average inner
+// iteration of the last one by the reference's own clock, in microseconds.
+void ref_set_threads(int threads) { g_threads = threads > 0 ? threads : 1; }
+double ref_last_iteration_us() { return g_last_iter_us; }
+
 // Returns the number of solves performed, or a negative value when the
 // buffers are too small.  Field outputs are per cell in the reference's cell
 // order.  The reference writes solution.vtk / solution_BC.vtk into the
@@ -133,7 +145,7 @@ int64_t ref_sod_run(int n_cells, double lx, double ly, double lz, int steps, int
                     double* dump_x0, double* dump_x, int* dump_method, double* p, double* u,
                     double* T, double* rho, double* h) {
     Cconfiguration c;
-    c.openmp_number_of_threads = 1;
+    c.openmp_number_of_threads = g_threads;
     c.eproblem_type = enums::Eproblem_type::Fluid_flow;
     c.eflow_type_visc = visc_type ? enums::Eflow_type_visc::Inviscid : enums::Eflow_type_visc::Viscous;
     c.etimestepping_type = enums::Etimestepping_type::Transient;
@@ -175,7 +187,9 @@ int64_t ref_sod_run(int n_cells, double lx, double ly, double lz, int steps, int
     g_rec.dump_x = dump_x;
     g_rec.dump_method = dump_method;
 
+    const unsigned long long before = root.total_elapsed_iteration_time;
     root.run_config(c);
+    g_last_iter_us = (double)(root.total_elapsed_iteration_time - before) / ((double)steps * inner);
 
     const CMatrix_solver& S = *root.Matrix_solver;
     const int64_t nnz = (int64_t)S.matrix.value.size();

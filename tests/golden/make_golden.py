@@ -62,8 +62,7 @@ def solver_cases():
               abssum_b=np.array(oracle.ref().ref_abssum(ctypes.c_int64(a.n), b.ctypes.data_as(_f))))
 
 
-SOD = dict(n_cells=1000, lx=1.0, ly=0.1, lz=0.1, dt=1e-6, p_left=1e5, t_left=348.4, p_right=1e4,
-           t_right=278.7, conv_type=1, visc_type=1, inner=10)
+SOD = oracle.SOD_SETTINGS
 
 
 def sod_run(steps, solver_type, solver_mode, max_dump):
