@@ -177,6 +177,20 @@ ocsl_status ocsl_solve_multi(ocsl_handle h, int method, int precision, int k, co
 ocsl_status ocsl_CG_solve(ocsl_handle h, double* res);
 ocsl_status ocsl_ILU_BICGSTAB_solve(ocsl_handle h, double* res);
 
+/* CMatrix_solver::ILU_BICGSTAB_solve (cmatrix_solver.cpp:682-762) for k
+ * right-hand sides of the handle's matrix at once, in REFERENCE mode: each
+ * right-hand side's solution, residual (res[c]) and history are bitwise
+ * those of ocsl_ILU_BICGSTAB_solve with that right-hand side and guess,
+ * while the factor is computed once and every matrix, factor and vector
+ * pass serves all k.  Replaces consecutive calls on one matrix, such as the
+ * three momentum solves of CFlow_solver::iterate (cflow_solver.cpp:259-301)
+ * when no no-slip viscous wall couples the components.  B and X hold k
+ * column blocks of n values (B[c*n + i]); X holds the guesses on entry and
+ * the solutions on return.  1 <= k <= 4; res may be NULL.  The history
+ * (ocsl_get_history) is the k histories one after another. */
+ocsl_status ocsl_ILU_BICGSTAB_solve_multi(ocsl_handle h, int k, const double* B, double* X,
+                                          ocsl_where where, double* res);
+
 /* Residual history of the last solve: REFERENCE mode, the value each
  * iteration computes (||r'||^2 for CG, ||r||^2/||b||^2 + DBL_MIN for
  * BiCGSTAB); TOLERANCE mode, ||r_k||/||b|| for k = 0..iterations. */

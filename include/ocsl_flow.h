@@ -100,6 +100,15 @@ typedef struct {
     int support;       /* ocsl_flow_support: where the SIMPLE step's support loops run */
     ocsl_options solve_options;
     int cell_order;    /* ocsl_flow_cell_order: the order the device keeps cells in */
+    /* 1 (default): in REFERENCE mode with ILU_BICGSTAB for momentum, the three
+     * momentum systems are solved by one ocsl_ILU_BICGSTAB_solve_multi call
+     * when they share their matrix and none reads another's solution, that is
+     * unless a viscous flow has a no-slip wall
+     * (cflow_solver_cell_face_loops.cpp:94-113); the results are bitwise
+     * those of the three consecutive solves of CFlow_solver::iterate
+     * (cflow_solver.cpp:259-301).  0: always three solves.  Device support
+     * loops only. */
+    int momentum_together;
 } ocsl_flow_params;
 
 /* The order in which the device stores cells (and interior faces, by their

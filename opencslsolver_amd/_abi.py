@@ -95,6 +95,8 @@ SIGNATURES = {
                                         ctypes.POINTER(Options), ctypes.POINTER(Report)]),
     "ocsl_CG_solve": (ctypes.c_int, [c_handle, _f64p]),
     "ocsl_ILU_BICGSTAB_solve": (ctypes.c_int, [c_handle, _f64p]),
+    "ocsl_ILU_BICGSTAB_solve_multi": (ctypes.c_int, [c_handle, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                     ctypes.c_int, _f64p]),
     "ocsl_get_history": (ctypes.c_int, [c_handle, _f64p, ctypes.c_int, _i32p]),
     "ocsl_ABSsum": (ctypes.c_int, [c_handle, ctypes.c_void_p, ctypes.c_int, _f64p]),
     "ocsl_sum": (ctypes.c_int, [c_handle, ctypes.c_void_p, ctypes.c_int, _f64p]),

@@ -242,6 +242,7 @@ ocsl_status ocsl_create(ocsl_handle* out, int64_t n, const uint64_t* row_ptr,
     if (!h) return OCSL_ERR_NOMEM;
     ocsl_status st = guarded([&] {
         OCSL_HIP(hipGetDevice(&h->device));
+        OCSL_HIP(hipDeviceGetAttribute(&h->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
         OCSL_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         OCSL_HIP(hipEventCreate(&h->ev0));
         OCSL_HIP(hipEventCreate(&h->ev1));
@@ -412,6 +413,36 @@ ocsl_status ocsl_ILU_BICGSTAB_solve(ocsl_handle h, double* res) {
     ocsl_status st = ocsl_solve(h, OCSL_ILU_BICGSTAB, OCSL_MODE_REFERENCE, OCSL_FP64, nullptr, &rep);
     if (st >= 0 && res) *res = rep.residual;
     return st;
+}
+
+ocsl_status ocsl_ILU_BICGSTAB_solve_multi(ocsl_handle h, int k, const double* B, double* X, ocsl_where where,
+                                          double* res) {
+    if (!h || !B || !X || k < 1 || k > 4) return OCSL_ERR_ARG;
+    std::vector<ocsl_report> r(k);
+    std::memset(r.data(), 0, sizeof(ocsl_report) * k);
+    ocsl_status st = guarded([&] {
+        OCSL_REQUIRE(h->have_vals, OCSL_ERR_STATE, "values not set");
+        const size_t cnt = (size_t)h->host.n * k;
+        DBuf<double> bd, xd;
+        const double* Bd = B;
+        double* Xd = X;
+        if (where == OCSL_HOST) {
+            bd.upload(B, cnt, h->stream);
+            xd.upload(X, cnt, h->stream);
+            Bd = bd.p;
+            Xd = xd.p;
+        }
+        bicgstab_reference_multi(h, k, Bd, Xd, r.data());
+        if (where == OCSL_HOST) OCSL_HIP(hipMemcpyAsync(X, Xd, sizeof(double) * cnt, hipMemcpyDeviceToHost, h->stream));
+        sync(h);
+    });
+    if (st != OCSL_OK) return st;
+    int worst = 0;
+    for (int c = 0; c < k; ++c) {
+        worst |= r[c].status;
+        if (res) res[c] = r[c].residual;
+    }
+    return (ocsl_status)worst;
 }
 
 ocsl_status ocsl_get_history(ocsl_handle h, double* out, int cap, int* len) {

@@ -297,13 +297,28 @@ constexpr int kTriSerialLds = 65536;   // LDS bytes of a serial solve, at most
 inline size_t tri_serial_lds(int n, int w, int nr) {
     return (size_t)n * (16 * (size_t)nr + 12 * (size_t)w + 16);
 }
-template <class VM, class X, int NR, bool kLower, int kW>
+// Vectors of kS interleaved columns (v[i kS + c]); workgroup b solves
+// columns (b mod (kS / NR)) NR .. + NR - 1 of in[b / (kS / NR)] into the
+// same of out[...] (two vector pairs: the paired solve), so independent
+// right-hand sides run on separate compute units at once.
+template <class X>
+struct SerialIO {
+    const X* in[2];
+    X* out[2];
+};
+
+template <class VM, class X, int NR, int kS, bool kLower, int kW>
 __global__ void __launch_bounds__(kTriSerialBlock) k_tri_serial_ell(LevelEll t, const VM* __restrict__ ev,
-                                                                    const VM* __restrict__ dv, const X* in, X* out,
+                                                                    const VM* __restrict__ dv, SerialIO<X> io,
                                                                     const Scalars* S) {
     extern __shared__ double lds[];
     if (S && S->halt) return;
     const int n = t.n, w = t.w;
+    constexpr int kG = kS / NR;  // workgroups per vector
+    const int pair = blockIdx.x / kG;
+    const int col0 = (blockIdx.x % kG) * NR;
+    const X* __restrict__ in = io.in[pair];
+    X* __restrict__ out = io.out[pair];
     double* xs = lds;               // n NR: the solution, by row
     double* xin = xs + n * NR;      // n NR: the input, by position
     double* vv = xin + n * NR;      // w n: factor values, entry e of position k at e n + k
@@ -321,13 +336,13 @@ __global__ void __launch_bounds__(kTriSerialBlock) k_tri_serial_ell(LevelEll t, 
         rc[k] = t.cnt[k];
         vd[k] = kLower ? 1.0 : (double)dv[k];
 #pragma unroll
-        for (int c = 0; c < NR; ++c) xin[k * NR + c] = (double)in[(size_t)i * NR + c];
+        for (int c = 0; c < NR; ++c) xin[k * NR + c] = (double)in[(size_t)i * kS + col0 + c];
     }
     __syncthreads();
     // A chain: position 0 has no entry, every later position one, on the row
     // of the position before it (the factors of 1D meshes).
     int chain = 1;
-    if constexpr (kW == 1 && NR == 1) {
+    if constexpr (kW == 1) {
         for (int k = threadIdx.x; k < n; k += blockDim.x)
             if (rc[k] != (k > 0 ? 1 : 0) || (k > 0 && rj[k] != ri[k - 1])) chain = 0;
         chain = __syncthreads_and(chain);
@@ -336,43 +351,51 @@ __global__ void __launch_bounds__(kTriSerialBlock) k_tri_serial_ell(LevelEll t, 
     }
     if (chain && threadIdx.x == 0) {
         // y_k = x_k - m_k y_{k-1} (upper: then / d_k), one dependent
-        // multiply-add per row; the terms as in the general loop below
-        constexpr int kC = 8;
-        double pv;
-        {
-            const double v = xin[0];
+        // multiply-add per row and right-hand side (the NR recurrences are
+        // independent, so they overlap); the terms as in the general loop
+        constexpr int kC = NR == 1 ? 8 : 4;
+        double pv[NR];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+            const double v = xin[c];
             const X r0 = (X)(kLower ? v : v / vd[0]);
-            pv = (double)r0;
-            xs[ri[0]] = pv;
+            pv[c] = (double)r0;
+            xs[ri[0] * NR + c] = pv[c];
         }
+        auto step = [&](double x, double m, double d, int c) {
+            double sum = 0.0;
+            sum += m * pv[c];
+            const double v = x - sum;
+            const X r = (X)(kLower ? v : v / d);
+            pv[c] = (double)r;
+        };
         int k0 = 1;
         for (; k0 + kC <= n; k0 += kC) {
-            double xv[kC], mv[kC], dd[kC];
+            double xv[kC][NR], mv[kC], dd[kC];
             int rr[kC];
 #pragma unroll
             for (int u = 0; u < kC; ++u) {
-                xv[u] = xin[k0 + u];
+#pragma unroll
+                for (int c = 0; c < NR; ++c) xv[u][c] = xin[(k0 + u) * NR + c];
                 mv[u] = vv[k0 + u];
                 dd[u] = kLower ? 1.0 : vd[k0 + u];
                 rr[u] = ri[k0 + u];
             }
 #pragma unroll
             for (int u = 0; u < kC; ++u) {
-                double sum = 0.0;
-                sum += mv[u] * pv;
-                const double v = xv[u] - sum;
-                const X r = (X)(kLower ? v : v / dd[u]);
-                pv = (double)r;
-                xs[rr[u]] = pv;
+#pragma unroll
+                for (int c = 0; c < NR; ++c) {
+                    step(xv[u][c], mv[u], dd[u], c);
+                    xs[rr[u] * NR + c] = pv[c];
+                }
             }
         }
         for (; k0 < n; ++k0) {
-            double sum = 0.0;
-            sum += vv[k0] * pv;
-            const double v = xin[k0] - sum;
-            const X r = (X)(kLower ? v : v / vd[k0]);
-            pv = (double)r;
-            xs[ri[k0]] = pv;
+#pragma unroll
+            for (int c = 0; c < NR; ++c) {
+                step(xin[k0 * NR + c], vv[k0], kLower ? 1.0 : vd[k0], c);
+                xs[ri[k0] * NR + c] = pv[c];
+            }
         }
     }
     if (!chain && threadIdx.x == 0) {
@@ -440,18 +463,23 @@ __global__ void __launch_bounds__(kTriSerialBlock) k_tri_serial_ell(LevelEll t, 
         }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < n * NR; q += blockDim.x) out[q] = (X)xs[q];
+    for (int q = threadIdx.x; q < n * NR; q += blockDim.x) out[(size_t)(q / NR) * kS + col0 + q % NR] = (X)xs[q];
 }
 
 // Launches the serial form: lower or upper, the one-entry chain or up to
-// kTriEll entries per row.
+// kTriEll entries per row; NR interleaved right-hand sides take NR
+// workgroups of one each (a serial solve is bound by its dependent chain,
+// so separate compute units run them in the time of one).
+// A second vector (in1, out1: not null) is solved in the same launch.
 template <class VM, class X, int NR>
 void launch_tri_serial(const LevelEll& te, const VM* ev, const VM* dv, const X* in, X* out, bool lower,
-                       const Scalars* S, hipStream_t st) {
-    const size_t lds = tri_serial_lds(te.n, te.w, NR);
+                       const Scalars* S, hipStream_t st, const X* in1 = nullptr, X* out1 = nullptr) {
+    const size_t lds = tri_serial_lds(te.n, te.w, 1);
+    const SerialIO<X> io{{in, in1 ? in1 : in}, {out, out1 ? out1 : out}};
+    const int grid = in1 ? 2 * NR : NR;
     auto go = [&](auto L, auto W) {
-        k_tri_serial_ell<VM, X, NR, decltype(L)::value, decltype(W)::value>
-            <<<1, kTriSerialBlock, lds, st>>>(te, ev, dv, in, out, S);
+        k_tri_serial_ell<VM, X, 1, NR, decltype(L)::value, decltype(W)::value>
+            <<<grid, kTriSerialBlock, lds, st>>>(te, ev, dv, io, S);
     };
     using T = std::true_type;
     using F = std::false_type;
@@ -531,8 +559,9 @@ bool use_block_tri(const ocsl_solver* h, const Levels& L, int n) {
 // arrays within kTriSerialLds, and levels of 8 rows or fewer on average
 // (each serial row costs ~50 cycles of LDS and multiply-add latency, a
 // workgroup level ~0.35 us).
-bool use_serial_tri(const ocsl_solver* h, const Levels& L, int n, int nr) {
-    if (!L.ell_w || tri_serial_lds(n, L.ell_w, nr) > (size_t)kTriSerialLds) return false;
+bool use_serial_tri(const ocsl_solver* h, const Levels& L, int n) {
+    const size_t cap = std::min((size_t)kTriSerialLds, (size_t)h->lds_max);
+    if (!L.ell_w || tri_serial_lds(n, L.ell_w, 1) > cap) return false;
     if (h->tri_form != 0) return h->tri_form == 3;
     return (int64_t)L.n_levels * 8 >= n;
 }
@@ -582,7 +611,7 @@ void tri_exact(ocsl_solver* h, const VM* m, const X* in, X* out, bool lower, con
     level_values<VM>(h, L, m, &ev, &dv);
     if (L.ell_w) {
         const LevelEll te{L.rows.p, L.cnt.p, L.ecw.p, n, L.ell_w};
-        if (use_serial_tri(h, L, n, NR)) {
+        if (use_serial_tri(h, L, n)) {
             launch_tri_serial<VM, X, NR>(te, ev, dv, in, out, lower, S, h->stream);
         } else if (use_block_tri(h, L, n)) {
             k_tri_block_ell<VM, X, NR><<<1, kTriBlock, 0, h->stream>>>(te, L.ptr.p, L.n_levels, ev, dv, in, out, lower,
@@ -614,7 +643,14 @@ void tri_exact_pair(ocsl_solver* h, const VM* m, const double* in0, double* out0
                     bool lower, const Scalars* S) {
     Levels& L = lower ? h->lev_fwd : h->lev_bwd;
     const int n = h->host.n;
-    if (use_serial_tri(h, L, n, 1) || use_block_tri(h, L, n)) {
+    if (use_serial_tri(h, L, n)) {
+        const VM *ev, *dv;
+        level_values<VM>(h, L, m, &ev, &dv);
+        const LevelEll te{L.rows.p, L.cnt.p, L.ecw.p, n, L.ell_w};
+        launch_tri_serial<VM, double, 1>(te, ev, dv, in0, out0, lower, S, h->stream, in1, out1);
+        return;
+    }
+    if (use_block_tri(h, L, n)) {
         tri_exact<VM, double, 1>(h, m, in0, out0, lower, S);
         tri_exact<VM, double, 1>(h, m, in1, out1, lower, S);
         return;
@@ -805,6 +841,269 @@ void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
     const double b_spmv = nnz * 12.0 + 16.0 * n;
     const double b_tri = nnz * 12.0 + 8.0 * n + 16.0 * n;
     rep->bytes = it * (2.0 * b_spmv + 3.0 * b_tri + 20.0 * 8.0 * n);
+}
+
+// ------------------------------------------------------------------ reference mode, k right-hand sides
+// ILU_BICGSTAB_solve (cmatrix_solver.cpp:682-762) for k right-hand sides of
+// one matrix at once: the factor is computed once (every call of the
+// reference computes the same one from the same values), and every matrix,
+// factor and vector pass serves all k, held interleaved (v[i k + c]).  Each
+// right-hand side keeps its own scalars (S[c]) and takes exactly the steps,
+// terms and reduction orders of the one-vector solve above, so its results
+// are bitwise those of that solve.  The triangular solves run for every
+// right-hand side; one whose solve has halted computes values that nothing
+// reads (its x and scalars are frozen).
+namespace {
+
+constexpr size_t kNoCoef = ~(size_t)0;
+
+__global__ void k_set_scalars_nr(Scalars* S, Scalars v, int k) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < k) S[threadIdx.x] = v;
+}
+
+// k_bref_scalars for S[c] (thread c); history entry it of c at hist[3 c + it].
+__global__ void k_bref_scalars_nr(Scalars* S, double* hist, int step, int it, int k) {
+    const int c = threadIdx.x;
+    if (blockIdx.x != 0 || c >= k) return;
+    Scalars* sc = S + c;
+    if (sc->halt) return;
+    if (step == 0) {
+        sc->alpha = sc->aux[0] / (sc->aux[1] + DBL_MIN);
+    } else if (step == 1) {
+        const double res = sc->aux[2] / sc->aux[3] + DBL_MIN;
+        sc->rr = res;
+        hist[3 * c + it] = res;
+        sc->iter = it + 1;
+        if (res < 1.0e-20) sc->halt = 1;
+    } else {
+        sc->omega = sc->aux[4] / (sc->aux[5] + 1.0e-40);
+    }
+}
+
+__device__ __forceinline__ double scalar_at(const Scalars* s, size_t off) {
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(s) + off);
+}
+
+// k_axpby_s per right-hand side: out = (ca a_c) x + (cb b_c) y, a_c and b_c
+// the fields at byte offsets oa, ob of S[c] (kNoCoef: 1).
+template <int NR>
+__global__ void k_axpby_nr(int n, double ca, size_t oa, const double* __restrict__ x, double cb, size_t ob,
+                           const double* __restrict__ y, double* out, const Scalars* S) {
+    double a[NR], b[NR];
+    bool on[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+        on[c] = !S[c].halt;
+        a[c] = oa != kNoCoef ? ca * scalar_at(S + c, oa) : ca;
+        b[c] = ob != kNoCoef ? cb * scalar_at(S + c, ob) : cb;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+#pragma unroll
+        for (int c = 0; c < NR; ++c)
+            if (on[c]) {
+                const size_t q = (size_t)i * NR + c;
+                out[q] = a[c] * x[q] + b[c] * y[q];
+            }
+}
+
+// k_dots with K products per right-hand side (kind 0, a.b) of interleaved
+// vectors: the same block and thread decomposition, sum c K + j in slot
+// c K + j.
+template <int NR, int K>
+__global__ void __launch_bounds__(kBlock) k_dots_nr(int n, DotArgs<K> args, RedScratch red) {
+    double acc[NR * K];
+#pragma unroll
+    for (int r = 0; r < NR * K; ++r) acc[r] = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c)
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const size_t q = (size_t)i * NR + c;
+                const double av = args.a[j][q];
+                acc[c * K + j] += args.kind[j] == 0 ? av * args.b[j][q] : (args.kind[j] == 1 ? av : fabs(av));
+            }
+    }
+    block_partials<NR * K>(acc, red);
+}
+
+// Totals c K + j to S[c].aux[slot + j].
+template <int NR, int K>
+struct FinAux {
+    Scalars* S;
+    int slot;
+    __device__ bool active() const { return true; }
+    __device__ void operator()(const double* t) const {
+        for (int c = 0; c < NR; ++c)
+            for (int j = 0; j < K; ++j) S[c].aux[slot + j] = t[c * K + j];
+    }
+};
+
+// k_sell_axpby of interleaved vectors: out = a (A x) + b z, z may be null.
+template <int NR>
+__global__ void __launch_bounds__(kBlock) k_sell_axpby_nr(SellView<double> A, double a, const double* __restrict__ x,
+                                                          double b, const double* __restrict__ z,
+                                                          double* __restrict__ out) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * kWavesPerBlock;
+    for (int s = wave; s < A.n_slices; s += nwaves) {
+        const int row = s * kWave + lane;
+        double ax[NR];
+        sell_rows_nr<double, double, NR>(A, s, lane, row, x, ax);
+        if (row < A.n)
+#pragma unroll
+            for (int c = 0; c < NR; ++c) {
+                const size_t q = (size_t)row * NR + c;
+                double o = (a == 1.0) ? ax[c] : a * ax[c];
+                if (z) o += b * z[q];
+                out[q] = o;
+            }
+    }
+}
+
+template <int NR>
+void bicgstab_reference_nr(ocsl_solver* h, const double* b, double* x, ocsl_report* reps) {
+    const int n = h->host.n;
+    ensure_csr_values(h, false);
+    ensure_levels(h);
+    Sell& s = sell_for(h, false);
+    ensure_sell_values(h, s, true, false);
+    Work& w = h->work;
+    const size_t nk = (size_t)n * NR;
+    for (int i = 0; i < 11; ++i) work_buf(h, w.d[i], nk);
+    double *r = w.d[0].p, *p = w.d[1].p, *v = w.d[2].p, *r0 = w.d[3].p, *sv = w.d[4].p, *tmp = w.d[5].p;
+    double *hh = w.d[6].p, *t = w.d[7].p, *y = w.d[8].p, *z = w.d[9].p, *lts = w.d[10].p;
+    Scalars* S = work_buf(h, h->S_multi, NR);
+    double* hist_d = work_buf(h, h->hist, 3 * NR);
+    const int gv = grid_for(n), gk = grid_for((int64_t)nk);
+    auto spmv = [&](double a, const double* xin, double bz, const double* zin, double* out) {
+        k_sell_axpby_nr<NR><<<grid_for((int64_t)s.n_slices * kWave), kBlock, 0, h->stream>>>(s.view<double>(), a,
+                                                                                             xin, bz, zin, out);
+    };
+    const int gd = red_blocks(kBlock * 4, n);
+    auto dot1 = [&](const double* a1, const double* b1, int slot) {
+        DotArgs<1> d{{a1}, {b1}, {0}};
+        k_dots_nr<NR, 1><<<gd, kBlock, 0, h->stream>>>(n, d, h->red());
+        k_finalize<NR, FinAux<NR, 1>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinAux<NR, 1>{S, slot});
+    };
+    auto dot2 = [&](const double* a1, const double* b1, const double* a2, const double* b2, int slot) {
+        DotArgs<2> d{{a1, a2}, {b1, b2}, {0, 0}};
+        k_dots_nr<NR, 2><<<gd, kBlock, 0, h->stream>>>(n, d, h->red());
+        k_finalize<2 * NR, FinAux<NR, 2>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinAux<NR, 2>{S, slot});
+    };
+    auto upd = [&](double ca, size_t oa, const double* xa, double cb, size_t ob, const double* ya, double* out) {
+        k_axpby_nr<NR><<<gv, kBlock, 0, h->stream>>>(n, ca, oa, xa, cb, ob, ya, out, S);
+    };
+    const size_t alpha = offsetof(Scalars, alpha), omega = offsetof(Scalars, omega);
+    auto lower = [&](const double* in, double* out) { tri_exact<double, double, NR>(h, h->ilu_m.p, in, out, true, nullptr); };
+    auto upper = [&](const double* in, double* out) {
+        tri_exact<double, double, NR>(h, h->ilu_m.p, in, out, false, nullptr);
+    };
+    // two independent lower solves: one launch in the serial form
+    auto lower_pair = [&](const double* in0, double* out0, const double* in1, double* out1) {
+        Levels& L = h->lev_fwd;
+        if (use_serial_tri(h, L, n)) {
+            const double *ev, *dv;
+            level_values<double>(h, L, h->ilu_m.p, &ev, &dv);
+            const LevelEll te{L.rows.p, L.cnt.p, L.ecw.p, n, L.ell_w};
+            launch_tri_serial<double, double, NR>(te, ev, dv, in0, out0, true, nullptr, h->stream, in1, out1);
+        } else {
+            lower(in0, out0);
+            lower(in1, out1);
+        }
+    };
+    Scalars init{};
+    init.alpha = 1.0;
+    init.omega = 1.0;
+    OCSL_HIP(hipEventRecord(h->ev0, h->stream));
+    auto body = [&] {
+        k_set_scalars_nr<<<1, 64, 0, h->stream>>>(S, init, NR);
+        ilu_decompose_device(h, 3, false);
+        k_fill<double><<<gk, kBlock, 0, h->stream>>>((int)nk, 0.0, p);
+        k_fill<double><<<gk, kBlock, 0, h->stream>>>((int)nk, 0.0, v);
+        spmv(-1.0, x, 1.0, b, r);
+        k_permute_in<double><<<gk, kBlock, 0, h->stream>>>((int)nk, nullptr, r, r0);
+        for (int it = 0; it < 3; ++it) {
+            dot1(r0, r, 0);
+            upd(1.0, kNoCoef, p, -1.0, omega, v, tmp);
+            upd(1.0, kNoCoef, r, 1.0, omega, tmp, p);
+            lower(p, tmp);
+            upper(tmp, y);
+            spmv(1.0, y, 0.0, nullptr, v);
+            dot1(r0, v, 1);
+            k_bref_scalars_nr<<<1, 64, 0, h->stream>>>(S, hist_d, 0, it, NR);
+            upd(1.0, kNoCoef, x, 1.0, alpha, y, hh);
+            dot2(r, r, b, b, 2);
+            k_bref_scalars_nr<<<1, 64, 0, h->stream>>>(S, hist_d, 1, it, NR);
+            upd(-1.0, alpha, v, 1.0, kNoCoef, r, sv);
+            lower(sv, tmp);
+            upper(tmp, z);
+            spmv(1.0, z, 0.0, nullptr, t);
+            lower_pair(t, tmp, sv, lts);
+            dot2(tmp, lts, tmp, tmp, 4);
+            k_bref_scalars_nr<<<1, 64, 0, h->stream>>>(S, hist_d, 2, it, NR);
+            upd(1.0, kNoCoef, hh, 1.0, omega, z, x);
+            upd(-1.0, omega, t, 1.0, kNoCoef, sv, r);
+        }
+        OCSL_HIP(hipGetLastError());
+    };
+    run_reference(h, 3, b, body, NR);
+    ++h->ilu_epoch;
+    OCSL_HIP(hipEventRecord(h->ev1, h->stream));
+    std::vector<Scalars> Sh(NR);
+    std::vector<double> hist(3 * NR);
+    OCSL_HIP(hipMemcpyAsync(Sh.data(), S, sizeof(Scalars) * NR, hipMemcpyDeviceToHost, h->stream));
+    OCSL_HIP(hipMemcpyAsync(hist.data(), hist_d, sizeof(double) * 3 * NR, hipMemcpyDeviceToHost, h->stream));
+    sync(h);
+    float ms = 0.f;
+    OCSL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    const double nnz = (double)h->host.col.size();
+    const double b_spmv = nnz * 12.0 + 16.0 * n * NR;
+    const double b_tri = nnz * 12.0 + 8.0 * n + 16.0 * n * NR;
+    // the history of the last call: right-hand side after right-hand side
+    h->hist_host.clear();
+    for (int c = 0; c < NR; ++c) {
+        const int it = Sh[c].iter;
+        const double res = Sh[c].rr;
+        ocsl_report& rep = reps[c];
+        rep.status = std::isfinite(res) ? OCSL_OK : OCSL_WARN_NONFINITE;
+        rep.iterations = it;
+        rep.residual = res;
+        rep.history_len = it;
+        rep.solve_ms = ms;
+        rep.true_relres = std::nan("");  // not computed: the call returns the reference's residual
+        // each right-hand side's share of the passes its iterations ran in
+        rep.bytes = it * (2.0 * b_spmv + 4.0 * b_tri + 20.0 * 8.0 * n * NR) / NR;
+        for (int i = 0; i < it; ++i) h->hist_host.push_back(hist[3 * c + i]);
+    }
+    h->hist_len = (int)h->hist_host.size();
+}
+
+}  // namespace
+
+template <class T>
+__global__ void k_interleave(int n, int k, const T* __restrict__ in, T* __restrict__ out, int to_rows);
+
+// B_dev, X_dev: k column blocks of n (X: guesses in, solutions out).
+void bicgstab_reference_multi(ocsl_solver* h, int k, const double* B_dev, double* X_dev, ocsl_report* reps) {
+    OCSL_REQUIRE(k >= 1 && k <= 4, OCSL_ERR_ARG, "1 to 4 right-hand sides");
+    const int n = h->host.n;
+    // interleaved copies at fixed addresses (the solve's graph reads them)
+    double* b_il = work_buf(h, h->work.d[14], (size_t)n * k);
+    double* x_il = work_buf(h, h->work.d[15], (size_t)n * k);
+    const int g = grid_for(n);
+    k_interleave<double><<<g, kBlock, 0, h->stream>>>(n, k, B_dev, b_il, 1);
+    k_interleave<double><<<g, kBlock, 0, h->stream>>>(n, k, X_dev, x_il, 1);
+    OCSL_HIP(hipGetLastError());
+    switch (k) {
+        case 1: bicgstab_reference_nr<1>(h, b_il, x_il, reps); break;
+        case 2: bicgstab_reference_nr<2>(h, b_il, x_il, reps); break;
+        case 3: bicgstab_reference_nr<3>(h, b_il, x_il, reps); break;
+        default: bicgstab_reference_nr<4>(h, b_il, x_il, reps); break;
+    }
+    k_interleave<double><<<g, kBlock, 0, h->stream>>>(n, k, x_il, X_dev, 0);
+    OCSL_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ tolerance mode
@@ -1582,7 +1881,7 @@ extern "C" ocsl_status ocsl_time_tri_solve_form(ocsl_handle h, int lower, int fo
         const int n = h->host.n;
         ensure_csr_values(h, false);
         ensure_levels(h);
-        OCSL_REQUIRE(form != 3 || use_serial_tri(h, lower ? h->lev_fwd : h->lev_bwd, n, 1), OCSL_ERR_ARG,
+        OCSL_REQUIRE(form != 3 || use_serial_tri(h, lower ? h->lev_fwd : h->lev_bwd, n), OCSL_ERR_ARG,
                      "the serial form does not fit this system");
         ilu_decompose_device(h, 3, false);
         DBuf<double> in, out;

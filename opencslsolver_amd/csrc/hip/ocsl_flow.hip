@@ -725,6 +725,7 @@ struct ocsl_flow_s {
     DBuf<double> d_gpc, d_mach, d_b_p, d_b_p_out;
     // assembled system
     DBuf<double> d_val, d_b, d_x0;
+    DBuf<double> d_b3, d_x3;  // momentum right-hand sides and solutions, solved together
     int64_t nnz_in = 0;
     std::vector<double> x_host, hist;
     bool initialized = false;
@@ -1185,7 +1186,7 @@ void assemble(ocsl_flow_s* fl, int eq) {
         down_c(fl, fl->st.ap, fl->d_ap, st);
         down_c(fl, fl->st.ap_vec, fl->d_ap_vec, st);
     }
-    OCSL_HIP(hipStreamSynchronize(st));
+    if (!fl->dev) OCSL_HIP(hipStreamSynchronize(st));  // the host reads ap, ap_vec
     ocsl_solver* h = fl->solver;
     h->val_in = fl->d_val.p;
     h->have_vals = true;
@@ -1235,6 +1236,19 @@ void solve(ocsl_flow_s* fl, int method) {
     if (st < 0) throw Error{st, std::string("flow solve: ") + ocsl_last_error()};
 }
 
+// The three momentum systems share one matrix and none reads another's
+// solution unless a viscous flow has a no-slip wall, whose terms couple the
+// components (cflow_solver_cell_face_loops.cpp:94-113); then one REFERENCE
+// solve of three right-hand sides replaces the three consecutive ones.
+bool momentum_together(const ocsl_flow_s* fl) {
+    const ocsl_flow_params& p = fl->prm;
+    if (!p.momentum_together || p.mode != OCSL_MODE_REFERENCE || p.method_momentum != OCSL_ILU_BICGSTAB) return false;
+    if (!p.viscous) return true;
+    for (int k = 0; k < p.n_bc; ++k)
+        if (p.bc[k].type == OCSL_BC_WALL && p.bc[k].no_slip) return false;
+    return true;
+}
+
 // CFlow_solver::iterate (cflow_solver.cpp:229-476) with the state resident in
 // device memory: the support loops as kernels on the solver's stream, the
 // solutions read from the handle's x.
@@ -1252,11 +1266,29 @@ void iterate_dev(ocsl_flow_s* fl) {
     k_fill<double><<<grid_for(n), kBlock, 0, st>>>(n, 0.0, s.ap);
     OCSL_HIP(hipGetLastError());
     if (fl->prm.viscous) viscous_stress(fl);
-    for (int d = 0; d < 3; ++d) {
-        assemble(fl, OCSL_EQ_MOMENTUM_X + d);
-        solve(fl, fl->prm.method_momentum);
-        x = fl->solver->x_own.p;
-        k_take_u<<<grid(n), kBlock, 0, st>>>(s, x, m.cperm, d, n);
+    if (momentum_together(fl)) {
+        // each component's right-hand side and guess into a column of B3, X3
+        double* B3 = (fl->d_b3.alloc(3 * (size_t)n), fl->d_b3.p);
+        double* X3 = (fl->d_x3.alloc(3 * (size_t)n), fl->d_x3.p);
+        const size_t bytes = sizeof(double) * n;
+        for (int d = 0; d < 3; ++d) {
+            assemble(fl, OCSL_EQ_MOMENTUM_X + d);
+            OCSL_HIP(hipMemcpyAsync(B3 + (size_t)d * n, fl->d_b.p, bytes, hipMemcpyDeviceToDevice, st));
+            OCSL_HIP(hipMemcpyAsync(X3 + (size_t)d * n, fl->d_x0.p, bytes, hipMemcpyDeviceToDevice, st));
+        }
+        ocsl_report reps[3];
+        bicgstab_reference_multi(fl->solver, 3, B3, X3, reps);
+        for (int d = 0; d < 3; ++d) {
+            fl->hist.push_back(reps[d].residual);
+            k_take_u<<<grid(n), kBlock, 0, st>>>(s, X3 + (size_t)d * n, m.cperm, d, n);
+        }
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            assemble(fl, OCSL_EQ_MOMENTUM_X + d);
+            solve(fl, fl->prm.method_momentum);
+            x = fl->solver->x_own.p;
+            k_take_u<<<grid(n), kBlock, 0, st>>>(s, x, m.cperm, d, n);
+        }
     }
     if (F) k_face_mdot<<<grid(F), kBlock, 0, st>>>(m, s, blend);
     if (B) k_bc_mdot<<<grid(B), kBlock, 0, st>>>(m, s, ph);
@@ -1494,6 +1526,7 @@ void ocsl_flow_default_params(ocsl_flow_params* p) {
     p->mode = OCSL_MODE_REFERENCE;
     p->precision = OCSL_FP64;
     ocsl_default_options(&p->solve_options);
+    p->momentum_together = 1;
 }
 
 ocsl_status ocsl_flow_create(ocsl_flow* out, const ocsl_mesh* mesh, const uint64_t* row_ptr,

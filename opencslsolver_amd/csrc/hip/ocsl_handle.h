@@ -150,6 +150,7 @@ struct ocsl_solver {
     // over the rows staged in LDS (ocsl_set_tri_form; the timing hook sets it
     // for the duration of a measurement)
     int tri_form = 0;
+    int lds_max = 65536;  // LDS bytes one workgroup may use on this device
     ocsl::IluTables ilu;
     bool ilu_built = false;
     ocsl::DBuf<double> ilu_m, ilu_old;
@@ -158,6 +159,7 @@ struct ocsl_solver {
 
     ocsl::Work work;
     ocsl::DBuf<ocsl::Scalars> S;
+    ocsl::DBuf<ocsl::Scalars> S_multi;  // per right-hand side: REFERENCE BiCGSTAB of k together
     ocsl::Scalars* S_host = nullptr;  // pinned mirror
     ocsl::DBuf<double> red_partial;
     ocsl::DBuf<double> dscal;          // small scalar outputs of k_dots
@@ -227,12 +229,12 @@ inline T* work_buf(ocsl_solver* h, DBuf<T>& b, size_t count) {
 // graph that later solves with the same address replay.  Returns after the
 // work is issued.
 template <class F>
-void run_reference(ocsl_solver* h, int family, const void* b, F&& body) {
+void run_reference(ocsl_solver* h, int family, const void* b, F&& body, int k = 0) {
     if (!h->ref_graphs) {
         body();
         return;
     }
-    const GraphKey key = reference_graph_key(family, b);
+    const GraphKey key = reference_graph_key(family, b, k);
     auto it = h->graphs.find(key);
     if (it != h->graphs.end()) {
         OCSL_HIP(hipGraphLaunch(it->second, h->stream));
@@ -275,6 +277,7 @@ inline int grid_for(int64_t n, int per_block = kBlock, int cap = 4096) {
 void cg_reference(ocsl_solver* h, ocsl_report* rep);
 void cg_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report* rep);
 void bicgstab_reference(ocsl_solver* h, ocsl_report* rep);
+void bicgstab_reference_multi(ocsl_solver* h, int k, const double* B_dev, double* X_dev, ocsl_report* reps);
 void bicgstab_tolerance(ocsl_solver* h, bool mixed, const ocsl_options& o, ocsl_report* rep);
 // The reference's ILU sweeps from M^0 = m0 (device, real-entry order), or
 // from A when m0 is null; the result lands in h->ilu_m.

@@ -169,9 +169,12 @@ inline GraphKey cg_graph_key(int kind, const CgLaunch& L, int chunk, int rpl) {
 // whole solve is one graph; it bakes in the right-hand side's address, the
 // one caller buffer a solve reads that can change between solves (bit 50
 // set: no key of the other families).
-inline GraphKey reference_graph_key(int family, const void* b) {
-    if (family < 1 || family > 2) throw Error{OCSL_ERR_ARG, "graph key: reference family"};
-    return {(int64_t)1 << 50 | family, 0, (uint64_t)(uintptr_t)b};
+// Family 3, the REFERENCE BiCGSTAB of k right-hand sides together, also
+// keys on k (1..4).
+inline GraphKey reference_graph_key(int family, const void* b, int k = 0) {
+    if (family < 1 || family > 3) throw Error{OCSL_ERR_ARG, "graph key: reference family"};
+    if (family == 3 ? (k < 1 || k > 4) : k != 0) throw Error{OCSL_ERR_ARG, "graph key: right-hand sides"};
+    return {(int64_t)1 << 50 | family, k, (uint64_t)(uintptr_t)b};
 }
 
 // Tolerance BiCGSTAB: its iteration form (< 2^40), the chunk length and the

@@ -58,7 +58,8 @@ class Params(ctypes.Structure):
                 ("t2", ctypes.c_double), ("n_bc", ctypes.c_int), ("bc", BC * MAX_BC),
                 ("method_momentum", ctypes.c_int), ("method_pressure", ctypes.c_int),
                 ("method_enthalpy", ctypes.c_int), ("mode", ctypes.c_int), ("precision", ctypes.c_int),
-                ("support", ctypes.c_int), ("solve_options", _abi.Options), ("cell_order", ctypes.c_int)]
+                ("support", ctypes.c_int), ("solve_options", _abi.Options), ("cell_order", ctypes.c_int),
+                ("momentum_together", ctypes.c_int)]
 
 
 CELL_FIELDS = {"u": 3, "u_old": 3, "rho": 1, "rho_old": 1, "p": 1, "p_old": 1, "p_corr": 1, "grad_p": 3,

@@ -205,6 +205,23 @@ class Solver:
         _check(self._lib.ocsl_ILU_BICGSTAB_solve(self._h, ctypes.byref(res)), allow_warn=True)
         return res.value
 
+    def ILU_BICGSTAB_solve_multi(self, B, X0):
+        """ILU_BICGSTAB_solve for the k rows of B (shape (k, n)) at once, X0
+        the k guesses: each row's solution and residual are bitwise those of
+        ILU_BICGSTAB_solve with that right-hand side and guess.  Returns (X,
+        residuals)."""
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        k = B.shape[0]
+        X = np.array(X0, dtype=np.float64, order="C")
+        if X.shape != B.shape:
+            raise ValueError("X0 must have the shape of B")
+        res = np.zeros(k)
+        st = self._lib.ocsl_ILU_BICGSTAB_solve_multi(self._h, k, B.ctypes.data_as(ctypes.c_void_p),
+                                                     X.ctypes.data_as(ctypes.c_void_p), _abi.HOST,
+                                                     res.ctypes.data_as(_f64p))
+        _check(st, allow_warn=True)
+        return X, res
+
     def history(self) -> np.ndarray:
         ln = ctypes.c_int()
         _check(self._lib.ocsl_get_history(self._h, None, 0, ctypes.byref(ln)))
@@ -426,13 +443,26 @@ class Batch:
         _check(self._lib.ocsl_batch_set_rhs(self._h, ptrs, _abi.HOST))
 
     def solve(self, precision="fp64", x0=None, **opts):
-        """Returns (solutions, reports, latency_ms)."""
-        xs = [np.zeros(n) if x0 is None else np.array(x0[k], dtype=np.float64) for k, n in enumerate(self.n)]
-        ptrs = (ctypes.c_void_p * self.m)(*[x.ctypes.data_as(ctypes.c_void_p) for x in xs])
+        """Returns (solutions, reports, latency_ms).  x0: None (zeros), host
+        arrays (copied), or torch CUDA float64 tensors, which hold the
+        initial guesses on entry and are overwritten by the solutions (the
+        returned solutions are those tensors)."""
+        if x0 is not None and all(_is_torch_cuda(x) for x in x0):
+            xs = list(x0)
+            for x, n in zip(xs, self.n):
+                _ptr(x)  # checks dtype and layout, orders torch's stream before the call
+                if x.numel() != n:
+                    raise ValueError("x0 sizes must match the systems")
+            where = _abi.DEVICE
+            ptrs = (ctypes.c_void_p * self.m)(*[x.data_ptr() for x in xs])
+        else:
+            xs = [np.zeros(n) if x0 is None else np.array(x0[k], dtype=np.float64) for k, n in enumerate(self.n)]
+            where = _abi.HOST
+            ptrs = (ctypes.c_void_p * self.m)(*[x.ctypes.data_as(ctypes.c_void_p) for x in xs])
         o = default_options(**opts)
         reps = (_abi.Report * self.m)()
         lat = np.zeros(self.m)
-        st = self._lib.ocsl_batch_solve(self._h, PRECISIONS[precision], ctypes.byref(o), ptrs, _abi.HOST,
+        st = self._lib.ocsl_batch_solve(self._h, PRECISIONS[precision], ctypes.byref(o), ptrs, where,
                                         reps, lat.ctypes.data_as(_f64p))
         _check(st, allow_warn=True)
         return xs, [r.as_dict() for r in reps], lat

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <set>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ocsl_launch.h"
@@ -168,6 +169,18 @@ void check_keys() {
     for (int fam : {1, 2})
         for (const void* b : {(const void*)b0, (const void*)b1, (const void*)nullptr})
             if (!keys.insert(reference_graph_key(fam, b)).second) fail("REFERENCE key collides");
+    for (int k = 1; k <= 4; ++k)
+        for (const void* b : {(const void*)b0, (const void*)b1})
+            if (!keys.insert(reference_graph_key(3, b, k)).second) fail("REFERENCE k-RHS key collides");
+    for (auto fk : {std::pair<int, int>{3, 0}, {3, 5}, {1, 2}, {4, 1}}) {
+        bool threw = false;
+        try {
+            reference_graph_key(fk.first, b0, fk.second);
+        } catch (const Error&) {
+            threw = true;
+        }
+        if (!threw) fail("reference_graph_key accepted family " + std::to_string(fk.first));
+    }
     for (int kind : {-1, 256}) {
         bool threw = false;
         try {

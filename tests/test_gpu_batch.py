@@ -34,6 +34,29 @@ def test_batch_matches_single_solves(precision):
         assert lat[k] > 0
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_device_resident_guesses(precision):
+    """Guesses and solutions in device memory (torch tensors, solved in
+    place) give bitwise the solutions of host arrays."""
+    import torch
+
+    sysl = _systems([7, 13, 10, 19])
+    rng = np.random.default_rng(5)
+    x0 = [rng.standard_normal(a.n) * 0.1 for a, _ in sysl]
+    with Batch([(a.row_ptr, a.col) for a, _ in sysl]) as bt:
+        bt.set_values([a.val for a, _ in sysl])
+        bt.set_rhs([b for _, b in sysl])
+        xs, reps, _ = bt.solve(precision, x0=x0, tol=1e-8, check_every=8)
+        xd = [torch.tensor(x, dtype=torch.float64, device="cuda") for x in x0]
+        out, reps_d, _ = bt.solve(precision, x0=xd, tol=1e-8, check_every=8)
+        assert all(o is t for o, t in zip(out, xd))
+        for k in range(len(sysl)):
+            assert np.array_equal(xs[k], xd[k].cpu().numpy()), k
+            assert reps_d[k]["iterations"] == reps[k]["iterations"]
+        with pytest.raises(ValueError):
+            bt.solve(precision, x0=[x[:-1] for x in xd], tol=1e-8)
+
+
 def test_batch_zero_rhs_and_reuse():
     sysl = _systems([5, 8, 12])
     bs = [b for _, b in sysl]

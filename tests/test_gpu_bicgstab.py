@@ -345,6 +345,56 @@ def test_serial_triangular_form_inside_the_solvers():
         assert np.linalg.norm(B[k] - a.matvec(res[0][2][k])) <= 1e-10 * np.linalg.norm(B[k])
 
 
+@pytest.mark.parametrize("kind,k", [("chain", 3), ("chain", 1), ("box", 3), ("perm32", 4), ("perm32", 2)])
+def test_reference_multi_rhs_equals_single_solves(kind, k):
+    """ocsl_ILU_BICGSTAB_solve_multi: each right-hand side's solution,
+    residual and history bitwise those of ocsl_ILU_BICGSTAB_solve with that
+    right-hand side and guess, on the 1000-row chain (serial triangular
+    form), the randomly numbered 8^3 box (one workgroup) and 32^3 system (a
+    launch per level).  The chain cases include a zero right-hand side (the
+    reference's 0/0 residual) and a guess that is already the solution (the
+    reference's early exit); a second call replays the captured graph."""
+    if kind == "chain":
+        a = gen.chain1d(1000, 5)
+    else:
+        a = gen.convdiff3d(8 if kind == "box" else 32, seed_perm=3)[0]
+    rng = np.random.default_rng(17)
+    B = rng.uniform(-1.0, 1.0, (k, a.n))
+    X0 = rng.standard_normal((k, a.n)) * 0.01
+    if kind == "chain" and k == 3:
+        B[1] = 0.0
+        X0[1] = 0.0
+        dense = np.zeros((a.n, a.n))
+        for i in range(a.n):
+            for q in range(int(a.row_ptr[i]), int(a.row_ptr[i + 1])):
+                if int(a.col[q]) < a.n:
+                    dense[i, int(a.col[q])] += a.val[q]
+        X0[2] = np.linalg.solve(dense, B[2])
+    singles = []
+    with Solver(a.row_ptr, a.col) as s:
+        s.set_values(a.val)
+        for c in range(k):
+            s.set_rhs(B[c])
+            s.set_x(X0[c])
+            r = s.ILU_BICGSTAB_solve()
+            singles.append((s.get_x(), r, s.history()))
+        X, res = s.ILU_BICGSTAB_solve_multi(B, X0)
+        hist = s.history()
+        X2, res2 = s.ILU_BICGSTAB_solve_multi(B, X0)
+    assert np.array_equal(X, X2, equal_nan=True) and np.array_equal(res, res2, equal_nan=True)
+    at = 0
+    for c in range(k):
+        x1, r1, h1 = singles[c]
+        assert np.array_equal(X[c], x1, equal_nan=True), c
+        assert np.array_equal(res[c], r1, equal_nan=True), (c, res[c], r1)
+        assert np.array_equal(hist[at:at + len(h1)], h1, equal_nan=True), c
+        at += len(h1)
+    assert at == len(hist)
+    if kind == "chain" and k == 3:
+        assert len(singles[2][2]) == 1  # the early exit after one residual
+        assert np.isnan(res[1])
+
+
 def test_graph_cache_follows_the_jacobi_weight():
     """TOLERANCE BiCGSTAB with damped-Jacobi triangular sweeps replays its
     iteration chunks as graphs whose sweeps take the weight as an argument: a

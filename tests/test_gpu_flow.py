@@ -157,6 +157,57 @@ def test_flow_3d_device_run_matches_host(viscous, blended, transient):
         assert _close(dev[key], host[key], 1e-9), key
 
 
+@pytest.mark.parametrize("viscous,walls", [(0, "no_slip"), (1, "slip")])
+def test_momentum_solved_together_changes_no_bit(viscous, walls):
+    """The three momentum systems solved by one REFERENCE solve of three
+    right-hand sides (the default where no no-slip wall of a viscous flow
+    couples them) against three consecutive solves: bitwise the same
+    fields and per-solve residuals, on a permuted 8^3 hex mesh."""
+    rng = np.random.default_rng(9)
+    n = 8
+    perm = rng.permutation(n ** 3)
+    mesh, rp, col = gen_hex(n, perm=perm)
+    bcs = [dict(type=flow.VELOCITY_INLET, velocity=(3.0, 0.5, -0.2), temperature=300.0),
+           dict(type=flow.PRESSURE_OUTLET, pressure=1e5),
+           dict(type=flow.WALL, no_slip=1 if walls == "no_slip" else 0), dict(type=flow.WALL, no_slip=0),
+           dict(type=flow.SYMMETRY), dict(type=flow.SYMMETRY)]
+    out = {}
+    for together in (0, 1):
+        prm = flow.default_params(viscous=viscous, blended=1, transient=1, dt=1e-4, p0=1e5, t0=300.0,
+                                  u0=(2.0, 0.0, 0.0), bcs=bcs, method_pressure=0, momentum_together=together)
+        with flow.Flow(mesh, rp, col, prm) as fl:
+            fl.initialize()
+            fl.run(2, 2)
+            out[together] = (fl.get_state(), fl.solve_history())
+    (s0, h0), (s1, h1) = out[0], out[1]
+    assert np.array_equal(h0, h1, equal_nan=True)
+    for key in ("p", "T", "rho", "h", "u", "f_mdot", "b_mdot"):
+        assert np.all(np.isfinite(s1[key])), key
+        assert np.array_equal(s0[key], s1[key]), key
+
+
+def test_sod_momentum_together_changes_no_bit():
+    """The shock tube (serial triangular solves of three interleaved
+    right-hand sides): two steps with the momentum systems solved together
+    and one by one, bitwise the same."""
+    d = np.load(os.path.join(GOLDEN, "sod_split_2steps.npz"))
+    mesh = np.load(os.path.join(GOLDEN, "flow_sod1000_mesh.npz"))
+    cfg = dict(FLOW_META["sod64"])
+    cfg.update(n_cells=SOD_META["n_cells"], dt=SOD_META["dt"], inner=SOD_META["inner"], solver_type=0,
+               solver_mode=1)
+    out = {}
+    for together in (0, 1):
+        prm = params_for(cfg)
+        prm.momentum_together = together
+        with flow.Flow(flow.mesh_from_npz(mesh), d["row_ptr"], d["col"], prm) as fl:
+            fl.initialize()
+            fl.run(2, SOD_META["inner"])
+            out[together] = (fl.get_state(), fl.solve_history())
+    assert np.array_equal(out[0][1], out[1][1], equal_nan=True)
+    for key in ("p", "T", "rho", "h", "u"):
+        assert np.array_equal(out[0][0][key], out[1][0][key]), key
+
+
 def _sod(tag, steps):
     d = np.load(os.path.join(GOLDEN, f"sod_{tag}_{steps}steps.npz"))
     mesh = np.load(os.path.join(GOLDEN, "flow_sod1000_mesh.npz"))
