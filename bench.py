@@ -869,7 +869,7 @@ def sod(args, out):
     import oracle
     from opencslsolver_amd import flow
 
-    steps, inner = 200, 10
+    steps, inner = args.sod_steps, 10
     _log(f"bench: shock tube, {steps} steps x {inner} inner iterations")
     mz = np.load(os.path.join(ROOT, "tests", "golden", "flow_sod1000_mesh.npz"))
     bcs = [dict(type=flow.WALL, no_slip=0), dict(type=flow.EMPTY), dict(type=flow.WALL, no_slip=0)]
@@ -982,6 +982,8 @@ def main():
                     help="apply the cached tuned launch when present instead of tuning again")
     ap.add_argument("--skip-headline", action="store_true",
                     help="run only the side sections (for profiling them); the JSON line then has no value")
+    ap.add_argument("--sod-steps", type=int, default=200,
+                    help="time steps of the shock tube (sod section); fewer for a profiled run")
     ap.add_argument("--skip-batch-sequential", action="store_true",
                     help="batch section: leave out the same systems solved one by one")
     ap.add_argument("--profile-kernels", action="store_true",
