@@ -707,13 +707,14 @@ static void apply_precond(ocsl_solver* h, const VM* m, const X* in, X* tmp, X* t
 
 // ------------------------------------------------------------------ reference mode
 // Device-resident scalars of the REFERENCE BiCGSTAB: the dot products land
-// in S->aux (0: r0.r, 1: r0.v, 2: r.r, 3: b.b, 4: (L^-1 t).(L^-1 s), 5: (L^-1 t).(L^-1 t)), a
-// one-thread kernel forms alpha, the residual and omega with the reference's
-// formulas, and the vector updates read their coefficients from S.  The
+// in S->aux (0: r0.r, 1: r0.v, 2: r.r, 3: b.b, 4: (L^-1 t).(L^-1 s), 5: (L^-1 t).(L^-1 t)), the
+// last stage of the reduction that completes a step's inputs forms alpha,
+// the residual or omega with the reference's formulas (bref_step), and the
+// vector updates read their coefficients from S.  The
 // reference's early exit (residual below 1e-20, before x = h) sets S->halt,
 // after which the updates and the triangular solves do nothing.
-__global__ void k_bref_scalars(Scalars* S, double* hist, int step, int it) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || S->halt) return;
+__device__ __forceinline__ void bref_step(Scalars* S, double* hist, int step, int it) {
+    if (step < 0 || S->halt) return;  // step < 0: no step
     if (step == 0) {
         S->alpha = S->aux[0] / (S->aux[1] + DBL_MIN);
     } else if (step == 1) {
@@ -726,6 +727,20 @@ __global__ void k_bref_scalars(Scalars* S, double* hist, int step, int it) {
         S->omega = S->aux[4] / (S->aux[5] + 1.0e-40);
     }
 }
+
+// The totals of the dot products a step reads (aux[slot ..]), then the
+// step: one launch for the reduction's last stage and the scalar.
+template <int NR>
+struct FinBref {
+    Scalars* S;
+    double* hist;
+    int slot, step, it;
+    __device__ bool active() const { return true; }
+    __device__ void operator()(const double* t) const {
+        for (int r = 0; r < NR; ++r) S->aux[slot + r] = t[r];
+        bref_step(S, hist, step, it);
+    }
+};
 
 void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
     const int n = h->host.n;
@@ -756,10 +771,17 @@ void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
         k_dots<1><<<gd, kBlock, 0, h->stream>>>(n, d, h->red());
         k_finalize<1, FinStore<1>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinStore<1>{aux + slot});
     };
-    auto dot2 = [&](const double* a1, const double* b1, const double* a2, const double* b2, int slot) {
+    // dot products whose totals a scalar step reads, with that step
+    auto dot1s = [&](const double* a1, const double* b1, int slot, int step, int it) {
+        DotArgs<1> d{{a1}, {b1}, {0}};
+        k_dots<1><<<gd, kBlock, 0, h->stream>>>(n, d, h->red());
+        k_finalize<1, FinBref<1>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinBref<1>{S, hist_d, slot, step, it});
+    };
+    auto dot2s = [&](const double* a1, const double* b1, const double* a2, const double* b2, int slot, int step,
+                     int it) {
         DotArgs<2> d{{a1, a2}, {b1, b2}, {0, 0}};
         k_dots<2><<<gd, kBlock, 0, h->stream>>>(n, d, h->red());
-        k_finalize<2, FinStore<2>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinStore<2>{aux + slot});
+        k_finalize<2, FinBref<2>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinBref<2>{S, hist_d, slot, step, it});
     };
     auto upd = [&](double ca, const double* pa, const double* xa, double cb, const double* pb, const double* ya,
                    double* out) {
@@ -788,18 +810,15 @@ void bicgstab_reference(ocsl_solver* h, ocsl_report* rep) {
         tri_exact<double, double, 1>(h, m, p, tmp, true, S);
         tri_exact<double, double, 1>(h, m, tmp, y, false, S);
         spmv(1.0, y, 0.0, nullptr, v);
-        dot1(r0, v, 1);
-        k_bref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 0, it);    // alpha
+        dot1s(r0, v, 1, 0, it);                                        // alpha
         upd(1.0, nullptr, x, 1.0, alpha, y, hh);                       // h = x + alpha y
-        dot2(r, r, b, b, 2);
-        k_bref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 1, it);    // residual, early exit
+        dot2s(r, r, b, b, 2, 1, it);                                   // residual, early exit
         upd(-1.0, alpha, v, 1.0, nullptr, r, sv);                      // s = r - alpha v
         tri_exact<double, double, 1>(h, m, sv, tmp, true, S);
         tri_exact<double, double, 1>(h, m, tmp, z, false, S);
         spmv(1.0, z, 0.0, nullptr, t);
         tri_exact_pair<double>(h, m, t, tmp, sv, lts, true, S);       // L^-1 t, L^-1 s
-        dot2(tmp, lts, tmp, tmp, 4);
-        k_bref_scalars<<<1, 64, 0, h->stream>>>(S, hist_d, 2, it);    // omega
+        dot2s(tmp, lts, tmp, tmp, 4, 2, it);                           // omega
         upd(1.0, nullptr, hh, 1.0, omega, z, x);                       // x = h + omega z
         upd(-1.0, omega, t, 1.0, nullptr, sv, r);                      // r = s - omega t
     }
@@ -861,24 +880,6 @@ __global__ void k_set_scalars_nr(Scalars* S, Scalars v, int k) {
     if (blockIdx.x == 0 && (int)threadIdx.x < k) S[threadIdx.x] = v;
 }
 
-// k_bref_scalars for S[c] (thread c); history entry it of c at hist[3 c + it].
-__global__ void k_bref_scalars_nr(Scalars* S, double* hist, int step, int it, int k) {
-    const int c = threadIdx.x;
-    if (blockIdx.x != 0 || c >= k) return;
-    Scalars* sc = S + c;
-    if (sc->halt) return;
-    if (step == 0) {
-        sc->alpha = sc->aux[0] / (sc->aux[1] + DBL_MIN);
-    } else if (step == 1) {
-        const double res = sc->aux[2] / sc->aux[3] + DBL_MIN;
-        sc->rr = res;
-        hist[3 * c + it] = res;
-        sc->iter = it + 1;
-        if (res < 1.0e-20) sc->halt = 1;
-    } else {
-        sc->omega = sc->aux[4] / (sc->aux[5] + 1.0e-40);
-    }
-}
 
 __device__ __forceinline__ double scalar_at(const Scalars* s, size_t off) {
     return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(s) + off);
@@ -927,15 +928,19 @@ __global__ void __launch_bounds__(kBlock) k_dots_nr(int n, DotArgs<K> args, RedS
     block_partials<NR * K>(acc, red);
 }
 
-// Totals c K + j to S[c].aux[slot + j].
+// Totals c K + j to S[c].aux[slot + j], then bref_step of each right-hand
+// side (history entry it of c at hist[3 c + it]).
 template <int NR, int K>
 struct FinAux {
     Scalars* S;
-    int slot;
+    double* hist;
+    int slot, step, it;
     __device__ bool active() const { return true; }
     __device__ void operator()(const double* t) const {
-        for (int c = 0; c < NR; ++c)
+        for (int c = 0; c < NR; ++c) {
             for (int j = 0; j < K; ++j) S[c].aux[slot + j] = t[c * K + j];
+            bref_step(S + c, hist + 3 * c, step, it);
+        }
     }
 };
 
@@ -982,15 +987,19 @@ void bicgstab_reference_nr(ocsl_solver* h, const double* b, double* x, ocsl_repo
                                                                                              xin, bz, zin, out);
     };
     const int gd = red_blocks(kBlock * 4, n);
-    auto dot1 = [&](const double* a1, const double* b1, int slot) {
+    // step < 0: the totals only; else with the scalar step that reads them
+    auto dot1 = [&](const double* a1, const double* b1, int slot, int step = -1, int it = 0) {
         DotArgs<1> d{{a1}, {b1}, {0}};
         k_dots_nr<NR, 1><<<gd, kBlock, 0, h->stream>>>(n, d, h->red());
-        k_finalize<NR, FinAux<NR, 1>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinAux<NR, 1>{S, slot});
+        k_finalize<NR, FinAux<NR, 1>><<<1, kBlock, 0, h->stream>>>(h->red(), gd,
+                                                                    FinAux<NR, 1>{S, hist_d, slot, step, it});
     };
-    auto dot2 = [&](const double* a1, const double* b1, const double* a2, const double* b2, int slot) {
+    auto dot2 = [&](const double* a1, const double* b1, const double* a2, const double* b2, int slot, int step,
+                    int it) {
         DotArgs<2> d{{a1, a2}, {b1, b2}, {0, 0}};
         k_dots_nr<NR, 2><<<gd, kBlock, 0, h->stream>>>(n, d, h->red());
-        k_finalize<2 * NR, FinAux<NR, 2>><<<1, kBlock, 0, h->stream>>>(h->red(), gd, FinAux<NR, 2>{S, slot});
+        k_finalize<2 * NR, FinAux<NR, 2>><<<1, kBlock, 0, h->stream>>>(h->red(), gd,
+                                                                        FinAux<NR, 2>{S, hist_d, slot, step, it});
     };
     auto upd = [&](double ca, size_t oa, const double* xa, double cb, size_t ob, const double* ya, double* out) {
         k_axpby_nr<NR><<<gv, kBlock, 0, h->stream>>>(n, ca, oa, xa, cb, ob, ya, out, S);
@@ -1031,18 +1040,15 @@ void bicgstab_reference_nr(ocsl_solver* h, const double* b, double* x, ocsl_repo
             lower(p, tmp);
             upper(tmp, y);
             spmv(1.0, y, 0.0, nullptr, v);
-            dot1(r0, v, 1);
-            k_bref_scalars_nr<<<1, 64, 0, h->stream>>>(S, hist_d, 0, it, NR);
+            dot1(r0, v, 1, 0, it);
             upd(1.0, kNoCoef, x, 1.0, alpha, y, hh);
-            dot2(r, r, b, b, 2);
-            k_bref_scalars_nr<<<1, 64, 0, h->stream>>>(S, hist_d, 1, it, NR);
+            dot2(r, r, b, b, 2, 1, it);
             upd(-1.0, alpha, v, 1.0, kNoCoef, r, sv);
             lower(sv, tmp);
             upper(tmp, z);
             spmv(1.0, z, 0.0, nullptr, t);
             lower_pair(t, tmp, sv, lts);
-            dot2(tmp, lts, tmp, tmp, 4);
-            k_bref_scalars_nr<<<1, 64, 0, h->stream>>>(S, hist_d, 2, it, NR);
+            dot2(tmp, lts, tmp, tmp, 4, 2, it);
             upd(1.0, kNoCoef, hh, 1.0, omega, z, x);
             upd(-1.0, omega, t, 1.0, kNoCoef, sv, r);
         }
